@@ -1,0 +1,301 @@
+#!/usr/bin/env python3
+"""Retained-operator throughput of the hip provider on one MI355X.
+
+One step = one retained-operator evaluation of the 256^2 x Nz65 x 3 workload
+with inputs resident in HBM:
+  1. pruned inverse c2r   compact retained (z, field, mode) -> real planes
+  2. pruned forward r2c   real planes -> compact retained (2/3-rule truncated)
+  3. K^2-grouped vertical GEMM on MFMA -> modal
+The headline runs in FP32 storage (FP32 transforms, FP32 MFMA vertical
+products); the plain FP64 GPU form is timed beside it and supplies the
+reference for the FP32 error.  The CPU column times the FP64 oracle
+(pocketfft + BLAS) on this machine's host cores on a bounded sample.
+
+Prints ONE JSON line.  `--gpus N` with N > 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "spectral-kernel-benchmarks_amd", "python"))
+
+METRIC = "retained-operator evals/s @256²×Nz65×3, FP32 vs FP64 at 1e-6 err; %HBM; p99 ms"
+HBM_PEAK_GBPS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_retained_operator_256_nz65_f3.json")
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--family", choices=["qr32", "k2"], default="qr32",
+                    help="qr32: 32 K^2 bins of 65x65 QR-Gaussian matrices; k2: reference 2045x(42x65) family")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--quiet", action="store_true", help="no progress lines on stderr")
+    return ap.parse_args()
+
+
+def log(args, msg):
+    if not args.quiet:
+        print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+class Operator:
+    """Buffers and plans of one retained-operator evaluation at one precision."""
+
+    def __init__(self, S, torch, w, modes, groups, nj, fwd, inv, x64, precision, device):
+        self.S, self.torch, self.w, self.nj = S, torch, w, nj
+        self.pruned = S.PrunedPlan(w, modes, precision)
+        self.vert = S.VerticalPlan(w, nj, groups, fwd, inv, precision)
+        cd, rd = S.complex_dtype(precision), S.real_dtype(precision)
+        self.x = torch.from_numpy(x64).to(device=device, dtype=cd)
+        self.real = torch.empty(w.real_elements, dtype=rd, device=device)
+        self.back = torch.empty(len(modes) * w.planes, dtype=cd, device=device)
+        self.modal = torch.empty(len(modes) * w.fields * nj, dtype=cd, device=device)
+        self.stages = [
+            ("inverse_columns", lambda: self.pruned.inverse_columns(self.x)),
+            ("inverse_rows", lambda: self.pruned.inverse_rows(self.real)),
+            ("forward_rows", lambda: self.pruned.forward_rows(self.real)),
+            ("forward_columns", lambda: self.pruned.forward_columns(self.back)),
+            ("vertical_gemm", lambda: self.vert.forward(self.back, self.modal)),
+        ]
+
+    def step(self):
+        for _, fn in self.stages:
+            fn()
+
+
+def timed(torch, op, steps):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        op.step()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def instrumented(torch, op, steps):
+    """Per-stage and per-eval device times from HIP events recorded on the
+    stream the kernels run on (torch's current stream)."""
+    names = [n for n, _ in op.stages]
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 1)] for _ in range(steps)]
+    torch.cuda.synchronize()
+    for s in range(steps):
+        ev[s][0].record()
+        for i, (_, fn) in enumerate(op.stages):
+            fn()
+            ev[s][i + 1].record()
+    torch.cuda.synchronize()
+    per_stage = {n: [] for n in names}
+    per_eval = []
+    for s in range(steps):
+        for i, n in enumerate(names):
+            per_stage[n].append(ev[s][i].elapsed_time(ev[s][i + 1]))
+        per_eval.append(ev[s][0].elapsed_time(ev[s][-1]))
+    return per_stage, per_eval
+
+
+def percentile(values, q):
+    v = sorted(values)
+    if not v:
+        return None
+    k = (len(v) - 1) * q
+    lo, hi = int(k), min(int(k) + 1, len(v) - 1)
+    return v[lo] + (v[hi] - v[lo]) * (k - lo)
+
+
+def cpu_baseline(args, w, x64, nj, groups, fwd):
+    """FP64 oracle timed on the host cores; also returns its output, used only
+    as the checker of the GPU results."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # checker / baseline only
+
+    workers = max(1, min(16, os.cpu_count() or 1))
+    ow = O.Workload(w.nx, w.ny, w.nz, w.fields)
+    om = O.retained_modes(ow)
+    first = groups["first_mode"].copy()
+    count = groups["mode_count"].copy()
+    out = O.retained_operator_fast(ow, om, x64, nj, first, count, fwd, workers)  # warm (plans, pages)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        O.retained_operator_fast(ow, om, x64, nj, first, count, fwd, workers)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= args.cpu_seconds or n >= 200:
+            break
+    rec = {
+        "value": n / el,
+        "unit": "evals/s",
+        "cores": workers,
+        "kind": "port",
+        "sample": "%d full 256x256xNz65x3 FP64 retained-operator evals in %.1f s: scipy pocketfft "
+                  "rfft2/irfft2 (workers=%d, not FFTW), C gather/embed, numpy BLAS vertical"
+                  % (n, el, workers),
+        "ms_per_eval": 1e3 * el / n,
+    }
+    return rec, out, O
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (requested --gpus %d)" % args.gpus,
+              file=sys.stderr)
+        return 2
+    import numpy as np
+    import torch
+
+    import skb_hip as S
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 2
+    device = torch.device("cuda:0")
+    w = S.Workload(256, 256, 65, 3)
+    log(args, "setup %s family=%s" % (w, args.family))
+    modes = S.retained_modes(w)
+    if args.family == "qr32":
+        groups = S.k2_binned_groups(modes, 32)
+        nj = w.nz
+        fwd, inv = S.qr_gaussian_family(w.nz, len(groups), 129)
+        family = "32 K^2 bins x 65x65 QR-Gaussian (seed 129)"
+    else:
+        groups = S.k2_groups(modes)
+        nj = w.nj
+        fwd, inv = S.k2_vertical_family(w.nz, nj, groups)
+        family = "%d exact-K^2 groups x %dx%d pair-rotated DCT-II" % (len(groups), nj, w.nz)
+    x64 = S.spectral_fixture(w, modes, 129, -1.0)
+
+    ops = {p: Operator(S, torch, w, modes, groups, nj, fwd, inv, x64, p, device) for p in (S.SKB_F32, S.SKB_F64)}
+    f32, f64 = ops[S.SKB_F32], ops[S.SKB_F64]
+
+    # FP32 versus the FP64 GPU form on identical inputs
+    f32.step()
+    f64.step()
+    torch.cuda.synchronize()
+    a = f32.modal.cpu().numpy().astype(np.complex128)
+    b = f64.modal.cpu().numpy()
+    err_max = float(np.abs(a - b).max() / max(1.0, np.abs(b).max()))
+    err_l2 = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    log(args, "fp32 vs fp64 max-scale err %.3e, rel L2 %.3e" % (err_max, err_l2))
+
+    log(args, "warmup %d" % args.warmup)
+    for _ in range(args.warmup):
+        f32.step()
+    log(args, "timed %d steps (fp32)" % args.steps)
+    wall = timed(torch, f32, args.steps)
+    value = args.steps / wall
+    log(args, "fp32 %.1f evals/s (%.2f us/eval)" % (value, 1e6 * wall / args.steps))
+    per_stage, per_eval = instrumented(torch, f32, args.steps)
+    log(args, "fp64 form")
+    for _ in range(max(2, args.warmup // 4)):
+        f64.step()
+    f64_steps = max(10, args.steps // 4)
+    wall64 = timed(torch, f64, f64_steps)
+    _, per_eval64 = instrumented(torch, f64, f64_steps)
+
+    # algorithmic (stage-boundary) bytes attributed to each kernel, FP32 storage
+    nkl, P = len(modes), w.planes
+    ret_b, real_b = nkl * P * 8, w.real_elements * 4
+    modal_b = nkl * w.fields * nj * 8
+    mat_b = len(groups) * nj * w.nz * 4
+    algo = {
+        "inverse_columns": ret_b,
+        "inverse_rows": real_b,
+        "forward_rows": real_b,
+        "forward_columns": ret_b,
+        "vertical_gemm": ret_b + modal_b + mat_b,
+    }
+    eval_bytes = sum(algo.values())
+    kernels = {}
+    for n, ms in per_stage.items():
+        avg = sum(ms) / len(ms)
+        kernels[n] = {"avg_us": 1e3 * avg, "algorithmic_bytes": algo[n], "GBps": algo[n] / (avg * 1e-3) / 1e9}
+    dominant = max(kernels, key=lambda n: kernels[n]["avg_us"])
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        try:
+            tf = json.load(open(TRAFFIC_FILE))
+            if tf.get("family") == args.family:
+                traffic = tf.get("kernels", {}).get(dominant, {}).get("hbm_bytes")
+        except ValueError:
+            traffic = None
+    dk = kernels[dominant]
+    roofline = {
+        "bound": "hbm",
+        "kernel": dominant,
+        "achieved": dk["GBps"],
+        "peak": HBM_PEAK_GBPS,
+        "unit": "GB/s",
+        "frac": dk["GBps"] / HBM_PEAK_GBPS,
+        "traffic": traffic,
+        "algorithmic_bytes_per_launch": dk["algorithmic_bytes"],
+    }
+    eval_avg_ms = sum(per_eval) / len(per_eval)
+    eval_roofline = {
+        "bytes_per_eval": eval_bytes,
+        "achieved": eval_bytes / (eval_avg_ms * 1e-3) / 1e9,
+        "frac": eval_bytes / (eval_avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+        "from": "sum of instrumented kernel times",
+        "wall_achieved": eval_bytes * value / 1e9,
+        "wall_frac": eval_bytes * value / 1e9 / HBM_PEAK_GBPS,
+    }
+
+    cpu = None
+    err_oracle = None
+    if not args.no_cpu_baseline:
+        log(args, "cpu baseline (oracle, bounded sample of %.0f s)" % args.cpu_seconds)
+        cpu, ref, O = cpu_baseline(args, w, x64, nj, groups, fwd)
+        err_oracle = {
+            "fp32_max_scale": O.max_relative_error(a, ref),
+            "fp32_relative_l2": O.relative_l2_error(a, ref),
+            "fp64_max_scale": O.max_relative_error(b, ref),
+        }
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded complex-Gaussian retained spectra x (1+|k|^2)^-1 (DC real), " + family,
+        "config": {
+            "workload": "retained-mode operator 256x256 Nz65 fields3: pruned c2r -> r2c (2/3 radial) -> "
+                        "K^2-grouped vertical GEMM",
+            "nx": 256, "ny": 256, "nz": 65, "fields": 3, "retained_modes": nkl, "vertical_family": args.family,
+            "vertical_groups": len(groups), "nj": nj,
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "eval_roofline": eval_roofline,
+        "latency_ms": {"p50": percentile(per_eval, 0.5), "p99": percentile(per_eval, 0.99),
+                       "mean": eval_avg_ms},
+        "kernels": kernels,
+        "error_vs_fp64_gpu": {"max_scale_normalized": err_max, "relative_l2": err_l2, "tolerance": 1e-6,
+                              "passed": err_max <= 1e-6 and err_l2 <= 1e-6},
+        "error_vs_fp64_oracle": err_oracle,
+        "fp64": {"value": f64_steps / wall64, "unit": "evals/s", "ms_per_step": 1e3 * wall64 / f64_steps,
+                 "p50_ms": percentile(per_eval64, 0.5), "p99_ms": percentile(per_eval64, 0.99),
+                 "bytes_per_eval": 2 * eval_bytes},
+        "device": torch.cuda.get_device_name(0),
+        "library": S.version(),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

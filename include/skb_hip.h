@@ -1,0 +1,190 @@
+/*
+ * skb_hip.h -- C ABI of the MI355X (gfx950) `hip` provider for skbench's
+ * retained-mode spectral-operator path.
+ *
+ * Every compute entry point works on DEVICE pointers and enqueues on the
+ * caller's hipStream_t (passed as void*; NULL = default stream).  Nothing
+ * here allocates after plan creation.  Plans are not re-entrant: one
+ * in-flight call per plan (the reference providers are single-caller too,
+ * include/skbench/skbench.hpp:364-372).
+ *
+ * Status codes map to the reference's exception types (src/fftw_provider.cpp
+ * :468-487, src/vertical_gemm.cpp:431-463):
+ *   SKB_INVALID_ARGUMENT -> std::invalid_argument  (bad shape / stride / size)
+ *   SKB_RUNTIME_ERROR    -> std::runtime_error     (HIP or device failure)
+ *   SKB_LOGIC_ERROR      -> std::logic_error       (wrong entry point / layout)
+ *   SKB_BAD_ALLOC        -> std::bad_alloc
+ *   SKB_UNSUPPORTED      -> provider capability() "unsupported" record
+ * skb_last_error() returns the calling thread's last message.
+ *
+ * Layouts are the reference's, element for element (src/model.cpp:219-244):
+ *   real       x + Nx*(y + Ny*(z + Nz*field))
+ *   WVM        z + Nz*field + P*(kx + (Nx/2+1)*ky),   P = Nz*fields
+ *   retained   z + Nz*field + P*mode
+ *   modal      j + Nj*field + Nj*fields*mode
+ * Complex values are interleaved (re, im) pairs of the plan's precision:
+ * SKB_F64 -> double[2] (binary-identical to skbench::Complex), SKB_F32 ->
+ * float[2].
+ */
+#ifndef SKB_HIP_H
+#define SKB_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum skb_status {
+    SKB_OK = 0,
+    SKB_INVALID_ARGUMENT = 1,
+    SKB_RUNTIME_ERROR = 2,
+    SKB_LOGIC_ERROR = 3,
+    SKB_BAD_ALLOC = 4,
+    SKB_UNSUPPORTED = 5
+};
+
+enum skb_precision { SKB_F64 = 0, SKB_F32 = 1 };
+
+/* skbench::Workload (include/skbench/skbench.hpp:23-39). */
+typedef struct skb_workload {
+    uint64_t nx, ny, nz, fields;
+    double lx, ly;
+    int32_t antialias;
+    int32_t reserved;
+} skb_workload;
+
+/* skbench::RetainedMode (include/skbench/skbench.hpp:41-48). */
+typedef struct skb_retained_mode {
+    int64_t k, l;
+    uint64_t stored_kx, stored_ky;
+    int32_t conjugates_stored_value;
+    int32_t reserved;
+    double radial;
+} skb_retained_mode;
+
+/* skbench::VerticalModeGroup (include/skbench/skbench.hpp:50-56). */
+typedef struct skb_mode_group {
+    uint64_t squared_key, first_mode, mode_count;
+} skb_mode_group;
+
+/* ---- errors / library identity --------------------------------------- */
+/* Copies the calling thread's last error message (NUL-terminated). */
+size_t skb_last_error(char* buffer, size_t capacity);
+/* Library identity string, e.g. "skb-hip gfx950" (FFTWProvider::version,
+ * include/skbench/skbench.hpp:415-416). */
+const char* skb_version(void);
+
+/* ---- host model (src/model.cpp) -------------------------------------- */
+/* retainedHorizontalModes (src/model.cpp:129-167).  Pass modes=NULL to get
+ * the count only. */
+int skb_retained_modes(const skb_workload* workload, skb_retained_mode* modes, uint64_t capacity,
+                       uint64_t* count);
+/* squaredWavenumberGroups (src/model.cpp:169-207). */
+int skb_k2_groups(const skb_retained_mode* modes, uint64_t mode_count, skb_mode_group* groups,
+                  uint64_t capacity, uint64_t* count);
+/* Contiguous bins of K^2 = k^2 + l^2 over the retained order:
+ * bin = floor(bins * K^2 / (K^2_max + 1)), empty bins dropped.  Builds the
+ * coarse grouping of the 32-bin vertical family; no reference counterpart. */
+int skb_k2_binned_groups(const skb_retained_mode* modes, uint64_t mode_count, uint64_t bins,
+                         skb_mode_group* groups, uint64_t capacity, uint64_t* count);
+/* FNV-1a hashes (src/model.cpp:209-217, 462-487). */
+uint64_t skb_mode_order_hash(const skb_retained_mode* modes, uint64_t count);
+uint64_t skb_group_order_hash(const skb_mode_group* groups, uint64_t count);
+uint64_t skb_wvm_order_hash(const skb_workload* workload);
+/* orthonormalVerticalFixture (src/model.cpp:489-506): forward nj x nz,
+ * inverse nz x nj, row-major. */
+int skb_orthonormal_vertical(uint64_t nz, uint64_t nj, double* forward, double* inverse);
+/* squaredWavenumberVerticalFixture (src/model.cpp:523-565): one nj x nz
+ * (forward) and nz x nj (inverse) matrix per group, contiguous. */
+int skb_k2_vertical_family(uint64_t nz, uint64_t nj, const skb_mode_group* groups, uint64_t group_count,
+                           double* forward, double* inverse);
+/* n x n orthogonal family from the QR factorisation of seeded Gaussian
+ * matrices (group g uses seed + g); forward = Q^T, inverse = Q.  No
+ * reference counterpart (32-bin 65x65 family). */
+int skb_qr_gaussian_family(uint64_t n, uint64_t group_count, uint64_t seed, double* forward, double* inverse);
+/* makeFixture (src/model.cpp:618-661).  kind: 0 impulse, 1 sinusoid (1,2),
+ * 2 random = uniform[-1,1) from the portable generator of
+ * src/benchmark.cpp:187-196, 3 dc, 4 nyquist, 5 sinusoid (5,17). */
+int skb_make_fixture(const skb_workload* workload, int kind, uint64_t seed, double* real);
+/* Compact retained spectrum: complex Gaussian (Box-Muller over the same
+ * generator) times (1 + k^2 + l^2)^exponent, DC imaginary part zero. */
+int skb_spectral_fixture(const skb_workload* workload, const skb_retained_mode* modes, uint64_t mode_count,
+                         uint64_t seed, double exponent, double* retained_interleaved);
+
+/* ---- full batched 2-D r2c / c2r in WVM order ---------------------------
+ * Replaces FFTWProvider (include/skbench/skbench.hpp:364-421;
+ * src/fftw_provider.cpp:304-316 forward/inverse, 537-610 guru64 geometry). */
+typedef struct skb_fft_plan skb_fft_plan;
+int skb_fft_plan_create(const skb_workload* workload, int precision, skb_fft_plan** plan);
+int skb_fft_plan_destroy(skb_fft_plan* plan);
+/* FFTWProvider::forward: unnormalised r2c, real -> WVM half spectrum. */
+int skb_fft_forward(skb_fft_plan* plan, const void* real, void* wvm, void* stream);
+/* FFTWProvider::inverse: unnormalised c2r; DESTROYS `wvm` (as FFTW c2r may,
+ * src/fftw_provider.cpp:1290-1292). */
+int skb_fft_inverse(skb_fft_plan* plan, void* wvm, void* real, void* stream);
+/* FFTWProvider::gatherRetainedOuter / embedRetainedOuter
+ * (src/fftw_provider.cpp:318-329; src/model.cpp:310-339). */
+int skb_fft_gather_retained(skb_fft_plan* plan, const skb_retained_mode* modes, uint64_t mode_count,
+                            const void* wvm, void* retained, void* stream);
+int skb_fft_embed_retained(skb_fft_plan* plan, const skb_retained_mode* modes, uint64_t mode_count,
+                           const void* retained, void* wvm, void* stream);
+
+/* ---- pruned retained-mode transforms -----------------------------------
+ * Replaces FFTWPrunedProvider (include/skbench/skbench.hpp:467-520;
+ * src/pruned_fftw_provider.cpp:230-241 forward, 265-276 inverse): only the
+ * active k_x band is transformed along y and only retained modes are read
+ * or written.  Caller inputs are preserved. */
+typedef struct skb_pruned_plan skb_pruned_plan;
+int skb_pruned_plan_create(const skb_workload* workload, const skb_retained_mode* modes, uint64_t mode_count,
+                           int precision, skb_pruned_plan** plan);
+int skb_pruned_plan_destroy(skb_pruned_plan* plan);
+/* T_h^+ = P_h F_xy: real planes -> compact retained (z, field, mode). */
+int skb_pruned_forward(skb_pruned_plan* plan, const void* real, void* retained, void* stream);
+/* T_h^- = F_xy^-1 E_h: compact retained -> real planes (unnormalised). */
+int skb_pruned_inverse(skb_pruned_plan* plan, const void* retained, void* real, void* stream);
+/* Stage-by-stage form of the two calls above, for per-kernel timing
+ * (FFTWPrunedProvider::executeForwardRows / executeForwardColumns +
+ * gatherForward / embedInverse + executeInverseColumns / executeInverseRows,
+ * include/skbench/skbench.hpp:478-489).  The intermediate lives in the
+ * plan's scratch, so a forward_rows must precede forward_columns and an
+ * inverse_columns must precede inverse_rows on the same stream. */
+int skb_pruned_forward_rows(skb_pruned_plan* plan, const void* real, void* stream);
+int skb_pruned_forward_columns(skb_pruned_plan* plan, void* retained, void* stream);
+int skb_pruned_inverse_columns(skb_pruned_plan* plan, const void* retained, void* stream);
+int skb_pruned_inverse_rows(skb_pruned_plan* plan, void* real, void* stream);
+/* FFTWPrunedProvider::activeKxCount / scratchBytes. */
+uint64_t skb_pruned_active_kx(const skb_pruned_plan* plan);
+uint64_t skb_pruned_scratch_bytes(const skb_pruned_plan* plan);
+
+/* ---- K^2-grouped vertical GEMM -----------------------------------------
+ * Replaces VerticalGemmProvider (include/skbench/skbench.hpp:152-218;
+ * src/vertical_gemm.cpp:534-596 grouped zgemm / split dgemm): per group g,
+ * modal[:, cols_g] = F_g (nj x nz) * physical[:, cols_g] and
+ * physical[:, cols_g] = I_g (nz x nj) * modal[:, cols_g], columns
+ * (field + fields*mode) of contiguous z (or j) vectors.  Runs on MFMA
+ * (f64 16x16x4 for SKB_F64, f32 16x16x4 for SKB_F32). */
+typedef struct skb_vertical_plan skb_vertical_plan;
+int skb_vertical_plan_create(const skb_workload* workload, uint64_t nj, const skb_mode_group* groups,
+                             uint64_t group_count, const double* forward, const double* inverse, int precision,
+                             skb_vertical_plan** plan);
+int skb_vertical_plan_destroy(skb_vertical_plan* plan);
+int skb_vertical_forward(skb_vertical_plan* plan, const void* physical, void* modal, void* stream);
+int skb_vertical_inverse(skb_vertical_plan* plan, const void* modal, void* physical, void* stream);
+/* VerticalGemmProvider::matrixBytesPerDirection. */
+uint64_t skb_vertical_matrix_bytes(const skb_vertical_plan* plan);
+
+/* ---- pointwise advection -----------------------------------------------
+ * Replaces PointwiseAdvectionExecutor::execute (src/pointwise_advection.hpp
+ * :18-43; src/pointwise_advection.cpp:152-166):
+ * target = -scale * (u*qx + v*qy + w*qz) with shared = [u|v|w] and
+ * derivative = [qx|qy|qz], each block `volume` elements. */
+int skb_pointwise_advection(uint64_t volume, double scale, const void* shared, const void* derivative,
+                            void* target, int precision, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SKB_HIP_H */

@@ -1,0 +1,424 @@
+"""CPU oracle for the retained-mode spectral operator -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+bench.py may import this module; the HIP product never does.  It wraps the C
+restatement in ``skb_oracle.c`` (built to ``oracle/build/liboracle.so``) and
+adds the FP64 full-size paths with scipy's pocketfft, which is an FFT
+implementation independent of both the C direct DFT and the HIP kernels.
+
+Reference citations are to the skbench source tree.
+
+Pinning summary (details in tests/test_oracle.py):
+  * retained_modes/mode_order_hash/k2_groups/group_hash/wvm_order_hash:
+    published golden hashes and counts (tests/golden/published_workloads.json).
+  * direct_r2c/direct_c2r: closed forms + agreement with pocketfft.
+  * fft_r2c_wvm/fft_c2r_wvm: agreement with the direct DFT on small grids,
+    round trip c2r(r2c(x)) == Nx*Ny*x.
+  * retained_operator: closed form NxNy * vertical(x) on Hermitian-consistent
+    inputs, and a direct-DFT composition on small grids.
+  * k2_binned_groups / qr_gaussian_family: no reference counterpart (the
+    32-bin QR-Gaussian family is a BASELINE-only variant).  Pinned by QR
+    uniqueness: Q^T Q = I and Q^T G upper triangular with positive diagonal.
+  * spectral_gaussian: no reference counterpart; "parity unpinned" beyond its
+    statistical and Hermitian properties (it only generates inputs).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+
+MODE_DTYPE = np.dtype(
+    [
+        ("k", "<i8"),
+        ("l", "<i8"),
+        ("stored_kx", "<u8"),
+        ("stored_ky", "<u8"),
+        ("conj", "<i4"),
+        ("pad", "<i4"),
+        ("radial", "<f8"),
+    ]
+)
+
+FIXTURES = {"impulse": 0, "sinusoid": 1, "random": 2, "dc": 3, "nyquist": 4, "sinusoid-5-17": 5}
+
+
+def build() -> str:
+    """Compile the C oracle with gcc (idempotent)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        u64, i64, dbl, vp = ctypes.c_uint64, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+        sig = {
+            "or_retained_vertical_modes": (u64, [u64]),
+            "or_retained_modes": (i64, [u64, u64, dbl, dbl, ctypes.c_int, vp, u64]),
+            "or_mode_order_hash": (u64, [vp, u64]),
+            "or_wvm_order_hash": (u64, [u64, u64, u64, u64]),
+            "or_squared_key": (u64, [i64, i64]),
+            "or_k2_groups": (i64, [vp, u64, vp, vp, vp]),
+            "or_group_hash": (u64, [vp, vp, vp, u64]),
+            "or_orthonormal_vertical": (None, [u64, u64, vp, vp]),
+            "or_k2_vertical_family": (None, [u64, u64, u64, vp, vp, vp]),
+            "or_signed_unit": (dbl, [vp]),
+            "or_make_fixture": (ctypes.c_int, [u64, u64, u64, u64, ctypes.c_int, u64, vp]),
+            "or_direct_r2c": (None, [u64, u64, u64, u64, vp, vp]),
+            "or_direct_c2r": (None, [u64, u64, u64, u64, vp, vp]),
+            "or_direct_retained": (None, [u64, u64, u64, u64, vp, u64, vp, vp]),
+            "or_gather_retained": (None, [u64, u64, u64, u64, vp, u64, vp, vp]),
+            "or_embed_retained": (None, [u64, u64, u64, u64, vp, u64, vp, vp]),
+            "or_vertical_forward": (None, [u64, u64, u64, u64, vp, vp, vp, vp, vp]),
+            "or_vertical_inverse": (None, [u64, u64, u64, u64, vp, vp, vp, vp, vp]),
+            "or_pointwise_advection": (None, [u64, dbl, vp, vp, vp]),
+            "or_max_relative_error_complex": (dbl, [vp, vp, u64]),
+            "or_max_relative_error_real": (dbl, [vp, vp, u64, dbl]),
+            "or_relative_l2_error_complex": (dbl, [vp, vp, u64]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+@dataclass(frozen=True)
+class Workload:
+    """skbench Workload (include/skbench/skbench.hpp:23-39)."""
+
+    nx: int
+    ny: int
+    nz: int
+    fields: int
+    lx: float = 1.0
+    ly: float = 1.0
+    antialias: bool = True
+
+    @property
+    def planes(self) -> int:
+        return self.nz * self.fields
+
+    @property
+    def nx_half(self) -> int:
+        return self.nx // 2 + 1
+
+    @property
+    def real_elements(self) -> int:
+        return self.nx * self.ny * self.planes
+
+    @property
+    def spectrum_elements(self) -> int:
+        return self.nx_half * self.ny * self.planes
+
+    @property
+    def nj(self) -> int:
+        return int(lib().or_retained_vertical_modes(self.nz))
+
+
+def fnv(h: int) -> str:
+    return "fnv1a64:%016x" % h
+
+
+# ---- modes, groups, hashes ------------------------------------------------
+def retained_modes(w: Workload) -> np.ndarray:
+    L = lib()
+    n = L.or_retained_modes(w.nx, w.ny, w.lx, w.ly, int(w.antialias), None, 0)
+    if n < 0:
+        raise ValueError("invalid workload")
+    out = np.zeros(n, dtype=MODE_DTYPE)
+    L.or_retained_modes(w.nx, w.ny, w.lx, w.ly, int(w.antialias), _p(out), n)
+    return out
+
+
+def mode_order_hash(modes: np.ndarray) -> str:
+    return fnv(lib().or_mode_order_hash(_p(modes), len(modes)))
+
+
+def wvm_order_hash(w: Workload) -> str:
+    return fnv(lib().or_wvm_order_hash(w.nx, w.ny, w.nz, w.fields))
+
+
+def k2_groups(modes: np.ndarray):
+    n = len(modes)
+    key = np.zeros(n, np.uint64)
+    first = np.zeros(n, np.uint64)
+    count = np.zeros(n, np.uint64)
+    g = lib().or_k2_groups(_p(modes), n, _p(key), _p(first), _p(count))
+    if g < 0:
+        raise ValueError("equal squared-wavenumber modes are not contiguous")
+    return key[:g].copy(), first[:g].copy(), count[:g].copy()
+
+
+def group_hash(key, first, count) -> str:
+    key, first, count = (np.ascontiguousarray(a, np.uint64) for a in (key, first, count))
+    return fnv(lib().or_group_hash(_p(key), _p(first), _p(count), len(key)))
+
+
+def squared_keys(modes: np.ndarray) -> np.ndarray:
+    k = modes["k"].astype(np.int64)
+    l = modes["l"].astype(np.int64)
+    return (k * k + l * l).astype(np.uint64)
+
+
+def k2_binned_groups(modes: np.ndarray, bins: int = 32):
+    """BASELINE-only grouping: bin b = floor(bins*K2/(K2max+1)) over the
+    retained order, empty bins dropped.  Returns (bin_ids, first, count)."""
+    k2 = squared_keys(modes).astype(np.int64)
+    b = (bins * k2) // (int(k2.max()) + 1)
+    ids, first, count = [], [], []
+    for i, bi in enumerate(b):
+        if not ids or ids[-1] != bi:
+            ids.append(int(bi))
+            first.append(i)
+            count.append(1)
+        else:
+            count[-1] += 1
+    if len(set(ids)) != len(ids):
+        raise ValueError("bins are not contiguous in the retained order")
+    return np.array(ids, np.uint64), np.array(first, np.uint64), np.array(count, np.uint64)
+
+
+# ---- vertical fixtures ----------------------------------------------------
+def orthonormal_vertical(nz: int, nj: int):
+    f = np.zeros(nj * nz)
+    i = np.zeros(nz * nj)
+    lib().or_orthonormal_vertical(nz, nj, _p(f), _p(i))
+    return f, i
+
+
+def k2_vertical_family(nz: int, nj: int, keys: np.ndarray):
+    keys = np.ascontiguousarray(keys, np.uint64)
+    g = len(keys)
+    f = np.zeros(g * nj * nz)
+    i = np.zeros(g * nz * nj)
+    lib().or_k2_vertical_family(nz, nj, g, _p(keys), _p(f), _p(i))
+    return f, i
+
+
+def signed_units(seed: int, count: int) -> np.ndarray:
+    state = ctypes.c_uint64(0x9E3779B97F4A7C15 if seed == 0 else seed)
+    fn = lib().or_signed_unit
+    ptr = ctypes.byref(state)
+    return np.array([fn(ptr) for _ in range(count)])
+
+
+def gaussians(seed: int, count: int) -> np.ndarray:
+    """Box-Muller pairs over the portable generator: pair i uses draws 2i and
+    2i+1; an odd count drops the last sine."""
+    pairs = (count + 1) // 2
+    u = signed_units(seed, 2 * pairs)
+    u1 = (u[0::2] + 1.0) * 0.5
+    u2 = (u[1::2] + 1.0) * 0.5
+    r = np.sqrt(-2.0 * np.log(1.0 - u1))
+    out = np.empty(2 * pairs)
+    out[0::2] = r * np.cos(2.0 * np.pi * u2)
+    out[1::2] = r * np.sin(2.0 * np.pi * u2)
+    return out[:count]
+
+
+def qr_gaussian_family(n: int, groups: int, seed: int) -> np.ndarray:
+    """BASELINE-only vertical family: per group, Q of the QR factorisation of
+    an n x n Gaussian matrix (row-major, drawn from gaussians(seed + group)),
+    with R's diagonal made positive.  Returns forward matrices (groups*n*n,
+    row-major Q^T so that modal = Q^T phys) -- the inverse is Q."""
+    out = np.zeros((groups, n, n))
+    for g in range(groups):
+        G = gaussians(seed + g, n * n).reshape(n, n)
+        Q, R = np.linalg.qr(G)
+        s = np.sign(np.diag(R))
+        s[s == 0] = 1.0
+        Q = Q * s[None, :]
+        out[g] = Q.T
+    return out.reshape(-1)
+
+
+# ---- fixtures ---------------------------------------------------------------
+def make_fixture(w: Workload, kind: str, seed: int) -> np.ndarray:
+    out = np.zeros(w.real_elements)
+    if lib().or_make_fixture(w.nx, w.ny, w.nz, w.fields, FIXTURES[kind], seed, _p(out)) != 0:
+        raise ValueError(kind)
+    return out
+
+
+def spectral_gaussian(w: Workload, modes: np.ndarray, seed: int, exponent: float = -1.0) -> np.ndarray:
+    """Compact retained input (z + Nz*field + P*mode): complex Gaussian times
+    (1 + k^2 + l^2)^exponent, DC imaginary part zero.  exponent=-1 gives the
+    (1+|k|^2)^-1 spectrum, exponent=-5/6 the |k|^-5/3 one."""
+    P = w.planes
+    g = gaussians(seed, 2 * P * len(modes)).reshape(len(modes), P, 2)
+    amp = (1.0 + squared_keys(modes).astype(np.float64)) ** exponent
+    out = (g[..., 0] + 1j * g[..., 1]) * amp[:, None]
+    dc = (modes["k"] == 0) & (modes["l"] == 0)
+    out[dc] = out[dc].real
+    return np.ascontiguousarray(out.reshape(-1))
+
+
+# ---- direct DFT and permutations (C) ----------------------------------------
+def direct_r2c(w: Workload, real: np.ndarray) -> np.ndarray:
+    real = np.ascontiguousarray(real, np.float64)
+    out = np.zeros(w.spectrum_elements, np.complex128)
+    lib().or_direct_r2c(w.nx, w.ny, w.nz, w.fields, _p(real), _p(out))
+    return out
+
+
+def direct_c2r(w: Workload, spec: np.ndarray) -> np.ndarray:
+    spec = np.ascontiguousarray(spec, np.complex128)
+    out = np.zeros(w.real_elements)
+    lib().or_direct_c2r(w.nx, w.ny, w.nz, w.fields, _p(spec), _p(out))
+    return out
+
+
+def direct_retained(w: Workload, modes: np.ndarray, real: np.ndarray) -> np.ndarray:
+    real = np.ascontiguousarray(real, np.float64)
+    out = np.zeros(len(modes) * w.planes, np.complex128)
+    lib().or_direct_retained(w.nx, w.ny, w.nz, w.fields, _p(modes), len(modes), _p(real), _p(out))
+    return out
+
+
+def gather_retained(w: Workload, modes: np.ndarray, full: np.ndarray) -> np.ndarray:
+    full = np.ascontiguousarray(full, np.complex128)
+    out = np.zeros(len(modes) * w.planes, np.complex128)
+    lib().or_gather_retained(w.nx, w.ny, w.nz, w.fields, _p(modes), len(modes), _p(full), _p(out))
+    return out
+
+
+def embed_retained(w: Workload, modes: np.ndarray, retained: np.ndarray) -> np.ndarray:
+    retained = np.ascontiguousarray(retained, np.complex128)
+    out = np.zeros(w.spectrum_elements, np.complex128)
+    lib().or_embed_retained(w.nx, w.ny, w.nz, w.fields, _p(modes), len(modes), _p(retained), _p(out))
+    return out
+
+
+def vertical_forward(w: Workload, nj: int, first, count, forward: np.ndarray, physical: np.ndarray) -> np.ndarray:
+    first = np.ascontiguousarray(first, np.uint64)
+    count = np.ascontiguousarray(count, np.uint64)
+    physical = np.ascontiguousarray(physical, np.complex128)
+    forward = np.ascontiguousarray(forward, np.float64)
+    nmodes = int(first[-1] + count[-1])
+    out = np.zeros(nmodes * w.fields * nj, np.complex128)
+    lib().or_vertical_forward(w.nz, nj, w.fields, len(first), _p(first), _p(count), _p(forward), _p(physical), _p(out))
+    return out
+
+
+def vertical_inverse(w: Workload, nj: int, first, count, inverse: np.ndarray, modal: np.ndarray) -> np.ndarray:
+    first = np.ascontiguousarray(first, np.uint64)
+    count = np.ascontiguousarray(count, np.uint64)
+    modal = np.ascontiguousarray(modal, np.complex128)
+    inverse = np.ascontiguousarray(inverse, np.float64)
+    nmodes = int(first[-1] + count[-1])
+    out = np.zeros(nmodes * w.planes, np.complex128)
+    lib().or_vertical_inverse(w.nz, nj, w.fields, len(first), _p(first), _p(count), _p(inverse), _p(modal), _p(out))
+    return out
+
+
+def pointwise_advection(volume: int, scale: float, shared: np.ndarray, derivative: np.ndarray) -> np.ndarray:
+    shared = np.ascontiguousarray(shared, np.float64)
+    derivative = np.ascontiguousarray(derivative, np.float64)
+    out = np.zeros(volume)
+    lib().or_pointwise_advection(volume, scale, _p(shared), _p(derivative), _p(out))
+    return out
+
+
+def max_relative_error(actual, expected, actual_scale: float = 1.0) -> float:
+    """maximumRelativeError (src/model.cpp:718-736)."""
+    a = np.ascontiguousarray(actual)
+    b = np.ascontiguousarray(expected)
+    if np.iscomplexobj(a) or np.iscomplexobj(b):
+        a = np.ascontiguousarray(a, np.complex128) * actual_scale
+        b = np.ascontiguousarray(b, np.complex128)
+        return float(lib().or_max_relative_error_complex(_p(a), _p(b), a.size))
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    return float(lib().or_max_relative_error_real(_p(a), _p(b), a.size, actual_scale))
+
+
+def relative_l2_error(actual, expected) -> float:
+    """relativeL2Error (src/model.cpp:738-751)."""
+    a = np.ascontiguousarray(actual, np.complex128)
+    b = np.ascontiguousarray(expected, np.complex128)
+    return float(lib().or_relative_l2_error_complex(_p(a), _p(b), a.size))
+
+
+# ---- FP64 FFT paths (pocketfft) ---------------------------------------------
+def _workers(workers):
+    return workers if workers else (os.cpu_count() or 1)
+
+
+def fft_r2c_wvm(w: Workload, real: np.ndarray, workers: int | None = None) -> np.ndarray:
+    """Full batched 2-D r2c written in WVM order z + Nz*field + P*(kx + Nxh*ky)
+    (geometry of src/fftw_provider.cpp:537-572)."""
+    import scipy.fft
+
+    a = np.asarray(real, np.float64).reshape(w.planes, w.ny, w.nx)
+    s = scipy.fft.rfft2(a, axes=(1, 2), workers=_workers(workers))
+    return np.ascontiguousarray(s.transpose(1, 2, 0)).reshape(-1)
+
+
+def fft_c2r_wvm(w: Workload, spec: np.ndarray, workers: int | None = None) -> np.ndarray:
+    """Unnormalised batched 2-D c2r from WVM order (src/fftw_provider.cpp:574-610):
+    columns c2c along ky, then rows c2r along kx ignoring Im of the kx=0 and
+    kx=Nx/2 entries, as FFTW's multi-dimensional c2r does."""
+    import scipy.fft
+
+    s = np.asarray(spec, np.complex128).reshape(w.ny, w.nx_half, w.planes).transpose(2, 0, 1)
+    out = scipy.fft.irfft2(s, s=(w.ny, w.nx), axes=(1, 2), norm="forward", workers=_workers(workers))
+    return np.ascontiguousarray(out).reshape(-1)
+
+
+def retained_inverse(w: Workload, modes: np.ndarray, retained: np.ndarray, workers=None) -> np.ndarray:
+    """T_h^- = F_xy^-1 E_h (src/pruned_fftw_provider.cpp:265-276)."""
+    return fft_c2r_wvm(w, embed_retained(w, modes, retained), workers)
+
+
+def retained_forward(w: Workload, modes: np.ndarray, real: np.ndarray, workers=None) -> np.ndarray:
+    """T_h^+ = P_h F_xy (src/pruned_fftw_provider.cpp:230-241)."""
+    return gather_retained(w, modes, fft_r2c_wvm(w, real, workers))
+
+
+def retained_operator(w: Workload, modes, retained, nj, first, count, forward, workers=None) -> np.ndarray:
+    """One retained-operator evaluation: pruned inverse c2r, forward r2c
+    truncated to the retained modes, K^2-grouped vertical forward GEMM."""
+    real = retained_inverse(w, modes, retained, workers)
+    back = retained_forward(w, modes, real, workers)
+    return vertical_forward(w, nj, first, count, forward, back)
+
+
+def vertical_forward_blas(w: Workload, nj: int, first, count, forward: np.ndarray, physical: np.ndarray) -> np.ndarray:
+    """Same product as vertical_forward, through numpy's BLAS (multi-threaded):
+    used only by the CPU baseline timing.  Equality with the C loop is
+    checked in tests/test_oracle.py."""
+    nmodes = int(first[-1] + count[-1])
+    P = np.asarray(physical, np.complex128).reshape(nmodes, w.fields, w.nz)
+    F = np.asarray(forward, np.float64).reshape(len(first), nj, w.nz)
+    out = np.empty((nmodes, w.fields, nj), np.complex128)
+    for g in range(len(first)):
+        a, b = int(first[g]), int(first[g] + count[g])
+        out[a:b] = P[a:b] @ F[g].T
+    return out.reshape(-1)
+
+
+def retained_operator_fast(w: Workload, modes, retained, nj, first, count, forward, workers=None) -> np.ndarray:
+    """retained_operator with the BLAS vertical product (CPU baseline leg)."""
+    real = retained_inverse(w, modes, retained, workers)
+    back = retained_forward(w, modes, real, workers)
+    return vertical_forward_blas(w, nj, first, count, forward, back)

@@ -1,0 +1,820 @@
+// Batched 2-D real FFTs in WVM order and the pruned retained-mode transforms
+// for gfx950.
+//
+// Design (all sizes powers of two, 4..1024 per axis):
+//   * Every pass works on a TILE OF CONSECUTIVE PLANES, so global traffic in
+//     the plane-fastest WVM / retained layouts is contiguous runs of
+//     tile*sizeof(complex) = 128 B.
+//   * Row pass (x): a workgroup owns one row index y of `tile` planes.  Two
+//     real rows are packed as one complex sequence (x_a + i x_b), transformed
+//     with a mixed-radix Stockham FFT in LDS and separated by Hermitian
+//     symmetry.  r2c keeps only kx < kc (kc = active k_x band for the pruned
+//     transform, Nx/2+1 for the full one); c2r synthesises the zero band in
+//     registers and never reads it.
+//   * Column pass (y): a workgroup owns one kx column of `tile` planes.  Input
+//     is either the strided intermediate / WVM spectrum or, for the pruned
+//     inverse, a gather straight from the compact retained array (with the
+//     k_x = 0 Hermitian partners); output is the strided array or a scatter
+//     into the compact retained array.  Rows that hold no retained mode are
+//     never read or written.
+//   * Intermediate between the passes ("T"): z + Nz*field + P*(kx + kc*y).
+//     For the full transforms T IS the WVM spectrum (kc = Nx/2+1) and the
+//     column pass runs in place.
+//   * Twiddles come from FP64 host tables rounded once to the working
+//     precision and staged in LDS.
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+
+namespace skb {
+
+// ---------------------------------------------------------------------------
+// Stockham FFT in LDS
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int pick_radix(int n) {
+    return (n % 8 == 0 && n != 16 && n != 32) ? 8 : (n % 4 == 0 ? 4 : 2);
+}
+
+// x * (-i) for the forward sign, x * (+i) for the inverse.
+template <bool INV, typename R>
+__device__ __forceinline__ cx<R> rot_i(cx<R> a) {
+    return INV ? cx<R>{-a.y, a.x} : cx<R>{a.y, -a.x};
+}
+
+template <bool INV, typename R>
+__device__ __forceinline__ void dft4(cx<R>& a0, cx<R>& a1, cx<R>& a2, cx<R>& a3) {
+    const cx<R> t0 = a0 + a2, t1 = a0 - a2, t2 = a1 + a3, t3 = rot_i<INV>(a1 - a3);
+    a0 = t0 + t2;
+    a2 = t0 - t2;
+    a1 = t1 + t3;
+    a3 = t1 - t3;
+}
+
+template <int RAD, bool INV, typename R>
+__device__ __forceinline__ void dft(cx<R>* v) {
+    if constexpr (RAD == 2) {
+        const cx<R> a = v[0], b = v[1];
+        v[0] = a + b;
+        v[1] = a - b;
+    } else if constexpr (RAD == 4) {
+        dft4<INV>(v[0], v[1], v[2], v[3]);
+    } else {
+        static_assert(RAD == 8, "radix");
+        cx<R> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+        cx<R> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+        dft4<INV>(e0, e1, e2, e3);
+        dft4<INV>(o0, o1, o2, o3);
+        const R h = R(0.707106781186547524400844362104849039);
+        // o_k *= exp(-+ 2 pi i k / 8)
+        o1 = INV ? cx<R>{h * (o1.x - o1.y), h * (o1.x + o1.y)} : cx<R>{h * (o1.x + o1.y), h * (o1.y - o1.x)};
+        o2 = rot_i<INV>(o2);
+        o3 = INV ? cx<R>{-h * (o3.x + o3.y), h * (o3.x - o3.y)} : cx<R>{h * (o3.y - o3.x), -h * (o3.x + o3.y)};
+        v[0] = e0 + o0;
+        v[4] = e0 - o0;
+        v[1] = e1 + o1;
+        v[5] = e1 - o1;
+        v[2] = e2 + o2;
+        v[6] = e2 - o2;
+        v[3] = e3 + o3;
+        v[7] = e3 - o3;
+    }
+}
+
+// One Stockham stage of radix RAD over S sequences of length N stored at
+// buf[s*SEQ ...].  NS = product of the radices already applied.  The stage
+// reads every input before the first barrier and writes after it, so it runs
+// in place.  Ends with a barrier.
+template <typename R, int N, int SEQ, int RAD, int NS, int S, int NT, bool INV>
+__device__ __forceinline__ void stockham_stage(cx<R>* buf, const cx<R>* tw) {
+    constexpr int T = N / RAD;
+    constexpr int TOTAL = S * T;
+    constexpr int PER = (TOTAL + NT - 1) / NT;
+    cx<R> v[PER][RAD];
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int b = tid + i * NT;
+        if (TOTAL % NT == 0 || b < TOTAL) {
+            const int s = b / T, j = b - s * T;
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) v[i][r] = buf[s * SEQ + j + r * T];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int b = tid + i * NT;
+        if (TOTAL % NT == 0 || b < TOTAL) {
+            const int s = b / T, j = b - s * T;
+            const int t = j & (NS - 1);
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < RAD; ++r) {
+                    const cx<R> w = tw[r * t * (N / (NS * RAD))];
+                    v[i][r] = INV ? cmulc(v[i][r], w) : cmul(v[i][r], w);
+                }
+            }
+            dft<RAD, INV>(v[i]);
+            const int base = s * SEQ + (j / NS) * NS * RAD + t;
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) buf[base + r * NS] = v[i][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename R, int N, int SEQ, int REM, int NS, int S, int NT, bool INV>
+__device__ __forceinline__ void stockham(cx<R>* buf, const cx<R>* tw) {
+    if constexpr (REM > 1) {
+        constexpr int RAD = pick_radix(REM);
+        stockham_stage<R, N, SEQ, RAD, NS, S, NT, INV>(buf, tw);
+        stockham<R, N, SEQ, REM / RAD, NS * RAD, S, NT, INV>(buf, tw);
+    }
+}
+
+constexpr int kThreads = 256;
+
+template <typename R>
+constexpr int row_tile() {
+    return sizeof(R) == 8 ? 8 : 16;  // planes per row workgroup (even)
+}
+template <typename R, int N>
+constexpr int col_tile() {
+    // planes per column workgroup; keeps the LDS tile at <= ~64 KiB
+    return sizeof(R) == 8 ? (N >= 1024 ? 4 : 8) : (N >= 1024 ? 8 : 16);
+}
+
+// Entry of a column's retained-mode list: the row (ky) it occupies, the
+// compact mode index and whether the compact value is the conjugate of the
+// stored half-spectrum value.
+struct ColEntry {
+    int row;
+    int mode;
+    int conj;
+    int pad;
+};
+
+// ---------------------------------------------------------------------------
+// Row pass: r2c along x.  real (x fastest, y, plane) -> T[p + P*(kx + kc*y)]
+// ---------------------------------------------------------------------------
+template <typename R, int NX>
+__global__ __launch_bounds__(kThreads) void row_r2c_kernel(const R* __restrict__ real, cx<R>* __restrict__ out,
+                                                          const cx<R>* __restrict__ tw, int ny, int planes, int kc) {
+    constexpr int TP = row_tile<R>();
+    constexpr int S = TP / 2;
+    __shared__ cx<R> buf[S * NX];
+    __shared__ cx<R> twl[NX];
+    const int y = blockIdx.x;
+    const int p0 = blockIdx.y * TP;
+    for (int i = threadIdx.x; i < NX; i += kThreads) twl[i] = tw[i];
+    for (int e = threadIdx.x; e < TP * NX; e += kThreads) {
+        const int pit = e / NX, n = e - pit * NX;
+        const int p = p0 + pit;
+        const R v = p < planes ? real[(static_cast<size_t>(p) * ny + y) * NX + n] : R(0);
+        reinterpret_cast<R*>(&buf[(pit >> 1) * NX + n])[pit & 1] = v;
+    }
+    __syncthreads();
+    stockham<R, NX, NX, NX, 1, S, kThreads, false>(buf, twl);
+    const size_t row_base = static_cast<size_t>(planes) * static_cast<size_t>(kc) * y;
+    for (int e = threadIdx.x; e < kc * TP; e += kThreads) {
+        const int pit = e % TP, kx = e / TP;
+        const int p = p0 + pit;
+        if (p >= planes) continue;
+        const cx<R> zk = buf[(pit >> 1) * NX + kx];
+        const cx<R> zn = buf[(pit >> 1) * NX + ((NX - kx) & (NX - 1))];
+        // X_a = (Z_k + conj Z_{N-k})/2,  X_b = (Z_k - conj Z_{N-k})/(2i)
+        const cx<R> x = (pit & 1) == 0 ? cx<R>{R(0.5) * (zk.x + zn.x), R(0.5) * (zk.y - zn.y)}
+                                       : cx<R>{R(0.5) * (zk.y + zn.y), R(0.5) * (zn.x - zk.x)};
+        out[row_base + static_cast<size_t>(planes) * kx + p] = x;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Row pass: c2r along x.  T[p + P*(kx + kc*y)] (kx < kc, zero above) -> real.
+// The imaginary parts at kx = 0 and kx = Nx/2 are ignored, as FFTW's c2r does.
+// ---------------------------------------------------------------------------
+template <typename R, int NX>
+__global__ __launch_bounds__(kThreads) void row_c2r_kernel(const cx<R>* __restrict__ in, R* __restrict__ real,
+                                                          const cx<R>* __restrict__ tw, int ny, int planes, int kc) {
+    constexpr int TP = row_tile<R>();
+    constexpr int S = TP / 2;
+    constexpr int H = NX / 2 + 1;
+    __shared__ cx<R> buf[S * NX];
+    __shared__ cx<R> twl[NX];
+    const int y = blockIdx.x;
+    const int p0 = blockIdx.y * TP;
+    for (int i = threadIdx.x; i < NX; i += kThreads) twl[i] = tw[i];
+    const size_t row_base = static_cast<size_t>(planes) * static_cast<size_t>(kc) * y;
+    for (int e = threadIdx.x; e < H * S; e += kThreads) {
+        const int s = e % S, kx = e / S;
+        const int pa = p0 + 2 * s, pb = pa + 1;
+        cx<R> xa{R(0), R(0)}, xb{R(0), R(0)};
+        if (kx < kc) {
+            const size_t base = row_base + static_cast<size_t>(planes) * kx;
+            if (pa < planes) xa = in[base + pa];
+            if (pb < planes) xb = in[base + pb];
+        }
+        const bool self = kx == 0 || kx == NX / 2;
+        if (self) {
+            xa.y = R(0);
+            xb.y = R(0);
+        }
+        // Z_k = X_a + i X_b ;  Z_{N-k} = conj X_a + i conj X_b
+        buf[s * NX + kx] = cx<R>{xa.x - xb.y, xa.y + xb.x};
+        if (!self) buf[s * NX + NX - kx] = cx<R>{xa.x + xb.y, xb.x - xa.y};
+    }
+    __syncthreads();
+    stockham<R, NX, NX, NX, 1, S, kThreads, true>(buf, twl);
+    for (int e = threadIdx.x; e < TP * NX; e += kThreads) {
+        const int pit = e / NX, n = e - pit * NX;
+        const int p = p0 + pit;
+        if (p >= planes) continue;
+        const cx<R> z = buf[(pit >> 1) * NX + n];
+        real[(static_cast<size_t>(p) * ny + y) * NX + n] = (pit & 1) ? z.y : z.x;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Column pass: c2c along y for column kx = blockIdx.x of `tile` planes.
+// ---------------------------------------------------------------------------
+enum ColIO { kStrided = 0, kRetained = 1 };
+
+template <typename R, int NY, bool INV, int IN, int OUT>
+__global__ __launch_bounds__(kThreads) void col_kernel(const cx<R>* in, cx<R>* out, const cx<R>* __restrict__ tw,
+                                                      int planes, int in_kc, int out_kc,
+                                                      const int* __restrict__ col_start,
+                                                      const ColEntry* __restrict__ entries) {
+    constexpr int TP = col_tile<R, NY>();
+    constexpr int SEQ = NY + 1;  // one-element pad: plane-fastest LDS writes hit distinct banks
+    __shared__ cx<R> buf[TP * SEQ];
+    __shared__ cx<R> twl[NY];
+    const int kx = blockIdx.x;
+    const int p0 = blockIdx.y * TP;
+    const size_t P = static_cast<size_t>(planes);
+    for (int i = threadIdx.x; i < NY; i += kThreads) twl[i] = tw[i];
+    if constexpr (IN == kStrided) {
+        for (int e = threadIdx.x; e < NY * TP; e += kThreads) {
+            const int pit = e % TP, yy = e / TP;
+            const int p = p0 + pit;
+            buf[pit * SEQ + yy] = p < planes ? in[P * (kx + static_cast<size_t>(in_kc) * yy) + p] : cx<R>{R(0), R(0)};
+        }
+    } else {
+        for (int e = threadIdx.x; e < TP * SEQ; e += kThreads) buf[e] = cx<R>{R(0), R(0)};
+        __syncthreads();
+        const int beg = col_start[kx], cnt = col_start[kx + 1] - beg;
+        for (int e = threadIdx.x; e < cnt * TP; e += kThreads) {
+            const int pit = e % TP;
+            const ColEntry ent = entries[beg + e / TP];
+            const int p = p0 + pit;
+            if (p >= planes) continue;
+            cx<R> v = in[P * ent.mode + p];
+            if (ent.conj) v.y = -v.y;
+            buf[pit * SEQ + ent.row] = v;
+        }
+    }
+    __syncthreads();
+    stockham<R, NY, SEQ, NY, 1, TP, kThreads, INV>(buf, twl);
+    if constexpr (OUT == kStrided) {
+        for (int e = threadIdx.x; e < NY * TP; e += kThreads) {
+            const int pit = e % TP, yy = e / TP;
+            const int p = p0 + pit;
+            if (p < planes) out[P * (kx + static_cast<size_t>(out_kc) * yy) + p] = buf[pit * SEQ + yy];
+        }
+    } else {
+        const int beg = col_start[kx], cnt = col_start[kx + 1] - beg;
+        for (int e = threadIdx.x; e < cnt * TP; e += kThreads) {
+            const int pit = e % TP;
+            const ColEntry ent = entries[beg + e / TP];
+            const int p = p0 + pit;
+            if (p >= planes) continue;
+            cx<R> v = buf[pit * SEQ + ent.row];
+            if (ent.conj) v.y = -v.y;
+            out[P * ent.mode + p] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Retained gather / embed on a full WVM spectrum (P_h, E_h).
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void gather_kernel(const cx<R>* __restrict__ wvm, cx<R>* __restrict__ retained,
+                              const ColEntry* __restrict__ modes /* row=ky, mode=kx */, int planes, int nxh,
+                              size_t total) {
+    for (size_t e = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; e < total;
+         e += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const size_t m = e / planes, p = e - m * planes;
+        const ColEntry md = modes[m];
+        cx<R> v = wvm[static_cast<size_t>(planes) * (md.mode + static_cast<size_t>(nxh) * md.row) + p];
+        if (md.conj) v.y = -v.y;
+        retained[e] = v;
+    }
+}
+
+template <typename R>
+__global__ void zero_kernel(cx<R>* __restrict__ out, size_t total) {
+    for (size_t e = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; e < total;
+         e += static_cast<size_t>(gridDim.x) * blockDim.x)
+        out[e] = cx<R>{R(0), R(0)};
+}
+
+// entries: (row=ky, mode=compact index, conj, pad=kx) incl. k_x = 0 partners
+template <typename R>
+__global__ void embed_kernel(const cx<R>* __restrict__ retained, cx<R>* __restrict__ wvm,
+                             const ColEntry* __restrict__ entries, int planes, int nxh, size_t total) {
+    for (size_t e = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; e < total;
+         e += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const size_t i = e / planes, p = e - i * planes;
+        const ColEntry en = entries[i];
+        cx<R> v = retained[static_cast<size_t>(planes) * en.mode + p];
+        if (en.conj) v.y = -v.y;
+        wvm[static_cast<size_t>(planes) * (en.pad + static_cast<size_t>(nxh) * en.row) + p] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side dispatch
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename R>
+std::vector<cx<R>> twiddles(int n) {
+    std::vector<cx<R>> t(n);
+    for (int i = 0; i < n; ++i) {
+        const double a = 2.0 * M_PI * static_cast<double>(i) / static_cast<double>(n);
+        t[i] = cx<R>{static_cast<R>(std::cos(a)), static_cast<R>(-std::sin(a))};
+    }
+    return t;
+}
+
+void check_launch(const char* what) { check_hip(hipGetLastError(), what); }
+
+template <typename R, int NX>
+void launch_row(bool inverse, const void* in, void* out, const void* tw, int ny, int planes, int kc,
+                hipStream_t stream) {
+    constexpr int TP = row_tile<R>();
+    const dim3 grid(ny, (planes + TP - 1) / TP);
+    if (inverse)
+        hipLaunchKernelGGL((row_c2r_kernel<R, NX>), grid, dim3(kThreads), 0, stream, static_cast<const cx<R>*>(in),
+                           static_cast<R*>(out), static_cast<const cx<R>*>(tw), ny, planes, kc);
+    else
+        hipLaunchKernelGGL((row_r2c_kernel<R, NX>), grid, dim3(kThreads), 0, stream, static_cast<const R*>(in),
+                           static_cast<cx<R>*>(out), static_cast<const cx<R>*>(tw), ny, planes, kc);
+    check_launch(inverse ? "row c2r launch" : "row r2c launch");
+}
+
+enum class ColVariant { FwdStrided, FwdToRetained, InvStrided, InvFromRetained };
+
+template <typename R, int NY>
+void launch_col(ColVariant v, const void* in, void* out, const void* tw, int planes, int kx_count, int in_kc,
+                int out_kc, const int* col_start, const ColEntry* entries, hipStream_t stream) {
+    constexpr int TP = col_tile<R, NY>();
+    const dim3 grid(kx_count, (planes + TP - 1) / TP);
+    const auto* i = static_cast<const cx<R>*>(in);
+    auto* o = static_cast<cx<R>*>(out);
+    const auto* t = static_cast<const cx<R>*>(tw);
+    switch (v) {
+        case ColVariant::FwdStrided:
+            hipLaunchKernelGGL((col_kernel<R, NY, false, kStrided, kStrided>), grid, dim3(kThreads), 0, stream, i, o, t,
+                               planes, in_kc, out_kc, col_start, entries);
+            break;
+        case ColVariant::FwdToRetained:
+            hipLaunchKernelGGL((col_kernel<R, NY, false, kStrided, kRetained>), grid, dim3(kThreads), 0, stream, i, o,
+                               t, planes, in_kc, out_kc, col_start, entries);
+            break;
+        case ColVariant::InvStrided:
+            hipLaunchKernelGGL((col_kernel<R, NY, true, kStrided, kStrided>), grid, dim3(kThreads), 0, stream, i, o, t,
+                               planes, in_kc, out_kc, col_start, entries);
+            break;
+        case ColVariant::InvFromRetained:
+            hipLaunchKernelGGL((col_kernel<R, NY, true, kRetained, kStrided>), grid, dim3(kThreads), 0, stream, i, o,
+                               t, planes, in_kc, out_kc, col_start, entries);
+            break;
+    }
+    check_launch("column launch");
+}
+
+#define SKB_POW2_CASES(MACRO) \
+    MACRO(4) MACRO(8) MACRO(16) MACRO(32) MACRO(64) MACRO(128) MACRO(256) MACRO(512) MACRO(1024)
+
+template <typename R>
+void row_pass(bool inverse, int nx, const void* in, void* out, const void* tw, int ny, int planes, int kc,
+              hipStream_t s) {
+    switch (nx) {
+#define SKB_ROW_CASE(N)                                                \
+    case N:                                                            \
+        launch_row<R, N>(inverse, in, out, tw, ny, planes, kc, s);     \
+        return;
+        SKB_POW2_CASES(SKB_ROW_CASE)
+#undef SKB_ROW_CASE
+    }
+    fail(SKB_INVALID_ARGUMENT, "unsupported Nx");
+}
+
+template <typename R>
+void col_pass(ColVariant v, int ny, const void* in, void* out, const void* tw, int planes, int kx_count, int in_kc,
+              int out_kc, const int* col_start, const ColEntry* entries, hipStream_t s) {
+    switch (ny) {
+#define SKB_COL_CASE(N)                                                                               \
+    case N:                                                                                           \
+        launch_col<R, N>(v, in, out, tw, planes, kx_count, in_kc, out_kc, col_start, entries, s);     \
+        return;
+        SKB_POW2_CASES(SKB_COL_CASE)
+#undef SKB_COL_CASE
+    }
+    fail(SKB_INVALID_ARGUMENT, "unsupported Ny");
+}
+
+void check_fft_sizes(const skb_workload& w) {
+    validate_workload(&w);
+    if (!is_pow2(w.nx) || !is_pow2(w.ny) || w.nx < kMinFft || w.ny < kMinFft || w.nx > kMaxFft || w.ny > kMaxFft)
+        fail(SKB_UNSUPPORTED, "the hip provider supports power-of-two Nx, Ny in [4, 1024]");
+    const uint64_t planes = w.nz * w.fields;
+    // grid.y = ceil(planes / tile) with tiles of >= 4 planes must stay below 65536
+    if (planes > (1u << 17)) fail(SKB_UNSUPPORTED, "plane count above 2^17");
+    if (planes * (w.nx / 2 + 1) * w.ny > (1ull << 31) || planes * w.nx * w.ny > (1ull << 31))
+        fail(SKB_UNSUPPORTED, "workload exceeds 2^31 elements per array");
+}
+
+// Twiddle tables for Nx and Ny in both precisions' layout.
+struct Twiddles {
+    DeviceBuffer x, y;
+    void build(int precision, uint64_t nx, uint64_t ny) {
+        if (precision == SKB_F64) {
+            auto tx = twiddles<double>(static_cast<int>(nx));
+            auto ty = twiddles<double>(static_cast<int>(ny));
+            x.upload(tx.data(), tx.size() * sizeof(tx[0]));
+            y.upload(ty.data(), ty.size() * sizeof(ty[0]));
+        } else {
+            auto tx = twiddles<float>(static_cast<int>(nx));
+            auto ty = twiddles<float>(static_cast<int>(ny));
+            x.upload(tx.data(), tx.size() * sizeof(tx[0]));
+            y.upload(ty.data(), ty.size() * sizeof(ty[0]));
+        }
+    }
+};
+
+// Column lists of retained modes.  forward: one entry per mode at its stored
+// (kx, ky).  inverse: additionally the k_x = 0 conjugate partners
+// (src/model.cpp:332-335).
+struct ColumnLists {
+    uint64_t kc = 0;  // active k_x band: max stored_kx + 1
+    std::vector<int> fwd_start, inv_start;
+    std::vector<ColEntry> fwd, inv;
+};
+
+ColumnLists build_column_lists(const skb_workload& w, const skb_retained_mode* modes, uint64_t n) {
+    if (!modes || n == 0) fail(SKB_INVALID_ARGUMENT, "The pruned provider requires retained modes.");
+    if (n > (1u << 30)) fail(SKB_UNSUPPORTED, "too many retained modes");
+    const uint64_t nxh = w.nx / 2 + 1;
+    ColumnLists c;
+    for (uint64_t m = 0; m < n; ++m) {
+        if (modes[m].stored_kx >= nxh || modes[m].stored_ky >= w.ny)
+            fail(SKB_INVALID_ARGUMENT, "A retained mode lies outside the half-spectrum.");
+        c.kc = std::max<uint64_t>(c.kc, modes[m].stored_kx + 1);
+    }
+    std::vector<std::vector<ColEntry>> f(c.kc), i(c.kc);
+    std::vector<uint8_t> used(c.kc * w.ny, 0);
+    auto claim = [&](uint64_t kx, uint64_t row) {
+        auto& u = used[kx * w.ny + row];
+        if (u) fail(SKB_INVALID_ARGUMENT, "two retained modes (or a mode and a Hermitian partner) share a stored slot");
+        u = 1;
+    };
+    for (uint64_t m = 0; m < n; ++m) {
+        const auto& md = modes[m];
+        const ColEntry e{static_cast<int>(md.stored_ky), static_cast<int>(m), md.conjugates_stored_value ? 1 : 0, 0};
+        claim(md.stored_kx, md.stored_ky);
+        f[md.stored_kx].push_back(e);
+        i[md.stored_kx].push_back(e);
+    }
+    for (uint64_t m = 0; m < n; ++m) {
+        const auto& md = modes[m];
+        if (md.stored_kx == 0 && md.stored_ky != 0 && 2 * md.stored_ky != w.ny) {
+            const uint64_t row = (w.ny - md.stored_ky) % w.ny;
+            claim(0, row);
+            i[0].push_back(ColEntry{static_cast<int>(row), static_cast<int>(m), md.conjugates_stored_value ? 0 : 1, 0});
+        }
+    }
+    c.fwd_start.push_back(0);
+    c.inv_start.push_back(0);
+    for (uint64_t kx = 0; kx < c.kc; ++kx) {
+        c.fwd.insert(c.fwd.end(), f[kx].begin(), f[kx].end());
+        c.inv.insert(c.inv.end(), i[kx].begin(), i[kx].end());
+        c.fwd_start.push_back(static_cast<int>(c.fwd.size()));
+        c.inv_start.push_back(static_cast<int>(c.inv.size()));
+    }
+    return c;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Plans
+// ---------------------------------------------------------------------------
+}  // namespace skb
+
+struct skb_fft_plan {
+    skb_workload w{};
+    int precision = SKB_F64;
+    skb::Twiddles tw;
+    // Device copy of the mode table last handed to gather/embed.
+    std::vector<skb_retained_mode> cached_modes;
+    skb::DeviceBuffer gather_table, embed_table;
+    uint64_t embed_entries = 0;
+};
+
+struct skb_pruned_plan {
+    skb_workload w{};
+    int precision = SKB_F64;
+    uint64_t modes = 0;
+    skb::Twiddles tw;
+    skb::ColumnLists lists;
+    skb::DeviceBuffer fwd_start, inv_start, fwd, inv;
+    skb::DeviceBuffer scratch;  // T: kc * Ny * P complex
+};
+
+using namespace skb;
+
+namespace {
+
+void refresh_mode_tables(skb_fft_plan* plan, const skb_retained_mode* modes, uint64_t n) {
+    if (!modes || n == 0) fail(SKB_INVALID_ARGUMENT, "retained modes required");
+    if (plan->cached_modes.size() == n &&
+        std::memcmp(plan->cached_modes.data(), modes, n * sizeof(skb_retained_mode)) == 0)
+        return;
+    const auto& w = plan->w;
+    const uint64_t nxh = w.nx / 2 + 1;
+    std::vector<ColEntry> g(n), e;
+    e.reserve(n + n / 4);
+    for (uint64_t m = 0; m < n; ++m) {
+        const auto& md = modes[m];
+        if (md.stored_kx >= nxh || md.stored_ky >= w.ny) fail(SKB_INVALID_ARGUMENT, "A retained mode lies outside the half-spectrum.");
+        const int cj = md.conjugates_stored_value ? 1 : 0;
+        g[m] = ColEntry{static_cast<int>(md.stored_ky), static_cast<int>(md.stored_kx), cj, 0};
+        e.push_back(ColEntry{static_cast<int>(md.stored_ky), static_cast<int>(m), cj, static_cast<int>(md.stored_kx)});
+    }
+    for (uint64_t m = 0; m < n; ++m) {
+        const auto& md = modes[m];
+        if (md.stored_kx == 0 && md.stored_ky != 0 && 2 * md.stored_ky != w.ny)
+            e.push_back(ColEntry{static_cast<int>((w.ny - md.stored_ky) % w.ny), static_cast<int>(m),
+                                 md.conjugates_stored_value ? 0 : 1, 0});
+    }
+    plan->gather_table.upload(g.data(), g.size() * sizeof(ColEntry));
+    plan->embed_table.upload(e.data(), e.size() * sizeof(ColEntry));
+    plan->embed_entries = e.size();
+    plan->cached_modes.assign(modes, modes + n);
+}
+
+template <typename R>
+void fft_forward(skb_fft_plan* p, const void* real, void* wvm, hipStream_t s) {
+    const auto& w = p->w;
+    const int planes = static_cast<int>(w.nz * w.fields);
+    const int nxh = static_cast<int>(w.nx / 2 + 1);
+    row_pass<R>(false, static_cast<int>(w.nx), real, wvm, p->tw.x.ptr, static_cast<int>(w.ny), planes, nxh, s);
+    col_pass<R>(ColVariant::FwdStrided, static_cast<int>(w.ny), wvm, wvm, p->tw.y.ptr, planes, nxh, nxh, nxh, nullptr,
+                nullptr, s);
+}
+
+template <typename R>
+void fft_inverse(skb_fft_plan* p, void* wvm, void* real, hipStream_t s) {
+    const auto& w = p->w;
+    const int planes = static_cast<int>(w.nz * w.fields);
+    const int nxh = static_cast<int>(w.nx / 2 + 1);
+    col_pass<R>(ColVariant::InvStrided, static_cast<int>(w.ny), wvm, wvm, p->tw.y.ptr, planes, nxh, nxh, nxh, nullptr,
+                nullptr, s);
+    row_pass<R>(true, static_cast<int>(w.nx), wvm, real, p->tw.x.ptr, static_cast<int>(w.ny), planes, nxh, s);
+}
+
+template <typename R>
+void pruned_forward(skb_pruned_plan* p, const void* real, void* retained, hipStream_t s) {
+    const auto& w = p->w;
+    const int planes = static_cast<int>(w.nz * w.fields);
+    const int kc = static_cast<int>(p->lists.kc);
+    row_pass<R>(false, static_cast<int>(w.nx), real, p->scratch.ptr, p->tw.x.ptr, static_cast<int>(w.ny), planes, kc, s);
+    col_pass<R>(ColVariant::FwdToRetained, static_cast<int>(w.ny), p->scratch.ptr, retained, p->tw.y.ptr, planes, kc, kc,
+                kc, p->fwd_start.as<int>(), p->fwd.as<ColEntry>(), s);
+}
+
+template <typename R>
+void pruned_inverse(skb_pruned_plan* p, const void* retained, void* real, hipStream_t s) {
+    const auto& w = p->w;
+    const int planes = static_cast<int>(w.nz * w.fields);
+    const int kc = static_cast<int>(p->lists.kc);
+    col_pass<R>(ColVariant::InvFromRetained, static_cast<int>(w.ny), retained, p->scratch.ptr, p->tw.y.ptr, planes, kc,
+                kc, kc, p->inv_start.as<int>(), p->inv.as<ColEntry>(), s);
+    row_pass<R>(true, static_cast<int>(w.nx), p->scratch.ptr, real, p->tw.x.ptr, static_cast<int>(w.ny), planes, kc, s);
+}
+
+void require_ptrs(const void* a, const void* b) {
+    if (!a || !b) fail(SKB_INVALID_ARGUMENT, "null device pointer");
+}
+
+dim3 stream_grid(size_t total) {
+    const size_t blocks = (total + 255) / 256;
+    return dim3(static_cast<unsigned>(std::min<size_t>(blocks, 8192)));
+}
+
+}  // namespace
+
+extern "C" {
+
+int skb_fft_plan_create(const skb_workload* workload, int precision, skb_fft_plan** plan) {
+    return guarded([&] {
+        if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan pointer");
+        *plan = nullptr;
+        check_precision(precision);
+        validate_workload(workload);
+        check_fft_sizes(*workload);
+        auto p = std::make_unique<skb_fft_plan>();
+        p->w = *workload;
+        p->precision = precision;
+        p->tw.build(precision, workload->nx, workload->ny);
+        *plan = p.release();
+    });
+}
+
+int skb_fft_plan_destroy(skb_fft_plan* plan) {
+    return guarded([&] { delete plan; });
+}
+
+int skb_fft_forward(skb_fft_plan* plan, const void* real, void* wvm, void* stream) {
+    return guarded([&] {
+        if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
+        require_ptrs(real, wvm);
+        if (plan->precision == SKB_F64) fft_forward<double>(plan, real, wvm, as_stream(stream));
+        else fft_forward<float>(plan, real, wvm, as_stream(stream));
+    });
+}
+
+int skb_fft_inverse(skb_fft_plan* plan, void* wvm, void* real, void* stream) {
+    return guarded([&] {
+        if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
+        require_ptrs(real, wvm);
+        if (plan->precision == SKB_F64) fft_inverse<double>(plan, wvm, real, as_stream(stream));
+        else fft_inverse<float>(plan, wvm, real, as_stream(stream));
+    });
+}
+
+int skb_fft_gather_retained(skb_fft_plan* plan, const skb_retained_mode* modes, uint64_t mode_count, const void* wvm,
+                            void* retained, void* stream) {
+    return guarded([&] {
+        if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
+        require_ptrs(wvm, retained);
+        refresh_mode_tables(plan, modes, mode_count);
+        const int planes = static_cast<int>(plan->w.nz * plan->w.fields);
+        const int nxh = static_cast<int>(plan->w.nx / 2 + 1);
+        const size_t total = mode_count * static_cast<size_t>(planes);
+        if (plan->precision == SKB_F64)
+            hipLaunchKernelGGL(gather_kernel<double>, stream_grid(total), dim3(256), 0, as_stream(stream),
+                               static_cast<const cx<double>*>(wvm), static_cast<cx<double>*>(retained),
+                               plan->gather_table.as<ColEntry>(), planes, nxh, total);
+        else
+            hipLaunchKernelGGL(gather_kernel<float>, stream_grid(total), dim3(256), 0, as_stream(stream),
+                               static_cast<const cx<float>*>(wvm), static_cast<cx<float>*>(retained),
+                               plan->gather_table.as<ColEntry>(), planes, nxh, total);
+        check_launch("gather launch");
+    });
+}
+
+int skb_fft_embed_retained(skb_fft_plan* plan, const skb_retained_mode* modes, uint64_t mode_count,
+                           const void* retained, void* wvm, void* stream) {
+    return guarded([&] {
+        if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
+        require_ptrs(wvm, retained);
+        refresh_mode_tables(plan, modes, mode_count);
+        const auto& w = plan->w;
+        const int planes = static_cast<int>(w.nz * w.fields);
+        const int nxh = static_cast<int>(w.nx / 2 + 1);
+        const size_t full = static_cast<size_t>(nxh) * w.ny * planes;
+        const size_t total = plan->embed_entries * static_cast<size_t>(planes);
+        if (plan->precision == SKB_F64) {
+            hipLaunchKernelGGL(zero_kernel<double>, stream_grid(full), dim3(256), 0, as_stream(stream),
+                               static_cast<cx<double>*>(wvm), full);
+            hipLaunchKernelGGL(embed_kernel<double>, stream_grid(total), dim3(256), 0, as_stream(stream),
+                               static_cast<const cx<double>*>(retained), static_cast<cx<double>*>(wvm),
+                               plan->embed_table.as<ColEntry>(), planes, nxh, total);
+        } else {
+            hipLaunchKernelGGL(zero_kernel<float>, stream_grid(full), dim3(256), 0, as_stream(stream),
+                               static_cast<cx<float>*>(wvm), full);
+            hipLaunchKernelGGL(embed_kernel<float>, stream_grid(total), dim3(256), 0, as_stream(stream),
+                               static_cast<const cx<float>*>(retained), static_cast<cx<float>*>(wvm),
+                               plan->embed_table.as<ColEntry>(), planes, nxh, total);
+        }
+        check_launch("embed launch");
+    });
+}
+
+int skb_pruned_plan_create(const skb_workload* workload, const skb_retained_mode* modes, uint64_t mode_count,
+                           int precision, skb_pruned_plan** plan) {
+    return guarded([&] {
+        if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan pointer");
+        *plan = nullptr;
+        check_precision(precision);
+        validate_workload(workload);
+        check_fft_sizes(*workload);
+        auto p = std::make_unique<skb_pruned_plan>();
+        p->w = *workload;
+        p->precision = precision;
+        p->modes = mode_count;
+        p->lists = build_column_lists(*workload, modes, mode_count);
+        p->tw.build(precision, workload->nx, workload->ny);
+        p->fwd_start.upload(p->lists.fwd_start.data(), p->lists.fwd_start.size() * sizeof(int));
+        p->inv_start.upload(p->lists.inv_start.data(), p->lists.inv_start.size() * sizeof(int));
+        p->fwd.upload(p->lists.fwd.data(), p->lists.fwd.size() * sizeof(ColEntry));
+        p->inv.upload(p->lists.inv.data(), p->lists.inv.size() * sizeof(ColEntry));
+        p->scratch.allocate(p->lists.kc * workload->ny * workload->nz * workload->fields * complex_bytes(precision));
+        *plan = p.release();
+    });
+}
+
+int skb_pruned_plan_destroy(skb_pruned_plan* plan) {
+    return guarded([&] { delete plan; });
+}
+
+int skb_pruned_forward(skb_pruned_plan* plan, const void* real, void* retained, void* stream) {
+    return guarded([&] {
+        if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
+        require_ptrs(real, retained);
+        if (plan->precision == SKB_F64) pruned_forward<double>(plan, real, retained, as_stream(stream));
+        else pruned_forward<float>(plan, real, retained, as_stream(stream));
+    });
+}
+
+int skb_pruned_inverse(skb_pruned_plan* plan, const void* retained, void* real, void* stream) {
+    return guarded([&] {
+        if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
+        require_ptrs(real, retained);
+        if (plan->precision == SKB_F64) pruned_inverse<double>(plan, retained, real, as_stream(stream));
+        else pruned_inverse<float>(plan, retained, real, as_stream(stream));
+    });
+}
+
+int skb_pruned_forward_rows(skb_pruned_plan* plan, const void* real, void* stream) {
+    return guarded([&] {
+        if (!plan || !real) fail(SKB_INVALID_ARGUMENT, "null argument");
+        const auto& w = plan->w;
+        const int planes = static_cast<int>(w.nz * w.fields), kc = static_cast<int>(plan->lists.kc);
+        if (plan->precision == SKB_F64)
+            row_pass<double>(false, static_cast<int>(w.nx), real, plan->scratch.ptr, plan->tw.x.ptr,
+                             static_cast<int>(w.ny), planes, kc, as_stream(stream));
+        else
+            row_pass<float>(false, static_cast<int>(w.nx), real, plan->scratch.ptr, plan->tw.x.ptr,
+                            static_cast<int>(w.ny), planes, kc, as_stream(stream));
+    });
+}
+
+int skb_pruned_forward_columns(skb_pruned_plan* plan, void* retained, void* stream) {
+    return guarded([&] {
+        if (!plan || !retained) fail(SKB_INVALID_ARGUMENT, "null argument");
+        const auto& w = plan->w;
+        const int planes = static_cast<int>(w.nz * w.fields), kc = static_cast<int>(plan->lists.kc);
+        if (plan->precision == SKB_F64)
+            col_pass<double>(ColVariant::FwdToRetained, static_cast<int>(w.ny), plan->scratch.ptr, retained,
+                             plan->tw.y.ptr, planes, kc, kc, kc, plan->fwd_start.as<int>(), plan->fwd.as<ColEntry>(),
+                             as_stream(stream));
+        else
+            col_pass<float>(ColVariant::FwdToRetained, static_cast<int>(w.ny), plan->scratch.ptr, retained,
+                            plan->tw.y.ptr, planes, kc, kc, kc, plan->fwd_start.as<int>(), plan->fwd.as<ColEntry>(),
+                            as_stream(stream));
+    });
+}
+
+int skb_pruned_inverse_columns(skb_pruned_plan* plan, const void* retained, void* stream) {
+    return guarded([&] {
+        if (!plan || !retained) fail(SKB_INVALID_ARGUMENT, "null argument");
+        const auto& w = plan->w;
+        const int planes = static_cast<int>(w.nz * w.fields), kc = static_cast<int>(plan->lists.kc);
+        if (plan->precision == SKB_F64)
+            col_pass<double>(ColVariant::InvFromRetained, static_cast<int>(w.ny), retained, plan->scratch.ptr,
+                             plan->tw.y.ptr, planes, kc, kc, kc, plan->inv_start.as<int>(), plan->inv.as<ColEntry>(),
+                             as_stream(stream));
+        else
+            col_pass<float>(ColVariant::InvFromRetained, static_cast<int>(w.ny), retained, plan->scratch.ptr,
+                            plan->tw.y.ptr, planes, kc, kc, kc, plan->inv_start.as<int>(), plan->inv.as<ColEntry>(),
+                            as_stream(stream));
+    });
+}
+
+int skb_pruned_inverse_rows(skb_pruned_plan* plan, void* real, void* stream) {
+    return guarded([&] {
+        if (!plan || !real) fail(SKB_INVALID_ARGUMENT, "null argument");
+        const auto& w = plan->w;
+        const int planes = static_cast<int>(w.nz * w.fields), kc = static_cast<int>(plan->lists.kc);
+        if (plan->precision == SKB_F64)
+            row_pass<double>(true, static_cast<int>(w.nx), plan->scratch.ptr, real, plan->tw.x.ptr,
+                             static_cast<int>(w.ny), planes, kc, as_stream(stream));
+        else
+            row_pass<float>(true, static_cast<int>(w.nx), plan->scratch.ptr, real, plan->tw.x.ptr,
+                            static_cast<int>(w.ny), planes, kc, as_stream(stream));
+    });
+}
+
+uint64_t skb_pruned_active_kx(const skb_pruned_plan* plan) { return plan ? plan->lists.kc : 0; }
+
+uint64_t skb_pruned_scratch_bytes(const skb_pruned_plan* plan) { return plan ? plan->scratch.bytes : 0; }
+
+}  // extern "C"

@@ -1,0 +1,451 @@
+"""ctypes binding of libskb_hip.so (the C ABI declared in include/skb_hip.h).
+
+Device buffers are torch tensors (torch is only the allocator and the stream
+provider here); every compute call goes through the C ABI into the
+hand-written gfx950 kernels.  If the library is missing this module raises on
+import -- there is no CPU fallback.
+
+Class names mirror the reference providers they replace:
+  FftPlan       <- skbench::FFTWProvider      (include/skbench/skbench.hpp:364-421)
+  PrunedPlan    <- skbench::FFTWPrunedProvider (include/skbench/skbench.hpp:467-520)
+  VerticalPlan  <- skbench::VerticalGemmProvider (include/skbench/skbench.hpp:152-218)
+  pointwise_advection <- PointwiseAdvectionExecutor::execute (src/pointwise_advection.hpp:18-43)
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "build", "libskb_hip.so")
+
+SKB_F64, SKB_F32 = 0, 1
+STATUS = {0: "ok", 1: "invalid_argument", 2: "runtime_error", 3: "logic_error", 4: "bad_alloc", 5: "unsupported"}
+
+
+class SkbError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__("%s: %s" % (STATUS.get(status, status), message))
+        self.status = status
+
+
+class InvalidArgument(SkbError, ValueError):
+    pass
+
+
+class Unsupported(SkbError):
+    pass
+
+
+class Workload_(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_uint64),
+        ("ny", ctypes.c_uint64),
+        ("nz", ctypes.c_uint64),
+        ("fields", ctypes.c_uint64),
+        ("lx", ctypes.c_double),
+        ("ly", ctypes.c_double),
+        ("antialias", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+MODE_DTYPE = np.dtype(
+    [
+        ("k", "<i8"),
+        ("l", "<i8"),
+        ("stored_kx", "<u8"),
+        ("stored_ky", "<u8"),
+        ("conj", "<i4"),
+        ("reserved", "<i4"),
+        ("radial", "<f8"),
+    ]
+)
+GROUP_DTYPE = np.dtype([("squared_key", "<u8"), ("first_mode", "<u8"), ("mode_count", "<u8")])
+
+FIXTURES = {"impulse": 0, "sinusoid": 1, "random": 2, "dc": 3, "nyquist": 4, "sinusoid-5-17": 5}
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libskb_hip.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u64, i32, dbl = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_double
+    W = ctypes.POINTER(Workload_)
+    pp = ctypes.POINTER(ctypes.c_void_p)
+    sigs = {
+        "skb_last_error": (ctypes.c_size_t, [ctypes.c_char_p, ctypes.c_size_t]),
+        "skb_version": (ctypes.c_char_p, []),
+        "skb_retained_modes": (i32, [W, vp, u64, ctypes.POINTER(u64)]),
+        "skb_k2_groups": (i32, [vp, u64, vp, u64, ctypes.POINTER(u64)]),
+        "skb_k2_binned_groups": (i32, [vp, u64, u64, vp, u64, ctypes.POINTER(u64)]),
+        "skb_mode_order_hash": (u64, [vp, u64]),
+        "skb_group_order_hash": (u64, [vp, u64]),
+        "skb_wvm_order_hash": (u64, [W]),
+        "skb_orthonormal_vertical": (i32, [u64, u64, vp, vp]),
+        "skb_k2_vertical_family": (i32, [u64, u64, vp, u64, vp, vp]),
+        "skb_qr_gaussian_family": (i32, [u64, u64, u64, vp, vp]),
+        "skb_make_fixture": (i32, [W, i32, u64, vp]),
+        "skb_spectral_fixture": (i32, [W, vp, u64, u64, dbl, vp]),
+        "skb_fft_plan_create": (i32, [W, i32, pp]),
+        "skb_fft_plan_destroy": (i32, [vp]),
+        "skb_fft_forward": (i32, [vp, vp, vp, vp]),
+        "skb_fft_inverse": (i32, [vp, vp, vp, vp]),
+        "skb_fft_gather_retained": (i32, [vp, vp, u64, vp, vp, vp]),
+        "skb_fft_embed_retained": (i32, [vp, vp, u64, vp, vp, vp]),
+        "skb_pruned_plan_create": (i32, [W, vp, u64, i32, pp]),
+        "skb_pruned_plan_destroy": (i32, [vp]),
+        "skb_pruned_forward": (i32, [vp, vp, vp, vp]),
+        "skb_pruned_inverse": (i32, [vp, vp, vp, vp]),
+        "skb_pruned_forward_rows": (i32, [vp, vp, vp]),
+        "skb_pruned_forward_columns": (i32, [vp, vp, vp]),
+        "skb_pruned_inverse_columns": (i32, [vp, vp, vp]),
+        "skb_pruned_inverse_rows": (i32, [vp, vp, vp]),
+        "skb_pruned_active_kx": (u64, [vp]),
+        "skb_pruned_scratch_bytes": (u64, [vp]),
+        "skb_vertical_plan_create": (i32, [W, u64, vp, u64, vp, vp, i32, pp]),
+        "skb_vertical_plan_destroy": (i32, [vp]),
+        "skb_vertical_forward": (i32, [vp, vp, vp, vp]),
+        "skb_vertical_inverse": (i32, [vp, vp, vp, vp]),
+        "skb_vertical_matrix_bytes": (u64, [vp]),
+        "skb_pointwise_advection": (i32, [u64, dbl, vp, vp, vp, i32, vp]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
+
+
+lib = _load()
+EXPORTED = [
+    "skb_last_error", "skb_version", "skb_retained_modes", "skb_k2_groups", "skb_k2_binned_groups",
+    "skb_mode_order_hash", "skb_group_order_hash", "skb_wvm_order_hash", "skb_orthonormal_vertical",
+    "skb_k2_vertical_family", "skb_qr_gaussian_family", "skb_make_fixture", "skb_spectral_fixture",
+    "skb_fft_plan_create", "skb_fft_plan_destroy", "skb_fft_forward", "skb_fft_inverse",
+    "skb_fft_gather_retained", "skb_fft_embed_retained", "skb_pruned_plan_create", "skb_pruned_plan_destroy",
+    "skb_pruned_forward", "skb_pruned_inverse", "skb_pruned_active_kx", "skb_pruned_scratch_bytes",
+    "skb_pruned_forward_rows", "skb_pruned_forward_columns", "skb_pruned_inverse_columns", "skb_pruned_inverse_rows",
+    "skb_vertical_plan_create", "skb_vertical_plan_destroy", "skb_vertical_forward", "skb_vertical_inverse",
+    "skb_vertical_matrix_bytes", "skb_pointwise_advection",
+]
+
+
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(4096)
+    lib.skb_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(status: int):
+    if status == 0:
+        return
+    msg = last_error()
+    if status == 1:
+        raise InvalidArgument(status, msg)
+    if status == 5:
+        raise Unsupported(status, msg)
+    raise SkbError(status, msg)
+
+
+def version() -> str:
+    return lib.skb_version().decode()
+
+
+@dataclass(frozen=True)
+class Workload:
+    """skbench::Workload (include/skbench/skbench.hpp:23-39)."""
+
+    nx: int
+    ny: int
+    nz: int
+    fields: int
+    lx: float = 1.0
+    ly: float = 1.0
+    antialias: bool = True
+
+    @property
+    def planes(self):
+        return self.nz * self.fields
+
+    @property
+    def nx_half(self):
+        return self.nx // 2 + 1
+
+    @property
+    def real_elements(self):
+        return self.nx * self.ny * self.planes
+
+    @property
+    def spectrum_elements(self):
+        return self.nx_half * self.ny * self.planes
+
+    @property
+    def nj(self):
+        return 0 if self.nz < 2 else 2 * (self.nz - 1) // 3
+
+    def c(self) -> Workload_:
+        return Workload_(self.nx, self.ny, self.nz, self.fields, self.lx, self.ly, int(self.antialias), 0)
+
+
+def _np(a):
+    assert a.flags["C_CONTIGUOUS"]
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+# ---- host model ---------------------------------------------------------------
+def retained_modes(w: Workload) -> np.ndarray:
+    wc = w.c()
+    n = ctypes.c_uint64()
+    check(lib.skb_retained_modes(ctypes.byref(wc), None, 0, ctypes.byref(n)))
+    out = np.zeros(n.value, MODE_DTYPE)
+    check(lib.skb_retained_modes(ctypes.byref(wc), _np(out), n.value, ctypes.byref(n)))
+    return out
+
+
+def k2_groups(modes: np.ndarray) -> np.ndarray:
+    out = np.zeros(len(modes), GROUP_DTYPE)
+    n = ctypes.c_uint64()
+    check(lib.skb_k2_groups(_np(modes), len(modes), _np(out), len(out), ctypes.byref(n)))
+    return out[: n.value].copy()
+
+
+def k2_binned_groups(modes: np.ndarray, bins: int = 32) -> np.ndarray:
+    out = np.zeros(len(modes), GROUP_DTYPE)
+    n = ctypes.c_uint64()
+    check(lib.skb_k2_binned_groups(_np(modes), len(modes), bins, _np(out), len(out), ctypes.byref(n)))
+    return out[: n.value].copy()
+
+
+def mode_order_hash(modes) -> str:
+    return "fnv1a64:%016x" % lib.skb_mode_order_hash(_np(modes), len(modes))
+
+
+def group_order_hash(groups) -> str:
+    return "fnv1a64:%016x" % lib.skb_group_order_hash(_np(groups), len(groups))
+
+
+def wvm_order_hash(w: Workload) -> str:
+    wc = w.c()
+    return "fnv1a64:%016x" % lib.skb_wvm_order_hash(ctypes.byref(wc))
+
+
+def orthonormal_vertical(nz: int, nj: int):
+    f = np.zeros(nj * nz)
+    i = np.zeros(nz * nj)
+    check(lib.skb_orthonormal_vertical(nz, nj, _np(f), _np(i)))
+    return f, i
+
+
+def k2_vertical_family(nz: int, nj: int, groups: np.ndarray):
+    f = np.zeros(len(groups) * nj * nz)
+    i = np.zeros(len(groups) * nz * nj)
+    check(lib.skb_k2_vertical_family(nz, nj, _np(groups), len(groups), _np(f), _np(i)))
+    return f, i
+
+
+def qr_gaussian_family(n: int, groups: int, seed: int):
+    f = np.zeros(groups * n * n)
+    i = np.zeros(groups * n * n)
+    check(lib.skb_qr_gaussian_family(n, groups, seed, _np(f), _np(i)))
+    return f, i
+
+
+def make_fixture(w: Workload, kind: str, seed: int) -> np.ndarray:
+    out = np.zeros(w.real_elements)
+    wc = w.c()
+    check(lib.skb_make_fixture(ctypes.byref(wc), FIXTURES[kind], seed, _np(out)))
+    return out
+
+
+def spectral_fixture(w: Workload, modes: np.ndarray, seed: int, exponent: float = -1.0) -> np.ndarray:
+    out = np.zeros(2 * len(modes) * w.planes)
+    wc = w.c()
+    check(lib.skb_spectral_fixture(ctypes.byref(wc), _np(modes), len(modes), seed, exponent, _np(out)))
+    return out.view(np.complex128)
+
+
+# ---- device helpers -------------------------------------------------------------
+def _torch():
+    import torch
+
+    return torch
+
+
+def real_dtype(precision):
+    t = _torch()
+    return t.float64 if precision == SKB_F64 else t.float32
+
+
+def complex_dtype(precision):
+    t = _torch()
+    return t.complex128 if precision == SKB_F64 else t.complex64
+
+
+def _dev(t, numel: int, dtype, name: str):
+    """Validate a device tensor on the host before any launch."""
+    torch = _torch()
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("%s must be a torch tensor" % name)
+    if t.device.type != "cuda":
+        raise ValueError("%s must be a device tensor" % name)
+    if t.dtype != dtype:
+        raise TypeError("%s has dtype %s, expected %s" % (name, t.dtype, dtype))
+    if not t.is_contiguous():
+        raise ValueError("%s must be contiguous" % name)
+    if t.numel() != numel:
+        raise ValueError("%s has %d elements, expected %d" % (name, t.numel(), numel))
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+class FftPlan:
+    """Full batched 2-D r2c / c2r in WVM order (FFTWProvider)."""
+
+    def __init__(self, w: Workload, precision: int = SKB_F64):
+        self.w, self.precision = w, precision
+        self._wc = w.c()
+        h = ctypes.c_void_p()
+        check(lib.skb_fft_plan_create(ctypes.byref(self._wc), precision, ctypes.byref(h)))
+        self.h = h
+
+    def forward(self, real, wvm, stream=None):
+        w = self.w
+        check(lib.skb_fft_forward(self.h, _dev(real, w.real_elements, real_dtype(self.precision), "real"),
+                                  _dev(wvm, w.spectrum_elements, complex_dtype(self.precision), "wvm"), _stream(stream)))
+
+    def inverse(self, wvm, real, stream=None):
+        """Destroys `wvm` (FFTW c2r semantics)."""
+        w = self.w
+        check(lib.skb_fft_inverse(self.h, _dev(wvm, w.spectrum_elements, complex_dtype(self.precision), "wvm"),
+                                  _dev(real, w.real_elements, real_dtype(self.precision), "real"), _stream(stream)))
+
+    def gather_retained(self, modes, wvm, retained, stream=None):
+        w = self.w
+        check(lib.skb_fft_gather_retained(
+            self.h, _np(modes), len(modes), _dev(wvm, w.spectrum_elements, complex_dtype(self.precision), "wvm"),
+            _dev(retained, len(modes) * w.planes, complex_dtype(self.precision), "retained"), _stream(stream)))
+
+    def embed_retained(self, modes, retained, wvm, stream=None):
+        w = self.w
+        check(lib.skb_fft_embed_retained(
+            self.h, _np(modes), len(modes),
+            _dev(retained, len(modes) * w.planes, complex_dtype(self.precision), "retained"),
+            _dev(wvm, w.spectrum_elements, complex_dtype(self.precision), "wvm"), _stream(stream)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.skb_fft_plan_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+
+class PrunedPlan:
+    """Pruned retained-mode transforms (FFTWPrunedProvider)."""
+
+    def __init__(self, w: Workload, modes: np.ndarray, precision: int = SKB_F64):
+        self.w, self.precision, self.modes = w, precision, modes
+        self._wc = w.c()
+        h = ctypes.c_void_p()
+        check(lib.skb_pruned_plan_create(ctypes.byref(self._wc), _np(modes), len(modes), precision, ctypes.byref(h)))
+        self.h = h
+
+    @property
+    def retained_elements(self):
+        return len(self.modes) * self.w.planes
+
+    def forward(self, real, retained, stream=None):
+        check(lib.skb_pruned_forward(self.h, _dev(real, self.w.real_elements, real_dtype(self.precision), "real"),
+                                     _dev(retained, self.retained_elements, complex_dtype(self.precision), "retained"),
+                                     _stream(stream)))
+
+    def inverse(self, retained, real, stream=None):
+        check(lib.skb_pruned_inverse(self.h,
+                                     _dev(retained, self.retained_elements, complex_dtype(self.precision), "retained"),
+                                     _dev(real, self.w.real_elements, real_dtype(self.precision), "real"),
+                                     _stream(stream)))
+
+    # stage-by-stage form (intermediate in the plan's scratch)
+    def forward_rows(self, real, stream=None):
+        check(lib.skb_pruned_forward_rows(
+            self.h, _dev(real, self.w.real_elements, real_dtype(self.precision), "real"), _stream(stream)))
+
+    def forward_columns(self, retained, stream=None):
+        check(lib.skb_pruned_forward_columns(
+            self.h, _dev(retained, self.retained_elements, complex_dtype(self.precision), "retained"), _stream(stream)))
+
+    def inverse_columns(self, retained, stream=None):
+        check(lib.skb_pruned_inverse_columns(
+            self.h, _dev(retained, self.retained_elements, complex_dtype(self.precision), "retained"), _stream(stream)))
+
+    def inverse_rows(self, real, stream=None):
+        check(lib.skb_pruned_inverse_rows(
+            self.h, _dev(real, self.w.real_elements, real_dtype(self.precision), "real"), _stream(stream)))
+
+    def active_kx(self) -> int:
+        return int(lib.skb_pruned_active_kx(self.h))
+
+    def scratch_bytes(self) -> int:
+        return int(lib.skb_pruned_scratch_bytes(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.skb_pruned_plan_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+
+class VerticalPlan:
+    """K^2-grouped vertical GEMM on MFMA (VerticalGemmProvider)."""
+
+    def __init__(self, w: Workload, nj: int, groups: np.ndarray, forward: np.ndarray, inverse: np.ndarray,
+                 precision: int = SKB_F64):
+        self.w, self.nj, self.precision = w, nj, precision
+        self.modes = int(groups["first_mode"][-1] + groups["mode_count"][-1])
+        self._wc = w.c()
+        forward = np.ascontiguousarray(forward, np.float64)
+        inverse = np.ascontiguousarray(inverse, np.float64)
+        if forward.size != len(groups) * nj * w.nz or inverse.size != forward.size:
+            raise ValueError("matrix family size does not match groups x nj x nz")
+        h = ctypes.c_void_p()
+        check(lib.skb_vertical_plan_create(ctypes.byref(self._wc), nj, _np(groups), len(groups), _np(forward),
+                                           _np(inverse), precision, ctypes.byref(h)))
+        self.h = h
+
+    def forward(self, physical, modal, stream=None):
+        cd = complex_dtype(self.precision)
+        check(lib.skb_vertical_forward(self.h, _dev(physical, self.modes * self.w.planes, cd, "physical"),
+                                       _dev(modal, self.modes * self.w.fields * self.nj, cd, "modal"), _stream(stream)))
+
+    def inverse(self, modal, physical, stream=None):
+        cd = complex_dtype(self.precision)
+        check(lib.skb_vertical_inverse(self.h, _dev(modal, self.modes * self.w.fields * self.nj, cd, "modal"),
+                                       _dev(physical, self.modes * self.w.planes, cd, "physical"), _stream(stream)))
+
+    def matrix_bytes(self) -> int:
+        return int(lib.skb_vertical_matrix_bytes(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.skb_vertical_plan_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+
+def pointwise_advection(volume: int, scale: float, shared, derivative, target, precision=SKB_F64, stream=None):
+    rd = real_dtype(precision)
+    check(lib.skb_pointwise_advection(volume, scale, _dev(shared, 3 * volume, rd, "shared"),
+                                      _dev(derivative, 3 * volume, rd, "derivative"), _dev(target, volume, rd, "target"),
+                                      precision, _stream(stream)))
