@@ -39,7 +39,64 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--quiet", action="store_true", help="no progress lines on stderr")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="short run for rocprofv3: warmup, --steps evals, then one 256 MiB device copy "
+                         "(the calibration kernel for FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="turn two rocprofv3 --pmc counter_collection.csv files (FETCH_SIZE run, WRITE_SIZE run) "
+                         "into " + os.path.relpath(TRAFFIC_FILE, ROOT) + " and exit")
     return ap.parse_args()
+
+
+COPY_BYTES = 256 << 20
+KERNEL_NAMES = {  # substring of the device kernel name -> stage
+    "col_kernel<float, 256, true, 1, 0>": "inverse_columns",
+    "row_c2r_kernel<float, 256>": "inverse_rows",
+    "row_r2c_kernel<float, 256>": "forward_rows",
+    "col_kernel<float, 256, false, 0, 1>": "forward_columns",
+    "grouped_gemm_kernel<float": "vertical_gemm",
+}
+
+
+def summarize_pmc(fetch_csv, write_csv, family):
+    """Per-kernel HBM bytes from rocprofv3 counter CSVs, scaled by what the same
+    counters read for a plain device copy of COPY_BYTES (read COPY_BYTES,
+    write COPY_BYTES)."""
+    import csv
+
+    def per_kernel(path, counter):
+        sums, counts = {}, {}
+        with open(path, newline="") as handle:
+            for row in csv.DictReader(handle):
+                if row.get("Counter_Name") != counter:
+                    continue
+                name = row.get("Kernel_Name", "")
+                sums[name] = sums.get(name, 0.0) + float(row["Counter_Value"])
+                counts[name] = counts.get(name, 0) + 1
+        return {k: sums[k] / counts[k] for k in sums}  # KiB per dispatch
+
+    fetch = per_kernel(fetch_csv, "FETCH_SIZE")
+    write = per_kernel(write_csv, "WRITE_SIZE")
+    copy_f = [v for k, v in fetch.items() if "copy" in k.lower() or "elementwise" in k.lower()]
+    copy_w = [v for k, v in write.items() if "copy" in k.lower() or "elementwise" in k.lower()]
+    if not copy_f or not copy_w:
+        raise SystemExit("no copy kernel found in the counter files")
+    f_scale = (COPY_BYTES / 1024.0) / max(copy_f)
+    w_scale = (COPY_BYTES / 1024.0) / max(copy_w)
+    out = {"family": family, "copy_bytes": COPY_BYTES, "fetch_scale": f_scale, "write_scale": w_scale,
+           "kernels": {}}
+    for name, stage in KERNEL_NAMES.items():
+        fk = [v for k, v in fetch.items() if name in k]
+        wk = [v for k, v in write.items() if name in k]
+        if fk and wk:
+            fb = fk[0] * 1024.0 * f_scale
+            wb = wk[0] * 1024.0 * w_scale
+            out["kernels"][stage] = {"fetch_bytes": fb, "write_bytes": wb, "hbm_bytes": fb + wb,
+                                     "fetch_kib_raw": fk[0], "write_kib_raw": wk[0]}
+    with open(TRAFFIC_FILE, "w") as handle:
+        json.dump(out, handle, indent=1)
+        handle.write("\n")
+    print(json.dumps(out))
 
 
 def log(args, msg):
@@ -145,6 +202,9 @@ def cpu_baseline(args, w, x64, nj, groups, fwd):
 
 def main():
     args = parse()
+    if args.summarize_pmc:
+        summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], args.family)
+        return 0
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (requested --gpus %d)" % args.gpus,
               file=sys.stderr)
@@ -185,6 +245,19 @@ def main():
     err_max = float(np.abs(a - b).max() / max(1.0, np.abs(b).max()))
     err_l2 = float(np.linalg.norm(a - b) / np.linalg.norm(b))
     log(args, "fp32 vs fp64 max-scale err %.3e, rel L2 %.3e" % (err_max, err_l2))
+
+    if args.profile_run:
+        for _ in range(args.warmup):
+            f32.step()
+        for _ in range(args.steps):
+            f32.step()
+        src = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device=device).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(args, "profile run done")
+        return 0
 
     log(args, "warmup %d" % args.warmup)
     for _ in range(args.warmup):

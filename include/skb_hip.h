@@ -154,6 +154,13 @@ int skb_pruned_forward_rows(skb_pruned_plan* plan, const void* real, void* strea
 int skb_pruned_forward_columns(skb_pruned_plan* plan, void* retained, void* stream);
 int skb_pruned_inverse_columns(skb_pruned_plan* plan, const void* retained, void* stream);
 int skb_pruned_inverse_rows(skb_pruned_plan* plan, void* real, void* stream);
+/* Launch shape of one stage (0 forward rows, 1 forward columns, 2 inverse
+ * columns, 3 inverse rows): the number of workgroups, 0 = one per work item
+ * (the default).  Fewer workgroups than items makes them persistent, each
+ * prefetching its next item.  Set by the launch autotuner; results do not
+ * depend on it.  No reference counterpart (FFTW plans choose internally). */
+int skb_pruned_plan_set_grid(skb_pruned_plan* plan, int stage, int workgroups);
+uint64_t skb_pruned_plan_items(const skb_pruned_plan* plan, int stage);
 /* FFTWPrunedProvider::activeKxCount / scratchBytes. */
 uint64_t skb_pruned_active_kx(const skb_pruned_plan* plan);
 uint64_t skb_pruned_scratch_bytes(const skb_pruned_plan* plan);
@@ -174,6 +181,11 @@ int skb_vertical_forward(skb_vertical_plan* plan, const void* physical, void* mo
 int skb_vertical_inverse(skb_vertical_plan* plan, const void* modal, void* physical, void* stream);
 /* VerticalGemmProvider::matrixBytesPerDirection. */
 uint64_t skb_vertical_matrix_bytes(const skb_vertical_plan* plan);
+/* Launch shape: wavefronts per launch, 0 = one per 16-column tile (the
+ * default); fewer makes them persistent.  Set by the launch autotuner; no
+ * reference counterpart. */
+int skb_vertical_plan_set_grid(skb_vertical_plan* plan, int wavefronts);
+uint64_t skb_vertical_plan_items(const skb_vertical_plan* plan);
 
 /* ---- pointwise advection -----------------------------------------------
  * Replaces PointwiseAdvectionExecutor::execute (src/pointwise_advection.hpp

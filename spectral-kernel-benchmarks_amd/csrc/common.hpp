@@ -84,6 +84,18 @@ struct DeviceBuffer {
 
 inline bool is_pow2(uint64_t v) { return v && !(v & (v - 1)); }
 
+// XCD-aware work mapping.  MI355X dispatches workgroup b to XCD b % 8, and
+// each XCD has its own L2.  With one workgroup per item, item ranges are
+// split into 8 contiguous chunks, one per XCD, so neighbouring items (which
+// share the cache lines at their boundaries) run behind the same L2.
+// Launch xcd_grid(items) workgroups; xcd_item() >= items means idle.
+constexpr int kXcds = 8;
+inline int xcd_grid(int items) { return ((items + kXcds - 1) / kXcds) * kXcds; }
+__device__ __forceinline__ int xcd_item(int b, int items) {
+    const int per = (items + kXcds - 1) / kXcds;
+    return (b % kXcds) * per + b / kXcds;
+}
+
 // Sizes the hand-written FFT kernels are instantiated for.
 constexpr uint64_t kMinFft = 4;
 constexpr uint64_t kMaxFft = 1024;

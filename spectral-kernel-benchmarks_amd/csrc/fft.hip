@@ -299,6 +299,397 @@ __global__ __launch_bounds__(kThreads) void col_kernel(const cx<R>* in, cx<R>* o
 }
 
 // ---------------------------------------------------------------------------
+// Two-pass register FFT for N = N1 * N2 (N >= 64)
+//
+// X[k1 + N1 k2] = sum_n2 W_N^(n2 k1) W_N2^(n2 k2) sum_n1 x[N2 n1 + n2] W_N1^(n1 k1)
+// Pass 1: a thread owns one n2 (of one sequence), loads x[N2 n1 + n2] for all
+//         n1 straight from global memory, does a length-N1 DFT in registers
+//         and applies W_N^(n2 k1).
+// Pass 2: after ONE transpose through LDS a thread owns one k1, does a
+//         length-N2 DFT in registers and stores X[k1 + N1 k2] straight to
+//         global memory.
+// One LDS round trip and one barrier per transform (plus the twiddle table).
+// ---------------------------------------------------------------------------
+
+// cos(2 pi m / 32), exact symmetric values from the first octant.
+__host__ __device__ constexpr double cos32(int m) {
+    constexpr double oct[9] = {1.0,
+                               0.98078528040323044912618223613424,
+                               0.92387953251128675612818318939679,
+                               0.83146961230254523707878837761791,
+                               0.70710678118654752440084436210485,
+                               0.55557023301960222474283081394853,
+                               0.38268343236508977172845998403040,
+                               0.19509032201612826784828486847702,
+                               0.0};
+    m &= 31;
+    if (m <= 8) return oct[m];
+    if (m <= 16) return -oct[16 - m];
+    if (m <= 24) return -oct[m - 16];
+    return oct[32 - m];
+}
+
+// x * exp(-+ 2 pi i m / R) for a compile-time-known m (after unrolling).
+template <int R, bool INV, typename T>
+__device__ __forceinline__ cx<T> twc(int m, cx<T> x) {
+    m = ((m % R) + R) % R;
+    if (m == 0) return x;
+    if (4 * m == R) return rot_i<INV>(x);
+    if (2 * m == R) return cx<T>{-x.x, -x.y};
+    if (4 * m == 3 * R) return rot_i<!INV>(x);
+    const int m32 = m * (32 / R);
+    const T c = static_cast<T>(cos32(m32));
+    const T sn = static_cast<T>(cos32(m32 - 8));  // sin(2 pi m / R)
+    const cx<T> w{c, INV ? sn : -sn};
+    return cmul(x, w);
+}
+
+template <int R>
+struct RSplit;
+template <>
+struct RSplit<16> {
+    static constexpr int A = 4;
+};
+template <>
+struct RSplit<32> {
+    static constexpr int A = 8;
+};
+
+// In-register DFT of length R in natural order (R = 2, 4, 8, 16, 32).
+template <int R, bool INV, typename T>
+__device__ __forceinline__ void rdft(cx<T>* v) {
+    if constexpr (R == 2 || R == 4 || R == 8) {
+        dft<R, INV>(v);
+    } else {
+        constexpr int A = RSplit<R>::A, B = R / A;
+        cx<T> y[R];
+#pragma unroll
+        for (int n2 = 0; n2 < B; ++n2) {
+            cx<T> u[A];
+#pragma unroll
+            for (int n1 = 0; n1 < A; ++n1) u[n1] = v[B * n1 + n2];
+            rdft<A, INV>(u);
+#pragma unroll
+            for (int k1 = 0; k1 < A; ++k1) y[k1 * B + n2] = twc<R, INV>(n2 * k1, u[k1]);
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) {
+            cx<T> w[B];
+#pragma unroll
+            for (int n2 = 0; n2 < B; ++n2) w[n2] = y[k1 * B + n2];
+            rdft<B, INV>(w);
+#pragma unroll
+            for (int k2 = 0; k2 < B; ++k2) v[k1 + A * k2] = w[k2];
+        }
+    }
+}
+
+template <int N>
+struct Split2 {
+    // N1 >= N2, both <= 32
+    static constexpr int N1 = N == 64 ? 8 : (N == 128 ? 16 : (N == 256 ? 16 : 32));
+    static constexpr int N2 = N / N1;
+};
+
+// Planes (column kernels) or plane pairs (row kernels) per workgroup, and
+// the workgroup size: keep each 16-byte-or-wider global access run at >= 128 B.
+template <typename R, int N>
+struct Geo2 {
+    static constexpr int N1 = Split2<N>::N1, N2 = Split2<N>::N2;
+    static constexpr int LANES = (sizeof(R) == 4 ? 256 : 128);
+    static constexpr int TP = LANES / N1;       // column kernel: planes per workgroup
+    static constexpr int PAIRS = LANES / N1;    // row kernels: plane pairs per workgroup
+    static constexpr int THREADS = LANES;       // TP * N1 >= TP * N2
+};
+
+// Row passes.  A work item is one row index y of a tile of TP = 2*PAIRS
+// consecutive planes; item = y * tiles + tile, so the workgroups running at
+// the same time cover whole plane runs of the same rows (DRAM-page locality
+// in the plane-fastest intermediate).  Persistent workgroups prefetch
+// the next item's inputs into registers while transforming the current one.
+template <typename R, int NX>
+__device__ __forceinline__ void r2c_load(cx<R>* v, const R* real, int item, int ny, int planes) {
+    using G = Geo2<R, NX>;
+    constexpr int N1 = G::N1, N2 = G::N2, S = G::PAIRS, TP = 2 * S;
+    const int t = threadIdx.x, n2 = t % N2, s1 = t / N2;
+    const int ntiles = (planes + TP - 1) / TP;
+    const int y = item / ntiles, p0 = (item % ntiles) * TP;
+    const int pa = p0 + 2 * s1, pb = pa + 1;
+    const bool act = s1 < S;
+    const R* ra = real + (static_cast<size_t>(pa) * ny + y) * NX + n2;
+    const R* rb = ra + static_cast<size_t>(ny) * NX;
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) {
+        v[n1].x = (act && pa < planes) ? ra[N2 * n1] : R(0);
+        v[n1].y = (act && pb < planes) ? rb[N2 * n1] : R(0);
+    }
+}
+
+template <typename R, int NX>
+__global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void row2_r2c_kernel(const R* __restrict__ real,
+                                                                         cx<R>* __restrict__ out,
+                                                                         const cx<R>* __restrict__ tw, int ny,
+                                                                         int planes, int kc, int items) {
+    using G = Geo2<R, NX>;
+    constexpr int N1 = G::N1, N2 = G::N2, S = G::PAIRS, TP = 2 * S, NT = G::THREADS;
+    // LDS strides in complex elements are odd, so the 16 (FP32) or 8 (FP64)
+    // plane pairs touched by one instruction fall on distinct banks.
+    constexpr int YS = N2 + 1;             // row of the transpose buffer
+    constexpr int SS = N1 * YS + 1;        // one pair's transpose buffer
+    constexpr int ZS = NX + 1;             // one pair's spectrum
+    constexpr int BUF = S * (SS > ZS ? SS : ZS);
+    __shared__ cx<R> buf[BUF];
+    __shared__ cx<R> twl[NX];
+    const int t = threadIdx.x;
+    // one workgroup per item: XCD-aware mapping; persistent: linear
+    int item = gridDim.x >= static_cast<unsigned>(items) ? xcd_item(blockIdx.x, items) : blockIdx.x;
+    cx<R> v[N1];
+    if (item < items) r2c_load<R, NX>(v, real, item, ny, planes);
+    for (int i = t; i < NX; i += NT) twl[i] = tw[i];
+    __syncthreads();
+    const int n2 = t % N2, s1 = t / N2;
+    const int k1 = t % N1, s2 = t / N1;
+    const int ntiles = (planes + TP - 1) / TP;
+    while (item < items) {
+        const int next = gridDim.x >= static_cast<unsigned>(items) ? items : item + gridDim.x;
+        cx<R> vn[N1];
+        if (next < items) r2c_load<R, NX>(vn, real, next, ny, planes);
+        const int y = item / ntiles, p0 = (item % ntiles) * TP;
+        // pass 1: thread (n2 fastest, pair s)
+        if (s1 < S) {
+            rdft<N1, false>(v);
+#pragma unroll
+            for (int k = 0; k < N1; ++k) buf[s1 * SS + k * YS + n2] = cmul(v[k], twl[n2 * k]);
+        }
+        __syncthreads();
+        // pass 2: thread (k1 fastest, pair s)
+        cx<R> w[N2];
+#pragma unroll
+        for (int m = 0; m < N2; ++m) w[m] = buf[s2 * SS + k1 * YS + m];
+        __syncthreads();  // buf becomes Z[s][k]
+        rdft<N2, false>(w);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) buf[s2 * ZS + k1 + N1 * k2] = w[k2];
+        __syncthreads();
+        // Hermitian split of each pair and plane-fastest store of kx < kc
+        const size_t row_base = static_cast<size_t>(planes) * static_cast<size_t>(kc) * y;
+        for (int e = t; e < kc * TP; e += NT) {
+            const int pit = e % TP, kx = e / TP;
+            const int p = p0 + pit;
+            if (p >= planes) continue;
+            const cx<R> zk = buf[(pit >> 1) * ZS + kx];
+            const cx<R> zn = buf[(pit >> 1) * ZS + ((NX - kx) & (NX - 1))];
+            const cx<R> x = (pit & 1) == 0 ? cx<R>{R(0.5) * (zk.x + zn.x), R(0.5) * (zk.y - zn.y)}
+                                           : cx<R>{R(0.5) * (zk.y + zn.y), R(0.5) * (zn.x - zk.x)};
+            out[row_base + static_cast<size_t>(planes) * kx + p] = x;
+        }
+        __syncthreads();  // buf is rewritten by the next item
+#pragma unroll
+        for (int m = 0; m < N1; ++m) v[m] = vn[m];
+        item = next;
+    }
+}
+
+// c2r pass-1 input: Z[k] = X_a[k] + i X_b[k] with the Hermitian extension;
+// Im of the kx = 0 and kx = Nx/2 bins is ignored and kx >= kc is zero (never
+// read).  Thread (pair s fastest, n2).
+template <typename R, int NX>
+__device__ __forceinline__ void c2r_load(cx<R>* v, const cx<R>* in, int item, int ny, int planes, int kc) {
+    using G = Geo2<R, NX>;
+    constexpr int N1 = G::N1, N2 = G::N2, S = G::PAIRS, TP = 2 * S;
+    const int t = threadIdx.x, s1 = t % S, n2 = t / S;
+    const int ntiles = (planes + TP - 1) / TP;
+    const int y = item / ntiles, p0 = (item % ntiles) * TP;
+    const int pa = p0 + 2 * s1;
+    const bool act = n2 < N2;
+    const size_t row_base = static_cast<size_t>(planes) * static_cast<size_t>(kc) * y;
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) {
+        const int k = N2 * n1 + n2;
+        const bool upper = k > NX / 2;
+        const int kx = upper ? NX - k : k;
+        cx<R> xa{R(0), R(0)}, xb{R(0), R(0)};
+        if (act && kx < kc) {
+            const size_t base = row_base + static_cast<size_t>(planes) * kx + pa;
+            if (pa < planes) xa = in[base];
+            if (pa + 1 < planes) xb = in[base + 1];
+        }
+        if (kx == 0 || kx == NX / 2) {
+            xa.y = R(0);
+            xb.y = R(0);
+        }
+        v[n1] = upper ? cx<R>{xa.x + xb.y, xb.x - xa.y} : cx<R>{xa.x - xb.y, xa.y + xb.x};
+    }
+}
+
+template <typename R, int NX>
+__global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void row2_c2r_kernel(const cx<R>* __restrict__ in,
+                                                                         R* __restrict__ real,
+                                                                         const cx<R>* __restrict__ tw, int ny,
+                                                                         int planes, int kc, int items) {
+    using G = Geo2<R, NX>;
+    constexpr int N1 = G::N1, N2 = G::N2, S = G::PAIRS, TP = 2 * S, NT = G::THREADS;
+    constexpr int YS = N2 + 1;
+    constexpr int SS = N1 * YS + 1;  // odd: conflict-free across pairs
+    __shared__ cx<R> buf[S * SS];
+    __shared__ cx<R> twl[NX];
+    const int t = threadIdx.x;
+    // one workgroup per item: XCD-aware mapping; persistent: linear
+    int item = gridDim.x >= static_cast<unsigned>(items) ? xcd_item(blockIdx.x, items) : blockIdx.x;
+    cx<R> v[N1];
+    if (item < items) c2r_load<R, NX>(v, in, item, ny, planes, kc);
+    for (int i = t; i < NX; i += NT) twl[i] = tw[i];
+    __syncthreads();
+    const int s1 = t % S, n2 = t / S;
+    const int k1 = t % N1, s2 = t / N1;
+    const int ntiles = (planes + TP - 1) / TP;
+    while (item < items) {
+        const int next = gridDim.x >= static_cast<unsigned>(items) ? items : item + gridDim.x;
+        cx<R> vn[N1];
+        if (next < items) c2r_load<R, NX>(vn, in, next, ny, planes, kc);
+        const int y = item / ntiles, p0 = (item % ntiles) * TP;
+        if (n2 < N2) {
+            rdft<N1, true>(v);
+#pragma unroll
+            for (int k = 0; k < N1; ++k) buf[s1 * SS + k * YS + n2] = cmulc(v[k], twl[n2 * k]);
+        }
+        __syncthreads();
+        // pass 2: thread (k1 fastest, pair s) -> real rows of planes 2s, 2s+1
+        cx<R> w[N2];
+#pragma unroll
+        for (int m = 0; m < N2; ++m) w[m] = buf[s2 * SS + k1 * YS + m];
+        rdft<N2, true>(w);
+        const int pa = p0 + 2 * s2;
+        R* ra = real + (static_cast<size_t>(pa) * ny + y) * NX + k1;
+        R* rb = ra + static_cast<size_t>(ny) * NX;
+        if (pa < planes) {
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) ra[N1 * k2] = w[k2].x;
+        }
+        if (pa + 1 < planes) {
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) rb[N1 * k2] = w[k2].y;
+        }
+        __syncthreads();  // buf is rewritten by the next item
+#pragma unroll
+        for (int m = 0; m < N1; ++m) v[m] = vn[m];
+        item = next;
+    }
+}
+
+// Column pass along y.  A work item is one column kx and a tile of TP
+// consecutive planes; item = kx * tiles + tile, so concurrently running
+// workgroups read whole contiguous plane runs of the same (kx, y) rows --
+// plane-tile-major order would leave each DRAM page 8% used per visit.  Workgroups are persistent:
+// each loops over items blockIdx.x, blockIdx.x + gridDim.x, ... and issues the
+// global loads of its NEXT item before transforming the current one, so HBM
+// traffic overlaps the register/LDS work instead of alternating with it.
+// IN  kStrided: in[P*(kx + in_kc*y) + p]   kRetained: in[P*mode + p] via map
+// OUT kStrided: out[P*(kx + out_kc*y) + p] kRetained: out[P*mode + p] via map
+// Maps: code = 2*mode + conj at [kx*NY + row], or -1 (no mode).
+template <typename R, int NY, int IN>
+__device__ __forceinline__ void col2_load(cx<R>* v, const cx<R>* in, int item, int kx_count, int planes, int in_kc,
+                                          const int* __restrict__ map) {
+    using G = Geo2<R, NY>;
+    constexpr int N1 = G::N1, N2 = G::N2, TP = G::TP;
+    const int t = threadIdx.x, pp = t % TP, n2 = t / TP;
+    const int ntiles = (planes + TP - 1) / TP;
+    const int kx = item / ntiles, p = (item % ntiles) * TP + pp;
+    const size_t P = static_cast<size_t>(planes);
+    if constexpr (IN == kStrided) {
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) {
+            const int yy = N2 * n1 + n2;
+            v[n1] = (n2 < N2 && p < planes) ? in[P * (kx + static_cast<size_t>(in_kc) * yy) + p] : cx<R>{R(0), R(0)};
+        }
+    } else {
+        // all codes first, then all values: two batched round trips
+        int code[N1];
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) code[n1] = n2 < N2 ? map[kx * NY + N2 * n1 + n2] : -1;
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) {
+            cx<R> x{R(0), R(0)};
+            if (code[n1] >= 0 && p < planes) {
+                x = in[P * (code[n1] >> 1) + p];
+                if (code[n1] & 1) x.y = -x.y;
+            }
+            v[n1] = x;
+        }
+    }
+}
+
+template <typename R, int NY, bool INV, int IN, int OUT>
+__global__ __launch_bounds__((Geo2<R, NY>::THREADS)) void col2_kernel(const cx<R>* in, cx<R>* out,
+                                                                     const cx<R>* __restrict__ tw, int planes,
+                                                                     int kx_count, int items, int in_kc, int out_kc,
+                                                                     const int* __restrict__ map) {
+    using G = Geo2<R, NY>;
+    constexpr int N1 = G::N1, N2 = G::N2, TP = G::TP, NT = G::THREADS;
+    __shared__ cx<R> buf[NY * TP];
+    __shared__ cx<R> twl[NY];
+    const int t = threadIdx.x;
+    const size_t P = static_cast<size_t>(planes);
+    const int ntiles = (planes + TP - 1) / TP;
+    // one workgroup per item: XCD-aware mapping; persistent: linear
+    int item = gridDim.x >= static_cast<unsigned>(items) ? xcd_item(blockIdx.x, items) : blockIdx.x;
+    cx<R> v[N1];
+    if (item < items) col2_load<R, NY, IN>(v, in, item, kx_count, planes, in_kc, map);
+    for (int i = t; i < NY; i += NT) twl[i] = tw[i];
+    __syncthreads();
+    const int pp = t % TP, n2 = t / TP, k1 = t / TP;
+    while (item < items) {
+        const int next = gridDim.x >= static_cast<unsigned>(items) ? items : item + gridDim.x;
+        cx<R> vn[N1];
+        if (next < items) col2_load<R, NY, IN>(vn, in, next, kx_count, planes, in_kc, map);
+        const int kx = item / ntiles, p = (item % ntiles) * TP + pp;
+        // Output codes are loaded with (not after) the item's inputs: on gfx9
+        // one in-order counter tracks loads and stores, so a code loaded
+        // between two stores would make the second store wait for the first.
+        int ocode[N2];
+        if constexpr (OUT == kRetained) {
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) ocode[k2] = map[kx * NY + k1 + N1 * k2];
+        }
+        // pass 1: thread (plane fastest, n2)
+        if (n2 < N2) {
+            rdft<N1, INV>(v);
+#pragma unroll
+            for (int k = 0; k < N1; ++k) {
+                const cx<R> w = twl[n2 * k];
+                buf[(k * N2 + n2) * TP + pp] = INV ? cmulc(v[k], w) : cmul(v[k], w);
+            }
+        }
+        __syncthreads();
+        // pass 2: thread (plane fastest, k1)
+        cx<R> w[N2];
+#pragma unroll
+        for (int m = 0; m < N2; ++m) w[m] = buf[(k1 * N2 + m) * TP + pp];
+        rdft<N2, INV>(w);
+        if (p < planes) {
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) {
+                const int row = k1 + N1 * k2;
+                if constexpr (OUT == kStrided) {
+                    out[P * (kx + static_cast<size_t>(out_kc) * row) + p] = w[k2];
+                } else {
+                    const int code = ocode[k2];
+                    if (code >= 0) {
+                        cx<R> x = w[k2];
+                        if (code & 1) x.y = -x.y;
+                        out[P * (code >> 1) + p] = x;
+                    }
+                }
+            }
+        }
+        __syncthreads();  // buf is rewritten by the next item
+#pragma unroll
+        for (int m = 0; m < N1; ++m) v[m] = vn[m];
+        item = next;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Retained gather / embed on a full WVM spectrum (P_h, E_h).
 // ---------------------------------------------------------------------------
 template <typename R>
@@ -353,47 +744,98 @@ std::vector<cx<R>> twiddles(int n) {
 
 void check_launch(const char* what) { check_hip(hipGetLastError(), what); }
 
+
+// Workgroups of a launch: `requested` (> 0) or one per work item, capped at
+// the item count.  Fewer workgroups than items makes them persistent.
+int launch_grid(int items, int requested) {
+    return requested > 0 && requested < items ? requested : xcd_grid(items);
+}
+
 template <typename R, int NX>
-void launch_row(bool inverse, const void* in, void* out, const void* tw, int ny, int planes, int kc,
+void launch_row(bool inverse, const void* in, void* out, const void* tw, int ny, int planes, int kc, int req,
                 hipStream_t stream) {
-    constexpr int TP = row_tile<R>();
-    const dim3 grid(ny, (planes + TP - 1) / TP);
-    if (inverse)
-        hipLaunchKernelGGL((row_c2r_kernel<R, NX>), grid, dim3(kThreads), 0, stream, static_cast<const cx<R>*>(in),
-                           static_cast<R*>(out), static_cast<const cx<R>*>(tw), ny, planes, kc);
-    else
-        hipLaunchKernelGGL((row_r2c_kernel<R, NX>), grid, dim3(kThreads), 0, stream, static_cast<const R*>(in),
-                           static_cast<cx<R>*>(out), static_cast<const cx<R>*>(tw), ny, planes, kc);
+    if constexpr (NX >= 64) {
+        using G = Geo2<R, NX>;
+        constexpr int TP = 2 * G::PAIRS;
+        const int items = ny * ((planes + TP - 1) / TP);
+        const dim3 grid(launch_grid(items, req));
+        if (inverse)
+            hipLaunchKernelGGL((row2_c2r_kernel<R, NX>), grid, dim3(G::THREADS), 0, stream,
+                               static_cast<const cx<R>*>(in), static_cast<R*>(out), static_cast<const cx<R>*>(tw), ny,
+                               planes, kc, items);
+        else
+            hipLaunchKernelGGL((row2_r2c_kernel<R, NX>), grid, dim3(G::THREADS), 0, stream,
+                               static_cast<const R*>(in), static_cast<cx<R>*>(out), static_cast<const cx<R>*>(tw), ny,
+                               planes, kc, items);
+    } else {
+        constexpr int TP = row_tile<R>();
+        const dim3 grid(ny, (planes + TP - 1) / TP);
+        if (inverse)
+            hipLaunchKernelGGL((row_c2r_kernel<R, NX>), grid, dim3(kThreads), 0, stream,
+                               static_cast<const cx<R>*>(in), static_cast<R*>(out), static_cast<const cx<R>*>(tw), ny,
+                               planes, kc);
+        else
+            hipLaunchKernelGGL((row_r2c_kernel<R, NX>), grid, dim3(kThreads), 0, stream, static_cast<const R*>(in),
+                               static_cast<cx<R>*>(out), static_cast<const cx<R>*>(tw), ny, planes, kc);
+    }
     check_launch(inverse ? "row c2r launch" : "row r2c launch");
 }
 
 enum class ColVariant { FwdStrided, FwdToRetained, InvStrided, InvFromRetained };
 
+// Stage indices of the pruned plan (skb_pruned_plan_set_grid).
+constexpr int kStageForwardRows = 0, kStageForwardColumns = 1, kStageInverseColumns = 2, kStageInverseRows = 3;
+
 template <typename R, int NY>
 void launch_col(ColVariant v, const void* in, void* out, const void* tw, int planes, int kx_count, int in_kc,
-                int out_kc, const int* col_start, const ColEntry* entries, hipStream_t stream) {
-    constexpr int TP = col_tile<R, NY>();
-    const dim3 grid(kx_count, (planes + TP - 1) / TP);
+                int out_kc, const int* col_start, const ColEntry* entries, const int* map, int req,
+                hipStream_t stream) {
     const auto* i = static_cast<const cx<R>*>(in);
     auto* o = static_cast<cx<R>*>(out);
     const auto* t = static_cast<const cx<R>*>(tw);
-    switch (v) {
-        case ColVariant::FwdStrided:
-            hipLaunchKernelGGL((col_kernel<R, NY, false, kStrided, kStrided>), grid, dim3(kThreads), 0, stream, i, o, t,
-                               planes, in_kc, out_kc, col_start, entries);
-            break;
-        case ColVariant::FwdToRetained:
-            hipLaunchKernelGGL((col_kernel<R, NY, false, kStrided, kRetained>), grid, dim3(kThreads), 0, stream, i, o,
-                               t, planes, in_kc, out_kc, col_start, entries);
-            break;
-        case ColVariant::InvStrided:
-            hipLaunchKernelGGL((col_kernel<R, NY, true, kStrided, kStrided>), grid, dim3(kThreads), 0, stream, i, o, t,
-                               planes, in_kc, out_kc, col_start, entries);
-            break;
-        case ColVariant::InvFromRetained:
-            hipLaunchKernelGGL((col_kernel<R, NY, true, kRetained, kStrided>), grid, dim3(kThreads), 0, stream, i, o,
-                               t, planes, in_kc, out_kc, col_start, entries);
-            break;
+    if constexpr (NY >= 64) {
+        using G = Geo2<R, NY>;
+        const int items = kx_count * ((planes + G::TP - 1) / G::TP);
+        const dim3 grid(launch_grid(items, req)), block(G::THREADS);
+        switch (v) {
+            case ColVariant::FwdStrided:
+                hipLaunchKernelGGL((col2_kernel<R, NY, false, kStrided, kStrided>), grid, block, 0, stream, i, o, t,
+                                   planes, kx_count, items, in_kc, out_kc, map);
+                break;
+            case ColVariant::FwdToRetained:
+                hipLaunchKernelGGL((col2_kernel<R, NY, false, kStrided, kRetained>), grid, block, 0, stream, i, o, t,
+                                   planes, kx_count, items, in_kc, out_kc, map);
+                break;
+            case ColVariant::InvStrided:
+                hipLaunchKernelGGL((col2_kernel<R, NY, true, kStrided, kStrided>), grid, block, 0, stream, i, o, t,
+                                   planes, kx_count, items, in_kc, out_kc, map);
+                break;
+            case ColVariant::InvFromRetained:
+                hipLaunchKernelGGL((col2_kernel<R, NY, true, kRetained, kStrided>), grid, block, 0, stream, i, o, t,
+                                   planes, kx_count, items, in_kc, out_kc, map);
+                break;
+        }
+    } else {
+        constexpr int TP = col_tile<R, NY>();
+        const dim3 grid(kx_count, (planes + TP - 1) / TP);
+        switch (v) {
+            case ColVariant::FwdStrided:
+                hipLaunchKernelGGL((col_kernel<R, NY, false, kStrided, kStrided>), grid, dim3(kThreads), 0, stream, i,
+                                   o, t, planes, in_kc, out_kc, col_start, entries);
+                break;
+            case ColVariant::FwdToRetained:
+                hipLaunchKernelGGL((col_kernel<R, NY, false, kStrided, kRetained>), grid, dim3(kThreads), 0, stream,
+                                   i, o, t, planes, in_kc, out_kc, col_start, entries);
+                break;
+            case ColVariant::InvStrided:
+                hipLaunchKernelGGL((col_kernel<R, NY, true, kStrided, kStrided>), grid, dim3(kThreads), 0, stream, i,
+                                   o, t, planes, in_kc, out_kc, col_start, entries);
+                break;
+            case ColVariant::InvFromRetained:
+                hipLaunchKernelGGL((col_kernel<R, NY, true, kRetained, kStrided>), grid, dim3(kThreads), 0, stream,
+                                   i, o, t, planes, in_kc, out_kc, col_start, entries);
+                break;
+        }
     }
     check_launch("column launch");
 }
@@ -402,12 +844,12 @@ void launch_col(ColVariant v, const void* in, void* out, const void* tw, int pla
     MACRO(4) MACRO(8) MACRO(16) MACRO(32) MACRO(64) MACRO(128) MACRO(256) MACRO(512) MACRO(1024)
 
 template <typename R>
-void row_pass(bool inverse, int nx, const void* in, void* out, const void* tw, int ny, int planes, int kc,
+void row_pass(bool inverse, int nx, const void* in, void* out, const void* tw, int ny, int planes, int kc, int req,
               hipStream_t s) {
     switch (nx) {
-#define SKB_ROW_CASE(N)                                                \
-    case N:                                                            \
-        launch_row<R, N>(inverse, in, out, tw, ny, planes, kc, s);     \
+#define SKB_ROW_CASE(N)                                                     \
+    case N:                                                                 \
+        launch_row<R, N>(inverse, in, out, tw, ny, planes, kc, req, s);     \
         return;
         SKB_POW2_CASES(SKB_ROW_CASE)
 #undef SKB_ROW_CASE
@@ -417,11 +859,11 @@ void row_pass(bool inverse, int nx, const void* in, void* out, const void* tw, i
 
 template <typename R>
 void col_pass(ColVariant v, int ny, const void* in, void* out, const void* tw, int planes, int kx_count, int in_kc,
-              int out_kc, const int* col_start, const ColEntry* entries, hipStream_t s) {
+              int out_kc, const int* col_start, const ColEntry* entries, const int* map, int req, hipStream_t s) {
     switch (ny) {
-#define SKB_COL_CASE(N)                                                                               \
-    case N:                                                                                           \
-        launch_col<R, N>(v, in, out, tw, planes, kx_count, in_kc, out_kc, col_start, entries, s);     \
+#define SKB_COL_CASE(N)                                                                                       \
+    case N:                                                                                                   \
+        launch_col<R, N>(v, in, out, tw, planes, kx_count, in_kc, out_kc, col_start, entries, map, req, s);   \
         return;
         SKB_POW2_CASES(SKB_COL_CASE)
 #undef SKB_COL_CASE
@@ -465,11 +907,13 @@ struct ColumnLists {
     uint64_t kc = 0;  // active k_x band: max stored_kx + 1
     std::vector<int> fwd_start, inv_start;
     std::vector<ColEntry> fwd, inv;
+    // dense maps [kx * Ny + ky] -> 2*mode + conj, or -1
+    std::vector<int> fmap, imap;
 };
 
 ColumnLists build_column_lists(const skb_workload& w, const skb_retained_mode* modes, uint64_t n) {
     if (!modes || n == 0) fail(SKB_INVALID_ARGUMENT, "The pruned provider requires retained modes.");
-    if (n > (1u << 30)) fail(SKB_UNSUPPORTED, "too many retained modes");
+    if (n > (1u << 29)) fail(SKB_UNSUPPORTED, "too many retained modes");
     const uint64_t nxh = w.nx / 2 + 1;
     ColumnLists c;
     for (uint64_t m = 0; m < n; ++m) {
@@ -498,6 +942,12 @@ ColumnLists build_column_lists(const skb_workload& w, const skb_retained_mode* m
             claim(0, row);
             i[0].push_back(ColEntry{static_cast<int>(row), static_cast<int>(m), md.conjugates_stored_value ? 0 : 1, 0});
         }
+    }
+    c.fmap.assign(c.kc * w.ny, -1);
+    c.imap.assign(c.kc * w.ny, -1);
+    for (uint64_t kx = 0; kx < c.kc; ++kx) {
+        for (const auto& e : f[kx]) c.fmap[kx * w.ny + e.row] = 2 * e.mode + e.conj;
+        for (const auto& e : i[kx]) c.imap[kx * w.ny + e.row] = 2 * e.mode + e.conj;
     }
     c.fwd_start.push_back(0);
     c.inv_start.push_back(0);
@@ -529,11 +979,12 @@ struct skb_fft_plan {
 
 struct skb_pruned_plan {
     skb_workload w{};
+    int grid[4] = {0, 0, 0, 0};  // per-stage workgroup count (0 = one per item)
     int precision = SKB_F64;
     uint64_t modes = 0;
     skb::Twiddles tw;
     skb::ColumnLists lists;
-    skb::DeviceBuffer fwd_start, inv_start, fwd, inv;
+    skb::DeviceBuffer fwd_start, inv_start, fwd, inv, fmap, imap;
     skb::DeviceBuffer scratch;  // T: kc * Ny * P complex
 };
 
@@ -574,9 +1025,9 @@ void fft_forward(skb_fft_plan* p, const void* real, void* wvm, hipStream_t s) {
     const auto& w = p->w;
     const int planes = static_cast<int>(w.nz * w.fields);
     const int nxh = static_cast<int>(w.nx / 2 + 1);
-    row_pass<R>(false, static_cast<int>(w.nx), real, wvm, p->tw.x.ptr, static_cast<int>(w.ny), planes, nxh, s);
+    row_pass<R>(false, static_cast<int>(w.nx), real, wvm, p->tw.x.ptr, static_cast<int>(w.ny), planes, nxh, 0, s);
     col_pass<R>(ColVariant::FwdStrided, static_cast<int>(w.ny), wvm, wvm, p->tw.y.ptr, planes, nxh, nxh, nxh, nullptr,
-                nullptr, s);
+                nullptr, nullptr, 0, s);
 }
 
 template <typename R>
@@ -585,8 +1036,8 @@ void fft_inverse(skb_fft_plan* p, void* wvm, void* real, hipStream_t s) {
     const int planes = static_cast<int>(w.nz * w.fields);
     const int nxh = static_cast<int>(w.nx / 2 + 1);
     col_pass<R>(ColVariant::InvStrided, static_cast<int>(w.ny), wvm, wvm, p->tw.y.ptr, planes, nxh, nxh, nxh, nullptr,
-                nullptr, s);
-    row_pass<R>(true, static_cast<int>(w.nx), wvm, real, p->tw.x.ptr, static_cast<int>(w.ny), planes, nxh, s);
+                nullptr, nullptr, 0, s);
+    row_pass<R>(true, static_cast<int>(w.nx), wvm, real, p->tw.x.ptr, static_cast<int>(w.ny), planes, nxh, 0, s);
 }
 
 template <typename R>
@@ -594,9 +1045,10 @@ void pruned_forward(skb_pruned_plan* p, const void* real, void* retained, hipStr
     const auto& w = p->w;
     const int planes = static_cast<int>(w.nz * w.fields);
     const int kc = static_cast<int>(p->lists.kc);
-    row_pass<R>(false, static_cast<int>(w.nx), real, p->scratch.ptr, p->tw.x.ptr, static_cast<int>(w.ny), planes, kc, s);
+    row_pass<R>(false, static_cast<int>(w.nx), real, p->scratch.ptr, p->tw.x.ptr, static_cast<int>(w.ny), planes, kc,
+                p->grid[kStageForwardRows], s);
     col_pass<R>(ColVariant::FwdToRetained, static_cast<int>(w.ny), p->scratch.ptr, retained, p->tw.y.ptr, planes, kc, kc,
-                kc, p->fwd_start.as<int>(), p->fwd.as<ColEntry>(), s);
+                kc, p->fwd_start.as<int>(), p->fwd.as<ColEntry>(), p->fmap.as<int>(), p->grid[kStageForwardColumns], s);
 }
 
 template <typename R>
@@ -605,8 +1057,9 @@ void pruned_inverse(skb_pruned_plan* p, const void* retained, void* real, hipStr
     const int planes = static_cast<int>(w.nz * w.fields);
     const int kc = static_cast<int>(p->lists.kc);
     col_pass<R>(ColVariant::InvFromRetained, static_cast<int>(w.ny), retained, p->scratch.ptr, p->tw.y.ptr, planes, kc,
-                kc, kc, p->inv_start.as<int>(), p->inv.as<ColEntry>(), s);
-    row_pass<R>(true, static_cast<int>(w.nx), p->scratch.ptr, real, p->tw.x.ptr, static_cast<int>(w.ny), planes, kc, s);
+                kc, kc, p->inv_start.as<int>(), p->inv.as<ColEntry>(), p->imap.as<int>(), p->grid[kStageInverseColumns], s);
+    row_pass<R>(true, static_cast<int>(w.nx), p->scratch.ptr, real, p->tw.x.ptr, static_cast<int>(w.ny), planes, kc,
+                p->grid[kStageInverseRows], s);
 }
 
 void require_ptrs(const void* a, const void* b) {
@@ -726,6 +1179,8 @@ int skb_pruned_plan_create(const skb_workload* workload, const skb_retained_mode
         p->inv_start.upload(p->lists.inv_start.data(), p->lists.inv_start.size() * sizeof(int));
         p->fwd.upload(p->lists.fwd.data(), p->lists.fwd.size() * sizeof(ColEntry));
         p->inv.upload(p->lists.inv.data(), p->lists.inv.size() * sizeof(ColEntry));
+        p->fmap.upload(p->lists.fmap.data(), p->lists.fmap.size() * sizeof(int));
+        p->imap.upload(p->lists.imap.data(), p->lists.imap.size() * sizeof(int));
         p->scratch.allocate(p->lists.kc * workload->ny * workload->nz * workload->fields * complex_bytes(precision));
         *plan = p.release();
     });
@@ -760,10 +1215,10 @@ int skb_pruned_forward_rows(skb_pruned_plan* plan, const void* real, void* strea
         const int planes = static_cast<int>(w.nz * w.fields), kc = static_cast<int>(plan->lists.kc);
         if (plan->precision == SKB_F64)
             row_pass<double>(false, static_cast<int>(w.nx), real, plan->scratch.ptr, plan->tw.x.ptr,
-                             static_cast<int>(w.ny), planes, kc, as_stream(stream));
+                             static_cast<int>(w.ny), planes, kc, plan->grid[kStageForwardRows], as_stream(stream));
         else
             row_pass<float>(false, static_cast<int>(w.nx), real, plan->scratch.ptr, plan->tw.x.ptr,
-                            static_cast<int>(w.ny), planes, kc, as_stream(stream));
+                            static_cast<int>(w.ny), planes, kc, plan->grid[kStageForwardRows], as_stream(stream));
     });
 }
 
@@ -775,11 +1230,11 @@ int skb_pruned_forward_columns(skb_pruned_plan* plan, void* retained, void* stre
         if (plan->precision == SKB_F64)
             col_pass<double>(ColVariant::FwdToRetained, static_cast<int>(w.ny), plan->scratch.ptr, retained,
                              plan->tw.y.ptr, planes, kc, kc, kc, plan->fwd_start.as<int>(), plan->fwd.as<ColEntry>(),
-                             as_stream(stream));
+                             plan->fmap.as<int>(), plan->grid[kStageForwardColumns], as_stream(stream));
         else
             col_pass<float>(ColVariant::FwdToRetained, static_cast<int>(w.ny), plan->scratch.ptr, retained,
                             plan->tw.y.ptr, planes, kc, kc, kc, plan->fwd_start.as<int>(), plan->fwd.as<ColEntry>(),
-                            as_stream(stream));
+                            plan->fmap.as<int>(), plan->grid[kStageForwardColumns], as_stream(stream));
     });
 }
 
@@ -791,11 +1246,11 @@ int skb_pruned_inverse_columns(skb_pruned_plan* plan, const void* retained, void
         if (plan->precision == SKB_F64)
             col_pass<double>(ColVariant::InvFromRetained, static_cast<int>(w.ny), retained, plan->scratch.ptr,
                              plan->tw.y.ptr, planes, kc, kc, kc, plan->inv_start.as<int>(), plan->inv.as<ColEntry>(),
-                             as_stream(stream));
+                             plan->imap.as<int>(), plan->grid[kStageInverseColumns], as_stream(stream));
         else
             col_pass<float>(ColVariant::InvFromRetained, static_cast<int>(w.ny), retained, plan->scratch.ptr,
                             plan->tw.y.ptr, planes, kc, kc, kc, plan->inv_start.as<int>(), plan->inv.as<ColEntry>(),
-                            as_stream(stream));
+                            plan->imap.as<int>(), plan->grid[kStageInverseColumns], as_stream(stream));
     });
 }
 
@@ -806,11 +1261,37 @@ int skb_pruned_inverse_rows(skb_pruned_plan* plan, void* real, void* stream) {
         const int planes = static_cast<int>(w.nz * w.fields), kc = static_cast<int>(plan->lists.kc);
         if (plan->precision == SKB_F64)
             row_pass<double>(true, static_cast<int>(w.nx), plan->scratch.ptr, real, plan->tw.x.ptr,
-                             static_cast<int>(w.ny), planes, kc, as_stream(stream));
+                             static_cast<int>(w.ny), planes, kc, plan->grid[kStageInverseRows], as_stream(stream));
         else
             row_pass<float>(true, static_cast<int>(w.nx), plan->scratch.ptr, real, plan->tw.x.ptr,
-                            static_cast<int>(w.ny), planes, kc, as_stream(stream));
+                            static_cast<int>(w.ny), planes, kc, plan->grid[kStageInverseRows], as_stream(stream));
     });
+}
+
+int skb_pruned_plan_set_grid(skb_pruned_plan* plan, int stage, int workgroups) {
+    return guarded([&] {
+        if (!plan || stage < 0 || stage > 3 || workgroups < 0) fail(SKB_INVALID_ARGUMENT, "bad stage or workgroup count");
+        plan->grid[stage] = workgroups;
+    });
+}
+
+uint64_t skb_pruned_plan_items(const skb_pruned_plan* plan, int stage) {
+    if (!plan || stage < 0 || stage > 3) return 0;
+    const auto& w = plan->w;
+    const uint64_t planes = w.nz * w.fields;
+    const bool f32 = plan->precision == SKB_F32;
+    const bool rows = stage == kStageForwardRows || stage == kStageInverseRows;
+    const uint64_t n = rows ? w.nx : w.ny;
+    uint64_t tile;
+    if (n >= 64) {
+        const uint64_t n1 = n == 64 ? 8 : (n <= 256 ? 16 : 32);
+        const uint64_t lanes = f32 ? 256 : 128;
+        tile = rows ? 2 * (lanes / n1) : lanes / n1;
+    } else {
+        tile = rows ? (f32 ? 16 : 8) : (f32 ? 16 : 8);
+    }
+    const uint64_t tiles = (planes + tile - 1) / tile;
+    return rows ? w.ny * tiles : plan->lists.kc * tiles;
 }
 
 uint64_t skb_pruned_active_kx(const skb_pruned_plan* plan) { return plan ? plan->lists.kc : 0; }

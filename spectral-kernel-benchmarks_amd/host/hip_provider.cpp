@@ -1,0 +1,535 @@
+// C++20 host mirror of the skbench providers over the C ABI of libskb_hip.so.
+#include "hip_provider.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <complex>
+#include <cstring>
+#include <iomanip>
+#include <new>
+#include <sstream>
+#include <stdexcept>
+
+namespace skbhip {
+
+static_assert(sizeof(Complex) == 2 * sizeof(double), "Complex must be two doubles");
+static_assert(sizeof(Complex) == sizeof(std::complex<double>), "Complex must match std::complex<double>");
+static_assert(sizeof(skb_mode_group) == sizeof(VerticalModeGroup), "group records must be layout-identical");
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double seconds_since(Clock::time_point t) { return std::chrono::duration<double>(Clock::now() - t).count(); }
+
+std::string last_error() {
+    char buf[1024];
+    skb_last_error(buf, sizeof buf);
+    return buf;
+}
+
+// Status -> the reference's exception classes (src/fftw_provider.cpp:468-487).
+void check(int status) {
+    switch (status) {
+        case SKB_OK: return;
+        case SKB_INVALID_ARGUMENT: throw std::invalid_argument(last_error());
+        case SKB_LOGIC_ERROR: throw std::logic_error(last_error());
+        case SKB_BAD_ALLOC: throw std::bad_alloc();
+        default: throw std::runtime_error(last_error());
+    }
+}
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+skb_workload to_c(const Workload& w) {
+    skb_workload c{};
+    c.nx = w.nx;
+    c.ny = w.ny;
+    c.nz = w.nz;
+    c.fields = w.fields;
+    c.lx = w.lx;
+    c.ly = w.ly;
+    c.antialias = w.antialias ? 1 : 0;
+    return c;
+}
+
+std::vector<skb_retained_mode> to_c(const std::vector<RetainedMode>& modes) {
+    std::vector<skb_retained_mode> out(modes.size());
+    for (std::size_t i = 0; i < modes.size(); ++i) {
+        out[i].k = modes[i].k;
+        out[i].l = modes[i].l;
+        out[i].stored_kx = modes[i].storedKx;
+        out[i].stored_ky = modes[i].storedKy;
+        out[i].conjugates_stored_value = modes[i].conjugatesStoredValue ? 1 : 0;
+        out[i].radial = modes[i].radialMode;
+    }
+    return out;
+}
+
+int c_precision(Precision p) { return p == Precision::float64 ? SKB_F64 : SKB_F32; }
+
+std::string fnv(std::uint64_t h) {
+    std::ostringstream s;
+    s << "fnv1a64:" << std::hex << std::setfill('0') << std::setw(16) << h;
+    return s.str();
+}
+
+// Device buffer plus host staging for the adapter scope.  FP64 host data is
+// converted to the plan precision on the host before upload.
+struct Staged {
+    void* dev = nullptr;
+    std::size_t elements = 0;  // real scalars
+    bool fp32 = false;
+    std::vector<float> host32;
+
+    Staged() = default;
+    Staged(const Staged&) = delete;
+    Staged& operator=(const Staged&) = delete;
+    ~Staged() {
+        if (dev) (void)hipFree(dev);
+    }
+    void allocate(std::size_t scalars, bool single) {
+        elements = scalars;
+        fp32 = single;
+        if (hipMalloc(&dev, scalars * (single ? 4 : 8)) != hipSuccess) {
+            dev = nullptr;
+            (void)hipGetLastError();
+            throw std::bad_alloc();
+        }
+        if (single) host32.resize(scalars);
+    }
+    void upload(const double* src, hipStream_t s) {
+        if (fp32) {
+            for (std::size_t i = 0; i < elements; ++i) host32[i] = static_cast<float>(src[i]);
+            hip_check(hipMemcpyAsync(dev, host32.data(), elements * 4, hipMemcpyHostToDevice, s), "upload");
+            hip_check(hipStreamSynchronize(s), "upload sync");
+        } else {
+            hip_check(hipMemcpyAsync(dev, src, elements * 8, hipMemcpyHostToDevice, s), "upload");
+        }
+    }
+    void download(double* dst, hipStream_t s) {
+        if (fp32) {
+            hip_check(hipMemcpyAsync(host32.data(), dev, elements * 4, hipMemcpyDeviceToHost, s), "download");
+            hip_check(hipStreamSynchronize(s), "download sync");
+            for (std::size_t i = 0; i < elements; ++i) dst[i] = host32[i];
+        } else {
+            hip_check(hipMemcpyAsync(dst, dev, elements * 8, hipMemcpyDeviceToHost, s), "download");
+            hip_check(hipStreamSynchronize(s), "download sync");
+        }
+    }
+};
+
+struct Stream {
+    hipStream_t s = nullptr;
+    Stream() { hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate"); }
+    ~Stream() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+    Stream(const Stream&) = delete;
+    Stream& operator=(const Stream&) = delete;
+};
+
+}  // namespace
+
+// ---- model ----------------------------------------------------------------------
+std::vector<RetainedMode> retainedHorizontalModes(const Workload& workload) {
+    const auto w = to_c(workload);
+    std::uint64_t n = 0;
+    check(skb_retained_modes(&w, nullptr, 0, &n));
+    std::vector<skb_retained_mode> c(n);
+    check(skb_retained_modes(&w, c.data(), n, &n));
+    std::vector<RetainedMode> out(n);
+    for (std::size_t i = 0; i < n; ++i)
+        out[i] = {c[i].k, c[i].l, c[i].stored_kx, c[i].stored_ky, c[i].conjugates_stored_value != 0, c[i].radial};
+    return out;
+}
+
+std::vector<VerticalModeGroup> squaredWavenumberGroups(const std::vector<RetainedMode>& modes) {
+    const auto c = to_c(modes);
+    std::vector<VerticalModeGroup> out(modes.size());
+    std::uint64_t n = 0;
+    check(skb_k2_groups(c.data(), c.size(), reinterpret_cast<skb_mode_group*>(out.data()), out.size(), &n));
+    out.resize(n);
+    return out;
+}
+
+std::string modeOrderHash(const std::vector<RetainedMode>& modes) {
+    const auto c = to_c(modes);
+    return fnv(skb_mode_order_hash(c.data(), c.size()));
+}
+
+std::string verticalModeGroupHash(const std::vector<VerticalModeGroup>& groups) {
+    return fnv(skb_group_order_hash(reinterpret_cast<const skb_mode_group*>(groups.data()), groups.size()));
+}
+
+std::string wvmSpectrumOrderHash(const Workload& workload) {
+    const auto w = to_c(workload);
+    return fnv(skb_wvm_order_hash(&w));
+}
+
+GroupedVerticalOperators squaredWavenumberVerticalFixture(const Workload& workload,
+                                                          const std::vector<RetainedMode>& modes) {
+    if (workload.lx != workload.ly)
+        throw std::invalid_argument("The exact integer K-squared fixture currently requires a square horizontal domain.");
+    GroupedVerticalOperators ops;
+    ops.id = "synthetic-k2-grouped-orthonormal-dct2-pair-rotation-v1";
+    ops.nz = workload.nz;
+    ops.nj = workload.retainedVerticalModes();
+    ops.groups = squaredWavenumberGroups(modes);
+    ops.forward.resize(ops.groups.size() * ops.nj * ops.nz);
+    ops.inverse.resize(ops.forward.size());
+    check(skb_k2_vertical_family(ops.nz, ops.nj, reinterpret_cast<const skb_mode_group*>(ops.groups.data()),
+                                 ops.groups.size(), ops.forward.data(), ops.inverse.data()));
+    return ops;
+}
+
+GroupedVerticalOperators binnedQrVerticalFixture(const Workload& workload, const std::vector<RetainedMode>& modes,
+                                                 std::size_t bins, std::uint64_t seed) {
+    GroupedVerticalOperators ops;
+    ops.id = "k2-binned-qr-gaussian-v1";
+    ops.nz = workload.nz;
+    ops.nj = workload.nz;
+    const auto c = to_c(modes);
+    ops.groups.resize(modes.size());
+    std::uint64_t n = 0;
+    check(skb_k2_binned_groups(c.data(), c.size(), bins, reinterpret_cast<skb_mode_group*>(ops.groups.data()),
+                               ops.groups.size(), &n));
+    ops.groups.resize(n);
+    ops.forward.resize(n * ops.nz * ops.nz);
+    ops.inverse.resize(ops.forward.size());
+    check(skb_qr_gaussian_family(ops.nz, n, seed, ops.forward.data(), ops.inverse.data()));
+    return ops;
+}
+
+std::vector<double> makeFixture(const Workload& workload, FixtureKind fixture, std::uint64_t seed) {
+    const auto w = to_c(workload);
+    std::vector<double> out(workload.realElements());
+    check(skb_make_fixture(&w, static_cast<int>(fixture), seed, out.data()));
+    return out;
+}
+
+// ---- HipProvider -------------------------------------------------------------------
+struct HipProvider::Impl {
+    Workload w;
+    Precision precision;
+    skb_fft_plan* plan = nullptr;
+    bool supported = true;
+    std::string capability = "hip provider: gfx950 Stockham r2c/c2r in WVM order";
+    double allocation_seconds = 0, planning_seconds = 0;
+    Stream stream;
+    Staged real, spectrum, retained;
+    std::vector<RetainedMode> cached_modes;
+    std::size_t retained_modes = 0;
+
+    Impl(const Workload& wl, Precision p) : w(wl), precision(p) {
+        const auto c = to_c(wl);
+        const auto t0 = Clock::now();
+        const int st = skb_fft_plan_create(&c, c_precision(p), &plan);
+        planning_seconds = seconds_since(t0);
+        if (st == SKB_UNSUPPORTED) {
+            supported = false;
+            capability = "unsupported: " + last_error();
+            return;
+        }
+        check(st);
+        const auto t1 = Clock::now();
+        const bool single = p == Precision::float32;
+        real.allocate(w.realElements(), single);
+        spectrum.allocate(2 * w.spectrumElements(), single);
+        allocation_seconds = seconds_since(t1);
+    }
+    ~Impl() {
+        if (plan) skb_fft_plan_destroy(plan);
+    }
+    void require() const {
+        if (!supported) throw std::logic_error(capability);
+    }
+    void ensure_retained(std::size_t modes) {
+        if (retained_modes == modes) return;
+        retained.~Staged();
+        new (&retained) Staged();
+        retained.allocate(2 * modes * w.planes(), precision == Precision::float32);
+        retained_modes = modes;
+    }
+};
+
+HipProvider::HipProvider(const Workload& workload, Precision precision)
+    : impl_(std::make_unique<Impl>(workload, precision)) {}
+HipProvider::~HipProvider() = default;
+HipProvider::HipProvider(HipProvider&&) noexcept = default;
+HipProvider& HipProvider::operator=(HipProvider&&) noexcept = default;
+
+void HipProvider::forward(const double* input, Complex* wvmSpectrum) {
+    auto& m = *impl_;
+    m.require();
+    m.real.upload(input, m.stream.s);
+    check(skb_fft_forward(m.plan, m.real.dev, m.spectrum.dev, m.stream.s));
+    m.spectrum.download(reinterpret_cast<double*>(wvmSpectrum), m.stream.s);
+}
+
+void HipProvider::inverse(Complex* wvmSpectrum, double* output) {
+    auto& m = *impl_;
+    m.require();
+    m.spectrum.upload(reinterpret_cast<const double*>(wvmSpectrum), m.stream.s);
+    check(skb_fft_inverse(m.plan, m.spectrum.dev, m.real.dev, m.stream.s));
+    m.real.download(output, m.stream.s);
+}
+
+void HipProvider::gatherRetainedOuter(const std::vector<RetainedMode>& modes, const Complex* wvmSpectrum,
+                                      Complex* retainedSpectrum) {
+    auto& m = *impl_;
+    m.require();
+    const auto c = to_c(modes);
+    m.ensure_retained(modes.size());
+    m.spectrum.upload(reinterpret_cast<const double*>(wvmSpectrum), m.stream.s);
+    check(skb_fft_gather_retained(m.plan, c.data(), c.size(), m.spectrum.dev, m.retained.dev, m.stream.s));
+    m.retained.download(reinterpret_cast<double*>(retainedSpectrum), m.stream.s);
+}
+
+void HipProvider::embedRetainedOuter(const std::vector<RetainedMode>& modes, const Complex* retainedSpectrum,
+                                     Complex* wvmSpectrum) {
+    auto& m = *impl_;
+    m.require();
+    const auto c = to_c(modes);
+    m.ensure_retained(modes.size());
+    m.retained.upload(reinterpret_cast<const double*>(retainedSpectrum), m.stream.s);
+    check(skb_fft_embed_retained(m.plan, c.data(), c.size(), m.retained.dev, m.spectrum.dev, m.stream.s));
+    m.spectrum.download(reinterpret_cast<double*>(wvmSpectrum), m.stream.s);
+}
+
+void HipProvider::forwardDevice(const void* real, void* wvm, void* stream) {
+    impl_->require();
+    check(skb_fft_forward(impl_->plan, real, wvm, stream));
+}
+
+void HipProvider::inverseDevice(void* wvm, void* real, void* stream) {
+    impl_->require();
+    check(skb_fft_inverse(impl_->plan, wvm, real, stream));
+}
+
+void HipProvider::executeSchedulerNoop() { hip_check(hipStreamSynchronize(impl_->stream.s), "sync"); }
+bool HipProvider::supported() const noexcept { return impl_->supported; }
+std::string HipProvider::capability() const { return impl_->capability; }
+double HipProvider::allocationSeconds() const noexcept { return impl_->allocation_seconds; }
+double HipProvider::planningSeconds() const noexcept { return impl_->planning_seconds; }
+std::size_t HipProvider::planningBytes() const noexcept { return (impl_->w.nx + impl_->w.ny) * 16; }
+std::size_t HipProvider::minimumAlignmentBytes() const noexcept { return 256; }
+std::string HipProvider::libraryIdentity() const { return "libskb_hip.so"; }
+std::string HipProvider::version() const { return skb_version(); }
+
+// ---- HipPrunedProvider ----------------------------------------------------------------
+struct HipPrunedProvider::Impl {
+    Workload w;
+    std::size_t modes;
+    skb_pruned_plan* plan = nullptr;
+    Stream stream;
+    Staged real, retained;
+
+    Impl(const Workload& wl, const std::vector<RetainedMode>& m, Precision p) : w(wl), modes(m.size()) {
+        const auto c = to_c(m);
+        const auto cw = to_c(wl);
+        check(skb_pruned_plan_create(&cw, c.data(), c.size(), c_precision(p), &plan));
+        const bool single = p == Precision::float32;
+        real.allocate(w.realElements(), single);
+        retained.allocate(2 * modes * w.planes(), single);
+    }
+    ~Impl() {
+        if (plan) skb_pruned_plan_destroy(plan);
+    }
+};
+
+HipPrunedProvider::HipPrunedProvider(const Workload& workload, const std::vector<RetainedMode>& modes,
+                                     Precision precision)
+    : impl_(std::make_unique<Impl>(workload, modes, precision)) {}
+HipPrunedProvider::~HipPrunedProvider() = default;
+HipPrunedProvider::HipPrunedProvider(HipPrunedProvider&&) noexcept = default;
+HipPrunedProvider& HipPrunedProvider::operator=(HipPrunedProvider&&) noexcept = default;
+
+void HipPrunedProvider::forward(const double* input, Complex* retainedSpectrum) {
+    executeForwardRows(input);
+    executeForwardColumns();
+    gatherForward(retainedSpectrum);
+}
+
+void HipPrunedProvider::inverse(const Complex* retainedSpectrum, double* output) {
+    embedInverse(retainedSpectrum);
+    executeInverseColumns();
+    executeInverseRows(output);
+}
+
+void HipPrunedProvider::executeForwardRows(const double* input) {
+    auto& m = *impl_;
+    m.real.upload(input, m.stream.s);
+    check(skb_pruned_forward_rows(m.plan, m.real.dev, m.stream.s));
+}
+
+void HipPrunedProvider::executeForwardColumns() {
+    auto& m = *impl_;
+    check(skb_pruned_forward_columns(m.plan, m.retained.dev, m.stream.s));
+}
+
+void HipPrunedProvider::gatherForward(Complex* retainedSpectrum) {
+    auto& m = *impl_;
+    m.retained.download(reinterpret_cast<double*>(retainedSpectrum), m.stream.s);
+}
+
+void HipPrunedProvider::embedInverse(const Complex* retainedSpectrum) {
+    auto& m = *impl_;
+    m.retained.upload(reinterpret_cast<const double*>(retainedSpectrum), m.stream.s);
+}
+
+void HipPrunedProvider::executeInverseColumns() {
+    auto& m = *impl_;
+    check(skb_pruned_inverse_columns(m.plan, m.retained.dev, m.stream.s));
+}
+
+void HipPrunedProvider::executeInverseRows(double* output) {
+    auto& m = *impl_;
+    check(skb_pruned_inverse_rows(m.plan, m.real.dev, m.stream.s));
+    m.real.download(output, m.stream.s);
+}
+
+void HipPrunedProvider::forwardDevice(const void* real, void* retained, void* stream) {
+    check(skb_pruned_forward(impl_->plan, real, retained, stream));
+}
+
+void HipPrunedProvider::inverseDevice(const void* retained, void* real, void* stream) {
+    check(skb_pruned_inverse(impl_->plan, retained, real, stream));
+}
+
+void HipPrunedProvider::executeSchedulerNoop() { hip_check(hipStreamSynchronize(impl_->stream.s), "sync"); }
+std::size_t HipPrunedProvider::activeKxCount() const noexcept { return skb_pruned_active_kx(impl_->plan); }
+std::size_t HipPrunedProvider::fullKxCount() const noexcept { return impl_->w.nxHalf(); }
+std::size_t HipPrunedProvider::rowTransformsPerDirection() const noexcept { return impl_->w.ny * impl_->w.planes(); }
+std::size_t HipPrunedProvider::columnTransformsPerDirection() const noexcept {
+    return activeKxCount() * impl_->w.planes();
+}
+std::size_t HipPrunedProvider::omittedColumnTransformsPerDirection() const noexcept {
+    return (fullKxCount() - activeKxCount()) * impl_->w.planes();
+}
+std::size_t HipPrunedProvider::scratchBytes() const noexcept { return skb_pruned_scratch_bytes(impl_->plan); }
+std::string HipPrunedProvider::libraryIdentity() const { return "libskb_hip.so"; }
+std::string HipPrunedProvider::version() const { return skb_version(); }
+
+// ---- HipVerticalGemmProvider ----------------------------------------------------------
+struct HipVerticalGemmProvider::Impl {
+    Workload w;
+    std::size_t nj, modes, groups;
+    skb_vertical_plan* plan = nullptr;
+    Stream stream;
+    Staged physical, modal;
+
+    Impl(const Workload& wl, const GroupedVerticalOperators& ops, Precision p)
+        : w(wl), nj(ops.nj), groups(ops.groups.size()) {
+        if (ops.nz != wl.nz) throw std::invalid_argument("Vertical operator dimensions do not match the workload.");
+        if (ops.groups.empty()) throw std::invalid_argument("Vertical GEMM requires at least one group.");
+        modes = ops.groups.back().firstMode + ops.groups.back().modeCount;
+        const auto cw = to_c(wl);
+        check(skb_vertical_plan_create(&cw, nj, reinterpret_cast<const skb_mode_group*>(ops.groups.data()), groups,
+                                       ops.forward.data(), ops.inverse.data(), c_precision(p), &plan));
+        const bool single = p == Precision::float32;
+        physical.allocate(2 * modes * wl.planes(), single);
+        modal.allocate(2 * modes * wl.fields * nj, single);
+    }
+    ~Impl() {
+        if (plan) skb_vertical_plan_destroy(plan);
+    }
+};
+
+HipVerticalGemmProvider::HipVerticalGemmProvider(const Workload& workload, const GroupedVerticalOperators& operators,
+                                                 Precision precision)
+    : impl_(std::make_unique<Impl>(workload, operators, precision)) {}
+HipVerticalGemmProvider::~HipVerticalGemmProvider() = default;
+HipVerticalGemmProvider::HipVerticalGemmProvider(HipVerticalGemmProvider&&) noexcept = default;
+HipVerticalGemmProvider& HipVerticalGemmProvider::operator=(HipVerticalGemmProvider&&) noexcept = default;
+
+bool HipVerticalGemmProvider::supported() const noexcept { return true; }
+std::string HipVerticalGemmProvider::capability() const { return "hip grouped MFMA GEMM (gfx950)"; }
+std::size_t HipVerticalGemmProvider::columns() const noexcept { return impl_->modes * impl_->w.fields; }
+std::size_t HipVerticalGemmProvider::physicalElements() const noexcept { return impl_->modes * impl_->w.planes(); }
+std::size_t HipVerticalGemmProvider::modalElements() const noexcept {
+    return impl_->modes * impl_->w.fields * impl_->nj;
+}
+std::size_t HipVerticalGemmProvider::groupCount() const noexcept { return impl_->groups; }
+std::size_t HipVerticalGemmProvider::gemmCallsPerExecution() const noexcept { return 1; }
+std::size_t HipVerticalGemmProvider::matrixBytesPerDirection() const noexcept {
+    return skb_vertical_matrix_bytes(impl_->plan);
+}
+void HipVerticalGemmProvider::loadPhysicalInput(const Complex* input) {
+    impl_->physical.upload(reinterpret_cast<const double*>(input), impl_->stream.s);
+}
+void HipVerticalGemmProvider::loadModalInput(const Complex* input) {
+    impl_->modal.upload(reinterpret_cast<const double*>(input), impl_->stream.s);
+}
+void HipVerticalGemmProvider::executeForward() {
+    check(skb_vertical_forward(impl_->plan, impl_->physical.dev, impl_->modal.dev, impl_->stream.s));
+}
+void HipVerticalGemmProvider::executeInverse() {
+    check(skb_vertical_inverse(impl_->plan, impl_->modal.dev, impl_->physical.dev, impl_->stream.s));
+}
+void HipVerticalGemmProvider::copyForwardOutput(Complex* output) const {
+    impl_->modal.download(reinterpret_cast<double*>(output), impl_->stream.s);
+}
+void HipVerticalGemmProvider::copyInverseOutput(Complex* output) const {
+    impl_->physical.download(reinterpret_cast<double*>(output), impl_->stream.s);
+}
+void HipVerticalGemmProvider::forwardDevice(const void* physical, void* modal, void* stream) {
+    check(skb_vertical_forward(impl_->plan, physical, modal, stream));
+}
+void HipVerticalGemmProvider::inverseDevice(const void* modal, void* physical, void* stream) {
+    check(skb_vertical_inverse(impl_->plan, modal, physical, stream));
+}
+std::string HipVerticalGemmProvider::libraryIdentity() const { return "libskb_hip.so"; }
+
+// ---- HipPointwiseAdvectionExecutor ------------------------------------------------------
+struct HipPointwiseAdvectionExecutor::Impl {
+    std::size_t volume;
+    double scale;
+    Precision precision;
+    Stream stream;
+    Staged shared, derivative, target;
+
+    Impl(std::size_t v, double s, Precision p) : volume(v), scale(s), precision(p) {
+        if (v == 0) throw std::invalid_argument("Pointwise advection volume must be positive.");
+        const bool single = p == Precision::float32;
+        shared.allocate(3 * v, single);
+        derivative.allocate(3 * v, single);
+        target.allocate(v, single);
+    }
+};
+
+HipPointwiseAdvectionExecutor::HipPointwiseAdvectionExecutor(std::size_t volumeElements, double scale,
+                                                             Precision precision)
+    : impl_(std::make_unique<Impl>(volumeElements, scale, precision)) {}
+HipPointwiseAdvectionExecutor::~HipPointwiseAdvectionExecutor() = default;
+HipPointwiseAdvectionExecutor::HipPointwiseAdvectionExecutor(HipPointwiseAdvectionExecutor&&) noexcept = default;
+HipPointwiseAdvectionExecutor& HipPointwiseAdvectionExecutor::operator=(HipPointwiseAdvectionExecutor&&) noexcept =
+    default;
+
+void HipPointwiseAdvectionExecutor::execute(const double* shared, const double* derivative, double* target) {
+    auto& m = *impl_;
+    m.shared.upload(shared, m.stream.s);
+    m.derivative.upload(derivative, m.stream.s);
+    check(skb_pointwise_advection(m.volume, m.scale, m.shared.dev, m.derivative.dev, m.target.dev,
+                                  c_precision(m.precision), m.stream.s));
+    m.target.download(target, m.stream.s);
+}
+
+void HipPointwiseAdvectionExecutor::executeDevice(const void* shared, const void* derivative, void* target,
+                                                  void* stream) {
+    auto& m = *impl_;
+    check(skb_pointwise_advection(m.volume, m.scale, shared, derivative, target, c_precision(m.precision), stream));
+}
+
+void HipPointwiseAdvectionExecutor::executeSchedulerNoop() {
+    hip_check(hipStreamSynchronize(impl_->stream.s), "sync");
+}
+std::size_t HipPointwiseAdvectionExecutor::persistentBytes() const noexcept {
+    const std::size_t e = impl_->precision == Precision::float32 ? 4 : 8;
+    return 7 * impl_->volume * e;
+}
+
+}  // namespace skbhip

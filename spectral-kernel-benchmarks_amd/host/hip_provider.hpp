@@ -1,0 +1,214 @@
+// C++20 host mirror of skbench's provider classes, backed by the gfx950 HIP
+// kernels of libskb_hip.so through the C ABI in include/skb_hip.h.
+//
+// Every class keeps the reference's method names, argument meaning, buffer
+// ownership and error behaviour:
+//   HipProvider                  <- skbench::FFTWProvider          (include/skbench/skbench.hpp:364-421)
+//   HipPrunedProvider            <- skbench::FFTWPrunedProvider    (include/skbench/skbench.hpp:467-520)
+//   HipVerticalGemmProvider      <- skbench::VerticalGemmProvider  (include/skbench/skbench.hpp:152-218)
+//   HipPointwiseAdvectionExecutor<- skbench::PointwiseAdvectionExecutor (src/pointwise_advection.hpp:18-43)
+// Host-pointer methods are the reference's "adapter" scope: they stage the
+// caller's FP64 arrays through device buffers owned by the provider (allocated
+// at construction; nothing is allocated per call).  The *Device methods are
+// the "primitive" scope on caller-owned device memory.
+//
+// The value types below are layout-identical to the reference's
+// (skbench::Complex, Workload, RetainedMode, VerticalModeGroup); the static
+// asserts in hip_provider.cpp pin that, so a skbench build can pass its own
+// objects through reinterpret_cast exactly as it does with fftw_complex.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "skb_hip.h"
+
+namespace skbhip {
+
+struct Complex {
+    double real = 0.0;
+    double imag = 0.0;
+};
+
+struct Workload {
+    std::size_t nx = 0, ny = 0, nz = 0, fields = 0;
+    double lx = 1.0, ly = 1.0;
+    bool antialias = true;
+
+    std::size_t planes() const { return nz * fields; }
+    std::size_t nxHalf() const { return nx / 2 + 1; }
+    std::size_t realElements() const { return nx * ny * planes(); }
+    std::size_t spectrumElements() const { return nxHalf() * ny * planes(); }
+    std::size_t retainedVerticalModes() const { return nz < 2 ? 0 : 2 * (nz - 1) / 3; }
+};
+
+struct RetainedMode {
+    std::int64_t k = 0, l = 0;
+    std::size_t storedKx = 0, storedKy = 0;
+    bool conjugatesStoredValue = false;
+    double radialMode = 0.0;
+};
+
+struct VerticalModeGroup {
+    std::uint64_t squaredModeKey = 0;
+    std::size_t firstMode = 0;
+    std::size_t modeCount = 0;
+    bool operator==(const VerticalModeGroup&) const = default;
+};
+
+struct GroupedVerticalOperators {
+    std::string id;
+    std::size_t nz = 0, nj = 0;
+    std::vector<VerticalModeGroup> groups;
+    std::vector<double> forward;  // groups x (nj x nz)
+    std::vector<double> inverse;  // groups x (nz x nj)
+};
+
+enum class Precision { float64, float32 };
+
+enum class FixtureKind { impulse, sinusoid, random, dc, nyquist, sinusoid5x17 };
+
+// ---- model (src/model.cpp) ------------------------------------------------
+std::vector<RetainedMode> retainedHorizontalModes(const Workload& workload);
+std::vector<VerticalModeGroup> squaredWavenumberGroups(const std::vector<RetainedMode>& modes);
+std::string modeOrderHash(const std::vector<RetainedMode>& modes);
+std::string verticalModeGroupHash(const std::vector<VerticalModeGroup>& groups);
+std::string wvmSpectrumOrderHash(const Workload& workload);
+GroupedVerticalOperators squaredWavenumberVerticalFixture(const Workload& workload,
+                                                          const std::vector<RetainedMode>& modes);
+// 32 contiguous K^2 bins of 65x65-style orthogonal matrices (no reference counterpart).
+GroupedVerticalOperators binnedQrVerticalFixture(const Workload& workload, const std::vector<RetainedMode>& modes,
+                                                 std::size_t bins, std::uint64_t seed);
+std::vector<double> makeFixture(const Workload& workload, FixtureKind fixture, std::uint64_t seed);
+
+// ---- providers --------------------------------------------------------------
+class HipProvider {
+public:
+    HipProvider(const Workload& workload, Precision precision = Precision::float64);
+    ~HipProvider();
+    HipProvider(HipProvider&&) noexcept;
+    HipProvider& operator=(HipProvider&&) noexcept;
+    HipProvider(const HipProvider&) = delete;
+    HipProvider& operator=(const HipProvider&) = delete;
+
+    // adapter scope (host FP64 arrays, WVM order)
+    void forward(const double* input, Complex* wvmSpectrum);
+    void inverse(Complex* wvmSpectrum, double* output);  // may treat wvmSpectrum as scratch
+    void gatherRetainedOuter(const std::vector<RetainedMode>& modes, const Complex* wvmSpectrum,
+                             Complex* retainedSpectrum);
+    void embedRetainedOuter(const std::vector<RetainedMode>& modes, const Complex* retainedSpectrum,
+                            Complex* wvmSpectrum);
+    // primitive scope (device arrays in the plan's precision)
+    void forwardDevice(const void* real, void* wvm, void* stream);
+    void inverseDevice(void* wvm, void* real, void* stream);
+
+    void executeSchedulerNoop();
+    bool supported() const noexcept;
+    std::string capability() const;
+    double allocationSeconds() const noexcept;
+    double planningSeconds() const noexcept;
+    std::size_t planningBytes() const noexcept;
+    std::size_t minimumAlignmentBytes() const noexcept;
+    std::string libraryIdentity() const;
+    std::string version() const;
+
+private:
+    struct Impl;
+    std::unique_ptr<Impl> impl_;
+};
+
+class HipPrunedProvider {
+public:
+    HipPrunedProvider(const Workload& workload, const std::vector<RetainedMode>& modes,
+                      Precision precision = Precision::float64);
+    ~HipPrunedProvider();
+    HipPrunedProvider(HipPrunedProvider&&) noexcept;
+    HipPrunedProvider& operator=(HipPrunedProvider&&) noexcept;
+    HipPrunedProvider(const HipPrunedProvider&) = delete;
+    HipPrunedProvider& operator=(const HipPrunedProvider&) = delete;
+
+    void forward(const double* input, Complex* retainedSpectrum);
+    void inverse(const Complex* retainedSpectrum, double* output);
+    void executeForwardRows(const double* input);
+    void executeForwardColumns();  // fused with gatherForward on the device
+    void gatherForward(Complex* retainedSpectrum);
+    void embedInverse(const Complex* retainedSpectrum);
+    void executeInverseColumns();
+    void executeInverseRows(double* output);
+    void forwardDevice(const void* real, void* retained, void* stream);
+    void inverseDevice(const void* retained, void* real, void* stream);
+    void executeSchedulerNoop();
+
+    std::size_t activeKxCount() const noexcept;
+    std::size_t fullKxCount() const noexcept;
+    std::size_t rowTransformsPerDirection() const noexcept;
+    std::size_t columnTransformsPerDirection() const noexcept;
+    std::size_t omittedColumnTransformsPerDirection() const noexcept;
+    std::size_t scratchBytes() const noexcept;
+    bool completeHalfSpectrumOutputMaterialized() const noexcept { return false; }
+    std::string libraryIdentity() const;
+    std::string version() const;
+
+private:
+    struct Impl;
+    std::unique_ptr<Impl> impl_;
+};
+
+class HipVerticalGemmProvider {
+public:
+    HipVerticalGemmProvider(const Workload& workload, const GroupedVerticalOperators& operators,
+                            Precision precision = Precision::float64);
+    ~HipVerticalGemmProvider();
+    HipVerticalGemmProvider(HipVerticalGemmProvider&&) noexcept;
+    HipVerticalGemmProvider& operator=(HipVerticalGemmProvider&&) noexcept;
+    HipVerticalGemmProvider(const HipVerticalGemmProvider&) = delete;
+    HipVerticalGemmProvider& operator=(const HipVerticalGemmProvider&) = delete;
+
+    bool supported() const noexcept;
+    std::string capability() const;
+    std::size_t columns() const noexcept;
+    std::size_t physicalElements() const noexcept;
+    std::size_t modalElements() const noexcept;
+    std::size_t groupCount() const noexcept;
+    std::size_t gemmCallsPerExecution() const noexcept;  // one grouped launch
+    std::size_t matrixBytesPerDirection() const noexcept;
+
+    void loadPhysicalInput(const Complex* input);
+    void loadModalInput(const Complex* input);
+    void executeForward();
+    void executeInverse();
+    void copyForwardOutput(Complex* output) const;
+    void copyInverseOutput(Complex* output) const;
+    void forwardDevice(const void* physical, void* modal, void* stream);
+    void inverseDevice(const void* modal, void* physical, void* stream);
+    std::string libraryIdentity() const;
+
+private:
+    struct Impl;
+    std::unique_ptr<Impl> impl_;
+};
+
+class HipPointwiseAdvectionExecutor {
+public:
+    HipPointwiseAdvectionExecutor(std::size_t volumeElements, double scale,
+                                  Precision precision = Precision::float64);
+    ~HipPointwiseAdvectionExecutor();
+    HipPointwiseAdvectionExecutor(HipPointwiseAdvectionExecutor&&) noexcept;
+    HipPointwiseAdvectionExecutor& operator=(HipPointwiseAdvectionExecutor&&) noexcept;
+    HipPointwiseAdvectionExecutor(const HipPointwiseAdvectionExecutor&) = delete;
+    HipPointwiseAdvectionExecutor& operator=(const HipPointwiseAdvectionExecutor&) = delete;
+
+    void execute(const double* shared, const double* derivative, double* target);
+    void executeDevice(const void* shared, const void* derivative, void* target, void* stream);
+    void executeSchedulerNoop();
+    std::size_t persistentBytes() const noexcept;
+
+private:
+    struct Impl;
+    std::unique_ptr<Impl> impl_;
+};
+
+}  // namespace skbhip

@@ -73,6 +73,12 @@ FIXTURES = {"impulse": 0, "sinusoid": 1, "random": 2, "dc": 3, "nyquist": 4, "si
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError("libskb_hip.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+    # torch carries its own HIP runtime (SONAME libamdhip64.so.7, loaded by
+    # file name).  Loading torch first lets the library's NEEDED entry bind to
+    # that same runtime, so the process has ONE HIP runtime and device
+    # pointers and streams from torch are valid in the library.
+    import torch  # noqa: F401
+
     L = ctypes.CDLL(LIB_PATH)
     vp, u64, i32, dbl = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_double
     W = ctypes.POINTER(Workload_)
@@ -106,6 +112,10 @@ def _load():
         "skb_pruned_inverse_columns": (i32, [vp, vp, vp]),
         "skb_pruned_inverse_rows": (i32, [vp, vp, vp]),
         "skb_pruned_active_kx": (u64, [vp]),
+        "skb_pruned_plan_set_grid": (i32, [vp, i32, i32]),
+        "skb_pruned_plan_items": (u64, [vp, i32]),
+        "skb_vertical_plan_set_grid": (i32, [vp, i32]),
+        "skb_vertical_plan_items": (u64, [vp]),
         "skb_pruned_scratch_bytes": (u64, [vp]),
         "skb_vertical_plan_create": (i32, [W, u64, vp, u64, vp, vp, i32, pp]),
         "skb_vertical_plan_destroy": (i32, [vp]),
@@ -131,7 +141,8 @@ EXPORTED = [
     "skb_pruned_forward", "skb_pruned_inverse", "skb_pruned_active_kx", "skb_pruned_scratch_bytes",
     "skb_pruned_forward_rows", "skb_pruned_forward_columns", "skb_pruned_inverse_columns", "skb_pruned_inverse_rows",
     "skb_vertical_plan_create", "skb_vertical_plan_destroy", "skb_vertical_forward", "skb_vertical_inverse",
-    "skb_vertical_matrix_bytes", "skb_pointwise_advection",
+    "skb_vertical_matrix_bytes", "skb_pointwise_advection", "skb_pruned_plan_set_grid", "skb_pruned_plan_items",
+    "skb_vertical_plan_set_grid", "skb_vertical_plan_items",
 ]
 
 
@@ -392,6 +403,15 @@ class PrunedPlan:
         check(lib.skb_pruned_inverse_rows(
             self.h, _dev(real, self.w.real_elements, real_dtype(self.precision), "real"), _stream(stream)))
 
+    STAGES = ("forward_rows", "forward_columns", "inverse_columns", "inverse_rows")
+
+    def set_grid(self, stage: int, workgroups: int):
+        """Launch shape of one stage (0 = one workgroup per work item)."""
+        check(lib.skb_pruned_plan_set_grid(self.h, stage, workgroups))
+
+    def items(self, stage: int) -> int:
+        return int(lib.skb_pruned_plan_items(self.h, stage))
+
     def active_kx(self) -> int:
         return int(lib.skb_pruned_active_kx(self.h))
 
@@ -435,6 +455,13 @@ class VerticalPlan:
 
     def matrix_bytes(self) -> int:
         return int(lib.skb_vertical_matrix_bytes(self.h))
+
+    def set_grid(self, wavefronts: int):
+        """Wavefronts per launch (0 = one per 16-column tile)."""
+        check(lib.skb_vertical_plan_set_grid(self.h, wavefronts))
+
+    def items(self) -> int:
+        return int(lib.skb_vertical_plan_items(self.h))
 
     def close(self):
         if getattr(self, "h", None):

@@ -1,0 +1,165 @@
+// GPU tests of the C++ provider classes (skbench-style API), in the style of
+// the reference's tests/skbench_tests.cpp: one main() of require() checks.
+// The oracle (oracle/skb_oracle.c) is linked as the checker.
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "hip_provider.hpp"
+
+extern "C" {
+typedef struct {
+    double re, im;
+} or_complex;
+void or_direct_r2c(uint64_t, uint64_t, uint64_t, uint64_t, const double*, or_complex*);
+void or_direct_c2r(uint64_t, uint64_t, uint64_t, uint64_t, const or_complex*, double*);
+void or_direct_retained(uint64_t, uint64_t, uint64_t, uint64_t, const void*, uint64_t, const double*, or_complex*);
+void or_embed_retained(uint64_t, uint64_t, uint64_t, uint64_t, const void*, uint64_t, const or_complex*, or_complex*);
+void or_vertical_forward(uint64_t, uint64_t, uint64_t, uint64_t, const uint64_t*, const uint64_t*, const double*,
+                         const or_complex*, or_complex*);
+double or_max_relative_error_complex(const void*, const void*, uint64_t);
+double or_max_relative_error_real(const double*, const double*, uint64_t, double);
+}
+
+namespace {
+
+int failures = 0;
+
+void require(bool ok, const std::string& what) {
+    if (!ok) {
+        std::fprintf(stderr, "FAILED: %s\n", what.c_str());
+        ++failures;
+    }
+}
+
+// or_mode records are layout-identical to skb_retained_mode
+struct OrMode {
+    int64_t k, l;
+    uint64_t skx, sky;
+    int32_t conj, pad;
+    double radial;
+};
+
+std::vector<OrMode> to_or(const std::vector<skbhip::RetainedMode>& modes) {
+    std::vector<OrMode> out(modes.size());
+    for (size_t i = 0; i < modes.size(); ++i)
+        out[i] = {modes[i].k, modes[i].l, modes[i].storedKx, modes[i].storedKy, modes[i].conjugatesStoredValue ? 1 : 0,
+                  0, modes[i].radialMode};
+    return out;
+}
+
+}  // namespace
+
+int main() {
+    try {
+        using namespace skbhip;
+        const Workload w{8, 8, 7, 2, 1.0, 1.0, true};
+        require(w.planes() == 14 && w.nxHalf() == 5 && w.retainedVerticalModes() == 4, "smoke geometry");
+        const auto modes = retainedHorizontalModes(w);
+        require(modes.front().k == 0 && modes.front().l == 0, "DC is first");
+        const auto groups = squaredWavenumberGroups(modes);
+        require(groups.size() == 5, "smoke K-squared group count");
+        require(verticalModeGroupHash(groups) == "fnv1a64:69843c1f93cead00", "K-squared group hash");
+        const Workload quick{256, 256, 65, 3, 1.0, 1.0, true};
+        const auto quickModes = retainedHorizontalModes(quick);
+        require(quickModes.size() == 11439, "256^2 retained-mode count");
+        require(modeOrderHash(quickModes) == "fnv1a64:22e98f1dcda40ad5", "256^2 mode order hash");
+        require(wvmSpectrumOrderHash(quick) == "fnv1a64:b1cdd8f25ebcbb72", "256^2 WVM order hash");
+
+        const auto om = to_or(modes);
+        HipProvider full(w);
+        require(full.supported(), "hip provider supported at 8x8");
+        HipPrunedProvider pruned(w, modes);
+        require(pruned.activeKxCount() < pruned.fullKxCount(), "pruned provider omits kx columns");
+        require(pruned.columnTransformsPerDirection() + pruned.omittedColumnTransformsPerDirection() ==
+                    w.nxHalf() * w.planes(),
+                "pruned column accounting");
+        std::vector<Complex> spec(w.spectrumElements()), oracle(w.spectrumElements()), embedded(w.spectrumElements());
+        std::vector<Complex> retained(modes.size() * w.planes()), retainedOracle(retained.size());
+        std::vector<double> real(w.realElements()), realOracle(w.realElements());
+        for (int f = 0; f < 5; ++f) {
+            const auto kind = static_cast<FixtureKind>(f);
+            const auto input = makeFixture(w, kind, 700 + f);
+            const auto preserved = input;
+            or_direct_r2c(8, 8, 7, 2, input.data(), reinterpret_cast<or_complex*>(oracle.data()));
+            full.forward(input.data(), spec.data());
+            require(or_max_relative_error_complex(spec.data(), oracle.data(), spec.size()) <= 1e-12,
+                    "hip forward vs direct DFT, fixture " + std::to_string(f));
+            auto destroyed = oracle;
+            full.inverse(destroyed.data(), real.data());
+            require(or_max_relative_error_real(real.data(), input.data(), real.size(), 1.0 / 64.0) <= 1e-12,
+                    "hip inverse round trip, fixture " + std::to_string(f));
+            or_direct_retained(8, 8, 7, 2, om.data(), om.size(), input.data(),
+                               reinterpret_cast<or_complex*>(retainedOracle.data()));
+            pruned.forward(input.data(), retained.data());
+            require(or_max_relative_error_complex(retained.data(), retainedOracle.data(), retained.size()) < 1e-12,
+                    "pruned forward vs direct retained DFT");
+            require(input == preserved, "pruned forward modified caller input");
+            or_embed_retained(8, 8, 7, 2, om.data(), om.size(), reinterpret_cast<or_complex*>(retainedOracle.data()),
+                              reinterpret_cast<or_complex*>(embedded.data()));
+            or_direct_c2r(8, 8, 7, 2, reinterpret_cast<or_complex*>(embedded.data()), realOracle.data());
+            pruned.inverse(retainedOracle.data(), real.data());
+            require(or_max_relative_error_real(real.data(), realOracle.data(), real.size(), 1.0) < 1e-12,
+                    "pruned inverse vs direct DFT");
+            std::vector<Complex> gathered(retained.size());
+            full.gatherRetainedOuter(modes, oracle.data(), gathered.data());
+            require(or_max_relative_error_complex(gathered.data(), retainedOracle.data(), retained.size()) < 1e-12,
+                    "gatherRetainedOuter");
+        }
+
+        const auto ops = squaredWavenumberVerticalFixture(w, modes);
+        HipVerticalGemmProvider vertical(w, ops);
+        require(vertical.groupCount() == 5 && vertical.gemmCallsPerExecution() == 1, "vertical accounting");
+        std::vector<Complex> phys(vertical.physicalElements()), modal(vertical.modalElements()),
+            modalOracle(vertical.modalElements());
+        for (size_t i = 0; i < phys.size(); ++i)
+            phys[i] = {static_cast<double>(i % 31) / 31.0, -static_cast<double>(i % 37) / 37.0};
+        vertical.loadPhysicalInput(phys.data());
+        vertical.executeForward();
+        vertical.copyForwardOutput(modal.data());
+        std::vector<uint64_t> first, count;
+        for (const auto& g : ops.groups) {
+            first.push_back(g.firstMode);
+            count.push_back(g.modeCount);
+        }
+        or_vertical_forward(w.nz, ops.nj, w.fields, first.size(), first.data(), count.data(), ops.forward.data(),
+                            reinterpret_cast<or_complex*>(phys.data()), reinterpret_cast<or_complex*>(modalOracle.data()));
+        require(or_max_relative_error_complex(modal.data(), modalOracle.data(), modal.size()) <= 1e-12,
+                "vertical forward vs scalar products");
+
+        HipPointwiseAdvectionExecutor advection(37, 0.03125);
+        std::vector<double> shared(111), derivative(111), target(37), expected(37);
+        for (size_t i = 0; i < 111; ++i) {
+            shared[i] = static_cast<double>(i % 17) - 8.0;
+            derivative[i] = static_cast<double>((3 * i + 5) % 23) - 11.0;
+        }
+        for (size_t p = 0; p < 37; ++p)
+            expected[p] = -0.03125 * (shared[p] * derivative[p] + shared[37 + p] * derivative[37 + p] +
+                                      shared[74 + p] * derivative[74 + p]);
+        advection.execute(shared.data(), derivative.data(), target.data());
+        require(or_max_relative_error_real(target.data(), expected.data(), 37, 1.0) <= 1e-15, "pointwise tail");
+
+        bool threw = false;
+        try {
+            HipPrunedProvider bad(w, {});
+        } catch (const std::invalid_argument&) {
+            threw = true;
+        }
+        require(threw, "empty mode list must throw std::invalid_argument");
+        HipProvider odd(Workload{12, 8, 2, 1, 1.0, 1.0, true});
+        require(!odd.supported() && !odd.capability().empty(), "non-power-of-two grid reports unsupported");
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "exception: %s\n", e.what());
+        return 2;
+    }
+    if (failures) {
+        std::fprintf(stderr, "%d check(s) failed\n", failures);
+        return 1;
+    }
+    std::printf("skbench_hip_tests: all checks passed\n");
+    return 0;
+}
