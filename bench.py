@@ -50,10 +50,10 @@ def parse():
 
 COPY_BYTES = 256 << 20
 KERNEL_NAMES = {  # substring of the device kernel name -> stage
-    "col_kernel<float, 256, true, 1, 0>": "inverse_columns",
-    "row_c2r_kernel<float, 256>": "inverse_rows",
-    "row_r2c_kernel<float, 256>": "forward_rows",
-    "col_kernel<float, 256, false, 0, 1>": "forward_columns",
+    "col2_kernel<float, 256, true, 1, 0>": "inverse_columns",
+    "row2_c2r_kernel<float, 256>": "inverse_rows",
+    "row2_r2c_kernel<float, 256>": "forward_rows",
+    "col2_kernel<float, 256, false, 0, 1>": "forward_columns",
     "grouped_gemm_kernel<float": "vertical_gemm",
 }
 
@@ -65,24 +65,28 @@ def summarize_pmc(fetch_csv, write_csv, family):
     import csv
 
     def per_kernel(path, counter):
-        sums, counts = {}, {}
+        """(mean KiB per dispatch by kernel name, largest single copy dispatch in KiB)"""
+        sums, counts, copy_max = {}, {}, 0.0
         with open(path, newline="") as handle:
             for row in csv.DictReader(handle):
                 if row.get("Counter_Name") != counter:
                     continue
                 name = row.get("Kernel_Name", "")
-                sums[name] = sums.get(name, 0.0) + float(row["Counter_Value"])
+                value = float(row["Counter_Value"])
+                sums[name] = sums.get(name, 0.0) + value
                 counts[name] = counts.get(name, 0) + 1
-        return {k: sums[k] / counts[k] for k in sums}  # KiB per dispatch
+                if "copybuffer" in name.lower():
+                    copy_max = max(copy_max, value)
+        return {k: sums[k] / counts[k] for k in sums}, copy_max
 
-    fetch = per_kernel(fetch_csv, "FETCH_SIZE")
-    write = per_kernel(write_csv, "WRITE_SIZE")
-    copy_f = [v for k, v in fetch.items() if "copy" in k.lower() or "elementwise" in k.lower()]
-    copy_w = [v for k, v in write.items() if "copy" in k.lower() or "elementwise" in k.lower()]
-    if not copy_f or not copy_w:
+    # The calibration dispatch is the COPY_BYTES device-to-device copy, by far
+    # the largest copy of the run; the other copies are small host transfers.
+    fetch, copy_f = per_kernel(fetch_csv, "FETCH_SIZE")
+    write, copy_w = per_kernel(write_csv, "WRITE_SIZE")
+    if copy_f <= 0 or copy_w <= 0:
         raise SystemExit("no copy kernel found in the counter files")
-    f_scale = (COPY_BYTES / 1024.0) / max(copy_f)
-    w_scale = (COPY_BYTES / 1024.0) / max(copy_w)
+    f_scale = (COPY_BYTES / 1024.0) / copy_f
+    w_scale = (COPY_BYTES / 1024.0) / copy_w
     out = {"family": family, "copy_bytes": COPY_BYTES, "fetch_scale": f_scale, "write_scale": w_scale,
            "kernels": {}}
     for name, stage in KERNEL_NAMES.items():
@@ -138,24 +142,37 @@ def timed(torch, op, steps):
     return time.perf_counter() - t0
 
 
-def instrumented(torch, op, steps):
-    """Per-stage and per-eval device times from HIP events recorded on the
-    stream the kernels run on (torch's current stream)."""
+def instrumented(torch, op, steps, reps=50):
+    """Device times from HIP events recorded on the stream the kernels run on
+    (torch's current stream, which the plans are given).
+
+    per_eval: one event pair around each evaluation (latency percentiles).
+    per_stage: for each kernel, `reps` back-to-back launches between one event
+    pair, averaged -- the kernel's launch duration without the event gaps that
+    per-kernel pairs inside an evaluation would add.  Every stage is
+    idempotent (it rewrites its output from its input), so repeating it alone
+    does the same work as inside the evaluation."""
     names = [n for n, _ in op.stages]
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 1)] for _ in range(steps)]
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     torch.cuda.synchronize()
     for s in range(steps):
         ev[s][0].record()
-        for i, (_, fn) in enumerate(op.stages):
-            fn()
-            ev[s][i + 1].record()
+        op.step()
+        ev[s][1].record()
     torch.cuda.synchronize()
-    per_stage = {n: [] for n in names}
-    per_eval = []
-    for s in range(steps):
-        for i, n in enumerate(names):
-            per_stage[n].append(ev[s][i].elapsed_time(ev[s][i + 1]))
-        per_eval.append(ev[s][0].elapsed_time(ev[s][-1]))
+    per_eval = [a.elapsed_time(b) for a, b in ev]
+    per_stage = {}
+    for name, fn in op.stages:
+        blocks = []
+        for _ in range(max(1, steps // reps)):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(reps):
+                fn()
+            b.record()
+            blocks.append((a, b))
+        torch.cuda.synchronize()
+        per_stage[name] = [a.elapsed_time(b) / reps for a, b in blocks]
     return per_stage, per_eval
 
 
