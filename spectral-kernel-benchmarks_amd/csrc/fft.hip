@@ -402,6 +402,15 @@ struct Geo2 {
     static constexpr int THREADS = LANES;       // TP * N1 >= TP * N2
 };
 
+// First work item of a workgroup.  One workgroup per item (LOOP = false):
+// the XCD-aware mapping; workgroups past the last item exit at once.
+// Persistent workgroups (LOOP = true) take items blockIdx.x, blockIdx.x +
+// gridDim.x, ... and prefetch the next item's inputs into registers.
+template <bool LOOP>
+__device__ __forceinline__ int first_item(int items) {
+    return LOOP ? static_cast<int>(blockIdx.x) : xcd_item(static_cast<int>(blockIdx.x), items);
+}
+
 // Row passes.  A work item is one row index y of a tile of TP = 2*PAIRS
 // consecutive planes; item = y * tiles + tile, so the workgroups running at
 // the same time cover whole plane runs of the same rows (DRAM-page locality
@@ -425,11 +434,14 @@ __device__ __forceinline__ void r2c_load(cx<R>* v, const R* real, int item, int 
     }
 }
 
-template <typename R, int NX>
+// Spectrum side: out[ps * (kx + kc * y) + p]; ps >= planes is the plane
+// stride (planes for WVM order, padded to whole cache lines for the pruned
+// intermediate).
+template <typename R, int NX, bool LOOP>
 __global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void row2_r2c_kernel(const R* __restrict__ real,
                                                                          cx<R>* __restrict__ out,
                                                                          const cx<R>* __restrict__ tw, int ny,
-                                                                         int planes, int kc, int items) {
+                                                                         int planes, int ps, int kc, int items) {
     using G = Geo2<R, NX>;
     constexpr int N1 = G::N1, N2 = G::N2, S = G::PAIRS, TP = 2 * S, NT = G::THREADS;
     // LDS strides in complex elements are odd, so the 16 (FP32) or 8 (FP64)
@@ -441,19 +453,21 @@ __global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void row2_r2c_kernel(const 
     __shared__ cx<R> buf[BUF];
     __shared__ cx<R> twl[NX];
     const int t = threadIdx.x;
-    // one workgroup per item: XCD-aware mapping; persistent: linear
-    int item = gridDim.x >= static_cast<unsigned>(items) ? xcd_item(blockIdx.x, items) : blockIdx.x;
+    int item = first_item<LOOP>(items);
+    if (item >= items) return;  // uniform across the workgroup
     cx<R> v[N1];
-    if (item < items) r2c_load<R, NX>(v, real, item, ny, planes);
+    r2c_load<R, NX>(v, real, item, ny, planes);
     for (int i = t; i < NX; i += NT) twl[i] = tw[i];
     __syncthreads();
     const int n2 = t % N2, s1 = t / N2;
     const int k1 = t % N1, s2 = t / N1;
     const int ntiles = (planes + TP - 1) / TP;
-    while (item < items) {
-        const int next = gridDim.x >= static_cast<unsigned>(items) ? items : item + gridDim.x;
-        cx<R> vn[N1];
-        if (next < items) r2c_load<R, NX>(vn, real, next, ny, planes);
+    for (;;) {
+        const int next = LOOP ? item + static_cast<int>(gridDim.x) : items;
+        cx<R> vn[LOOP ? N1 : 1];
+        if constexpr (LOOP) {
+            if (next < items) r2c_load<R, NX>(vn, real, next, ny, planes);
+        }
         const int y = item / ntiles, p0 = (item % ntiles) * TP;
         // pass 1: thread (n2 fastest, pair s)
         if (s1 < S) {
@@ -472,7 +486,7 @@ __global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void row2_r2c_kernel(const 
         for (int k2 = 0; k2 < N2; ++k2) buf[s2 * ZS + k1 + N1 * k2] = w[k2];
         __syncthreads();
         // Hermitian split of each pair and plane-fastest store of kx < kc
-        const size_t row_base = static_cast<size_t>(planes) * static_cast<size_t>(kc) * y;
+        const size_t row_base = static_cast<size_t>(ps) * static_cast<size_t>(kc) * y;
         for (int e = t; e < kc * TP; e += NT) {
             const int pit = e % TP, kx = e / TP;
             const int p = p0 + pit;
@@ -481,11 +495,14 @@ __global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void row2_r2c_kernel(const 
             const cx<R> zn = buf[(pit >> 1) * ZS + ((NX - kx) & (NX - 1))];
             const cx<R> x = (pit & 1) == 0 ? cx<R>{R(0.5) * (zk.x + zn.x), R(0.5) * (zk.y - zn.y)}
                                            : cx<R>{R(0.5) * (zk.y + zn.y), R(0.5) * (zn.x - zk.x)};
-            out[row_base + static_cast<size_t>(planes) * kx + p] = x;
+            out[row_base + static_cast<size_t>(ps) * kx + p] = x;
         }
+        if (!LOOP || next >= items) break;
         __syncthreads();  // buf is rewritten by the next item
+        if constexpr (LOOP) {
 #pragma unroll
-        for (int m = 0; m < N1; ++m) v[m] = vn[m];
+            for (int m = 0; m < N1; ++m) v[m] = vn[m];
+        }
         item = next;
     }
 }
@@ -494,7 +511,7 @@ __global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void row2_r2c_kernel(const 
 // Im of the kx = 0 and kx = Nx/2 bins is ignored and kx >= kc is zero (never
 // read).  Thread (pair s fastest, n2).
 template <typename R, int NX>
-__device__ __forceinline__ void c2r_load(cx<R>* v, const cx<R>* in, int item, int ny, int planes, int kc) {
+__device__ __forceinline__ void c2r_load(cx<R>* v, const cx<R>* in, int item, int ny, int planes, int ps, int kc) {
     using G = Geo2<R, NX>;
     constexpr int N1 = G::N1, N2 = G::N2, S = G::PAIRS, TP = 2 * S;
     const int t = threadIdx.x, s1 = t % S, n2 = t / S;
@@ -502,7 +519,10 @@ __device__ __forceinline__ void c2r_load(cx<R>* v, const cx<R>* in, int item, in
     const int y = item / ntiles, p0 = (item % ntiles) * TP;
     const int pa = p0 + 2 * s1;
     const bool act = n2 < N2;
-    const size_t row_base = static_cast<size_t>(planes) * static_cast<size_t>(kc) * y;
+    const size_t row_base = static_cast<size_t>(ps) * static_cast<size_t>(kc) * y;
+    // FP32 with an even plane stride: the pair (pa, pa + 1) is one aligned
+    // 16-byte load (pa is even).
+    const bool wide = sizeof(R) == 4 && (ps & 1) == 0 && pa + 1 < planes;
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1) {
         const int k = N2 * n1 + n2;
@@ -510,9 +530,20 @@ __device__ __forceinline__ void c2r_load(cx<R>* v, const cx<R>* in, int item, in
         const int kx = upper ? NX - k : k;
         cx<R> xa{R(0), R(0)}, xb{R(0), R(0)};
         if (act && kx < kc) {
-            const size_t base = row_base + static_cast<size_t>(planes) * kx + pa;
-            if (pa < planes) xa = in[base];
-            if (pa + 1 < planes) xb = in[base + 1];
+            const size_t base = row_base + static_cast<size_t>(ps) * kx + pa;
+            if constexpr (sizeof(R) == 4) {
+                if (wide) {
+                    const float4 q = *reinterpret_cast<const float4*>(in + base);
+                    xa = cx<R>{q.x, q.y};
+                    xb = cx<R>{q.z, q.w};
+                } else {
+                    if (pa < planes) xa = in[base];
+                    if (pa + 1 < planes) xb = in[base + 1];
+                }
+            } else {
+                if (pa < planes) xa = in[base];
+                if (pa + 1 < planes) xb = in[base + 1];
+            }
         }
         if (kx == 0 || kx == NX / 2) {
             xa.y = R(0);
@@ -522,11 +553,11 @@ __device__ __forceinline__ void c2r_load(cx<R>* v, const cx<R>* in, int item, in
     }
 }
 
-template <typename R, int NX>
+template <typename R, int NX, bool LOOP>
 __global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void row2_c2r_kernel(const cx<R>* __restrict__ in,
                                                                          R* __restrict__ real,
                                                                          const cx<R>* __restrict__ tw, int ny,
-                                                                         int planes, int kc, int items) {
+                                                                         int planes, int ps, int kc, int items) {
     using G = Geo2<R, NX>;
     constexpr int N1 = G::N1, N2 = G::N2, S = G::PAIRS, TP = 2 * S, NT = G::THREADS;
     constexpr int YS = N2 + 1;
@@ -534,19 +565,21 @@ __global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void row2_c2r_kernel(const 
     __shared__ cx<R> buf[S * SS];
     __shared__ cx<R> twl[NX];
     const int t = threadIdx.x;
-    // one workgroup per item: XCD-aware mapping; persistent: linear
-    int item = gridDim.x >= static_cast<unsigned>(items) ? xcd_item(blockIdx.x, items) : blockIdx.x;
+    int item = first_item<LOOP>(items);
+    if (item >= items) return;  // uniform across the workgroup
     cx<R> v[N1];
-    if (item < items) c2r_load<R, NX>(v, in, item, ny, planes, kc);
+    c2r_load<R, NX>(v, in, item, ny, planes, ps, kc);
     for (int i = t; i < NX; i += NT) twl[i] = tw[i];
     __syncthreads();
     const int s1 = t % S, n2 = t / S;
     const int k1 = t % N1, s2 = t / N1;
     const int ntiles = (planes + TP - 1) / TP;
-    while (item < items) {
-        const int next = gridDim.x >= static_cast<unsigned>(items) ? items : item + gridDim.x;
-        cx<R> vn[N1];
-        if (next < items) c2r_load<R, NX>(vn, in, next, ny, planes, kc);
+    for (;;) {
+        const int next = LOOP ? item + static_cast<int>(gridDim.x) : items;
+        cx<R> vn[LOOP ? N1 : 1];
+        if constexpr (LOOP) {
+            if (next < items) c2r_load<R, NX>(vn, in, next, ny, planes, ps, kc);
+        }
         const int y = item / ntiles, p0 = (item % ntiles) * TP;
         if (n2 < N2) {
             rdft<N1, true>(v);
@@ -570,9 +603,12 @@ __global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void row2_c2r_kernel(const 
 #pragma unroll
             for (int k2 = 0; k2 < N2; ++k2) rb[N1 * k2] = w[k2].y;
         }
+        if (!LOOP || next >= items) break;
         __syncthreads();  // buf is rewritten by the next item
+        if constexpr (LOOP) {
 #pragma unroll
-        for (int m = 0; m < N1; ++m) v[m] = vn[m];
+            for (int m = 0; m < N1; ++m) v[m] = vn[m];
+        }
         item = next;
     }
 }
@@ -584,12 +620,13 @@ __global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void row2_c2r_kernel(const 
 // each loops over items blockIdx.x, blockIdx.x + gridDim.x, ... and issues the
 // global loads of its NEXT item before transforming the current one, so HBM
 // traffic overlaps the register/LDS work instead of alternating with it.
-// IN  kStrided: in[P*(kx + in_kc*y) + p]   kRetained: in[P*mode + p] via map
-// OUT kStrided: out[P*(kx + out_kc*y) + p] kRetained: out[P*mode + p] via map
+// IN  kStrided: in[ps*(kx + in_kc*y) + p]   kRetained: in[P*mode + p] via map
+// OUT kStrided: out[ps*(kx + out_kc*y) + p] kRetained: out[P*mode + p] via map
+// (P = planes; ps >= P is the plane stride of the strided operand)
 // Maps: code = 2*mode + conj at [kx*NY + row], or -1 (no mode).
 template <typename R, int NY, int IN>
-__device__ __forceinline__ void col2_load(cx<R>* v, const cx<R>* in, int item, int kx_count, int planes, int in_kc,
-                                          const int* __restrict__ map) {
+__device__ __forceinline__ void col2_load(cx<R>* v, const cx<R>* in, int item, int kx_count, int planes, int ps,
+                                          int in_kc, const int* __restrict__ map) {
     using G = Geo2<R, NY>;
     constexpr int N1 = G::N1, N2 = G::N2, TP = G::TP;
     const int t = threadIdx.x, pp = t % TP, n2 = t / TP;
@@ -600,7 +637,8 @@ __device__ __forceinline__ void col2_load(cx<R>* v, const cx<R>* in, int item, i
 #pragma unroll
         for (int n1 = 0; n1 < N1; ++n1) {
             const int yy = N2 * n1 + n2;
-            v[n1] = (n2 < N2 && p < planes) ? in[P * (kx + static_cast<size_t>(in_kc) * yy) + p] : cx<R>{R(0), R(0)};
+            v[n1] = (n2 < N2 && p < planes) ? in[static_cast<size_t>(ps) * (kx + static_cast<size_t>(in_kc) * yy) + p]
+                                            : cx<R>{R(0), R(0)};
         }
     } else {
         // all codes first, then all values: two batched round trips
@@ -619,9 +657,9 @@ __device__ __forceinline__ void col2_load(cx<R>* v, const cx<R>* in, int item, i
     }
 }
 
-template <typename R, int NY, bool INV, int IN, int OUT>
+template <typename R, int NY, bool INV, int IN, int OUT, bool LOOP>
 __global__ __launch_bounds__((Geo2<R, NY>::THREADS)) void col2_kernel(const cx<R>* in, cx<R>* out,
-                                                                     const cx<R>* __restrict__ tw, int planes,
+                                                                     const cx<R>* __restrict__ tw, int planes, int ps,
                                                                      int kx_count, int items, int in_kc, int out_kc,
                                                                      const int* __restrict__ map) {
     using G = Geo2<R, NY>;
@@ -631,17 +669,19 @@ __global__ __launch_bounds__((Geo2<R, NY>::THREADS)) void col2_kernel(const cx<R
     const int t = threadIdx.x;
     const size_t P = static_cast<size_t>(planes);
     const int ntiles = (planes + TP - 1) / TP;
-    // one workgroup per item: XCD-aware mapping; persistent: linear
-    int item = gridDim.x >= static_cast<unsigned>(items) ? xcd_item(blockIdx.x, items) : blockIdx.x;
+    int item = first_item<LOOP>(items);
+    if (item >= items) return;  // uniform across the workgroup
     cx<R> v[N1];
-    if (item < items) col2_load<R, NY, IN>(v, in, item, kx_count, planes, in_kc, map);
+    col2_load<R, NY, IN>(v, in, item, kx_count, planes, ps, in_kc, map);
     for (int i = t; i < NY; i += NT) twl[i] = tw[i];
     __syncthreads();
     const int pp = t % TP, n2 = t / TP, k1 = t / TP;
-    while (item < items) {
-        const int next = gridDim.x >= static_cast<unsigned>(items) ? items : item + gridDim.x;
-        cx<R> vn[N1];
-        if (next < items) col2_load<R, NY, IN>(vn, in, next, kx_count, planes, in_kc, map);
+    for (;;) {
+        const int next = LOOP ? item + static_cast<int>(gridDim.x) : items;
+        cx<R> vn[LOOP ? N1 : 1];
+        if constexpr (LOOP) {
+            if (next < items) col2_load<R, NY, IN>(vn, in, next, kx_count, planes, ps, in_kc, map);
+        }
         const int kx = item / ntiles, p = (item % ntiles) * TP + pp;
         // Output codes are loaded with (not after) the item's inputs: on gfx9
         // one in-order counter tracks loads and stores, so a code loaded
@@ -671,7 +711,7 @@ __global__ __launch_bounds__((Geo2<R, NY>::THREADS)) void col2_kernel(const cx<R
             for (int k2 = 0; k2 < N2; ++k2) {
                 const int row = k1 + N1 * k2;
                 if constexpr (OUT == kStrided) {
-                    out[P * (kx + static_cast<size_t>(out_kc) * row) + p] = w[k2];
+                    out[static_cast<size_t>(ps) * (kx + static_cast<size_t>(out_kc) * row) + p] = w[k2];
                 } else {
                     const int code = ocode[k2];
                     if (code >= 0) {
@@ -682,9 +722,12 @@ __global__ __launch_bounds__((Geo2<R, NY>::THREADS)) void col2_kernel(const cx<R
                 }
             }
         }
+        if (!LOOP || next >= items) break;
         __syncthreads();  // buf is rewritten by the next item
+        if constexpr (LOOP) {
 #pragma unroll
-        for (int m = 0; m < N1; ++m) v[m] = vn[m];
+            for (int m = 0; m < N1; ++m) v[m] = vn[m];
+        }
         item = next;
     }
 }
@@ -751,23 +794,34 @@ int launch_grid(int items, int requested) {
     return requested > 0 && requested < items ? requested : xcd_grid(items);
 }
 
+template <typename R, int NX, bool LOOP>
+void launch_row2(bool inverse, const void* in, void* out, const void* tw, int ny, int planes, int ps, int kc,
+                 int items, int grid, hipStream_t stream) {
+    using G = Geo2<R, NX>;
+    if (inverse)
+        hipLaunchKernelGGL((row2_c2r_kernel<R, NX, LOOP>), dim3(grid), dim3(G::THREADS), 0, stream,
+                           static_cast<const cx<R>*>(in), static_cast<R*>(out), static_cast<const cx<R>*>(tw), ny,
+                           planes, ps, kc, items);
+    else
+        hipLaunchKernelGGL((row2_r2c_kernel<R, NX, LOOP>), dim3(grid), dim3(G::THREADS), 0, stream,
+                           static_cast<const R*>(in), static_cast<cx<R>*>(out), static_cast<const cx<R>*>(tw), ny,
+                           planes, ps, kc, items);
+}
+
+// ps: plane stride of the spectrum-side operand (>= planes); the Stockham
+// kernels for Nx < 64 take ps == planes only.
 template <typename R, int NX>
-void launch_row(bool inverse, const void* in, void* out, const void* tw, int ny, int planes, int kc, int req,
+void launch_row(bool inverse, const void* in, void* out, const void* tw, int ny, int planes, int ps, int kc, int req,
                 hipStream_t stream) {
     if constexpr (NX >= 64) {
         using G = Geo2<R, NX>;
         constexpr int TP = 2 * G::PAIRS;
         const int items = ny * ((planes + TP - 1) / TP);
-        const dim3 grid(launch_grid(items, req));
-        if (inverse)
-            hipLaunchKernelGGL((row2_c2r_kernel<R, NX>), grid, dim3(G::THREADS), 0, stream,
-                               static_cast<const cx<R>*>(in), static_cast<R*>(out), static_cast<const cx<R>*>(tw), ny,
-                               planes, kc, items);
-        else
-            hipLaunchKernelGGL((row2_r2c_kernel<R, NX>), grid, dim3(G::THREADS), 0, stream,
-                               static_cast<const R*>(in), static_cast<cx<R>*>(out), static_cast<const cx<R>*>(tw), ny,
-                               planes, kc, items);
+        const int grid = launch_grid(items, req);
+        if (grid < items) launch_row2<R, NX, true>(inverse, in, out, tw, ny, planes, ps, kc, items, grid, stream);
+        else launch_row2<R, NX, false>(inverse, in, out, tw, ny, planes, ps, kc, items, grid, stream);
     } else {
+        if (ps != planes) fail(SKB_RUNTIME_ERROR, "padded plane stride needs Nx >= 64");
         constexpr int TP = row_tile<R>();
         const dim3 grid(ny, (planes + TP - 1) / TP);
         if (inverse)
@@ -786,8 +840,33 @@ enum class ColVariant { FwdStrided, FwdToRetained, InvStrided, InvFromRetained }
 // Stage indices of the pruned plan (skb_pruned_plan_set_grid).
 constexpr int kStageForwardRows = 0, kStageForwardColumns = 1, kStageInverseColumns = 2, kStageInverseRows = 3;
 
+template <typename R, int NY, bool LOOP>
+void launch_col2(ColVariant v, const cx<R>* i, cx<R>* o, const cx<R>* t, int planes, int ps, int kx_count, int items,
+                 int in_kc, int out_kc, const int* map, int grid_x, hipStream_t stream) {
+    using G = Geo2<R, NY>;
+    const dim3 grid(grid_x), block(G::THREADS);
+    switch (v) {
+        case ColVariant::FwdStrided:
+            hipLaunchKernelGGL((col2_kernel<R, NY, false, kStrided, kStrided, LOOP>), grid, block, 0, stream, i, o, t,
+                               planes, ps, kx_count, items, in_kc, out_kc, map);
+            break;
+        case ColVariant::FwdToRetained:
+            hipLaunchKernelGGL((col2_kernel<R, NY, false, kStrided, kRetained, LOOP>), grid, block, 0, stream, i, o, t,
+                               planes, ps, kx_count, items, in_kc, out_kc, map);
+            break;
+        case ColVariant::InvStrided:
+            hipLaunchKernelGGL((col2_kernel<R, NY, true, kStrided, kStrided, LOOP>), grid, block, 0, stream, i, o, t,
+                               planes, ps, kx_count, items, in_kc, out_kc, map);
+            break;
+        case ColVariant::InvFromRetained:
+            hipLaunchKernelGGL((col2_kernel<R, NY, true, kRetained, kStrided, LOOP>), grid, block, 0, stream, i, o, t,
+                               planes, ps, kx_count, items, in_kc, out_kc, map);
+            break;
+    }
+}
+
 template <typename R, int NY>
-void launch_col(ColVariant v, const void* in, void* out, const void* tw, int planes, int kx_count, int in_kc,
+void launch_col(ColVariant v, const void* in, void* out, const void* tw, int planes, int ps, int kx_count, int in_kc,
                 int out_kc, const int* col_start, const ColEntry* entries, const int* map, int req,
                 hipStream_t stream) {
     const auto* i = static_cast<const cx<R>*>(in);
@@ -796,26 +875,11 @@ void launch_col(ColVariant v, const void* in, void* out, const void* tw, int pla
     if constexpr (NY >= 64) {
         using G = Geo2<R, NY>;
         const int items = kx_count * ((planes + G::TP - 1) / G::TP);
-        const dim3 grid(launch_grid(items, req)), block(G::THREADS);
-        switch (v) {
-            case ColVariant::FwdStrided:
-                hipLaunchKernelGGL((col2_kernel<R, NY, false, kStrided, kStrided>), grid, block, 0, stream, i, o, t,
-                                   planes, kx_count, items, in_kc, out_kc, map);
-                break;
-            case ColVariant::FwdToRetained:
-                hipLaunchKernelGGL((col2_kernel<R, NY, false, kStrided, kRetained>), grid, block, 0, stream, i, o, t,
-                                   planes, kx_count, items, in_kc, out_kc, map);
-                break;
-            case ColVariant::InvStrided:
-                hipLaunchKernelGGL((col2_kernel<R, NY, true, kStrided, kStrided>), grid, block, 0, stream, i, o, t,
-                                   planes, kx_count, items, in_kc, out_kc, map);
-                break;
-            case ColVariant::InvFromRetained:
-                hipLaunchKernelGGL((col2_kernel<R, NY, true, kRetained, kStrided>), grid, block, 0, stream, i, o, t,
-                                   planes, kx_count, items, in_kc, out_kc, map);
-                break;
-        }
+        const int grid = launch_grid(items, req);
+        if (grid < items) launch_col2<R, NY, true>(v, i, o, t, planes, ps, kx_count, items, in_kc, out_kc, map, grid, stream);
+        else launch_col2<R, NY, false>(v, i, o, t, planes, ps, kx_count, items, in_kc, out_kc, map, grid, stream);
     } else {
+        if (ps != planes) fail(SKB_RUNTIME_ERROR, "padded plane stride needs Ny >= 64");
         constexpr int TP = col_tile<R, NY>();
         const dim3 grid(kx_count, (planes + TP - 1) / TP);
         switch (v) {
@@ -844,12 +908,12 @@ void launch_col(ColVariant v, const void* in, void* out, const void* tw, int pla
     MACRO(4) MACRO(8) MACRO(16) MACRO(32) MACRO(64) MACRO(128) MACRO(256) MACRO(512) MACRO(1024)
 
 template <typename R>
-void row_pass(bool inverse, int nx, const void* in, void* out, const void* tw, int ny, int planes, int kc, int req,
-              hipStream_t s) {
+void row_pass(bool inverse, int nx, const void* in, void* out, const void* tw, int ny, int planes, int ps, int kc,
+              int req, hipStream_t s) {
     switch (nx) {
-#define SKB_ROW_CASE(N)                                                     \
-    case N:                                                                 \
-        launch_row<R, N>(inverse, in, out, tw, ny, planes, kc, req, s);     \
+#define SKB_ROW_CASE(N)                                                         \
+    case N:                                                                     \
+        launch_row<R, N>(inverse, in, out, tw, ny, planes, ps, kc, req, s);     \
         return;
         SKB_POW2_CASES(SKB_ROW_CASE)
 #undef SKB_ROW_CASE
@@ -858,12 +922,13 @@ void row_pass(bool inverse, int nx, const void* in, void* out, const void* tw, i
 }
 
 template <typename R>
-void col_pass(ColVariant v, int ny, const void* in, void* out, const void* tw, int planes, int kx_count, int in_kc,
-              int out_kc, const int* col_start, const ColEntry* entries, const int* map, int req, hipStream_t s) {
+void col_pass(ColVariant v, int ny, const void* in, void* out, const void* tw, int planes, int ps, int kx_count,
+              int in_kc, int out_kc, const int* col_start, const ColEntry* entries, const int* map, int req,
+              hipStream_t s) {
     switch (ny) {
-#define SKB_COL_CASE(N)                                                                                       \
-    case N:                                                                                                   \
-        launch_col<R, N>(v, in, out, tw, planes, kx_count, in_kc, out_kc, col_start, entries, map, req, s);   \
+#define SKB_COL_CASE(N)                                                                                           \
+    case N:                                                                                                       \
+        launch_col<R, N>(v, in, out, tw, planes, ps, kx_count, in_kc, out_kc, col_start, entries, map, req, s);   \
         return;
         SKB_POW2_CASES(SKB_COL_CASE)
 #undef SKB_COL_CASE
@@ -982,10 +1047,16 @@ struct skb_pruned_plan {
     int grid[4] = {0, 0, 0, 0};  // per-stage workgroup count (0 = one per item)
     int precision = SKB_F64;
     uint64_t modes = 0;
+    int planes = 0;  // P = Nz * fields
+    int kc = 0;      // active k_x band
+    // Plane stride of T: P rounded up to whole 128-byte lines (16 FP32 or 8
+    // FP64 complex values) when both passes use the two-pass kernels, so every
+    // plane tile of the column and row kernels starts on a cache line.
+    int ps = 0;
     skb::Twiddles tw;
     skb::ColumnLists lists;
     skb::DeviceBuffer fwd_start, inv_start, fwd, inv, fmap, imap;
-    skb::DeviceBuffer scratch;  // T: kc * Ny * P complex
+    skb::DeviceBuffer scratch;  // T: kc * Ny * ps complex, element (kx, y, p) at ps * (kx + kc * y) + p
 };
 
 using namespace skb;
@@ -1025,9 +1096,10 @@ void fft_forward(skb_fft_plan* p, const void* real, void* wvm, hipStream_t s) {
     const auto& w = p->w;
     const int planes = static_cast<int>(w.nz * w.fields);
     const int nxh = static_cast<int>(w.nx / 2 + 1);
-    row_pass<R>(false, static_cast<int>(w.nx), real, wvm, p->tw.x.ptr, static_cast<int>(w.ny), planes, nxh, 0, s);
-    col_pass<R>(ColVariant::FwdStrided, static_cast<int>(w.ny), wvm, wvm, p->tw.y.ptr, planes, nxh, nxh, nxh, nullptr,
-                nullptr, nullptr, 0, s);
+    row_pass<R>(false, static_cast<int>(w.nx), real, wvm, p->tw.x.ptr, static_cast<int>(w.ny), planes, planes, nxh, 0,
+                s);
+    col_pass<R>(ColVariant::FwdStrided, static_cast<int>(w.ny), wvm, wvm, p->tw.y.ptr, planes, planes, nxh, nxh, nxh,
+                nullptr, nullptr, nullptr, 0, s);
 }
 
 template <typename R>
@@ -1035,32 +1107,50 @@ void fft_inverse(skb_fft_plan* p, void* wvm, void* real, hipStream_t s) {
     const auto& w = p->w;
     const int planes = static_cast<int>(w.nz * w.fields);
     const int nxh = static_cast<int>(w.nx / 2 + 1);
-    col_pass<R>(ColVariant::InvStrided, static_cast<int>(w.ny), wvm, wvm, p->tw.y.ptr, planes, nxh, nxh, nxh, nullptr,
-                nullptr, nullptr, 0, s);
-    row_pass<R>(true, static_cast<int>(w.nx), wvm, real, p->tw.x.ptr, static_cast<int>(w.ny), planes, nxh, 0, s);
+    col_pass<R>(ColVariant::InvStrided, static_cast<int>(w.ny), wvm, wvm, p->tw.y.ptr, planes, planes, nxh, nxh, nxh,
+                nullptr, nullptr, nullptr, 0, s);
+    row_pass<R>(true, static_cast<int>(w.nx), wvm, real, p->tw.x.ptr, static_cast<int>(w.ny), planes, planes, nxh, 0,
+                s);
+}
+
+// The four stages of the pruned transforms; T is the plan's kc x Ny x ps
+// intermediate (plane stride ps, see skb_pruned_plan).
+template <typename R>
+void stage_forward_rows(skb_pruned_plan* p, const void* real, hipStream_t s) {
+    const auto& w = p->w;
+    row_pass<R>(false, static_cast<int>(w.nx), real, p->scratch.ptr, p->tw.x.ptr, static_cast<int>(w.ny), p->planes,
+                p->ps, p->kc, p->grid[kStageForwardRows], s);
 }
 
 template <typename R>
-void pruned_forward(skb_pruned_plan* p, const void* real, void* retained, hipStream_t s) {
-    const auto& w = p->w;
-    const int planes = static_cast<int>(w.nz * w.fields);
-    const int kc = static_cast<int>(p->lists.kc);
-    row_pass<R>(false, static_cast<int>(w.nx), real, p->scratch.ptr, p->tw.x.ptr, static_cast<int>(w.ny), planes, kc,
-                p->grid[kStageForwardRows], s);
-    col_pass<R>(ColVariant::FwdToRetained, static_cast<int>(w.ny), p->scratch.ptr, retained, p->tw.y.ptr, planes, kc, kc,
-                kc, p->fwd_start.as<int>(), p->fwd.as<ColEntry>(), p->fmap.as<int>(), p->grid[kStageForwardColumns], s);
+void stage_forward_columns(skb_pruned_plan* p, void* retained, hipStream_t s) {
+    col_pass<R>(ColVariant::FwdToRetained, static_cast<int>(p->w.ny), p->scratch.ptr, retained, p->tw.y.ptr, p->planes,
+                p->ps, p->kc, p->kc, p->kc, p->fwd_start.as<int>(), p->fwd.as<ColEntry>(), p->fmap.as<int>(),
+                p->grid[kStageForwardColumns], s);
 }
 
 template <typename R>
-void pruned_inverse(skb_pruned_plan* p, const void* retained, void* real, hipStream_t s) {
-    const auto& w = p->w;
-    const int planes = static_cast<int>(w.nz * w.fields);
-    const int kc = static_cast<int>(p->lists.kc);
-    col_pass<R>(ColVariant::InvFromRetained, static_cast<int>(w.ny), retained, p->scratch.ptr, p->tw.y.ptr, planes, kc,
-                kc, kc, p->inv_start.as<int>(), p->inv.as<ColEntry>(), p->imap.as<int>(), p->grid[kStageInverseColumns], s);
-    row_pass<R>(true, static_cast<int>(w.nx), p->scratch.ptr, real, p->tw.x.ptr, static_cast<int>(w.ny), planes, kc,
-                p->grid[kStageInverseRows], s);
+void stage_inverse_columns(skb_pruned_plan* p, const void* retained, hipStream_t s) {
+    col_pass<R>(ColVariant::InvFromRetained, static_cast<int>(p->w.ny), retained, p->scratch.ptr, p->tw.y.ptr,
+                p->planes, p->ps, p->kc, p->kc, p->kc, p->inv_start.as<int>(), p->inv.as<ColEntry>(),
+                p->imap.as<int>(), p->grid[kStageInverseColumns], s);
 }
+
+template <typename R>
+void stage_inverse_rows(skb_pruned_plan* p, void* real, hipStream_t s) {
+    const auto& w = p->w;
+    row_pass<R>(true, static_cast<int>(w.nx), p->scratch.ptr, real, p->tw.x.ptr, static_cast<int>(w.ny), p->planes,
+                p->ps, p->kc, p->grid[kStageInverseRows], s);
+}
+
+// Runs stage F<float> or F<double> by the plan's precision.
+#define SKB_BY_PRECISION(plan, F, ...)                \
+    do {                                              \
+        if ((plan)->precision == SKB_F64)             \
+            F<double>(__VA_ARGS__);                   \
+        else                                          \
+            F<float>(__VA_ARGS__);                    \
+    } while (0)
 
 void require_ptrs(const void* a, const void* b) {
     if (!a || !b) fail(SKB_INVALID_ARGUMENT, "null device pointer");
@@ -1181,7 +1271,11 @@ int skb_pruned_plan_create(const skb_workload* workload, const skb_retained_mode
         p->inv.upload(p->lists.inv.data(), p->lists.inv.size() * sizeof(ColEntry));
         p->fmap.upload(p->lists.fmap.data(), p->lists.fmap.size() * sizeof(int));
         p->imap.upload(p->lists.imap.data(), p->lists.imap.size() * sizeof(int));
-        p->scratch.allocate(p->lists.kc * workload->ny * workload->nz * workload->fields * complex_bytes(precision));
+        p->planes = static_cast<int>(workload->nz * workload->fields);
+        p->kc = static_cast<int>(p->lists.kc);
+        const int line = precision == SKB_F64 ? 8 : 16;
+        p->ps = workload->nx >= 64 && workload->ny >= 64 ? (p->planes + line - 1) / line * line : p->planes;
+        p->scratch.allocate(p->lists.kc * workload->ny * static_cast<uint64_t>(p->ps) * complex_bytes(precision));
         *plan = p.release();
     });
 }
@@ -1194,8 +1288,8 @@ int skb_pruned_forward(skb_pruned_plan* plan, const void* real, void* retained, 
     return guarded([&] {
         if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
         require_ptrs(real, retained);
-        if (plan->precision == SKB_F64) pruned_forward<double>(plan, real, retained, as_stream(stream));
-        else pruned_forward<float>(plan, real, retained, as_stream(stream));
+        SKB_BY_PRECISION(plan, stage_forward_rows, plan, real, as_stream(stream));
+        SKB_BY_PRECISION(plan, stage_forward_columns, plan, retained, as_stream(stream));
     });
 }
 
@@ -1203,68 +1297,36 @@ int skb_pruned_inverse(skb_pruned_plan* plan, const void* retained, void* real, 
     return guarded([&] {
         if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
         require_ptrs(real, retained);
-        if (plan->precision == SKB_F64) pruned_inverse<double>(plan, retained, real, as_stream(stream));
-        else pruned_inverse<float>(plan, retained, real, as_stream(stream));
+        SKB_BY_PRECISION(plan, stage_inverse_columns, plan, retained, as_stream(stream));
+        SKB_BY_PRECISION(plan, stage_inverse_rows, plan, real, as_stream(stream));
     });
 }
 
 int skb_pruned_forward_rows(skb_pruned_plan* plan, const void* real, void* stream) {
     return guarded([&] {
         if (!plan || !real) fail(SKB_INVALID_ARGUMENT, "null argument");
-        const auto& w = plan->w;
-        const int planes = static_cast<int>(w.nz * w.fields), kc = static_cast<int>(plan->lists.kc);
-        if (plan->precision == SKB_F64)
-            row_pass<double>(false, static_cast<int>(w.nx), real, plan->scratch.ptr, plan->tw.x.ptr,
-                             static_cast<int>(w.ny), planes, kc, plan->grid[kStageForwardRows], as_stream(stream));
-        else
-            row_pass<float>(false, static_cast<int>(w.nx), real, plan->scratch.ptr, plan->tw.x.ptr,
-                            static_cast<int>(w.ny), planes, kc, plan->grid[kStageForwardRows], as_stream(stream));
+        SKB_BY_PRECISION(plan, stage_forward_rows, plan, real, as_stream(stream));
     });
 }
 
 int skb_pruned_forward_columns(skb_pruned_plan* plan, void* retained, void* stream) {
     return guarded([&] {
         if (!plan || !retained) fail(SKB_INVALID_ARGUMENT, "null argument");
-        const auto& w = plan->w;
-        const int planes = static_cast<int>(w.nz * w.fields), kc = static_cast<int>(plan->lists.kc);
-        if (plan->precision == SKB_F64)
-            col_pass<double>(ColVariant::FwdToRetained, static_cast<int>(w.ny), plan->scratch.ptr, retained,
-                             plan->tw.y.ptr, planes, kc, kc, kc, plan->fwd_start.as<int>(), plan->fwd.as<ColEntry>(),
-                             plan->fmap.as<int>(), plan->grid[kStageForwardColumns], as_stream(stream));
-        else
-            col_pass<float>(ColVariant::FwdToRetained, static_cast<int>(w.ny), plan->scratch.ptr, retained,
-                            plan->tw.y.ptr, planes, kc, kc, kc, plan->fwd_start.as<int>(), plan->fwd.as<ColEntry>(),
-                            plan->fmap.as<int>(), plan->grid[kStageForwardColumns], as_stream(stream));
+        SKB_BY_PRECISION(plan, stage_forward_columns, plan, retained, as_stream(stream));
     });
 }
 
 int skb_pruned_inverse_columns(skb_pruned_plan* plan, const void* retained, void* stream) {
     return guarded([&] {
         if (!plan || !retained) fail(SKB_INVALID_ARGUMENT, "null argument");
-        const auto& w = plan->w;
-        const int planes = static_cast<int>(w.nz * w.fields), kc = static_cast<int>(plan->lists.kc);
-        if (plan->precision == SKB_F64)
-            col_pass<double>(ColVariant::InvFromRetained, static_cast<int>(w.ny), retained, plan->scratch.ptr,
-                             plan->tw.y.ptr, planes, kc, kc, kc, plan->inv_start.as<int>(), plan->inv.as<ColEntry>(),
-                             plan->imap.as<int>(), plan->grid[kStageInverseColumns], as_stream(stream));
-        else
-            col_pass<float>(ColVariant::InvFromRetained, static_cast<int>(w.ny), retained, plan->scratch.ptr,
-                            plan->tw.y.ptr, planes, kc, kc, kc, plan->inv_start.as<int>(), plan->inv.as<ColEntry>(),
-                            plan->imap.as<int>(), plan->grid[kStageInverseColumns], as_stream(stream));
+        SKB_BY_PRECISION(plan, stage_inverse_columns, plan, retained, as_stream(stream));
     });
 }
 
 int skb_pruned_inverse_rows(skb_pruned_plan* plan, void* real, void* stream) {
     return guarded([&] {
         if (!plan || !real) fail(SKB_INVALID_ARGUMENT, "null argument");
-        const auto& w = plan->w;
-        const int planes = static_cast<int>(w.nz * w.fields), kc = static_cast<int>(plan->lists.kc);
-        if (plan->precision == SKB_F64)
-            row_pass<double>(true, static_cast<int>(w.nx), plan->scratch.ptr, real, plan->tw.x.ptr,
-                             static_cast<int>(w.ny), planes, kc, plan->grid[kStageInverseRows], as_stream(stream));
-        else
-            row_pass<float>(true, static_cast<int>(w.nx), plan->scratch.ptr, real, plan->tw.x.ptr,
-                            static_cast<int>(w.ny), planes, kc, plan->grid[kStageInverseRows], as_stream(stream));
+        SKB_BY_PRECISION(plan, stage_inverse_rows, plan, real, as_stream(stream));
     });
 }
 
