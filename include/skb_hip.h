@@ -150,6 +150,33 @@ int skb_pruned_inverse(skb_pruned_plan* plan, const void* retained, void* real, 
  * include/skbench/skbench.hpp:478-489).  The intermediate lives in the
  * plan's scratch, so a forward_rows must precede forward_columns and an
  * inverse_columns must precede inverse_rows on the same stream. */
+/* Split compact form: separate real and imaginary arrays.
+ * forward_split / inverse_split: element (plane, mode) at plane + P*mode
+ * (FFTWPrunedProvider::forwardSplit / inverseSplit and
+ * FFTWStreamingPrunedSplitProvider::forwardSplit / inverseSplit,
+ * include/skbench/skbench.hpp:484-492, 563-566;
+ * src/pruned_fftw_provider.cpp:1053-1066).  Forward outputs are multiplied
+ * by `scale`. */
+int skb_pruned_forward_split(skb_pruned_plan* plan, const void* real, void* retained_real, void* retained_imag,
+                             double scale, void* stream);
+int skb_pruned_inverse_split(skb_pruned_plan* plan, const void* retained_real, const void* retained_imag, void* real,
+                             void* stream);
+/* One field of a split compact array: element (z, mode) of the field at
+ * real[z + mode_stride * mode] and imaginary[...], device pointers in the
+ * plan's precision, mode_stride >= Nz (FieldView / ConstFieldView,
+ * include/skbench/skbench.hpp:536-546; src/pruned_fftw_provider.cpp:1068-1083,
+ * 1176-1196).  field_count must equal the workload's fields (at most 16). */
+typedef struct skb_field_view {
+    void* real;
+    void* imaginary;
+    uint64_t mode_stride;
+} skb_field_view;
+/* FFTWStreamingPrunedSplitProvider::forwardSplitFields / inverseSplitFields
+ * (include/skbench/skbench.hpp:567-570; src/pruned_fftw_provider.cpp:787-805). */
+int skb_pruned_forward_split_fields(skb_pruned_plan* plan, const void* real, const skb_field_view* fields,
+                                    uint64_t field_count, double scale, void* stream);
+int skb_pruned_inverse_split_fields(skb_pruned_plan* plan, const skb_field_view* fields, uint64_t field_count,
+                                    void* real, void* stream);
 int skb_pruned_forward_rows(skb_pruned_plan* plan, const void* real, void* stream);
 int skb_pruned_forward_columns(skb_pruned_plan* plan, void* retained, void* stream);
 int skb_pruned_inverse_columns(skb_pruned_plan* plan, const void* retained, void* stream);

@@ -241,13 +241,49 @@ __global__ __launch_bounds__(kThreads) void row_c2r_kernel(const cx<R>* __restri
 // ---------------------------------------------------------------------------
 // Column pass: c2c along y for column kx = blockIdx.x of `tile` planes.
 // ---------------------------------------------------------------------------
-enum ColIO { kStrided = 0, kRetained = 1 };
+enum ColIO { kStrided = 0, kRetained = 1, kSplit = 2 };
+
+// Compact retained operand in split form (kSplit): separate real and
+// imaginary arrays, one view per field.  Plane p = z + nz * f of mode m lives
+// at re[f][z + stride[f] * m] and im[f][...].  The plain split layout
+// (plane + P * mode) is one view with nz = P and stride = P.  Forward outputs
+// are multiplied by `scale` (src/pruned_fftw_provider.cpp:1053-1083).
+constexpr int kMaxSplitViews = 16;
+template <typename R>
+struct SplitViews {
+    R* re[kMaxSplitViews];
+    R* im[kMaxSplitViews];
+    long long stride[kMaxSplitViews];
+    int nz;
+    R scale;
+};
+
+// The view of plane p, with z folded into the pointers (lanes hold one plane).
+template <typename R>
+struct ViewLane {
+    R* re;
+    R* im;
+    size_t stride;
+};
+template <typename R>
+__device__ __forceinline__ ViewLane<R> view_lane(const SplitViews<R>& v, int p) {
+    const int f = p / v.nz, z = p - f * v.nz;
+    ViewLane<R> out{v.re[0], v.im[0], static_cast<size_t>(v.stride[0])};
+    // unrolled select: a lane-varying index into the argument array would go
+    // through private memory
+#pragma unroll
+    for (int i = 1; i < kMaxSplitViews; ++i)
+        if (f == i) out = ViewLane<R>{v.re[i], v.im[i], static_cast<size_t>(v.stride[i])};
+    out.re += z;
+    out.im += z;
+    return out;
+}
 
 template <typename R, int NY, bool INV, int IN, int OUT>
 __global__ __launch_bounds__(kThreads) void col_kernel(const cx<R>* in, cx<R>* out, const cx<R>* __restrict__ tw,
                                                       int planes, int in_kc, int out_kc,
                                                       const int* __restrict__ col_start,
-                                                      const ColEntry* __restrict__ entries) {
+                                                      const ColEntry* __restrict__ entries, const SplitViews<R> views) {
     constexpr int TP = col_tile<R, NY>();
     constexpr int SEQ = NY + 1;  // one-element pad: plane-fastest LDS writes hit distinct banks
     __shared__ cx<R> buf[TP * SEQ];
@@ -271,7 +307,13 @@ __global__ __launch_bounds__(kThreads) void col_kernel(const cx<R>* in, cx<R>* o
             const ColEntry ent = entries[beg + e / TP];
             const int p = p0 + pit;
             if (p >= planes) continue;
-            cx<R> v = in[P * ent.mode + p];
+            cx<R> v;
+            if constexpr (IN == kSplit) {
+                const ViewLane<R> vl = view_lane(views, p);
+                v = cx<R>{vl.re[vl.stride * ent.mode], vl.im[vl.stride * ent.mode]};
+            } else {
+                v = in[P * ent.mode + p];
+            }
             if (ent.conj) v.y = -v.y;
             buf[pit * SEQ + ent.row] = v;
         }
@@ -293,7 +335,13 @@ __global__ __launch_bounds__(kThreads) void col_kernel(const cx<R>* in, cx<R>* o
             if (p >= planes) continue;
             cx<R> v = buf[pit * SEQ + ent.row];
             if (ent.conj) v.y = -v.y;
-            out[P * ent.mode + p] = v;
+            if constexpr (OUT == kSplit) {
+                const ViewLane<R> vl = view_lane(views, p);
+                vl.re[vl.stride * ent.mode] = views.scale * v.x;
+                vl.im[vl.stride * ent.mode] = views.scale * v.y;
+            } else {
+                out[P * ent.mode + p] = v;
+            }
         }
     }
 }
@@ -626,7 +674,7 @@ __global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void row2_c2r_kernel(const 
 // Maps: code = 2*mode + conj at [kx*NY + row], or -1 (no mode).
 template <typename R, int NY, int IN>
 __device__ __forceinline__ void col2_load(cx<R>* v, const cx<R>* in, int item, int kx_count, int planes, int ps,
-                                          int in_kc, const int* __restrict__ map) {
+                                          int in_kc, const int* __restrict__ map, const SplitViews<R>& views) {
     using G = Geo2<R, NY>;
     constexpr int N1 = G::N1, N2 = G::N2, TP = G::TP;
     const int t = threadIdx.x, pp = t % TP, n2 = t / TP;
@@ -645,11 +693,20 @@ __device__ __forceinline__ void col2_load(cx<R>* v, const cx<R>* in, int item, i
         int code[N1];
 #pragma unroll
         for (int n1 = 0; n1 < N1; ++n1) code[n1] = n2 < N2 ? map[kx * NY + N2 * n1 + n2] : -1;
+        ViewLane<R> vl{};
+        if constexpr (IN == kSplit) {
+            if (p < planes) vl = view_lane(views, p);
+        }
 #pragma unroll
         for (int n1 = 0; n1 < N1; ++n1) {
             cx<R> x{R(0), R(0)};
             if (code[n1] >= 0 && p < planes) {
-                x = in[P * (code[n1] >> 1) + p];
+                if constexpr (IN == kSplit) {
+                    const size_t at = vl.stride * static_cast<size_t>(code[n1] >> 1);
+                    x = cx<R>{vl.re[at], vl.im[at]};
+                } else {
+                    x = in[P * (code[n1] >> 1) + p];
+                }
                 if (code[n1] & 1) x.y = -x.y;
             }
             v[n1] = x;
@@ -661,7 +718,8 @@ template <typename R, int NY, bool INV, int IN, int OUT, bool LOOP>
 __global__ __launch_bounds__((Geo2<R, NY>::THREADS)) void col2_kernel(const cx<R>* in, cx<R>* out,
                                                                      const cx<R>* __restrict__ tw, int planes, int ps,
                                                                      int kx_count, int items, int in_kc, int out_kc,
-                                                                     const int* __restrict__ map) {
+                                                                     const int* __restrict__ map,
+                                                                     const SplitViews<R> views) {
     using G = Geo2<R, NY>;
     constexpr int N1 = G::N1, N2 = G::N2, TP = G::TP, NT = G::THREADS;
     __shared__ cx<R> buf[NY * TP];
@@ -672,7 +730,7 @@ __global__ __launch_bounds__((Geo2<R, NY>::THREADS)) void col2_kernel(const cx<R
     int item = first_item<LOOP>(items);
     if (item >= items) return;  // uniform across the workgroup
     cx<R> v[N1];
-    col2_load<R, NY, IN>(v, in, item, kx_count, planes, ps, in_kc, map);
+    col2_load<R, NY, IN>(v, in, item, kx_count, planes, ps, in_kc, map, views);
     for (int i = t; i < NY; i += NT) twl[i] = tw[i];
     __syncthreads();
     const int pp = t % TP, n2 = t / TP, k1 = t / TP;
@@ -680,16 +738,20 @@ __global__ __launch_bounds__((Geo2<R, NY>::THREADS)) void col2_kernel(const cx<R
         const int next = LOOP ? item + static_cast<int>(gridDim.x) : items;
         cx<R> vn[LOOP ? N1 : 1];
         if constexpr (LOOP) {
-            if (next < items) col2_load<R, NY, IN>(vn, in, next, kx_count, planes, ps, in_kc, map);
+            if (next < items) col2_load<R, NY, IN>(vn, in, next, kx_count, planes, ps, in_kc, map, views);
         }
         const int kx = item / ntiles, p = (item % ntiles) * TP + pp;
         // Output codes are loaded with (not after) the item's inputs: on gfx9
         // one in-order counter tracks loads and stores, so a code loaded
         // between two stores would make the second store wait for the first.
         int ocode[N2];
-        if constexpr (OUT == kRetained) {
+        if constexpr (OUT != kStrided) {
 #pragma unroll
             for (int k2 = 0; k2 < N2; ++k2) ocode[k2] = map[kx * NY + k1 + N1 * k2];
+        }
+        ViewLane<R> ovl{};
+        if constexpr (OUT == kSplit) {
+            if (p < planes) ovl = view_lane(views, p);
         }
         // pass 1: thread (plane fastest, n2)
         if (n2 < N2) {
@@ -717,7 +779,13 @@ __global__ __launch_bounds__((Geo2<R, NY>::THREADS)) void col2_kernel(const cx<R
                     if (code >= 0) {
                         cx<R> x = w[k2];
                         if (code & 1) x.y = -x.y;
-                        out[P * (code >> 1) + p] = x;
+                        if constexpr (OUT == kSplit) {
+                            const size_t at = ovl.stride * static_cast<size_t>(code >> 1);
+                            ovl.re[at] = views.scale * x.x;
+                            ovl.im[at] = views.scale * x.y;
+                        } else {
+                            out[P * (code >> 1) + p] = x;
+                        }
                     }
                 }
             }
@@ -835,40 +903,34 @@ void launch_row(bool inverse, const void* in, void* out, const void* tw, int ny,
     check_launch(inverse ? "row c2r launch" : "row r2c launch");
 }
 
-enum class ColVariant { FwdStrided, FwdToRetained, InvStrided, InvFromRetained };
+enum class ColVariant { FwdStrided, FwdToRetained, FwdToSplit, InvStrided, InvFromRetained, InvFromSplit };
 
 // Stage indices of the pruned plan (skb_pruned_plan_set_grid).
 constexpr int kStageForwardRows = 0, kStageForwardColumns = 1, kStageInverseColumns = 2, kStageInverseRows = 3;
 
 template <typename R, int NY, bool LOOP>
 void launch_col2(ColVariant v, const cx<R>* i, cx<R>* o, const cx<R>* t, int planes, int ps, int kx_count, int items,
-                 int in_kc, int out_kc, const int* map, int grid_x, hipStream_t stream) {
+                 int in_kc, int out_kc, const int* map, const SplitViews<R>& views, int grid_x, hipStream_t stream) {
     using G = Geo2<R, NY>;
     const dim3 grid(grid_x), block(G::THREADS);
+#define SKB_COL2(INV, IN, OUT)                                                                                    \
+    hipLaunchKernelGGL((col2_kernel<R, NY, INV, IN, OUT, LOOP>), grid, block, 0, stream, i, o, t, planes, ps,     \
+                       kx_count, items, in_kc, out_kc, map, views)
     switch (v) {
-        case ColVariant::FwdStrided:
-            hipLaunchKernelGGL((col2_kernel<R, NY, false, kStrided, kStrided, LOOP>), grid, block, 0, stream, i, o, t,
-                               planes, ps, kx_count, items, in_kc, out_kc, map);
-            break;
-        case ColVariant::FwdToRetained:
-            hipLaunchKernelGGL((col2_kernel<R, NY, false, kStrided, kRetained, LOOP>), grid, block, 0, stream, i, o, t,
-                               planes, ps, kx_count, items, in_kc, out_kc, map);
-            break;
-        case ColVariant::InvStrided:
-            hipLaunchKernelGGL((col2_kernel<R, NY, true, kStrided, kStrided, LOOP>), grid, block, 0, stream, i, o, t,
-                               planes, ps, kx_count, items, in_kc, out_kc, map);
-            break;
-        case ColVariant::InvFromRetained:
-            hipLaunchKernelGGL((col2_kernel<R, NY, true, kRetained, kStrided, LOOP>), grid, block, 0, stream, i, o, t,
-                               planes, ps, kx_count, items, in_kc, out_kc, map);
-            break;
+        case ColVariant::FwdStrided: SKB_COL2(false, kStrided, kStrided); break;
+        case ColVariant::FwdToRetained: SKB_COL2(false, kStrided, kRetained); break;
+        case ColVariant::FwdToSplit: SKB_COL2(false, kStrided, kSplit); break;
+        case ColVariant::InvStrided: SKB_COL2(true, kStrided, kStrided); break;
+        case ColVariant::InvFromRetained: SKB_COL2(true, kRetained, kStrided); break;
+        case ColVariant::InvFromSplit: SKB_COL2(true, kSplit, kStrided); break;
     }
+#undef SKB_COL2
 }
 
 template <typename R, int NY>
 void launch_col(ColVariant v, const void* in, void* out, const void* tw, int planes, int ps, int kx_count, int in_kc,
-                int out_kc, const int* col_start, const ColEntry* entries, const int* map, int req,
-                hipStream_t stream) {
+                int out_kc, const int* col_start, const ColEntry* entries, const int* map,
+                const SplitViews<R>& views, int req, hipStream_t stream) {
     const auto* i = static_cast<const cx<R>*>(in);
     auto* o = static_cast<cx<R>*>(out);
     const auto* t = static_cast<const cx<R>*>(tw);
@@ -876,30 +938,26 @@ void launch_col(ColVariant v, const void* in, void* out, const void* tw, int pla
         using G = Geo2<R, NY>;
         const int items = kx_count * ((planes + G::TP - 1) / G::TP);
         const int grid = launch_grid(items, req);
-        if (grid < items) launch_col2<R, NY, true>(v, i, o, t, planes, ps, kx_count, items, in_kc, out_kc, map, grid, stream);
-        else launch_col2<R, NY, false>(v, i, o, t, planes, ps, kx_count, items, in_kc, out_kc, map, grid, stream);
+        if (grid < items)
+            launch_col2<R, NY, true>(v, i, o, t, planes, ps, kx_count, items, in_kc, out_kc, map, views, grid, stream);
+        else
+            launch_col2<R, NY, false>(v, i, o, t, planes, ps, kx_count, items, in_kc, out_kc, map, views, grid, stream);
     } else {
         if (ps != planes) fail(SKB_RUNTIME_ERROR, "padded plane stride needs Ny >= 64");
         constexpr int TP = col_tile<R, NY>();
         const dim3 grid(kx_count, (planes + TP - 1) / TP);
+#define SKB_COL(INV, IN, OUT)                                                                                     \
+    hipLaunchKernelGGL((col_kernel<R, NY, INV, IN, OUT>), grid, dim3(kThreads), 0, stream, i, o, t, planes, in_kc, \
+                       out_kc, col_start, entries, views)
         switch (v) {
-            case ColVariant::FwdStrided:
-                hipLaunchKernelGGL((col_kernel<R, NY, false, kStrided, kStrided>), grid, dim3(kThreads), 0, stream, i,
-                                   o, t, planes, in_kc, out_kc, col_start, entries);
-                break;
-            case ColVariant::FwdToRetained:
-                hipLaunchKernelGGL((col_kernel<R, NY, false, kStrided, kRetained>), grid, dim3(kThreads), 0, stream,
-                                   i, o, t, planes, in_kc, out_kc, col_start, entries);
-                break;
-            case ColVariant::InvStrided:
-                hipLaunchKernelGGL((col_kernel<R, NY, true, kStrided, kStrided>), grid, dim3(kThreads), 0, stream, i,
-                                   o, t, planes, in_kc, out_kc, col_start, entries);
-                break;
-            case ColVariant::InvFromRetained:
-                hipLaunchKernelGGL((col_kernel<R, NY, true, kRetained, kStrided>), grid, dim3(kThreads), 0, stream,
-                                   i, o, t, planes, in_kc, out_kc, col_start, entries);
-                break;
+            case ColVariant::FwdStrided: SKB_COL(false, kStrided, kStrided); break;
+            case ColVariant::FwdToRetained: SKB_COL(false, kStrided, kRetained); break;
+            case ColVariant::FwdToSplit: SKB_COL(false, kStrided, kSplit); break;
+            case ColVariant::InvStrided: SKB_COL(true, kStrided, kStrided); break;
+            case ColVariant::InvFromRetained: SKB_COL(true, kRetained, kStrided); break;
+            case ColVariant::InvFromSplit: SKB_COL(true, kSplit, kStrided); break;
         }
+#undef SKB_COL
     }
     check_launch("column launch");
 }
@@ -924,11 +982,12 @@ void row_pass(bool inverse, int nx, const void* in, void* out, const void* tw, i
 template <typename R>
 void col_pass(ColVariant v, int ny, const void* in, void* out, const void* tw, int planes, int ps, int kx_count,
               int in_kc, int out_kc, const int* col_start, const ColEntry* entries, const int* map, int req,
-              hipStream_t s) {
+              hipStream_t s, const SplitViews<R>& views = SplitViews<R>{}) {
     switch (ny) {
-#define SKB_COL_CASE(N)                                                                                           \
-    case N:                                                                                                       \
-        launch_col<R, N>(v, in, out, tw, planes, ps, kx_count, in_kc, out_kc, col_start, entries, map, req, s);   \
+#define SKB_COL_CASE(N)                                                                                        \
+    case N:                                                                                                    \
+        launch_col<R, N>(v, in, out, tw, planes, ps, kx_count, in_kc, out_kc, col_start, entries, map, views,  \
+                         req, s);                                                                              \
         return;
         SKB_POW2_CASES(SKB_COL_CASE)
 #undef SKB_COL_CASE
@@ -1143,6 +1202,58 @@ void stage_inverse_rows(skb_pruned_plan* p, void* real, hipStream_t s) {
                 p->ps, p->kc, p->grid[kStageInverseRows], s);
 }
 
+// Forward columns into / inverse columns from a split compact operand.
+template <typename R>
+void stage_forward_columns_split(skb_pruned_plan* p, const SplitViews<R>& views, hipStream_t s) {
+    col_pass<R>(ColVariant::FwdToSplit, static_cast<int>(p->w.ny), p->scratch.ptr, nullptr, p->tw.y.ptr, p->planes,
+                p->ps, p->kc, p->kc, p->kc, p->fwd_start.as<int>(), p->fwd.as<ColEntry>(), p->fmap.as<int>(),
+                p->grid[kStageForwardColumns], s, views);
+}
+
+template <typename R>
+void stage_inverse_columns_split(skb_pruned_plan* p, const SplitViews<R>& views, hipStream_t s) {
+    col_pass<R>(ColVariant::InvFromSplit, static_cast<int>(p->w.ny), nullptr, p->scratch.ptr, p->tw.y.ptr, p->planes,
+                p->ps, p->kc, p->kc, p->kc, p->inv_start.as<int>(), p->inv.as<ColEntry>(), p->imap.as<int>(),
+                p->grid[kStageInverseColumns], s, views);
+}
+
+// The plain split layout (plane + P * mode) as one view.
+template <typename R>
+SplitViews<R> plain_split(const void* re, const void* im, int planes, double scale) {
+    if (!re || !im) fail(SKB_INVALID_ARGUMENT, "null split array");
+    SplitViews<R> v{};
+    v.re[0] = static_cast<R*>(const_cast<void*>(re));
+    v.im[0] = static_cast<R*>(const_cast<void*>(im));
+    v.stride[0] = planes;
+    v.nz = planes;
+    v.scale = static_cast<R>(scale);
+    return v;
+}
+
+// One view per field; validation follows src/pruned_fftw_provider.cpp:929-958.
+template <typename R>
+SplitViews<R> field_split(const skb_pruned_plan* p, const skb_field_view* fields, uint64_t count, double scale,
+                          bool output) {
+    if (!fields || count != p->w.fields)
+        fail(SKB_INVALID_ARGUMENT, output ? "Streaming pruned output field views must match the workload field count."
+                                          : "Streaming pruned input field views must match the workload field count.");
+    if (count > static_cast<uint64_t>(kMaxSplitViews))
+        fail(SKB_UNSUPPORTED, "split field views support at most 16 fields");
+    SplitViews<R> v{};
+    for (uint64_t f = 0; f < count; ++f) {
+        if (!fields[f].real || !fields[f].imaginary || fields[f].mode_stride < p->w.nz)
+            fail(SKB_INVALID_ARGUMENT, output ? "Streaming pruned output field view is invalid."
+                                              : "Streaming pruned input field view is invalid.");
+        if (fields[f].mode_stride > (1ull << 40)) fail(SKB_UNSUPPORTED, "field view mode stride too large");
+        v.re[f] = static_cast<R*>(fields[f].real);
+        v.im[f] = static_cast<R*>(fields[f].imaginary);
+        v.stride[f] = static_cast<long long>(fields[f].mode_stride);
+    }
+    v.nz = static_cast<int>(p->w.nz);
+    v.scale = static_cast<R>(scale);
+    return v;
+}
+
 // Runs stage F<float> or F<double> by the plan's precision.
 #define SKB_BY_PRECISION(plan, F, ...)                \
     do {                                              \
@@ -1299,6 +1410,76 @@ int skb_pruned_inverse(skb_pruned_plan* plan, const void* retained, void* real, 
         require_ptrs(real, retained);
         SKB_BY_PRECISION(plan, stage_inverse_columns, plan, retained, as_stream(stream));
         SKB_BY_PRECISION(plan, stage_inverse_rows, plan, real, as_stream(stream));
+    });
+}
+
+int skb_pruned_forward_split(skb_pruned_plan* plan, const void* real, void* retained_real, void* retained_imag,
+                             double scale, void* stream) {
+    return guarded([&] {
+        if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
+        require_ptrs(real, retained_real);
+        const hipStream_t s = as_stream(stream);
+        if (plan->precision == SKB_F64) {
+            const auto v = plain_split<double>(retained_real, retained_imag, plan->planes, scale);
+            stage_forward_rows<double>(plan, real, s);
+            stage_forward_columns_split<double>(plan, v, s);
+        } else {
+            const auto v = plain_split<float>(retained_real, retained_imag, plan->planes, scale);
+            stage_forward_rows<float>(plan, real, s);
+            stage_forward_columns_split<float>(plan, v, s);
+        }
+    });
+}
+
+int skb_pruned_inverse_split(skb_pruned_plan* plan, const void* retained_real, const void* retained_imag, void* real,
+                             void* stream) {
+    return guarded([&] {
+        if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
+        require_ptrs(real, retained_real);
+        const hipStream_t s = as_stream(stream);
+        if (plan->precision == SKB_F64) {
+            const auto v = plain_split<double>(retained_real, retained_imag, plan->planes, 1.0);
+            stage_inverse_columns_split<double>(plan, v, s);
+            stage_inverse_rows<double>(plan, real, s);
+        } else {
+            const auto v = plain_split<float>(retained_real, retained_imag, plan->planes, 1.0);
+            stage_inverse_columns_split<float>(plan, v, s);
+            stage_inverse_rows<float>(plan, real, s);
+        }
+    });
+}
+
+int skb_pruned_forward_split_fields(skb_pruned_plan* plan, const void* real, const skb_field_view* fields,
+                                    uint64_t field_count, double scale, void* stream) {
+    return guarded([&] {
+        if (!plan || !real) fail(SKB_INVALID_ARGUMENT, "null argument");
+        const hipStream_t s = as_stream(stream);
+        if (plan->precision == SKB_F64) {
+            const auto v = field_split<double>(plan, fields, field_count, scale, true);
+            stage_forward_rows<double>(plan, real, s);
+            stage_forward_columns_split<double>(plan, v, s);
+        } else {
+            const auto v = field_split<float>(plan, fields, field_count, scale, true);
+            stage_forward_rows<float>(plan, real, s);
+            stage_forward_columns_split<float>(plan, v, s);
+        }
+    });
+}
+
+int skb_pruned_inverse_split_fields(skb_pruned_plan* plan, const skb_field_view* fields, uint64_t field_count,
+                                    void* real, void* stream) {
+    return guarded([&] {
+        if (!plan || !real) fail(SKB_INVALID_ARGUMENT, "null argument");
+        const hipStream_t s = as_stream(stream);
+        if (plan->precision == SKB_F64) {
+            const auto v = field_split<double>(plan, fields, field_count, 1.0, false);
+            stage_inverse_columns_split<double>(plan, v, s);
+            stage_inverse_rows<double>(plan, real, s);
+        } else {
+            const auto v = field_split<float>(plan, fields, field_count, 1.0, false);
+            stage_inverse_columns_split<float>(plan, v, s);
+            stage_inverse_rows<float>(plan, real, s);
+        }
     });
 }
 

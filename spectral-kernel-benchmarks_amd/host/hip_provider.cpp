@@ -5,6 +5,7 @@
 
 #include <chrono>
 #include <complex>
+#include <cstddef>
 #include <cstring>
 #include <iomanip>
 #include <new>
@@ -326,7 +327,8 @@ struct HipPrunedProvider::Impl {
     std::size_t modes;
     skb_pruned_plan* plan = nullptr;
     Stream stream;
-    Staged real, retained;
+    Staged real, retained, split_re, split_im;
+    std::vector<double> host_split;  // interleave / deinterleave of the split adapters
 
     Impl(const Workload& wl, const std::vector<RetainedMode>& m, Precision p) : w(wl), modes(m.size()) {
         const auto c = to_c(m);
@@ -335,6 +337,9 @@ struct HipPrunedProvider::Impl {
         const bool single = p == Precision::float32;
         real.allocate(w.realElements(), single);
         retained.allocate(2 * modes * w.planes(), single);
+        split_re.allocate(modes * w.planes(), single);
+        split_im.allocate(modes * w.planes(), single);
+        host_split.resize(2 * modes * w.planes());
     }
     ~Impl() {
         if (plan) skb_pruned_plan_destroy(plan);
@@ -392,6 +397,55 @@ void HipPrunedProvider::executeInverseRows(double* output) {
     m.real.download(output, m.stream.s);
 }
 
+// gatherForwardSplit / embedInverseSplit act on the interleaved device
+// result of executeForwardColumns / input of executeInverseColumns; the split
+// is done on the host side of the adapter (src/pruned_fftw_provider.cpp:430-440).
+void HipPrunedProvider::gatherForwardSplit(double* retainedReal, double* retainedImag, double scale) {
+    auto& m = *impl_;
+    m.retained.download(m.host_split.data(), m.stream.s);
+    const std::size_t n = m.modes * m.w.planes();
+    for (std::size_t i = 0; i < n; ++i) {
+        retainedReal[i] = scale * m.host_split[2 * i];
+        retainedImag[i] = scale * m.host_split[2 * i + 1];
+    }
+}
+
+void HipPrunedProvider::embedInverseSplit(const double* retainedReal, const double* retainedImag) {
+    auto& m = *impl_;
+    const std::size_t n = m.modes * m.w.planes();
+    for (std::size_t i = 0; i < n; ++i) {
+        m.host_split[2 * i] = retainedReal[i];
+        m.host_split[2 * i + 1] = retainedImag[i];
+    }
+    m.retained.upload(m.host_split.data(), m.stream.s);
+}
+
+void HipPrunedProvider::forwardSplit(const double* input, double* retainedReal, double* retainedImag, double scale) {
+    auto& m = *impl_;
+    m.real.upload(input, m.stream.s);
+    check(skb_pruned_forward_split(m.plan, m.real.dev, m.split_re.dev, m.split_im.dev, scale, m.stream.s));
+    m.split_re.download(retainedReal, m.stream.s);
+    m.split_im.download(retainedImag, m.stream.s);
+}
+
+void HipPrunedProvider::inverseSplit(const double* retainedReal, const double* retainedImag, double* output) {
+    auto& m = *impl_;
+    m.split_re.upload(retainedReal, m.stream.s);
+    m.split_im.upload(retainedImag, m.stream.s);
+    check(skb_pruned_inverse_split(m.plan, m.split_re.dev, m.split_im.dev, m.real.dev, m.stream.s));
+    m.real.download(output, m.stream.s);
+}
+
+void HipPrunedProvider::forwardSplitDevice(const void* real, void* retainedReal, void* retainedImag, double scale,
+                                           void* stream) {
+    check(skb_pruned_forward_split(impl_->plan, real, retainedReal, retainedImag, scale, stream));
+}
+
+void HipPrunedProvider::inverseSplitDevice(const void* retainedReal, const void* retainedImag, void* real,
+                                           void* stream) {
+    check(skb_pruned_inverse_split(impl_->plan, retainedReal, retainedImag, real, stream));
+}
+
 void HipPrunedProvider::forwardDevice(const void* real, void* retained, void* stream) {
     check(skb_pruned_forward(impl_->plan, real, retained, stream));
 }
@@ -413,6 +467,172 @@ std::size_t HipPrunedProvider::omittedColumnTransformsPerDirection() const noexc
 std::size_t HipPrunedProvider::scratchBytes() const noexcept { return skb_pruned_scratch_bytes(impl_->plan); }
 std::string HipPrunedProvider::libraryIdentity() const { return "libskb_hip.so"; }
 std::string HipPrunedProvider::version() const { return skb_version(); }
+
+// ---- HipStreamingPrunedSplitProvider ------------------------------------------------
+static_assert(sizeof(HipStreamingPrunedSplitProvider::FieldView) == sizeof(skb_field_view) &&
+                  offsetof(HipStreamingPrunedSplitProvider::FieldView, imaginary) ==
+                      offsetof(skb_field_view, imaginary) &&
+                  offsetof(HipStreamingPrunedSplitProvider::FieldView, modeStride) ==
+                      offsetof(skb_field_view, mode_stride),
+              "FieldView must match skb_field_view");
+
+struct HipStreamingPrunedSplitProvider::Impl {
+    Workload w;
+    std::size_t modes;
+    std::size_t tile;
+    StreamingInversePreparationPolicy policy;
+    skb_pruned_plan* plan = nullptr;
+    Stream stream;
+    Staged real, split_re, split_im;
+    std::vector<double> host_re, host_im;
+
+    Impl(const Workload& wl, const std::vector<RetainedMode>& m, Precision p, std::size_t tileWidth,
+         StreamingInversePreparationPolicy pol)
+        : w(wl), modes(m.size()), tile(tileWidth), policy(pol) {
+        // src/pruned_fftw_provider.cpp:668-677
+        if (tileWidth == 0 || tileWidth > 16)
+            throw std::invalid_argument("Streaming pruned FFTW tile width must lie in [1, 16].");
+        if (tileWidth == 1 && pol != StreamingInversePreparationPolicy::fullZero)
+            throw std::invalid_argument(
+                "Alternative streaming inverse preparation requires tile width greater than one.");
+        const auto c = to_c(m);
+        const auto cw = to_c(wl);
+        check(skb_pruned_plan_create(&cw, c.data(), c.size(), c_precision(p), &plan));
+        const bool single = p == Precision::float32;
+        real.allocate(w.realElements(), single);
+        split_re.allocate(modes * w.planes(), single);
+        split_im.allocate(modes * w.planes(), single);
+        host_re.resize(modes * w.planes());
+        host_im.resize(modes * w.planes());
+    }
+    ~Impl() {
+        if (plan) skb_pruned_plan_destroy(plan);
+    }
+    // src/pruned_fftw_provider.cpp:929-958
+    template <typename V>
+    void validate(const V* fields, std::size_t count, bool output) const {
+        if (fields == nullptr || count != w.fields)
+            throw std::invalid_argument(output
+                                            ? "Streaming pruned output field views must match the workload field count."
+                                            : "Streaming pruned input field views must match the workload field count.");
+        for (std::size_t f = 0; f < count; ++f)
+            if (fields[f].real == nullptr || fields[f].imaginary == nullptr || fields[f].modeStride < w.nz)
+                throw std::invalid_argument(output ? "Streaming pruned output field view is invalid."
+                                                   : "Streaming pruned input field view is invalid.");
+    }
+};
+
+HipStreamingPrunedSplitProvider::HipStreamingPrunedSplitProvider(const Workload& workload,
+                                                                 const std::vector<RetainedMode>& modes,
+                                                                 Precision precision, std::size_t tileWidth,
+                                                                 StreamingInversePreparationPolicy policy)
+    : impl_(std::make_unique<Impl>(workload, modes, precision, tileWidth, policy)) {}
+HipStreamingPrunedSplitProvider::~HipStreamingPrunedSplitProvider() = default;
+HipStreamingPrunedSplitProvider::HipStreamingPrunedSplitProvider(HipStreamingPrunedSplitProvider&&) noexcept = default;
+HipStreamingPrunedSplitProvider& HipStreamingPrunedSplitProvider::operator=(
+    HipStreamingPrunedSplitProvider&&) noexcept = default;
+
+void HipStreamingPrunedSplitProvider::forwardSplit(const double* input, double* retainedReal, double* retainedImag,
+                                                   double scale) {
+    auto& m = *impl_;
+    m.real.upload(input, m.stream.s);
+    check(skb_pruned_forward_split(m.plan, m.real.dev, m.split_re.dev, m.split_im.dev, scale, m.stream.s));
+    m.split_re.download(retainedReal, m.stream.s);
+    m.split_im.download(retainedImag, m.stream.s);
+}
+
+void HipStreamingPrunedSplitProvider::inverseSplit(const double* retainedReal, const double* retainedImag,
+                                                   double* output) {
+    auto& m = *impl_;
+    m.split_re.upload(retainedReal, m.stream.s);
+    m.split_im.upload(retainedImag, m.stream.s);
+    check(skb_pruned_inverse_split(m.plan, m.split_re.dev, m.split_im.dev, m.real.dev, m.stream.s));
+    m.real.download(output, m.stream.s);
+}
+
+// Host field views are gathered into / scattered from the plain split staging
+// arrays (plane + P * mode, plane = z + Nz * field).
+void HipStreamingPrunedSplitProvider::forwardSplitFields(const double* input, const FieldView* fields,
+                                                         std::size_t fieldCount, double scale) {
+    auto& m = *impl_;
+    m.validate(fields, fieldCount, true);
+    m.real.upload(input, m.stream.s);
+    check(skb_pruned_forward_split(m.plan, m.real.dev, m.split_re.dev, m.split_im.dev, scale, m.stream.s));
+    m.split_re.download(m.host_re.data(), m.stream.s);
+    m.split_im.download(m.host_im.data(), m.stream.s);
+    const std::size_t P = m.w.planes(), nz = m.w.nz;
+    for (std::size_t f = 0; f < fieldCount; ++f)
+        for (std::size_t mode = 0; mode < m.modes; ++mode)
+            for (std::size_t z = 0; z < nz; ++z) {
+                const std::size_t src = z + nz * f + P * mode, dst = z + fields[f].modeStride * mode;
+                fields[f].real[dst] = m.host_re[src];
+                fields[f].imaginary[dst] = m.host_im[src];
+            }
+}
+
+void HipStreamingPrunedSplitProvider::inverseSplitFields(const ConstFieldView* fields, std::size_t fieldCount,
+                                                         double* output) {
+    auto& m = *impl_;
+    m.validate(fields, fieldCount, false);
+    const std::size_t P = m.w.planes(), nz = m.w.nz;
+    for (std::size_t f = 0; f < fieldCount; ++f)
+        for (std::size_t mode = 0; mode < m.modes; ++mode)
+            for (std::size_t z = 0; z < nz; ++z) {
+                const std::size_t dst = z + nz * f + P * mode, src = z + fields[f].modeStride * mode;
+                m.host_re[dst] = fields[f].real[src];
+                m.host_im[dst] = fields[f].imaginary[src];
+            }
+    m.split_re.upload(m.host_re.data(), m.stream.s);
+    m.split_im.upload(m.host_im.data(), m.stream.s);
+    check(skb_pruned_inverse_split(m.plan, m.split_re.dev, m.split_im.dev, m.real.dev, m.stream.s));
+    m.real.download(output, m.stream.s);
+}
+
+void HipStreamingPrunedSplitProvider::forwardSplitDevice(const void* real, void* retainedReal, void* retainedImag,
+                                                         double scale, void* stream) {
+    check(skb_pruned_forward_split(impl_->plan, real, retainedReal, retainedImag, scale, stream));
+}
+
+void HipStreamingPrunedSplitProvider::inverseSplitDevice(const void* retainedReal, const void* retainedImag,
+                                                         void* real, void* stream) {
+    check(skb_pruned_inverse_split(impl_->plan, retainedReal, retainedImag, real, stream));
+}
+
+void HipStreamingPrunedSplitProvider::forwardSplitFieldsDevice(const void* real, const skb_field_view* fields,
+                                                               std::size_t fieldCount, double scale, void* stream) {
+    check(skb_pruned_forward_split_fields(impl_->plan, real, fields, fieldCount, scale, stream));
+}
+
+void HipStreamingPrunedSplitProvider::inverseSplitFieldsDevice(const skb_field_view* fields, std::size_t fieldCount,
+                                                               void* real, void* stream) {
+    check(skb_pruned_inverse_split_fields(impl_->plan, fields, fieldCount, real, stream));
+}
+
+void HipStreamingPrunedSplitProvider::executeSchedulerNoop() {
+    hip_check(hipStreamSynchronize(impl_->stream.s), "sync");
+}
+std::size_t HipStreamingPrunedSplitProvider::activeKxCount() const noexcept {
+    return skb_pruned_active_kx(impl_->plan);
+}
+std::size_t HipStreamingPrunedSplitProvider::fullKxCount() const noexcept { return impl_->w.nxHalf(); }
+std::size_t HipStreamingPrunedSplitProvider::rowTransformsPerDirection() const noexcept {
+    return impl_->w.ny * impl_->w.planes();
+}
+std::size_t HipStreamingPrunedSplitProvider::columnTransformsPerDirection() const noexcept {
+    return activeKxCount() * impl_->w.planes();
+}
+std::size_t HipStreamingPrunedSplitProvider::omittedColumnTransformsPerDirection() const noexcept {
+    return (fullKxCount() - activeKxCount()) * impl_->w.planes();
+}
+std::size_t HipStreamingPrunedSplitProvider::scratchBytes() const noexcept {
+    return skb_pruned_scratch_bytes(impl_->plan);
+}
+std::size_t HipStreamingPrunedSplitProvider::tileWidth() const noexcept { return impl_->tile; }
+StreamingInversePreparationPolicy HipStreamingPrunedSplitProvider::inversePreparationPolicy() const noexcept {
+    return impl_->policy;
+}
+std::string HipStreamingPrunedSplitProvider::libraryIdentity() const { return "libskb_hip.so"; }
+std::string HipStreamingPrunedSplitProvider::version() const { return skb_version(); }
 
 // ---- HipVerticalGemmProvider ----------------------------------------------------------
 struct HipVerticalGemmProvider::Impl {

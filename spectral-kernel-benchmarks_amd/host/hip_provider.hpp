@@ -7,6 +7,8 @@
 //   HipPrunedProvider            <- skbench::FFTWPrunedProvider    (include/skbench/skbench.hpp:467-520)
 //   HipVerticalGemmProvider      <- skbench::VerticalGemmProvider  (include/skbench/skbench.hpp:152-218)
 //   HipPointwiseAdvectionExecutor<- skbench::PointwiseAdvectionExecutor (src/pointwise_advection.hpp:18-43)
+//   HipStreamingPrunedSplitProvider <- skbench::FFTWStreamingPrunedSplitProvider
+//                                      (include/skbench/skbench.hpp:534-621)
 // Host-pointer methods are the reference's "adapter" scope: they stage the
 // caller's FP64 arrays through device buffers owned by the provider (allocated
 // at construction; nothing is allocated per call).  The *Device methods are
@@ -138,8 +140,15 @@ public:
     void embedInverse(const Complex* retainedSpectrum);
     void executeInverseColumns();
     void executeInverseRows(double* output);
+    // split compact form (include/skbench/skbench.hpp:481-492)
+    void gatherForwardSplit(double* retainedReal, double* retainedImag, double scale = 1.0);
+    void forwardSplit(const double* input, double* retainedReal, double* retainedImag, double scale = 1.0);
+    void embedInverseSplit(const double* retainedReal, const double* retainedImag);
+    void inverseSplit(const double* retainedReal, const double* retainedImag, double* output);
     void forwardDevice(const void* real, void* retained, void* stream);
     void inverseDevice(const void* retained, void* real, void* stream);
+    void forwardSplitDevice(const void* real, void* retainedReal, void* retainedImag, double scale, void* stream);
+    void inverseSplitDevice(const void* retainedReal, const void* retainedImag, void* real, void* stream);
     void executeSchedulerNoop();
 
     std::size_t activeKxCount() const noexcept;
@@ -149,6 +158,69 @@ public:
     std::size_t omittedColumnTransformsPerDirection() const noexcept;
     std::size_t scratchBytes() const noexcept;
     bool completeHalfSpectrumOutputMaterialized() const noexcept { return false; }
+    std::string libraryIdentity() const;
+    std::string version() const;
+
+private:
+    struct Impl;
+    std::unique_ptr<Impl> impl_;
+};
+
+// The reference's CPU scratch-clearing strategies for the streaming inverse
+// (include/skbench/skbench.hpp:522-527).  On the GPU the inverse column pass
+// reads retained values through a (k_x, k_y) -> mode map and never clears a
+// half-spectrum, so every policy runs the same kernels; the policy is kept
+// for API parity and reported back.
+enum class StreamingInversePreparationPolicy { fullZero, activeReset, compactPreserved, fullPreserved };
+
+class HipStreamingPrunedSplitProvider {
+public:
+    // Layout-identical to skb_field_view (static_assert in hip_provider.cpp).
+    struct ConstFieldView {
+        const double* real = nullptr;
+        const double* imaginary = nullptr;
+        std::size_t modeStride = 0;
+    };
+    struct FieldView {
+        double* real = nullptr;
+        double* imaginary = nullptr;
+        std::size_t modeStride = 0;
+    };
+
+    HipStreamingPrunedSplitProvider(const Workload& workload, const std::vector<RetainedMode>& modes,
+                                    Precision precision = Precision::float64, std::size_t tileWidth = 1,
+                                    StreamingInversePreparationPolicy inversePreparationPolicy =
+                                        StreamingInversePreparationPolicy::fullZero);
+    ~HipStreamingPrunedSplitProvider();
+    HipStreamingPrunedSplitProvider(HipStreamingPrunedSplitProvider&&) noexcept;
+    HipStreamingPrunedSplitProvider& operator=(HipStreamingPrunedSplitProvider&&) noexcept;
+    HipStreamingPrunedSplitProvider(const HipStreamingPrunedSplitProvider&) = delete;
+    HipStreamingPrunedSplitProvider& operator=(const HipStreamingPrunedSplitProvider&) = delete;
+
+    // adapter scope (host FP64 arrays)
+    void forwardSplit(const double* input, double* retainedReal, double* retainedImag, double scale = 1.0);
+    void inverseSplit(const double* retainedReal, const double* retainedImag, double* output);
+    void forwardSplitFields(const double* input, const FieldView* fields, std::size_t fieldCount,
+                            double scale = 1.0);
+    void inverseSplitFields(const ConstFieldView* fields, std::size_t fieldCount, double* output);
+    // primitive scope (device arrays in the plan's precision)
+    void forwardSplitDevice(const void* real, void* retainedReal, void* retainedImag, double scale, void* stream);
+    void inverseSplitDevice(const void* retainedReal, const void* retainedImag, void* real, void* stream);
+    void forwardSplitFieldsDevice(const void* real, const skb_field_view* fields, std::size_t fieldCount,
+                                  double scale, void* stream);
+    void inverseSplitFieldsDevice(const skb_field_view* fields, std::size_t fieldCount, void* real, void* stream);
+    void executeSchedulerNoop();
+
+    std::size_t activeKxCount() const noexcept;
+    std::size_t fullKxCount() const noexcept;
+    std::size_t rowTransformsPerDirection() const noexcept;
+    std::size_t columnTransformsPerDirection() const noexcept;
+    std::size_t omittedColumnTransformsPerDirection() const noexcept;
+    std::size_t scratchBytes() const noexcept;
+    std::size_t tileWidth() const noexcept;
+    StreamingInversePreparationPolicy inversePreparationPolicy() const noexcept;
+    bool rowInversePreservesInput() const noexcept { return true; }
+    bool completeHalfSpectrumMaterialized() const noexcept { return false; }
     std::string libraryIdentity() const;
     std::string version() const;
 

@@ -65,6 +65,11 @@ MODE_DTYPE = np.dtype(
         ("radial", "<f8"),
     ]
 )
+class FieldView_(ctypes.Structure):
+    """skb_field_view: one field of a split compact array (device pointers)."""
+    _fields_ = [("real", ctypes.c_void_p), ("imaginary", ctypes.c_void_p), ("mode_stride", ctypes.c_uint64)]
+
+
 GROUP_DTYPE = np.dtype([("squared_key", "<u8"), ("first_mode", "<u8"), ("mode_count", "<u8")])
 
 FIXTURES = {"impulse": 0, "sinusoid": 1, "random": 2, "dc": 3, "nyquist": 4, "sinusoid-5-17": 5}
@@ -107,6 +112,10 @@ def _load():
         "skb_pruned_plan_destroy": (i32, [vp]),
         "skb_pruned_forward": (i32, [vp, vp, vp, vp]),
         "skb_pruned_inverse": (i32, [vp, vp, vp, vp]),
+        "skb_pruned_forward_split": (i32, [vp, vp, vp, vp, dbl, vp]),
+        "skb_pruned_inverse_split": (i32, [vp, vp, vp, vp, vp]),
+        "skb_pruned_forward_split_fields": (i32, [vp, vp, vp, u64, dbl, vp]),
+        "skb_pruned_inverse_split_fields": (i32, [vp, vp, u64, vp, vp]),
         "skb_pruned_forward_rows": (i32, [vp, vp, vp]),
         "skb_pruned_forward_columns": (i32, [vp, vp, vp]),
         "skb_pruned_inverse_columns": (i32, [vp, vp, vp]),
@@ -142,7 +151,8 @@ EXPORTED = [
     "skb_pruned_forward_rows", "skb_pruned_forward_columns", "skb_pruned_inverse_columns", "skb_pruned_inverse_rows",
     "skb_vertical_plan_create", "skb_vertical_plan_destroy", "skb_vertical_forward", "skb_vertical_inverse",
     "skb_vertical_matrix_bytes", "skb_pointwise_advection", "skb_pruned_plan_set_grid", "skb_pruned_plan_items",
-    "skb_vertical_plan_set_grid", "skb_vertical_plan_items",
+    "skb_vertical_plan_set_grid", "skb_vertical_plan_items", "skb_pruned_forward_split", "skb_pruned_inverse_split",
+    "skb_pruned_forward_split_fields", "skb_pruned_inverse_split_fields",
 ]
 
 
@@ -385,6 +395,52 @@ class PrunedPlan:
                                      _dev(retained, self.retained_elements, complex_dtype(self.precision), "retained"),
                                      _dev(real, self.w.real_elements, real_dtype(self.precision), "real"),
                                      _stream(stream)))
+
+    # split compact form (FFTWPrunedProvider / FFTWStreamingPrunedSplitProvider)
+    def forward_split(self, real, retained_real, retained_imag, scale=1.0, stream=None):
+        rd = real_dtype(self.precision)
+        check(lib.skb_pruned_forward_split(
+            self.h, _dev(real, self.w.real_elements, rd, "real"),
+            _dev(retained_real, self.retained_elements, rd, "retained_real"),
+            _dev(retained_imag, self.retained_elements, rd, "retained_imag"), float(scale), _stream(stream)))
+
+    def inverse_split(self, retained_real, retained_imag, real, stream=None):
+        rd = real_dtype(self.precision)
+        check(lib.skb_pruned_inverse_split(
+            self.h, _dev(retained_real, self.retained_elements, rd, "retained_real"),
+            _dev(retained_imag, self.retained_elements, rd, "retained_imag"),
+            _dev(real, self.w.real_elements, rd, "real"), _stream(stream)))
+
+    def _views(self, views):
+        """[(real tensor, imag tensor, mode_stride)] -> skb_field_view array.
+        Each tensor must cover element (Nz - 1) + mode_stride * (modes - 1)."""
+        rd = real_dtype(self.precision)
+        arr = (FieldView_ * max(1, len(views)))()
+        for f, (re, im, stride) in enumerate(views):
+            stride = int(stride)
+            if stride < self.w.nz:
+                raise InvalidArgument(1, "field view mode stride below Nz")
+            need = self.w.nz + stride * (len(self.modes) - 1)
+            for t, nm in ((re, "field real"), (im, "field imaginary")):
+                torch = _torch()
+                if not isinstance(t, torch.Tensor) or t.device.type != "cuda" or t.dtype != rd:
+                    raise TypeError("%s must be a %s device tensor" % (nm, rd))
+                if not t.is_contiguous() or t.numel() < need:
+                    raise ValueError("%s must be contiguous with >= %d elements" % (nm, need))
+            arr[f] = FieldView_(re.data_ptr(), im.data_ptr(), stride)
+        return arr
+
+    def forward_split_fields(self, real, views, scale=1.0, stream=None):
+        arr = self._views(views)
+        check(lib.skb_pruned_forward_split_fields(
+            self.h, _dev(real, self.w.real_elements, real_dtype(self.precision), "real"), arr, len(views),
+            float(scale), _stream(stream)))
+
+    def inverse_split_fields(self, views, real, stream=None):
+        arr = self._views(views)
+        check(lib.skb_pruned_inverse_split_fields(
+            self.h, arr, len(views), _dev(real, self.w.real_elements, real_dtype(self.precision), "real"),
+            _stream(stream)))
 
     # stage-by-stage form (intermediate in the plan's scratch)
     def forward_rows(self, real, stream=None):

@@ -111,6 +111,92 @@ int main() {
                     "gatherRetainedOuter");
         }
 
+        // Split compact forms (tests/skbench_tests.cpp:1341-1460): direct split,
+        // 0.25 normalisation, field views with mode stride Nz + 3.
+        {
+            HipStreamingPrunedSplitProvider streaming(w, modes);
+            require(streaming.activeKxCount() == pruned.activeKxCount() && streaming.tileWidth() == 1,
+                    "streaming split accounting");
+            const std::size_t n = modes.size() * w.planes();
+            std::vector<double> re(n), im(n), oracleRe(n), oracleIm(n), re4(n), im4(n);
+            const std::size_t stride = w.nz + 3;
+            const std::size_t per = w.nz + stride * (modes.size() - 1);
+            std::vector<double> viewRe(w.fields * per, 7.25), viewIm(w.fields * per, 7.25);
+            std::vector<HipStreamingPrunedSplitProvider::FieldView> out(w.fields);
+            std::vector<HipStreamingPrunedSplitProvider::ConstFieldView> in(w.fields);
+            for (std::size_t f = 0; f < w.fields; ++f) {
+                out[f] = {viewRe.data() + f * per, viewIm.data() + f * per, stride};
+                in[f] = {viewRe.data() + f * per, viewIm.data() + f * per, stride};
+            }
+            for (int f = 0; f < 5; ++f) {
+                const auto input = makeFixture(w, static_cast<FixtureKind>(f), 900 + f);
+                or_direct_retained(8, 8, 7, 2, om.data(), om.size(), input.data(),
+                                   reinterpret_cast<or_complex*>(retainedOracle.data()));
+                for (std::size_t i = 0; i < n; ++i) {
+                    oracleRe[i] = retainedOracle[i].real;
+                    oracleIm[i] = retainedOracle[i].imag;
+                }
+                const std::string tag = ", fixture " + std::to_string(f);
+                pruned.forwardSplit(input.data(), re.data(), im.data());
+                require(or_max_relative_error_real(re.data(), oracleRe.data(), n, 1.0) < 1e-12 &&
+                            or_max_relative_error_real(im.data(), oracleIm.data(), n, 1.0) < 1e-12,
+                        "pruned retained-split forward vs direct retained DFT" + tag);
+                pruned.forwardSplit(input.data(), re4.data(), im4.data(), 0.25);
+                bool scaled = true;
+                for (std::size_t i = 0; i < n; ++i) scaled = scaled && re4[i] == 0.25 * re[i] && im4[i] == 0.25 * im[i];
+                require(scaled, "retained-split normalisation equals a separate scale" + tag);
+                streaming.forwardSplit(input.data(), re4.data(), im4.data());
+                require(re4 == re && im4 == im, "streaming split forward equals pruned split forward" + tag);
+                or_embed_retained(8, 8, 7, 2, om.data(), om.size(), reinterpret_cast<or_complex*>(retainedOracle.data()),
+                                  reinterpret_cast<or_complex*>(embedded.data()));
+                or_direct_c2r(8, 8, 7, 2, reinterpret_cast<or_complex*>(embedded.data()), realOracle.data());
+                pruned.inverseSplit(oracleRe.data(), oracleIm.data(), real.data());
+                require(or_max_relative_error_real(real.data(), realOracle.data(), real.size(), 1.0) < 1e-12,
+                        "pruned retained-split inverse vs direct DFT" + tag);
+                streaming.inverseSplit(oracleRe.data(), oracleIm.data(), real.data());
+                require(or_max_relative_error_real(real.data(), realOracle.data(), real.size(), 1.0) < 1e-12,
+                        "streaming split inverse vs direct DFT" + tag);
+                streaming.forwardSplitFields(input.data(), out.data(), out.size());
+                bool views = true, gaps = true;
+                for (std::size_t fld = 0; fld < w.fields; ++fld)
+                    for (std::size_t m = 0; m < modes.size(); ++m)
+                        for (std::size_t z = 0; z < (m + 1 < modes.size() ? stride : w.nz); ++z) {
+                            const std::size_t at = fld * per + z + stride * m;
+                            if (z < w.nz) {
+                                const std::size_t src = z + w.nz * fld + w.planes() * m;
+                                views = views && viewRe[at] == re[src] && viewIm[at] == im[src];
+                            } else {
+                                gaps = gaps && viewRe[at] == 7.25 && viewIm[at] == 7.25;
+                            }
+                        }
+                require(views, "field views hold the split forward" + tag);
+                require(gaps, "field-view gaps untouched" + tag);
+                std::vector<double> real2(real.size());
+                streaming.inverseSplit(re.data(), im.data(), real.data());
+                streaming.inverseSplitFields(in.data(), in.size(), real2.data());
+                require(real == real2, "split-field inverse equals split inverse" + tag);
+            }
+            bool threwCount = false, threwStride = false, threwTile = false;
+            try {
+                streaming.forwardSplitFields(makeFixture(w, FixtureKind::dc, 1).data(), out.data(), 1);
+            } catch (const std::invalid_argument&) {
+                threwCount = true;
+            }
+            auto badStride = out;
+            badStride[1].modeStride = w.nz - 1;
+            try {
+                streaming.forwardSplitFields(makeFixture(w, FixtureKind::dc, 1).data(), badStride.data(), badStride.size());
+            } catch (const std::invalid_argument&) {
+                threwStride = true;
+            }
+            try {
+                HipStreamingPrunedSplitProvider wide(w, modes, Precision::float64, 17);
+            } catch (const std::invalid_argument&) {
+                threwTile = true;
+            }
+            require(threwCount && threwStride && threwTile, "split field-view and tile-width validation");
+        }
+
         const auto ops = squaredWavenumberVerticalFixture(w, modes);
         HipVerticalGemmProvider vertical(w, ops);
         require(vertical.groupCount() == 5 && vertical.gemmCallsPerExecution() == 1, "vertical accounting");
