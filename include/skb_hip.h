@@ -208,11 +208,36 @@ int skb_vertical_forward(skb_vertical_plan* plan, const void* physical, void* mo
 int skb_vertical_inverse(skb_vertical_plan* plan, const void* modal, void* physical, void* stream);
 /* VerticalGemmProvider::matrixBytesPerDirection. */
 uint64_t skb_vertical_matrix_bytes(const skb_vertical_plan* plan);
-/* Launch shape: wavefronts per launch, 0 = one per 16-column tile (the
+/* Launch shape: workgroups per launch, 0 = one per 16-column tile (the
  * default); fewer makes them persistent.  Set by the launch autotuner; no
  * reference counterpart. */
-int skb_vertical_plan_set_grid(skb_vertical_plan* plan, int wavefronts);
+int skb_vertical_plan_set_grid(skb_vertical_plan* plan, int workgroups);
 uint64_t skb_vertical_plan_items(const skb_vertical_plan* plan);
+
+/* ---- WVM-direct vertical GEMM ------------------------------------------
+ * Replaces WvmDirectVerticalGemmProvider (include/skbench/skbench.hpp:220-257;
+ * src/vertical_gemm.cpp:935-1273): the same grouped products applied in place
+ * on FULL WVM arrays.  Mode m at frequency q = stored_kx + (Nx/2+1)*stored_ky
+ * reads the Nz x fields block at spectrum + P*q and writes the Nj x fields
+ * block at modal + Nj*fields*q (z / j fastest); for k_x = 0 modes with
+ * k_y != 0, 2k_y != Ny the conjugate is also written at (0, Ny - k_y)
+ * (src/vertical_gemm.cpp:1081-1131).  Entries of non-retained frequencies are
+ * not written: initialise the output once with the initialize calls
+ * (initializeModalOutput / initializeSpectrumOutput, which zero the whole
+ * array).  nj must equal the workload's retained vertical modes
+ * 2(Nz-1)/3 and every mode needs a unique stored frequency
+ * (src/vertical_gemm.cpp:965-1034).  The compact and WVM entry points reject
+ * a plan of the other kind with SKB_LOGIC_ERROR. */
+int skb_vertical_wvm_plan_create(const skb_workload* workload, const skb_retained_mode* modes, uint64_t mode_count,
+                                 uint64_t nj, const skb_mode_group* groups, uint64_t group_count,
+                                 const double* forward, const double* inverse, int precision,
+                                 skb_vertical_plan** plan);
+int skb_vertical_wvm_forward(skb_vertical_plan* plan, const void* full_spectrum, void* full_modal, void* stream);
+int skb_vertical_wvm_inverse(skb_vertical_plan* plan, const void* full_modal, void* full_spectrum, void* stream);
+/* WvmDirectVerticalGemmProvider::modalSpectrumElements: (Nx/2+1)*Ny*Nj*fields. */
+uint64_t skb_vertical_wvm_modal_elements(const skb_vertical_plan* plan);
+int skb_vertical_wvm_initialize_modal(skb_vertical_plan* plan, void* full_modal, void* stream);
+int skb_vertical_wvm_initialize_spectrum(skb_vertical_plan* plan, void* full_spectrum, void* stream);
 
 /* ---- pointwise advection -----------------------------------------------
  * Replaces PointwiseAdvectionExecutor::execute (src/pointwise_advection.hpp

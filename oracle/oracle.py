@@ -403,6 +403,35 @@ def retained_operator(w: Workload, modes, retained, nj, first, count, forward, w
     return vertical_forward(w, nj, first, count, forward, back)
 
 
+def wvm_direct_vertical(w: Workload, modes: np.ndarray, nj: int, first, count, mats: np.ndarray, full: np.ndarray,
+                        forward: bool = True) -> np.ndarray:
+    """WvmDirectVerticalGemmProvider::executeForward / executeInverse
+    (src/vertical_gemm.cpp:1133-1157, boundary copies 1107-1131) on FULL WVM
+    arrays: for each retained mode at stored frequency q = kx + (Nx/2+1) ky the
+    block full[q] (K x fields, K fastest) is multiplied by the mode's group
+    matrix (M x K row-major: Nj x Nz forward, Nz x Nj inverse) -- on the STORED
+    value, without the mode's conjugation flag -- and written at q; for
+    kx = 0, ky != 0, 2 ky != Ny the conjugate is also written at (0, Ny - ky).
+    Entries of other frequencies are zero (initializeModalOutput /
+    initializeSpectrumOutput)."""
+    nxh, F = w.nx // 2 + 1, w.fields
+    K, M = (w.nz, nj) if forward else (nj, w.nz)
+    first = np.asarray(first, np.int64)
+    count = np.asarray(count, np.int64)
+    src = np.asarray(full, np.complex128).reshape(nxh * w.ny, F, K)
+    A = np.asarray(mats, np.float64).reshape(len(first), M, K)
+    group_of = np.repeat(np.arange(len(first)), count)
+    kx = modes["stored_kx"].astype(np.int64)
+    ky = modes["stored_ky"].astype(np.int64)
+    q = kx + nxh * ky
+    out = np.zeros((nxh * w.ny, F, M), np.complex128)
+    for m in range(len(modes)):
+        out[q[m]] = src[q[m]] @ A[group_of[m]].T
+    for m in np.nonzero((kx == 0) & (ky != 0) & (2 * ky != w.ny))[0]:
+        out[nxh * ((w.ny - ky[m]) % w.ny)] = np.conj(out[q[m]])
+    return out.reshape(-1)
+
+
 def vertical_forward_blas(w: Workload, nj: int, first, count, forward: np.ndarray, physical: np.ndarray) -> np.ndarray:
     """Same product as vertical_forward, through numpy's BLAS (multi-threaded):
     used only by the CPU baseline timing.  Equality with the C loop is

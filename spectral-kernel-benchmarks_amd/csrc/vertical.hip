@@ -8,10 +8,25 @@
 // the split-dgemm formulation of src/vertical_gemm.cpp:558-596, which the
 // reference shows equals its complex zgemm (src/vertical_gemm.cpp:534-556).
 //
-// Work decomposition: one wavefront per 16 consecutive complex columns of a
-// group computes all ceil(M/16) row blocks with 16x16x4 MFMA
-// (v_mfma_f64_16x16x4_f64 for FP64, v_mfma_f32_16x16x4_f32 for FP32) into
-// two accumulators (Re, Im).  See grouped_gemm_kernel for the operand order.
+// Two column layouts share one kernel:
+//   compact    column c at in + K*c, out + M*c (VerticalGemmProvider,
+//              src/vertical_gemm.cpp:395-422);
+//   WVM direct column (mode, field) at in + K*(F*q + field), out + M*(F*q +
+//              field) with q = stored_kx + (Nx/2+1)*stored_ky the mode's
+//              frequency, and the k_x = 0 Hermitian partner written as the
+//              conjugate (WvmDirectVerticalGemmProvider,
+//              src/vertical_gemm.cpp:1081-1157).
+//
+// Work decomposition: one workgroup (4 wavefronts) per tile of 16 consecutive
+// columns of one group.  The tile's B block (16 x K complex) is loaded with
+// coalesced reads into LDS; each wavefront owns row blocks mb = wave,
+// wave + 4, ... of 16 output rows and runs 16x16x4 MFMA
+// (v_mfma_f32_16x16x4_f32 / v_mfma_f64_16x16x4_f64) over K in steps of 4,
+// two accumulators (Re, Im) per row block, A fragments held in registers.
+// When M = 16*MB + r with r <= 4 (M = 65: four blocks and one row), the r
+// leftover rows are dot products on the vector ALU instead of a mostly-empty
+// fifth MFMA block.  The 16 x M result is staged in the same LDS and written
+// with coalesced stores.
 //
 // MFMA 16x16x4 operand layout (64 lanes, lane = l), measured on gfx950:
 //   A[i][k]: i = l % 16, k = l / 16        B[k][j]: j = l % 16, k = l / 16
@@ -47,7 +62,7 @@ struct Acc<double> {
     }
 };
 
-// A tile is 16 consecutive complex columns of one group (one wavefront).
+// A tile is up to 16 consecutive complex columns of one group.
 struct GemmTile {
     int group;
     int col_begin;
@@ -55,158 +70,219 @@ struct GemmTile {
     int pad;
 };
 
-constexpr int kMaxK = 260;
+// Row blocks per wavefront held at once: M <= 132 gives at most 9 blocks.
+constexpr int kMaxSlots = 3;
 
-// The reduction index is permuted so that every lane's operands are
-// contiguous: MFMA k-step s, lane group q = lane / 16 covers
-//     z = KS * q + s,   s = 0 .. KS-1,   KS = ceil(K / 4).
-// B (the input columns): lane (j = lane % 16, q) needs z = KS*q .. KS*q+KS-1
-// of column j -- one contiguous run of KS complex values.  A is stored by the
-// plan in fragment order
-//     Ap[g][mb][s][lane] = A_g[16*mb + lane % 16][KS * (lane / 16) + s]
-// (zero outside M x K), so each A fragment is one coalesced 64-element read.
-//
-// Wavefronts are persistent over tiles (tile = wave, wave + waves, ...).
-// Memory ordering matters on gfx9, where one in-order counter covers loads
-// and stores: every load a wavefront will wait on is issued BEFORE the
-// stores that precede its use -- the next tile's B and the next row block's A
-// are prefetched, and all stores of a tile are issued after its last MFMA.
+// LDS column stride of the B block, in complex elements.  With KP = 4*KS
+// the MFMA B reads of one instruction (16 columns x 4 k) hit distinct banks:
+// FP32 (8-byte reads, two half-waves) needs CS/2 odd, FP64 (16-byte reads,
+// four quarter-waves) needs CS odd.
 template <typename R, int KS>
-__device__ __forceinline__ void gemm_load_b(cx<R>* b, const cx<R>* in, const GemmTile& t, int K, int lane) {
-    const int j = lane & 15, q = lane >> 4;
-    const bool col_ok = j < t.ncols;
-    const int z0 = KS * q;
-    const cx<R>* src = in + static_cast<size_t>(K) * (t.col_begin + (col_ok ? j : 0)) + z0;
-    if (col_ok && z0 + KS <= K) {
-#pragma unroll
-        for (int s = 0; s < KS; ++s) b[s] = src[s];
-    } else {
-#pragma unroll
-        for (int s = 0; s < KS; ++s) b[s] = (col_ok && z0 + s < K) ? src[s] : cx<R>{R(0), R(0)};
-    }
+constexpr int gemm_cs() {
+    return sizeof(R) == 4 ? 4 * KS + 2 : 4 * KS + 1;
 }
 
-template <typename R, int KS, int RB>
-__global__ __launch_bounds__(256) void grouped_gemm_kernel(const cx<R>* __restrict__ in, cx<R>* __restrict__ out,
-                                                          const R* __restrict__ ap,
-                                                          const GemmTile* __restrict__ tiles, int ntiles, int M,
-                                                          int K) {
+template <typename R>
+struct GemmArgs {
+    const cx<R>* in;
+    cx<R>* out;
+    const R* ap;    // [group][MB][KS][64]: A[16*mb + l%16][4*s + l/16] (zero outside M x K)
+    const R* arow;  // [group][VR][4*KS]: rows 16*MB .. M-1 (zero for k >= K)
+    const GemmTile* tiles;
+    int ntiles, M, K, MB, VR, F;
+    const int* qmap;     // WVM direct: frequency q of each mode
+    const int* partner;  // WVM direct: frequency of the k_x = 0 conjugate partner, or -1
+};
+
+template <typename R, int KS, bool WVM>
+__global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmArgs<R> g) {
     using A4 = typename Acc<R>::type;
-    // per-wave output stage: 16 columns x (16*RB) rows, complex
-    __shared__ cx<R> stage[4][16 * RB * 16];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int waves = gridDim.x * 4;
-    // wave-uniform tile index -> scalar descriptor loads (counted apart from
-    // the vector loads and stores)
-    // One wavefront per tile: XCD-aware (each XCD takes a contiguous range of
-    // workgroups' tiles, whose input and output blocks share boundary lines).
-    const int wgs_needed = (ntiles + 3) / 4;
-    const int wg = gridDim.x >= static_cast<unsigned>(wgs_needed) ? xcd_item(blockIdx.x, wgs_needed) : blockIdx.x;
-    int tile = __builtin_amdgcn_readfirstlane(wg * 4 + wave);
-    if (wg >= wgs_needed || tile >= ntiles) return;
-    const int j = lane & 15;
-    cx<R>* st = stage[wave];
-    while (true) {
-        const GemmTile t = tiles[tile];
-        cx<R> b[KS];
-        gemm_load_b<R, KS>(b, in, t, K, lane);
-        const R* a_ptr = ap + static_cast<size_t>(t.group) * RB * KS * 64 + lane;
-        A4 re[RB], im[RB];
-        R a[KS], an[KS];
+    constexpr int KP = 4 * KS;
+    constexpr int CS = gemm_cs<R, KS>();
+    constexpr int NL = (16 * KP + 255) / 256;  // B elements loaded per thread
+    extern __shared__ __align__(16) unsigned char gemm_lds[];
+    cx<R>* sb = reinterpret_cast<cx<R>*>(gemm_lds);  // B block, then the output stage
+    __shared__ int cbase[16], obase[16], pbase[16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int j = lane & 15, q = lane >> 4;
+    const bool one = gridDim.x >= static_cast<unsigned>(g.ntiles);
+    const int step = one ? g.ntiles : static_cast<int>(gridDim.x);
+    for (int tile = one ? xcd_item(blockIdx.x, g.ntiles) : static_cast<int>(blockIdx.x); tile < g.ntiles;
+         tile += step) {
+        const GemmTile t = g.tiles[__builtin_amdgcn_readfirstlane(tile)];
+        // A fragments of this wavefront's first row block: issued before the
+        // B block so both loads are in flight together.
+        R a0[KS];
+        if (wave < g.MB) {
+            const R* a_ptr = g.ap + (static_cast<size_t>(t.group) * g.MB + wave) * KS * 64 + lane;
 #pragma unroll
-        for (int s = 0; s < KS; ++s) a[s] = a_ptr[s * 64];
-#pragma unroll
-        for (int mb = 0; mb < RB; ++mb) {
-            if (mb + 1 < RB) {
-#pragma unroll
-                for (int s = 0; s < KS; ++s) an[s] = a_ptr[((mb + 1) * KS + s) * 64];
+            for (int s = 0; s < KS; ++s) a0[s] = a_ptr[s * 64];
+        }
+        if constexpr (WVM) {
+            if (tid < 16) {
+                int cb = 0, ob = 0, pb = -1;
+                if (tid < t.ncols) {
+                    const int col = t.col_begin + tid, mode = col / g.F, f = col - mode * g.F;
+                    const int qf = g.qmap[mode], pf = g.partner[mode];
+                    cb = g.K * (g.F * qf + f);
+                    ob = g.M * (g.F * qf + f);
+                    pb = pf >= 0 ? g.M * (g.F * pf + f) : -1;
+                }
+                cbase[tid] = cb;
+                obase[tid] = ob;
+                pbase[tid] = pb;
             }
-            A4 r = {0, 0, 0, 0}, i = {0, 0, 0, 0};
+            __syncthreads();
+        }
+        // B block -> LDS, zero outside the tile's columns and for k >= K
+        cx<R> bv[NL];
 #pragma unroll
-            for (int s = 0; s < KS; ++s) {
-                r = Acc<R>::mfma(a[s], b[s].x, r);
-                i = Acc<R>::mfma(a[s], b[s].y, i);
-            }
-            re[mb] = r;
-            im[mb] = i;
-            if (mb + 1 < RB) {
-#pragma unroll
-                for (int s = 0; s < KS; ++s) a[s] = an[s];
+        for (int n = 0; n < NL; ++n) {
+            const int e = tid + 256 * n, c = e / KP, z = e - c * KP;
+            bv[n] = cx<R>{R(0), R(0)};
+            if (e < 16 * KP && c < t.ncols && z < g.K) {
+                const size_t src = WVM ? static_cast<size_t>(cbase[c]) + z
+                                       : static_cast<size_t>(g.K) * (t.col_begin + c) + z;
+                bv[n] = g.in[src];
             }
         }
-        // Stage the 16-column block column-major with row stride M (the
-        // output layout), then store it as one contiguous run.
 #pragma unroll
-        for (int mb = 0; mb < RB; ++mb)
+        for (int n = 0; n < NL; ++n) {
+            const int e = tid + 256 * n, c = e / KP, z = e - c * KP;
+            if (e < 16 * KP) sb[c * CS + z] = bv[n];
+        }
+        __syncthreads();
+        // MFMA row blocks mb = wave + 4 * slot
+        A4 re[kMaxSlots], im[kMaxSlots];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = mb * 16 + Acc<R>::row(lane, r);
-                if (row < M) st[j * M + row] = cx<R>{re[mb][r], im[mb][r]};
+        for (int sl = 0; sl < kMaxSlots; ++sl) {
+            const int mb = wave + 4 * sl;
+            if (mb < g.MB) {
+                R a[KS];
+                if (sl == 0) {
+#pragma unroll
+                    for (int s = 0; s < KS; ++s) a[s] = a0[s];
+                } else {
+                    const R* a_ptr = g.ap + (static_cast<size_t>(t.group) * g.MB + mb) * KS * 64 + lane;
+#pragma unroll
+                    for (int s = 0; s < KS; ++s) a[s] = a_ptr[s * 64];
+                }
+                A4 r = {0, 0, 0, 0}, i = {0, 0, 0, 0};
+#pragma unroll
+                for (int s = 0; s < KS; ++s) {
+                    const cx<R> b = sb[j * CS + 4 * s + q];
+                    r = Acc<R>::mfma(a[s], b.x, r);
+                    i = Acc<R>::mfma(a[s], b.y, i);
+                }
+                re[sl] = r;
+                im[sl] = i;
             }
-        // the stage is private to this wavefront: order its LDS writes
-        // before the reads of other lanes
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        cx<R>* dst = out + static_cast<size_t>(M) * t.col_begin;
-        const int total = M * t.ncols;
-        for (int e = lane; e < total; e += 64) dst[e] = st[e];
-        tile += waves;
-        if (tile >= ntiles) break;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        // Leftover rows 16*MB + wave (wave < VR): lane (j, q) sums k in
+        // [q*KS, q*KS + KS) of column j; the four partial sums are added
+        // across lanes j, j+16, j+32, j+48.
+        cx<R> vrow{R(0), R(0)};
+        if (wave < g.VR) {
+            const R* ar = g.arow + (static_cast<size_t>(t.group) * g.VR + wave) * KP + q * KS;
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const R a = ar[s];
+                const cx<R> b = sb[j * CS + q * KS + s];
+                vrow.x += a * b.x;
+                vrow.y += a * b.y;
+            }
+            vrow.x += __shfl_xor(vrow.x, 16);
+            vrow.y += __shfl_xor(vrow.y, 16);
+            vrow.x += __shfl_xor(vrow.x, 32);
+            vrow.y += __shfl_xor(vrow.y, 32);
+        }
+        __syncthreads();  // every wavefront is done reading B
+        // stage the 16 x M result column-major (the output layout)
+        cx<R>* st = sb;
+#pragma unroll
+        for (int sl = 0; sl < kMaxSlots; ++sl) {
+            const int mb = wave + 4 * sl;
+            if (mb < g.MB) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * mb + Acc<R>::row(lane, r);
+                    if (row < g.M) st[j * g.M + row] = cx<R>{re[sl][r], im[sl][r]};
+                }
+            }
+        }
+        if (wave < g.VR && q == 0) st[j * g.M + 16 * g.MB + wave] = vrow;
+        __syncthreads();
+        const int total = g.M * t.ncols;
+        for (int e = tid; e < total; e += 256) {
+            const cx<R> v = st[e];
+            if constexpr (WVM) {
+                const int c = e / g.M, row = e - c * g.M;
+                g.out[static_cast<size_t>(obase[c]) + row] = v;
+                if (pbase[c] >= 0) g.out[static_cast<size_t>(pbase[c]) + row] = conjg(v);
+            } else {
+                g.out[static_cast<size_t>(g.M) * t.col_begin + e] = v;
+            }
+        }
+        __syncthreads();  // the stage and the column bases are reused by the next tile
     }
 }
 
-// Host: matrices (row-major M x K per group) -> fragment order.
 template <typename R>
-std::vector<R> fragment_order(const double* mats, uint64_t groups, int M, int K) {
-    const int KS = (K + 3) / 4, rblocks = (M + 15) / 16;
-    std::vector<R> ap(static_cast<size_t>(groups) * rblocks * KS * 64, R(0));
-    for (uint64_t g = 0; g < groups; ++g)
-        for (int mb = 0; mb < rblocks; ++mb)
+__global__ void fill_zero_kernel(cx<R>* __restrict__ out, size_t total) {
+    for (size_t e = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; e < total;
+         e += static_cast<size_t>(gridDim.x) * blockDim.x)
+        out[e] = cx<R>{R(0), R(0)};
+}
+
+// Split of M output rows into MB MFMA row blocks and VR <= 4 vector rows.
+struct RowSplit {
+    int mb, vr;
+};
+inline RowSplit row_split(int M) {
+    if (M >= 16 && M % 16 <= 4) return RowSplit{M / 16, M % 16};
+    return RowSplit{(M + 15) / 16, 0};
+}
+
+// Host: matrices (row-major M x K per group) -> fragment order and vector rows.
+template <typename R>
+void operand_order(const double* mats, uint64_t groups, int M, int K, std::vector<R>& ap, std::vector<R>& arow) {
+    const int KS = (K + 3) / 4, KP = 4 * KS;
+    const RowSplit sp = row_split(M);
+    ap.assign(static_cast<size_t>(groups) * sp.mb * KS * 64, R(0));
+    arow.assign(static_cast<size_t>(groups) * std::max(sp.vr, 1) * KP, R(0));
+    for (uint64_t g = 0; g < groups; ++g) {
+        for (int mb = 0; mb < sp.mb; ++mb)
             for (int s = 0; s < KS; ++s)
                 for (int l = 0; l < 64; ++l) {
-                    const int i = 16 * mb + (l & 15), k = KS * (l >> 4) + s;
+                    const int i = 16 * mb + (l & 15), k = 4 * s + (l >> 4);
                     if (i < M && k < K)
-                        ap[((g * rblocks + mb) * KS + s) * 64 + l] =
+                        ap[((g * sp.mb + mb) * KS + s) * 64 + l] =
                             static_cast<R>(mats[(g * M + i) * static_cast<uint64_t>(K) + k]);
                 }
-    return ap;
+        for (int r = 0; r < sp.vr; ++r)
+            for (int k = 0; k < K; ++k)
+                arow[(g * sp.vr + r) * KP + k] =
+                    static_cast<R>(mats[(g * M + 16 * sp.mb + r) * static_cast<uint64_t>(K) + k]);
+    }
 }
 
 namespace {
 
-template <typename R, int KS, int RB>
-void launch_gemm_rb(const void* in, void* out, const void* ap, const GemmTile* tiles, int ntiles, int M, int K,
-                    int req, hipStream_t s) {
-    const int wgs = (ntiles + 3) / 4;
-    const int grid = req > 0 && (req + 3) / 4 < wgs ? (req + 3) / 4 : xcd_grid(wgs);
-    hipLaunchKernelGGL((grouped_gemm_kernel<R, KS, RB>), dim3(grid), dim3(256), 0, s,
-                       static_cast<const cx<R>*>(in), static_cast<cx<R>*>(out), static_cast<const R*>(ap), tiles,
-                       ntiles, M, K);
+template <typename R, int KS, bool WVM>
+void launch_gemm_ks(const GemmArgs<R>& a, int req, hipStream_t s) {
+    constexpr int CS = gemm_cs<R, KS>();
+    const size_t lds = static_cast<size_t>(std::max(16 * CS, 16 * a.M)) * sizeof(cx<R>);
+    const int grid = req > 0 && req < a.ntiles ? req : xcd_grid(a.ntiles);
+    hipLaunchKernelGGL((grouped_gemm_kernel<R, KS, WVM>), dim3(grid), dim3(256), lds, s, a);
     check_hip(hipGetLastError(), "vertical GEMM launch");
 }
 
-template <typename R, int KS>
-void launch_gemm_ks(const void* in, void* out, const void* ap, const GemmTile* tiles, int ntiles, int M, int K,
-                    int req, hipStream_t s) {
-    switch ((M + 15) / 16) {
-#define SKB_RB(N) \
-    case N: launch_gemm_rb<R, KS, N>(in, out, ap, tiles, ntiles, M, K, req, s); return;
-        SKB_RB(1) SKB_RB(2) SKB_RB(3) SKB_RB(4) SKB_RB(5) SKB_RB(6) SKB_RB(7) SKB_RB(8) SKB_RB(9)
-#undef SKB_RB
-    }
-    fail(SKB_UNSUPPORTED, "vertical GEMM output dimension above 144");
-}
-
-template <typename R>
-void launch_gemm(const void* in, void* out, const void* ap, const GemmTile* tiles, int ntiles, int M, int K, int req,
-                 hipStream_t s) {
-    switch ((K + 3) / 4) {
+template <typename R, bool WVM>
+void launch_gemm(const GemmArgs<R>& a, int req, hipStream_t s) {
+    if (a.ntiles == 0) return;
+    switch ((a.K + 3) / 4) {
 #define SKB_KS(N) \
-    case N: launch_gemm_ks<R, N>(in, out, ap, tiles, ntiles, M, K, req, s); return;
+    case N: launch_gemm_ks<R, N, WVM>(a, req, s); return;
         SKB_KS(1) SKB_KS(2) SKB_KS(3) SKB_KS(4) SKB_KS(5) SKB_KS(6) SKB_KS(7) SKB_KS(8) SKB_KS(9) SKB_KS(10)
         SKB_KS(11) SKB_KS(12) SKB_KS(13) SKB_KS(14) SKB_KS(15) SKB_KS(16) SKB_KS(17) SKB_KS(22) SKB_KS(33)
 #undef SKB_KS
@@ -215,6 +291,30 @@ void launch_gemm(const void* in, void* out, const void* ap, const GemmTile* tile
 }
 
 }  // namespace
+
+// Matrices of one direction on the device.
+struct GemmOperand {
+    DeviceBuffer ap, arow;
+    int M = 0, K = 0;
+    RowSplit split{0, 0};
+    void build(const double* mats, uint64_t groups, int M_, int K_, int precision) {
+        M = M_;
+        K = K_;
+        split = row_split(M);
+        if (precision == SKB_F64) {
+            std::vector<double> a, r;
+            operand_order<double>(mats, groups, M, K, a, r);
+            ap.upload(a.data(), a.size() * sizeof(double));
+            arow.upload(r.data(), r.size() * sizeof(double));
+        } else {
+            std::vector<float> a, r;
+            operand_order<float>(mats, groups, M, K, a, r);
+            ap.upload(a.data(), a.size() * sizeof(float));
+            arow.upload(r.data(), r.size() * sizeof(float));
+        }
+    }
+};
+
 }  // namespace skb
 
 struct skb_vertical_plan {
@@ -223,12 +323,119 @@ struct skb_vertical_plan {
     int precision = SKB_F64;
     uint64_t groups = 0;
     int ntiles = 0;
-    int waves = 0;  // wavefronts per launch (0 = one per tile)
+    int grid = 0;  // workgroups per launch (0 = one per tile)
     uint64_t matrix_bytes = 0;  // logical matrix bytes per direction
-    skb::DeviceBuffer fwd, inv, tiles;  // matrices in fragment order
+    skb::GemmOperand fwd, inv;
+    skb::DeviceBuffer tiles;
+    // WVM-direct plans: per-mode frequency and k_x = 0 partner frequency
+    bool wvm = false;
+    uint64_t modes = 0;
+    skb::DeviceBuffer qmap, partner;
 };
 
 using namespace skb;
+
+namespace {
+
+void validate_groups(const skb_mode_group* groups, uint64_t group_count, uint64_t* modes_out) {
+    if (!groups || group_count == 0) fail(SKB_INVALID_ARGUMENT, "vertical GEMM requires at least one group");
+    // Groups must tile [0, modes) contiguously (src/model.cpp:191-206).
+    uint64_t next = 0;
+    for (uint64_t g = 0; g < group_count; ++g) {
+        if (groups[g].first_mode != next || groups[g].mode_count == 0)
+            fail(SKB_INVALID_ARGUMENT, "vertical groups must be contiguous, non-empty and start at mode 0");
+        next += groups[g].mode_count;
+    }
+    *modes_out = next;
+}
+
+std::unique_ptr<skb_vertical_plan> make_plan(const skb_workload* workload, uint64_t nj, const skb_mode_group* groups,
+                                             uint64_t group_count, const double* forward, const double* inverse,
+                                             int precision) {
+    check_precision(precision);
+    validate_workload(workload);
+    if (!forward || !inverse) fail(SKB_INVALID_ARGUMENT, "vertical GEMM requires forward and inverse matrices");
+    const uint64_t nz = workload->nz;
+    if (nj == 0 || nz == 0) fail(SKB_INVALID_ARGUMENT, "Vertical operator dimensions must be positive.");
+    if (nz > 132 || nj > 132) fail(SKB_UNSUPPORTED, "vertical dimensions above 132 (Nz, Nj) are not instantiated");
+    uint64_t modes = 0;
+    validate_groups(groups, group_count, &modes);
+    const uint64_t fields = workload->fields;
+    if (modes * fields * std::max(nz, nj) > (1ull << 31)) fail(SKB_UNSUPPORTED, "too many vertical columns");
+    std::vector<GemmTile> tiles;
+    for (uint64_t g = 0; g < group_count; ++g) {
+        const uint64_t c0 = groups[g].first_mode * fields;
+        const uint64_t nc = groups[g].mode_count * fields;
+        for (uint64_t c = 0; c < nc; c += 16)
+            tiles.push_back(GemmTile{static_cast<int>(g), static_cast<int>(c0 + c),
+                                     static_cast<int>(std::min<uint64_t>(16, nc - c)), 0});
+    }
+    auto p = std::make_unique<skb_vertical_plan>();
+    p->w = *workload;
+    p->nj = nj;
+    p->precision = precision;
+    p->groups = group_count;
+    p->modes = modes;
+    p->ntiles = static_cast<int>(tiles.size());
+    const int inj = static_cast<int>(nj), inz = static_cast<int>(nz);
+    p->fwd.build(forward, group_count, inj, inz, precision);
+    p->inv.build(inverse, group_count, inz, inj, precision);
+    p->tiles.upload(tiles.data(), tiles.size() * sizeof(GemmTile));
+    p->matrix_bytes = nj * nz * group_count * (precision == SKB_F64 ? 8 : 4);
+    return p;
+}
+
+template <typename R>
+GemmArgs<R> args(const skb_vertical_plan* p, const GemmOperand& op, const void* in, void* out) {
+    GemmArgs<R> a{};
+    a.in = static_cast<const cx<R>*>(in);
+    a.out = static_cast<cx<R>*>(out);
+    a.ap = op.ap.as<R>();
+    a.arow = op.arow.as<R>();
+    a.tiles = p->tiles.as<GemmTile>();
+    a.ntiles = p->ntiles;
+    a.M = op.M;
+    a.K = op.K;
+    a.MB = op.split.mb;
+    a.VR = op.split.vr;
+    a.F = static_cast<int>(p->w.fields);
+    a.qmap = p->qmap.as<int>();
+    a.partner = p->partner.as<int>();
+    return a;
+}
+
+void run(const skb_vertical_plan* p, bool forward, const void* in, void* out, void* stream, bool wvm) {
+    if (!p || !in || !out) fail(SKB_INVALID_ARGUMENT, "null argument");
+    if (p->wvm != wvm)
+        fail(SKB_LOGIC_ERROR, wvm ? "WVM-direct entry point called on a compact vertical plan"
+                                  : "compact entry point called on a WVM-direct vertical plan");
+    const GemmOperand& op = forward ? p->fwd : p->inv;
+    const hipStream_t s = as_stream(stream);
+    if (p->precision == SKB_F64) {
+        const auto a = args<double>(p, op, in, out);
+        if (wvm) launch_gemm<double, true>(a, p->grid, s);
+        else launch_gemm<double, false>(a, p->grid, s);
+    } else {
+        const auto a = args<float>(p, op, in, out);
+        if (wvm) launch_gemm<float, true>(a, p->grid, s);
+        else launch_gemm<float, false>(a, p->grid, s);
+    }
+}
+
+void fill_zero(const skb_vertical_plan* p, void* out, uint64_t elements, void* stream) {
+    if (!p || !out) fail(SKB_INVALID_ARGUMENT, "null argument");
+    if (!p->wvm) fail(SKB_LOGIC_ERROR, "output initialisation belongs to WVM-direct vertical plans");
+    const dim3 grid(static_cast<unsigned>(std::max<uint64_t>(1, std::min<uint64_t>((elements + 255) / 256, 8192))));
+    if (p->precision == SKB_F64)
+        hipLaunchKernelGGL(fill_zero_kernel<double>, grid, dim3(256), 0, as_stream(stream),
+                           static_cast<cx<double>*>(out), static_cast<size_t>(elements));
+    else
+        hipLaunchKernelGGL(fill_zero_kernel<float>, grid, dim3(256), 0, as_stream(stream),
+                           static_cast<cx<float>*>(out), static_cast<size_t>(elements));
+    check_hip(hipGetLastError(), "fill launch");
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -238,51 +445,49 @@ int skb_vertical_plan_create(const skb_workload* workload, uint64_t nj, const sk
     return guarded([&] {
         if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan pointer");
         *plan = nullptr;
-        check_precision(precision);
+        *plan = make_plan(workload, nj, groups, group_count, forward, inverse, precision).release();
+    });
+}
+
+int skb_vertical_wvm_plan_create(const skb_workload* workload, const skb_retained_mode* modes, uint64_t mode_count,
+                                 uint64_t nj, const skb_mode_group* groups, uint64_t group_count,
+                                 const double* forward, const double* inverse, int precision,
+                                 skb_vertical_plan** plan) {
+    return guarded([&] {
+        if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan pointer");
+        *plan = nullptr;
         validate_workload(workload);
-        if (!groups || group_count == 0) fail(SKB_INVALID_ARGUMENT, "vertical GEMM requires at least one group");
-        if (!forward || !inverse) fail(SKB_INVALID_ARGUMENT, "vertical GEMM requires forward and inverse matrices");
-        const uint64_t nz = workload->nz;
-        if (nj == 0 || nz == 0) fail(SKB_INVALID_ARGUMENT, "Vertical operator dimensions must be positive.");
-        if (nz > 132 || nj > 132)
-            fail(SKB_UNSUPPORTED, "vertical dimensions above 132 (Nz, Nj) are not instantiated");
-        // Groups must tile [0, modes) contiguously (src/model.cpp:191-206).
-        uint64_t next = 0;
-        for (uint64_t g = 0; g < group_count; ++g) {
-            if (groups[g].first_mode != next || groups[g].mode_count == 0)
-                fail(SKB_INVALID_ARGUMENT, "vertical groups must be contiguous, non-empty and start at mode 0");
-            next += groups[g].mode_count;
+        // src/vertical_gemm.cpp:965-1034
+        if (nj != (workload->nz < 2 ? 0 : 2 * (workload->nz - 1) / 3))
+            fail(SKB_INVALID_ARGUMENT, "Direct WVM vertical GEMM operator dimensions do not match the workload.");
+        if (!modes || mode_count == 0 || !groups || group_count == 0)
+            fail(SKB_INVALID_ARGUMENT, "Direct WVM vertical GEMM requires modes, groups, and fields.");
+        auto p = make_plan(workload, nj, groups, group_count, forward, inverse, precision);
+        if (p->modes != mode_count)
+            fail(SKB_INVALID_ARGUMENT, "Direct WVM vertical GEMM groups do not cover every retained mode.");
+        const uint64_t nxh = workload->nx / 2 + 1, ny = workload->ny;
+        const uint64_t frequencies = nxh * ny;
+        const uint64_t P = workload->nz * workload->fields;
+        if (frequencies * P > (1ull << 31) || frequencies * nj * workload->fields > (1ull << 31))
+            fail(SKB_UNSUPPORTED, "WVM arrays above 2^31 elements");
+        std::vector<int> q(mode_count), partner(mode_count, -1);
+        std::vector<uint8_t> used(frequencies, 0);
+        for (uint64_t m = 0; m < mode_count; ++m) {
+            const auto& md = modes[m];
+            if (md.stored_kx >= nxh || md.stored_ky >= ny)
+                fail(SKB_INVALID_ARGUMENT, "A retained mode lies outside the half-spectrum.");
+            const uint64_t f = md.stored_kx + nxh * md.stored_ky;
+            if (used[f])
+                fail(SKB_INVALID_ARGUMENT,
+                     "Direct WVM vertical GEMM requires a unique stored frequency for every retained mode.");
+            used[f] = 1;
+            q[m] = static_cast<int>(f);
+            if (md.stored_kx == 0 && md.stored_ky != 0 && 2 * md.stored_ky != ny)
+                partner[m] = static_cast<int>(nxh * ((ny - md.stored_ky) % ny));
         }
-        const uint64_t fields = workload->fields;
-        if (next * fields > (1ull << 31)) fail(SKB_UNSUPPORTED, "too many vertical columns");
-        std::vector<GemmTile> tiles;
-        for (uint64_t g = 0; g < group_count; ++g) {
-            const uint64_t c0 = groups[g].first_mode * fields;
-            const uint64_t nc = groups[g].mode_count * fields;
-            for (uint64_t c = 0; c < nc; c += 16)
-                tiles.push_back(GemmTile{static_cast<int>(g), static_cast<int>(c0 + c),
-                                         static_cast<int>(std::min<uint64_t>(16, nc - c)), 0});
-        }
-        auto p = std::make_unique<skb_vertical_plan>();
-        p->w = *workload;
-        p->nj = nj;
-        p->precision = precision;
-        p->groups = group_count;
-        p->ntiles = static_cast<int>(tiles.size());
-        const int inj = static_cast<int>(nj), inz = static_cast<int>(nz);
-        if (precision == SKB_F64) {
-            const auto f = fragment_order<double>(forward, group_count, inj, inz);
-            const auto i = fragment_order<double>(inverse, group_count, inz, inj);
-            p->fwd.upload(f.data(), f.size() * sizeof(double));
-            p->inv.upload(i.data(), i.size() * sizeof(double));
-        } else {
-            const auto f = fragment_order<float>(forward, group_count, inj, inz);
-            const auto i = fragment_order<float>(inverse, group_count, inz, inj);
-            p->fwd.upload(f.data(), f.size() * sizeof(float));
-            p->inv.upload(i.data(), i.size() * sizeof(float));
-        }
-        p->tiles.upload(tiles.data(), tiles.size() * sizeof(GemmTile));
-        p->matrix_bytes = nj * nz * group_count * (precision == SKB_F64 ? 8 : 4);
+        p->wvm = true;
+        p->qmap.upload(q.data(), q.size() * sizeof(int));
+        p->partner.upload(partner.data(), partner.size() * sizeof(int));
         *plan = p.release();
     });
 }
@@ -292,37 +497,44 @@ int skb_vertical_plan_destroy(skb_vertical_plan* plan) {
 }
 
 int skb_vertical_forward(skb_vertical_plan* plan, const void* physical, void* modal, void* stream) {
-    return guarded([&] {
-        if (!plan || !physical || !modal) fail(SKB_INVALID_ARGUMENT, "null argument");
-        const int M = static_cast<int>(plan->nj), K = static_cast<int>(plan->w.nz);
-        if (plan->precision == SKB_F64)
-            launch_gemm<double>(physical, modal, plan->fwd.ptr, plan->tiles.as<GemmTile>(), plan->ntiles, M, K, plan->waves,
-                                as_stream(stream));
-        else
-            launch_gemm<float>(physical, modal, plan->fwd.ptr, plan->tiles.as<GemmTile>(), plan->ntiles, M, K, plan->waves,
-                               as_stream(stream));
-    });
+    return guarded([&] { run(plan, true, physical, modal, stream, false); });
 }
 
 int skb_vertical_inverse(skb_vertical_plan* plan, const void* modal, void* physical, void* stream) {
+    return guarded([&] { run(plan, false, modal, physical, stream, false); });
+}
+
+int skb_vertical_wvm_forward(skb_vertical_plan* plan, const void* full_spectrum, void* full_modal, void* stream) {
+    return guarded([&] { run(plan, true, full_spectrum, full_modal, stream, true); });
+}
+
+int skb_vertical_wvm_inverse(skb_vertical_plan* plan, const void* full_modal, void* full_spectrum, void* stream) {
+    return guarded([&] { run(plan, false, full_modal, full_spectrum, stream, true); });
+}
+
+uint64_t skb_vertical_wvm_modal_elements(const skb_vertical_plan* plan) {
+    if (!plan) return 0;
+    return (plan->w.nx / 2 + 1) * plan->w.ny * plan->nj * plan->w.fields;
+}
+
+int skb_vertical_wvm_initialize_modal(skb_vertical_plan* plan, void* full_modal, void* stream) {
+    return guarded([&] { fill_zero(plan, full_modal, skb_vertical_wvm_modal_elements(plan), stream); });
+}
+
+int skb_vertical_wvm_initialize_spectrum(skb_vertical_plan* plan, void* full_spectrum, void* stream) {
     return guarded([&] {
-        if (!plan || !physical || !modal) fail(SKB_INVALID_ARGUMENT, "null argument");
-        const int M = static_cast<int>(plan->w.nz), K = static_cast<int>(plan->nj);
-        if (plan->precision == SKB_F64)
-            launch_gemm<double>(modal, physical, plan->inv.ptr, plan->tiles.as<GemmTile>(), plan->ntiles, M, K, plan->waves,
-                                as_stream(stream));
-        else
-            launch_gemm<float>(modal, physical, plan->inv.ptr, plan->tiles.as<GemmTile>(), plan->ntiles, M, K, plan->waves,
-                               as_stream(stream));
+        if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
+        const auto& w = plan->w;
+        fill_zero(plan, full_spectrum, (w.nx / 2 + 1) * w.ny * w.nz * w.fields, stream);
     });
 }
 
 uint64_t skb_vertical_matrix_bytes(const skb_vertical_plan* plan) { return plan ? plan->matrix_bytes : 0; }
 
-int skb_vertical_plan_set_grid(skb_vertical_plan* plan, int wavefronts) {
+int skb_vertical_plan_set_grid(skb_vertical_plan* plan, int workgroups) {
     return guarded([&] {
-        if (!plan || wavefronts < 0) fail(SKB_INVALID_ARGUMENT, "bad wavefront count");
-        plan->waves = wavefronts;
+        if (!plan || workgroups < 0) fail(SKB_INVALID_ARGUMENT, "bad workgroup count");
+        plan->grid = workgroups;
     });
 }
 

@@ -37,6 +37,10 @@ class InvalidArgument(SkbError, ValueError):
     pass
 
 
+class LogicError(SkbError):
+    """SKB_LOGIC_ERROR: wrong entry point for the plan (std::logic_error)."""
+
+
 class Unsupported(SkbError):
     pass
 
@@ -132,6 +136,12 @@ def _load():
         "skb_vertical_inverse": (i32, [vp, vp, vp, vp]),
         "skb_vertical_matrix_bytes": (u64, [vp]),
         "skb_pointwise_advection": (i32, [u64, dbl, vp, vp, vp, i32, vp]),
+        "skb_vertical_wvm_plan_create": (i32, [W, vp, u64, u64, vp, u64, vp, vp, i32, pp]),
+        "skb_vertical_wvm_forward": (i32, [vp, vp, vp, vp]),
+        "skb_vertical_wvm_inverse": (i32, [vp, vp, vp, vp]),
+        "skb_vertical_wvm_modal_elements": (u64, [vp]),
+        "skb_vertical_wvm_initialize_modal": (i32, [vp, vp, vp]),
+        "skb_vertical_wvm_initialize_spectrum": (i32, [vp, vp, vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -152,7 +162,9 @@ EXPORTED = [
     "skb_vertical_plan_create", "skb_vertical_plan_destroy", "skb_vertical_forward", "skb_vertical_inverse",
     "skb_vertical_matrix_bytes", "skb_pointwise_advection", "skb_pruned_plan_set_grid", "skb_pruned_plan_items",
     "skb_vertical_plan_set_grid", "skb_vertical_plan_items", "skb_pruned_forward_split", "skb_pruned_inverse_split",
-    "skb_pruned_forward_split_fields", "skb_pruned_inverse_split_fields",
+    "skb_pruned_forward_split_fields", "skb_pruned_inverse_split_fields", "skb_vertical_wvm_plan_create",
+    "skb_vertical_wvm_forward", "skb_vertical_wvm_inverse", "skb_vertical_wvm_modal_elements",
+    "skb_vertical_wvm_initialize_modal", "skb_vertical_wvm_initialize_spectrum",
 ]
 
 
@@ -168,6 +180,8 @@ def check(status: int):
     msg = last_error()
     if status == 1:
         raise InvalidArgument(status, msg)
+    if status == 3:
+        raise LogicError(status, msg)
     if status == 5:
         raise Unsupported(status, msg)
     raise SkbError(status, msg)
@@ -512,12 +526,62 @@ class VerticalPlan:
     def matrix_bytes(self) -> int:
         return int(lib.skb_vertical_matrix_bytes(self.h))
 
-    def set_grid(self, wavefronts: int):
-        """Wavefronts per launch (0 = one per 16-column tile)."""
-        check(lib.skb_vertical_plan_set_grid(self.h, wavefronts))
+    def set_grid(self, workgroups: int):
+        """Workgroups per launch (0 = one per 16-column tile)."""
+        check(lib.skb_vertical_plan_set_grid(self.h, workgroups))
 
     def items(self) -> int:
         return int(lib.skb_vertical_plan_items(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.skb_vertical_plan_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+
+class WvmVerticalPlan:
+    """Vertical GEMM applied in place on full WVM arrays
+    (WvmDirectVerticalGemmProvider): retained modes only, k_x = 0 partners
+    written as conjugates; other entries are left as initialised."""
+
+    def __init__(self, w: Workload, modes: np.ndarray, nj: int, groups: np.ndarray, forward: np.ndarray,
+                 inverse: np.ndarray, precision: int = SKB_F64):
+        self.w, self.nj, self.precision = w, nj, precision
+        self._wc = w.c()
+        forward = np.ascontiguousarray(forward, np.float64)
+        inverse = np.ascontiguousarray(inverse, np.float64)
+        if forward.size != len(groups) * nj * w.nz or inverse.size != forward.size:
+            raise ValueError("matrix family size does not match groups x nj x nz")
+        h = ctypes.c_void_p()
+        check(lib.skb_vertical_wvm_plan_create(ctypes.byref(self._wc), _np(modes), len(modes), nj, _np(groups),
+                                               len(groups), _np(forward), _np(inverse), precision, ctypes.byref(h)))
+        self.h = h
+
+    @property
+    def modal_elements(self) -> int:
+        return int(lib.skb_vertical_wvm_modal_elements(self.h))
+
+    def forward(self, full_spectrum, full_modal, stream=None):
+        cd = complex_dtype(self.precision)
+        check(lib.skb_vertical_wvm_forward(self.h, _dev(full_spectrum, self.w.spectrum_elements, cd, "spectrum"),
+                                           _dev(full_modal, self.modal_elements, cd, "modal"), _stream(stream)))
+
+    def inverse(self, full_modal, full_spectrum, stream=None):
+        cd = complex_dtype(self.precision)
+        check(lib.skb_vertical_wvm_inverse(self.h, _dev(full_modal, self.modal_elements, cd, "modal"),
+                                           _dev(full_spectrum, self.w.spectrum_elements, cd, "spectrum"),
+                                           _stream(stream)))
+
+    def initialize_modal(self, full_modal, stream=None):
+        check(lib.skb_vertical_wvm_initialize_modal(
+            self.h, _dev(full_modal, self.modal_elements, complex_dtype(self.precision), "modal"), _stream(stream)))
+
+    def initialize_spectrum(self, full_spectrum, stream=None):
+        check(lib.skb_vertical_wvm_initialize_spectrum(
+            self.h, _dev(full_spectrum, self.w.spectrum_elements, complex_dtype(self.precision), "spectrum"),
+            _stream(stream)))
 
     def close(self):
         if getattr(self, "h", None):

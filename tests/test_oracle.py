@@ -316,3 +316,40 @@ def test_blas_vertical_equals_loop_vertical():
     c = O.retained_operator_fast(w, modes, x, w.nz, first, count, f, 2)
     d = O.retained_operator(w, modes, x, w.nz, first, count, f, 2)
     assert O.max_relative_error(c, d) <= 1e-14
+
+
+def test_wvm_direct_vertical_matches_compact_path_and_partner_symmetry():
+    """The WVM-direct product works on stored values; the compact path works on
+    the gathered (k < 0: conjugated) values.  With real matrices
+    A conj(x) = conj(A x), so gather(WVM-direct output) must equal the compact
+    product of gather(input), and every k_x = 0 partner must hold the
+    conjugate of its mode."""
+    w = O.Workload(16, 16, 7, 2)
+    modes = O.retained_modes(w)
+    keys, first, count = O.k2_groups(modes)
+    nj = 2 * (w.nz - 1) // 3
+    fwd, inv = O.k2_vertical_family(w.nz, nj, keys)
+    rng = np.random.default_rng(5)
+    full = rng.standard_normal(w.spectrum_elements) + 1j * rng.standard_normal(w.spectrum_elements)
+    wm = O.Workload(16, 16, nj, 2)  # modal arrays use the WVM layout with Nj in place of Nz
+    out = O.wvm_direct_vertical(w, modes, nj, first, count, fwd, full, True)
+    compact = O.vertical_forward(w, nj, first, count, fwd, O.gather_retained(w, modes, full))
+    assert O.max_relative_error(O.gather_retained(wm, modes, out), compact) < 1e-13
+    back = O.wvm_direct_vertical(w, modes, nj, first, count, inv, out, False)
+    compact_back = O.vertical_inverse(w, nj, first, count, inv, compact)
+    assert O.max_relative_error(O.gather_retained(w, modes, back), compact_back) < 1e-13
+    nxh = w.nx // 2 + 1
+    blocks = out.reshape(nxh * w.ny, w.fields, nj)
+    seen = 0
+    for m in modes:
+        if m["stored_kx"] == 0 and m["stored_ky"] != 0 and 2 * m["stored_ky"] != w.ny:
+            q = nxh * int(m["stored_ky"])
+            qp = nxh * ((w.ny - int(m["stored_ky"])) % w.ny)
+            assert np.array_equal(blocks[qp], np.conj(blocks[q]))
+            seen += 1
+    assert seen > 0
+    # frequencies that are neither retained nor partners stay zero
+    touched = set(int(m["stored_kx"]) + nxh * int(m["stored_ky"]) for m in modes)
+    touched |= set(nxh * ((w.ny - int(m["stored_ky"])) % w.ny) for m in modes if m["stored_kx"] == 0)
+    untouched = [f for f in range(nxh * w.ny) if f not in touched]
+    assert untouched and np.all(blocks[untouched] == 0)
