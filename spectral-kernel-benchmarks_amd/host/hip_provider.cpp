@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <complex>
 #include <cstddef>
@@ -467,6 +468,94 @@ std::size_t HipPrunedProvider::omittedColumnTransformsPerDirection() const noexc
 std::size_t HipPrunedProvider::scratchBytes() const noexcept { return skb_pruned_scratch_bytes(impl_->plan); }
 std::string HipPrunedProvider::libraryIdentity() const { return "libskb_hip.so"; }
 std::string HipPrunedProvider::version() const { return skb_version(); }
+
+// ---- HipWvmDirectVerticalGemmProvider ----------------------------------------------
+struct HipWvmDirectVerticalGemmProvider::Impl {
+    Workload w;
+    std::size_t nj, modal_elements;
+    skb_vertical_plan* plan = nullptr;
+    Stream stream;
+    Staged spectrum, modal;  // full WVM arrays (the adapter transfers them whole)
+
+    Impl(const Workload& wl, const std::vector<RetainedMode>& modes, const GroupedVerticalOperators& ops, Precision p)
+        : w(wl), nj(ops.nj) {
+        // src/vertical_gemm.cpp:965-972
+        if (ops.nz != wl.nz || ops.nj != wl.retainedVerticalModes())
+            throw std::invalid_argument("Direct WVM vertical GEMM operator dimensions do not match the workload.");
+        const auto cw = to_c(wl);
+        const auto cm = to_c(modes);
+        check(skb_vertical_wvm_plan_create(&cw, cm.data(), cm.size(), nj,
+                                           reinterpret_cast<const skb_mode_group*>(ops.groups.data()),
+                                           ops.groups.size(), ops.forward.data(), ops.inverse.data(), c_precision(p),
+                                           &plan));
+        modal_elements = skb_vertical_wvm_modal_elements(plan);
+        const bool single = p == Precision::float32;
+        spectrum.allocate(2 * wl.spectrumElements(), single);
+        modal.allocate(2 * modal_elements, single);
+    }
+    ~Impl() {
+        if (plan) skb_vertical_plan_destroy(plan);
+    }
+};
+
+HipWvmDirectVerticalGemmProvider::HipWvmDirectVerticalGemmProvider(const Workload& workload,
+                                                                   const std::vector<RetainedMode>& modes,
+                                                                   const GroupedVerticalOperators& operators,
+                                                                   Precision precision)
+    : impl_(std::make_unique<Impl>(workload, modes, operators, precision)) {}
+HipWvmDirectVerticalGemmProvider::~HipWvmDirectVerticalGemmProvider() = default;
+HipWvmDirectVerticalGemmProvider::HipWvmDirectVerticalGemmProvider(HipWvmDirectVerticalGemmProvider&&) noexcept =
+    default;
+HipWvmDirectVerticalGemmProvider& HipWvmDirectVerticalGemmProvider::operator=(
+    HipWvmDirectVerticalGemmProvider&&) noexcept = default;
+
+std::string HipWvmDirectVerticalGemmProvider::capability() const {
+    return "supported: frequency-major direct grouped MFMA GEMM (gfx950)";
+}
+void HipWvmDirectVerticalGemmProvider::initializeModalOutput(Complex* fullModalSpectrum) const {
+    std::fill_n(fullModalSpectrum, impl_->modal_elements, Complex{});
+}
+void HipWvmDirectVerticalGemmProvider::initializeSpectrumOutput(Complex* fullSpectrum) const {
+    std::fill_n(fullSpectrum, impl_->w.spectrumElements(), Complex{});
+}
+// The adapter stages whole arrays, so entries the GEMM does not write keep
+// the caller's values, as in the reference.
+void HipWvmDirectVerticalGemmProvider::executeForward(const Complex* fullSpectrum, Complex* fullModalSpectrum) {
+    auto& m = *impl_;
+    m.spectrum.upload(reinterpret_cast<const double*>(fullSpectrum), m.stream.s);
+    m.modal.upload(reinterpret_cast<const double*>(fullModalSpectrum), m.stream.s);
+    check(skb_vertical_wvm_forward(m.plan, m.spectrum.dev, m.modal.dev, m.stream.s));
+    m.modal.download(reinterpret_cast<double*>(fullModalSpectrum), m.stream.s);
+}
+void HipWvmDirectVerticalGemmProvider::executeInverse(const Complex* fullModalSpectrum, Complex* fullSpectrum) {
+    auto& m = *impl_;
+    m.modal.upload(reinterpret_cast<const double*>(fullModalSpectrum), m.stream.s);
+    m.spectrum.upload(reinterpret_cast<const double*>(fullSpectrum), m.stream.s);
+    check(skb_vertical_wvm_inverse(m.plan, m.modal.dev, m.spectrum.dev, m.stream.s));
+    m.spectrum.download(reinterpret_cast<double*>(fullSpectrum), m.stream.s);
+}
+void HipWvmDirectVerticalGemmProvider::initializeModalOutputDevice(void* fullModalSpectrum, void* stream) const {
+    check(skb_vertical_wvm_initialize_modal(impl_->plan, fullModalSpectrum, stream));
+}
+void HipWvmDirectVerticalGemmProvider::initializeSpectrumOutputDevice(void* fullSpectrum, void* stream) const {
+    check(skb_vertical_wvm_initialize_spectrum(impl_->plan, fullSpectrum, stream));
+}
+void HipWvmDirectVerticalGemmProvider::executeForwardDevice(const void* fullSpectrum, void* fullModalSpectrum,
+                                                            void* stream) {
+    check(skb_vertical_wvm_forward(impl_->plan, fullSpectrum, fullModalSpectrum, stream));
+}
+void HipWvmDirectVerticalGemmProvider::executeInverseDevice(const void* fullModalSpectrum, void* fullSpectrum,
+                                                            void* stream) {
+    check(skb_vertical_wvm_inverse(impl_->plan, fullModalSpectrum, fullSpectrum, stream));
+}
+void HipWvmDirectVerticalGemmProvider::executeSchedulerNoop() {
+    hip_check(hipStreamSynchronize(impl_->stream.s), "sync");
+}
+std::size_t HipWvmDirectVerticalGemmProvider::modalSpectrumElements() const noexcept { return impl_->modal_elements; }
+std::size_t HipWvmDirectVerticalGemmProvider::matrixBytesPerDirection() const noexcept {
+    return skb_vertical_matrix_bytes(impl_->plan);
+}
+std::string HipWvmDirectVerticalGemmProvider::libraryIdentity() const { return "libskb_hip.so"; }
 
 // ---- HipStreamingPrunedSplitProvider ------------------------------------------------
 static_assert(sizeof(HipStreamingPrunedSplitProvider::FieldView) == sizeof(skb_field_view) &&

@@ -7,6 +7,8 @@
 //   HipPrunedProvider            <- skbench::FFTWPrunedProvider    (include/skbench/skbench.hpp:467-520)
 //   HipVerticalGemmProvider      <- skbench::VerticalGemmProvider  (include/skbench/skbench.hpp:152-218)
 //   HipPointwiseAdvectionExecutor<- skbench::PointwiseAdvectionExecutor (src/pointwise_advection.hpp:18-43)
+//   HipWvmDirectVerticalGemmProvider <- skbench::WvmDirectVerticalGemmProvider
+//                                      (include/skbench/skbench.hpp:220-257)
 //   HipStreamingPrunedSplitProvider <- skbench::FFTWStreamingPrunedSplitProvider
 //                                      (include/skbench/skbench.hpp:534-621)
 // Host-pointer methods are the reference's "adapter" scope: they stage the
@@ -256,6 +258,40 @@ public:
     void copyInverseOutput(Complex* output) const;
     void forwardDevice(const void* physical, void* modal, void* stream);
     void inverseDevice(const void* modal, void* physical, void* stream);
+    std::string libraryIdentity() const;
+
+private:
+    struct Impl;
+    std::unique_ptr<Impl> impl_;
+};
+
+class HipWvmDirectVerticalGemmProvider {
+public:
+    HipWvmDirectVerticalGemmProvider(const Workload& workload, const std::vector<RetainedMode>& modes,
+                                     const GroupedVerticalOperators& operators,
+                                     Precision precision = Precision::float64);
+    ~HipWvmDirectVerticalGemmProvider();
+    HipWvmDirectVerticalGemmProvider(HipWvmDirectVerticalGemmProvider&&) noexcept;
+    HipWvmDirectVerticalGemmProvider& operator=(HipWvmDirectVerticalGemmProvider&&) noexcept;
+    HipWvmDirectVerticalGemmProvider(const HipWvmDirectVerticalGemmProvider&) = delete;
+    HipWvmDirectVerticalGemmProvider& operator=(const HipWvmDirectVerticalGemmProvider&) = delete;
+
+    bool supported() const noexcept { return true; }
+    std::string capability() const;
+    // adapter scope (host FP64 full WVM arrays)
+    void initializeModalOutput(Complex* fullModalSpectrum) const;
+    void initializeSpectrumOutput(Complex* fullSpectrum) const;
+    void executeForward(const Complex* fullSpectrum, Complex* fullModalSpectrum);
+    void executeInverse(const Complex* fullModalSpectrum, Complex* fullSpectrum);
+    // primitive scope (device full WVM arrays in the plan's precision)
+    void initializeModalOutputDevice(void* fullModalSpectrum, void* stream) const;
+    void initializeSpectrumOutputDevice(void* fullSpectrum, void* stream) const;
+    void executeForwardDevice(const void* fullSpectrum, void* fullModalSpectrum, void* stream);
+    void executeInverseDevice(const void* fullModalSpectrum, void* fullSpectrum, void* stream);
+    void executeSchedulerNoop();
+    std::size_t modalSpectrumElements() const noexcept;
+    std::size_t gemmCallsPerExecution() const noexcept { return 1; }  // one grouped launch
+    std::size_t matrixBytesPerDirection() const noexcept;
     std::string libraryIdentity() const;
 
 private:

@@ -21,6 +21,8 @@ void or_direct_retained(uint64_t, uint64_t, uint64_t, uint64_t, const void*, uin
 void or_embed_retained(uint64_t, uint64_t, uint64_t, uint64_t, const void*, uint64_t, const or_complex*, or_complex*);
 void or_vertical_forward(uint64_t, uint64_t, uint64_t, uint64_t, const uint64_t*, const uint64_t*, const double*,
                          const or_complex*, or_complex*);
+void or_gather_retained(uint64_t, uint64_t, uint64_t, uint64_t, const void*, uint64_t, const or_complex*,
+                        or_complex*);
 double or_max_relative_error_complex(const void*, const void*, uint64_t);
 double or_max_relative_error_real(const double*, const double*, uint64_t, double);
 }
@@ -216,6 +218,50 @@ int main() {
                             reinterpret_cast<or_complex*>(phys.data()), reinterpret_cast<or_complex*>(modalOracle.data()));
         require(or_max_relative_error_complex(modal.data(), modalOracle.data(), modal.size()) <= 1e-12,
                 "vertical forward vs scalar products");
+
+        // WVM-direct vertical GEMM (src/vertical_gemm.cpp:1133-1157): on stored
+        // values; with real matrices gather(out) equals the compact product of
+        // gather(in) (gather conjugates k < 0 modes), and the k_x = 0 partner
+        // holds the conjugate.
+        {
+            HipWvmDirectVerticalGemmProvider wvm(w, modes, ops);
+            std::vector<Complex> full(w.spectrumElements()), fullModal(wvm.modalSpectrumElements());
+            for (size_t i = 0; i < full.size(); ++i)
+                full[i] = {std::sin(0.37 * static_cast<double>(i)), std::cos(0.11 * static_cast<double>(i) + 0.5)};
+            wvm.initializeModalOutput(fullModal.data());
+            wvm.executeForward(full.data(), fullModal.data());
+            std::vector<Complex> gathered(modes.size() * w.planes()), compact(modes.size() * w.fields * ops.nj),
+                gatheredModal(compact.size());
+            or_gather_retained(8, 8, 7, 2, om.data(), om.size(), reinterpret_cast<const or_complex*>(full.data()),
+                               reinterpret_cast<or_complex*>(gathered.data()));
+            or_vertical_forward(w.nz, ops.nj, w.fields, first.size(), first.data(), count.data(), ops.forward.data(),
+                                reinterpret_cast<or_complex*>(gathered.data()),
+                                reinterpret_cast<or_complex*>(compact.data()));
+            or_gather_retained(8, 8, ops.nj, 2, om.data(), om.size(),
+                               reinterpret_cast<const or_complex*>(fullModal.data()),
+                               reinterpret_cast<or_complex*>(gatheredModal.data()));
+            require(or_max_relative_error_complex(gatheredModal.data(), compact.data(), compact.size()) <= 1e-12,
+                    "WVM-direct forward vs compact product of gathered input");
+            bool partner = true;
+            const std::size_t nxh = w.nxHalf(), per = w.fields * ops.nj;
+            for (const auto& md : modes)
+                if (md.storedKx == 0 && md.storedKy != 0 && 2 * md.storedKy != w.ny)
+                    for (std::size_t e = 0; e < per; ++e) {
+                        const auto a = fullModal[nxh * md.storedKy * per + e];
+                        const auto b = fullModal[nxh * ((w.ny - md.storedKy) % w.ny) * per + e];
+                        partner = partner && a.real == b.real && a.imag == -b.imag;
+                    }
+            require(partner, "WVM-direct k_x = 0 partners hold the conjugate");
+            bool threw = false;
+            try {
+                auto bad = ops;
+                bad.nj = w.nz;
+                HipWvmDirectVerticalGemmProvider wrong(w, modes, bad);
+            } catch (const std::invalid_argument&) {
+                threw = true;
+            }
+            require(threw, "WVM-direct requires Nj = 2(Nz-1)/3");
+        }
 
         HipPointwiseAdvectionExecutor advection(37, 0.03125);
         std::vector<double> shared(111), derivative(111), target(37), expected(37);
