@@ -239,6 +239,26 @@ uint64_t skb_vertical_wvm_modal_elements(const skb_vertical_plan* plan);
 int skb_vertical_wvm_initialize_modal(skb_vertical_plan* plan, void* full_modal, void* stream);
 int skb_vertical_wvm_initialize_spectrum(skb_vertical_plan* plan, void* full_spectrum, void* stream);
 
+/* ---- antialiased nonlinear flux graph ------------------------------------
+ * Replaces the production-lifetime flux graph of runProductionLifetimeFluxBenchmark
+ * (src/benchmark.cpp:3957-4234; serial oracle 4182-4234).  workload->fields
+ * is the number of targets T; the input modal array has 3 + 3T fields
+ * [U, V, W, q0x, q0y, q0z, ...] and the output modal array T fields, both in
+ * the modal layout j + Nj*field + Nj*fields*mode.  One evaluation: vertical
+ * inverse (K^2 family, Nj x Nz per group) -> pruned horizontal inverse ->
+ * per target -(u*qx + v*qy + w*qz)/(Nx*Ny)^2 -> raw pruned forward ->
+ * vertical forward.  The grid must be square (src/benchmark.cpp:3981-3984).
+ * By default the physical-space stage is one fused row pass (inverse rows,
+ * products and forward rows in registers; Nx >= 64); skb_flux_set_fused(plan, 0)
+ * selects separate kernels with the real fields in HBM. */
+typedef struct skb_flux_plan skb_flux_plan;
+int skb_flux_plan_create(const skb_workload* workload, const skb_retained_mode* modes, uint64_t mode_count,
+                         uint64_t nj, const skb_mode_group* groups, uint64_t group_count, const double* forward,
+                         const double* inverse, int precision, skb_flux_plan** plan);
+int skb_flux_plan_destroy(skb_flux_plan* plan);
+int skb_flux_set_fused(skb_flux_plan* plan, int fused);
+int skb_flux_execute(skb_flux_plan* plan, const void* input_modal, void* output_modal, void* stream);
+
 /* ---- pointwise advection -----------------------------------------------
  * Replaces PointwiseAdvectionExecutor::execute (src/pointwise_advection.hpp
  * :18-43; src/pointwise_advection.cpp:152-166):

@@ -432,6 +432,47 @@ def wvm_direct_vertical(w: Workload, modes: np.ndarray, nj: int, first, count, m
     return out.reshape(-1)
 
 
+def flux_modal_fixture(w: Workload, modes: np.ndarray, nj: int, fields: int, seed: int) -> np.ndarray:
+    """Modal flux input: uniform[-0.125, 0.125) values, DC imaginary part zero
+    (the distribution of src/benchmark.cpp:4043-4055, drawn from the portable
+    generator of src/benchmark.cpp:187-196 instead of std::mt19937_64)."""
+    n = len(modes) * fields * nj
+    u = 0.125 * signed_units(seed, 2 * n)
+    out = (u[0::2] + 1j * u[1::2]).reshape(len(modes), fields, nj)
+    dc = (modes["k"] == 0) & (modes["l"] == 0)
+    out[dc] = out[dc].real
+    return out.reshape(-1)
+
+
+def production_flux(w: Workload, modes: np.ndarray, nj: int, first, count, forward: np.ndarray,
+                    inverse: np.ndarray, modal_in: np.ndarray, targets: int, use_direct: bool = False) -> np.ndarray:
+    """The serial flux oracle of src/benchmark.cpp:4182-4234 restated: vertical
+    inverse of the 3 + 3T input fields, pruned horizontal inverse of every
+    field, per target -(u*qx + v*qy + w*qz)/(Nx*Ny)^2, raw forward with
+    gather, vertical forward of the T outputs.  `use_direct` swaps pocketfft
+    for the O(N^4) direct DFT (small grids)."""
+    nin = 3 + 3 * targets
+    win = Workload(w.nx, w.ny, w.nz, nin, w.lx, w.ly)
+    wt = Workload(w.nx, w.ny, w.nz, targets, w.lx, w.ly)
+    phys = vertical_inverse(win, nj, first, count, inverse, modal_in)
+    if use_direct:
+        real = direct_c2r(win, embed_retained(win, modes, phys))
+    else:
+        real = retained_inverse(win, modes, phys)
+    vol = w.nx * w.ny * w.nz
+    real = real.reshape(nin, vol)
+    scale = 1.0 / float(w.nx * w.ny) ** 2
+    out_real = np.empty((targets, vol))
+    for t in range(targets):
+        d = 3 + 3 * t
+        out_real[t] = -scale * (real[0] * real[d] + real[1] * real[d + 1] + real[2] * real[d + 2])
+    if use_direct:
+        compact = gather_retained(wt, modes, direct_r2c(wt, out_real.reshape(-1)))
+    else:
+        compact = retained_forward(wt, modes, out_real.reshape(-1))
+    return vertical_forward(wt, nj, first, count, forward, compact)
+
+
 def vertical_forward_blas(w: Workload, nj: int, first, count, forward: np.ndarray, physical: np.ndarray) -> np.ndarray:
     """Same product as vertical_forward, through numpy's BLAS (multi-threaded):
     used only by the CPU baseline timing.  Equality with the C loop is

@@ -558,19 +558,18 @@ __global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void row2_r2c_kernel(const 
 // c2r pass-1 input: Z[k] = X_a[k] + i X_b[k] with the Hermitian extension;
 // Im of the kx = 0 and kx = Nx/2 bins is ignored and kx >= kc is zero (never
 // read).  Thread (pair s fastest, n2).
+// Core: the pair (pa, pa + 1) of row y (row_base = ps*kc*y); has_a / has_b
+// say which of the two planes exist.  Thread (s1 = t % S, n2 = t / S).
 template <typename R, int NX>
-__device__ __forceinline__ void c2r_load(cx<R>* v, const cx<R>* in, int item, int ny, int planes, int ps, int kc) {
+__device__ __forceinline__ void c2r_gather(cx<R>* v, const cx<R>* in, size_t row_base, int ps, int kc, int pa,
+                                           bool has_a, bool has_b) {
     using G = Geo2<R, NX>;
-    constexpr int N1 = G::N1, N2 = G::N2, S = G::PAIRS, TP = 2 * S;
-    const int t = threadIdx.x, s1 = t % S, n2 = t / S;
-    const int ntiles = (planes + TP - 1) / TP;
-    const int y = item / ntiles, p0 = (item % ntiles) * TP;
-    const int pa = p0 + 2 * s1;
+    constexpr int N1 = G::N1, N2 = G::N2, S = G::PAIRS;
+    const int n2 = static_cast<int>(threadIdx.x) / S;
     const bool act = n2 < N2;
-    const size_t row_base = static_cast<size_t>(ps) * static_cast<size_t>(kc) * y;
-    // FP32 with an even plane stride: the pair (pa, pa + 1) is one aligned
-    // 16-byte load (pa is even).
-    const bool wide = sizeof(R) == 4 && (ps & 1) == 0 && pa + 1 < planes;
+    // FP32 with an even plane stride and an even pa: the pair is one aligned
+    // 16-byte load.
+    const bool wide = sizeof(R) == 4 && ((ps | pa) & 1) == 0 && has_b;
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1) {
         const int k = N2 * n1 + n2;
@@ -585,12 +584,12 @@ __device__ __forceinline__ void c2r_load(cx<R>* v, const cx<R>* in, int item, in
                     xa = cx<R>{q.x, q.y};
                     xb = cx<R>{q.z, q.w};
                 } else {
-                    if (pa < planes) xa = in[base];
-                    if (pa + 1 < planes) xb = in[base + 1];
+                    if (has_a) xa = in[base];
+                    if (has_b) xb = in[base + 1];
                 }
             } else {
-                if (pa < planes) xa = in[base];
-                if (pa + 1 < planes) xb = in[base + 1];
+                if (has_a) xa = in[base];
+                if (has_b) xb = in[base + 1];
             }
         }
         if (kx == 0 || kx == NX / 2) {
@@ -599,6 +598,18 @@ __device__ __forceinline__ void c2r_load(cx<R>* v, const cx<R>* in, int item, in
         }
         v[n1] = upper ? cx<R>{xa.x + xb.y, xb.x - xa.y} : cx<R>{xa.x - xb.y, xa.y + xb.x};
     }
+}
+
+template <typename R, int NX>
+__device__ __forceinline__ void c2r_load(cx<R>* v, const cx<R>* in, int item, int ny, int planes, int ps, int kc) {
+    using G = Geo2<R, NX>;
+    constexpr int S = G::PAIRS, TP = 2 * S;
+    const int s1 = static_cast<int>(threadIdx.x) % S;
+    const int ntiles = (planes + TP - 1) / TP;
+    const int y = item / ntiles, p0 = (item % ntiles) * TP;
+    const int pa = p0 + 2 * s1;
+    c2r_gather<R, NX>(v, in, static_cast<size_t>(ps) * static_cast<size_t>(kc) * y, ps, kc, pa, pa < planes,
+                      pa + 1 < planes);
 }
 
 template <typename R, int NX, bool LOOP>
@@ -658,6 +669,141 @@ __global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void row2_c2r_kernel(const 
             for (int m = 0; m < N1; ++m) v[m] = vn[m];
         }
         item = next;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused physical-space stage of the flux graph (csrc/flux.hip).
+// T_in holds 3 + 3*targets fields of nz planes (plane = z + nz*field, plane
+// stride ps_in); T_out holds `targets` fields.  A work item is row y of a
+// tile of TP = 2*S consecutive z.  For each target t
+//     out_t = -scale * (u*q_t,x + v*q_t,y + w*q_t,z)
+// with every factor the c2r row transform of its field; the product is
+// formed in registers and transformed forward straight into T_out, so the
+// physical fields never reach HBM.
+// ---------------------------------------------------------------------------
+
+// c2r of one field's tile: on return thread (k1 = t % N1, s2 = t / N1) holds
+// out[k2] = (plane 2*s2, plane 2*s2 + 1) at x = k1 + N1*k2, k2 < N2.
+template <typename R, int NX>
+__device__ __forceinline__ void flux_c2r(cx<R>* out, const cx<R>* tin, cx<R>* buf, const cx<R>* twl, size_t row_base,
+                                         int ps, int kc, int pbase, int z0, int nz) {
+    using G = Geo2<R, NX>;
+    constexpr int N1 = G::N1, N2 = G::N2, S = G::PAIRS;
+    constexpr int YS = N2 + 1, SS = N1 * YS + 1;
+    const int t = threadIdx.x, s1 = t % S, n2 = t / S;
+    const int k1 = t % N1, s2 = t / N1;
+    const int za = z0 + 2 * s1;
+    cx<R> v[N1];
+    c2r_gather<R, NX>(v, tin, row_base, ps, kc, pbase + 2 * s1, za < nz, za + 1 < nz);
+    __syncthreads();  // buf and twl ready / free
+    if (n2 < N2) {
+        rdft<N1, true>(v);
+#pragma unroll
+        for (int k = 0; k < N1; ++k) buf[s1 * SS + k * YS + n2] = cmulc(v[k], twl[n2 * k]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < N2; ++m) out[m] = buf[s2 * SS + k1 * YS + m];
+    rdft<N2, true>(out);
+}
+
+// Forward transform of the register rows left by flux_c2r (x = k1 + N1*k2):
+// N = A*B with a = k1 (A = N1), b = k2 (B = N2):
+//     X[kk + N2*k2'] = sum_a W_N1^(a k2') W_N^(a kk) sum_b x[a + N1 b] W_N2^(b kk)
+// then the Hermitian split of each pair and the kx < kc store into T_out.
+template <typename R, int NX>
+__device__ __forceinline__ void flux_r2c_store(cx<R>* acc, cx<R>* tout, cx<R>* buf, const cx<R>* twl,
+                                               size_t row_base, int ps, int kc, int pbase, int z0, int nz) {
+    using G = Geo2<R, NX>;
+    constexpr int N1 = G::N1, N2 = G::N2, S = G::PAIRS, TP = 2 * S, NT = G::THREADS;
+    constexpr int YS2 = N1 + 1, SS2 = N2 * YS2 + 1, ZS = NX + 1;
+    const int t = threadIdx.x;
+    const int k1 = t % N1, s2 = t / N1;
+    const int kk = t % N2, s = t / N2;
+    rdft<N2, false>(acc);
+#pragma unroll
+    for (int k = 0; k < N2; ++k) acc[k] = cmul(acc[k], twl[k1 * k]);
+    __syncthreads();  // buf free
+#pragma unroll
+    for (int k = 0; k < N2; ++k) buf[s2 * SS2 + k * YS2 + k1] = acc[k];
+    __syncthreads();
+    cx<R> y[N1];
+    if (s < S) {
+#pragma unroll
+        for (int m = 0; m < N1; ++m) y[m] = buf[s * SS2 + kk * YS2 + m];
+        rdft<N1, false>(y);
+    }
+    __syncthreads();
+    if (s < S) {
+#pragma unroll
+        for (int k = 0; k < N1; ++k) buf[s * ZS + kk + N2 * k] = y[k];
+    }
+    __syncthreads();
+    for (int e = t; e < kc * TP; e += NT) {
+        const int pit = e % TP, kx = e / TP;
+        const int z = z0 + pit;
+        if (z >= nz) continue;
+        const cx<R> zk = buf[(pit >> 1) * ZS + kx];
+        const cx<R> zn = buf[(pit >> 1) * ZS + ((NX - kx) & (NX - 1))];
+        const cx<R> x = (pit & 1) == 0 ? cx<R>{R(0.5) * (zk.x + zn.x), R(0.5) * (zk.y - zn.y)}
+                                       : cx<R>{R(0.5) * (zk.y + zn.y), R(0.5) * (zn.x - zk.x)};
+        tout[row_base + static_cast<size_t>(ps) * kx + pbase + pit] = x;
+    }
+}
+
+template <typename R, int NX>
+struct FluxGeo {
+    using G = Geo2<R, NX>;
+    static constexpr int S = G::PAIRS;
+    static constexpr int SS = G::N1 * (G::N2 + 1) + 1, SS2 = G::N2 * (G::N1 + 1) + 1, ZS = NX + 1;
+    static constexpr int M1 = SS > SS2 ? SS : SS2;
+    static constexpr int BUF = S * (M1 > ZS ? M1 : ZS);
+};
+
+// KEEP: hold the c2r rows of u, v, w in registers for all targets (FP32);
+// otherwise recompute them per target (FP64, register budget).
+template <typename R, int NX, bool KEEP>
+__global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void flux_rows_kernel(const cx<R>* __restrict__ tin,
+                                                                          cx<R>* __restrict__ tout,
+                                                                          const cx<R>* __restrict__ tw, int nz,
+                                                                          int ps_in, int ps_out, int kc, int targets,
+                                                                          R scale, int items) {
+    using G = Geo2<R, NX>;
+    constexpr int N2 = G::N2, TP = 2 * G::PAIRS, NT = G::THREADS;
+    __shared__ cx<R> buf[FluxGeo<R, NX>::BUF];
+    __shared__ cx<R> twl[NX];
+    const int item = xcd_item(static_cast<int>(blockIdx.x), items);
+    if (item >= items) return;  // uniform across the workgroup
+    const int ztiles = (nz + TP - 1) / TP;
+    const int y = item / ztiles, z0 = (item % ztiles) * TP;
+    for (int i = threadIdx.x; i < NX; i += NT) twl[i] = tw[i];
+    const size_t rin = static_cast<size_t>(ps_in) * static_cast<size_t>(kc) * y;
+    const size_t rout = static_cast<size_t>(ps_out) * static_cast<size_t>(kc) * y;
+    cx<R> sh[KEEP ? 3 : 1][N2];
+    if constexpr (KEEP) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) flux_c2r<R, NX>(sh[d], tin, buf, twl, rin, ps_in, kc, d * nz + z0, z0, nz);
+    }
+    for (int tg = 0; tg < targets; ++tg) {
+        cx<R> acc[N2];
+#pragma unroll
+        for (int k = 0; k < N2; ++k) acc[k] = cx<R>{R(0), R(0)};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            cx<R>* sd = sh[KEEP ? d : 0];
+            if constexpr (!KEEP) flux_c2r<R, NX>(sd, tin, buf, twl, rin, ps_in, kc, d * nz + z0, z0, nz);
+            cx<R> q[N2];
+            flux_c2r<R, NX>(q, tin, buf, twl, rin, ps_in, kc, (3 + 3 * tg + d) * nz + z0, z0, nz);
+#pragma unroll
+            for (int k = 0; k < N2; ++k) {
+                acc[k].x += sd[k].x * q[k].x;
+                acc[k].y += sd[k].y * q[k].y;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < N2; ++k) acc[k] = cx<R>{-scale * acc[k].x, -scale * acc[k].y};
+        flux_r2c_store<R, NX>(acc, tout, buf, twl, rout, ps_out, kc, tg * nz + z0, z0, nz);
     }
 }
 
@@ -1510,6 +1656,59 @@ int skb_pruned_inverse_rows(skb_pruned_plan* plan, void* real, void* stream) {
         SKB_BY_PRECISION(plan, stage_inverse_rows, plan, real, as_stream(stream));
     });
 }
+
+}  // extern "C"
+
+namespace skb {
+
+bool flux_rows_supported(const skb_pruned_plan* pin, const skb_pruned_plan* pout) {
+    if (!pin || !pout) return false;
+    const auto &a = pin->w, &b = pout->w;
+    return a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.nx >= 64 && pin->kc == pout->kc &&
+           pin->precision == pout->precision && a.fields == 3 + 3 * b.fields;
+}
+
+namespace {
+template <typename R, int NX>
+void launch_flux_rows(skb_pruned_plan* pin, skb_pruned_plan* pout, int targets, double scale, hipStream_t s) {
+    if constexpr (NX >= 64) {
+        using G = Geo2<R, NX>;
+        constexpr int TP = 2 * G::PAIRS;
+        const int nz = static_cast<int>(pin->w.nz);
+        const int items = static_cast<int>(pin->w.ny) * ((nz + TP - 1) / TP);
+        hipLaunchKernelGGL((flux_rows_kernel<R, NX, sizeof(R) == 4>), dim3(xcd_grid(items)), dim3(G::THREADS), 0, s,
+                           static_cast<const cx<R>*>(pin->scratch.ptr), static_cast<cx<R>*>(pout->scratch.ptr),
+                           pin->tw.x.as<cx<R>>(), nz, pin->ps, pout->ps, pin->kc, targets, static_cast<R>(scale),
+                           items);
+        check_launch("flux rows launch");
+    } else {
+        fail(SKB_UNSUPPORTED, "fused flux rows need Nx >= 64");
+    }
+}
+
+template <typename R>
+void flux_rows_t(skb_pruned_plan* pin, skb_pruned_plan* pout, int targets, double scale, hipStream_t s) {
+    switch (pin->w.nx) {
+#define SKB_FLUX_CASE(N)                                             \
+    case N:                                                          \
+        launch_flux_rows<R, N>(pin, pout, targets, scale, s);        \
+        return;
+        SKB_POW2_CASES(SKB_FLUX_CASE)
+#undef SKB_FLUX_CASE
+    }
+    fail(SKB_INVALID_ARGUMENT, "unsupported Nx");
+}
+}  // namespace
+
+void flux_rows(skb_pruned_plan* pin, skb_pruned_plan* pout, int targets, double scale, void* stream) {
+    if (!flux_rows_supported(pin, pout)) fail(SKB_RUNTIME_ERROR, "fused flux rows: mismatched plans");
+    if (pin->precision == SKB_F64) flux_rows_t<double>(pin, pout, targets, scale, as_stream(stream));
+    else flux_rows_t<float>(pin, pout, targets, scale, as_stream(stream));
+}
+
+}  // namespace skb
+
+extern "C" {
 
 int skb_pruned_plan_set_grid(skb_pruned_plan* plan, int stage, int workgroups) {
     return guarded([&] {

@@ -1,0 +1,130 @@
+// Antialiased nonlinear flux graph: the production-lifetime flux of
+// src/benchmark.cpp:3957-4234, composed from the provider's plans.
+//
+// Inputs are 3 + 3T modal fields [U, V, W, q0x, q0y, q0z, ..., q(T-1)z]
+// (modal layout j + Nj*field + Nj*F*mode); outputs are T modal fields.  Per
+// evaluation:
+//   1. vertical inverse of all input fields     modal -> compact physical
+//   2. pruned horizontal inverse                 compact -> real volumes
+//   3. per target t: target = -(u*qtx + v*qty + w*qtz) / (Nx*Ny)^2
+//      (both inverses are unnormalised; src/benchmark.cpp:4040-4041)
+//   4. pruned horizontal forward (raw)           real -> compact
+//   5. vertical forward of the T outputs         compact -> modal
+// Steps 2-4 run either as separate kernels (real volumes in HBM) or fused
+// into one row pass that keeps the physical fields in registers
+// (skb_flux_set_fused).
+#include <memory>
+
+#include "common.hpp"
+#include "flux_rows.hpp"
+
+struct skb_flux_plan {
+    skb_workload w{};  // fields = targets (the output workload)
+    int precision = SKB_F64;
+    uint64_t targets = 0, modes = 0, nj = 0;
+    skb_vertical_plan* vin = nullptr;   // 3 + 3T fields, inverse direction
+    skb_vertical_plan* vout = nullptr;  // T fields, forward direction
+    skb_pruned_plan* pin = nullptr;     // 3 + 3T fields
+    skb_pruned_plan* pout = nullptr;    // T fields
+    skb::DeviceBuffer phys_in, phys_out, real_in, real_out;
+    bool fused = true;
+    ~skb_flux_plan() {
+        if (vin) skb_vertical_plan_destroy(vin);
+        if (vout) skb_vertical_plan_destroy(vout);
+        if (pin) skb_pruned_plan_destroy(pin);
+        if (pout) skb_pruned_plan_destroy(pout);
+    }
+};
+
+using namespace skb;
+
+namespace {
+
+void check_status(int status) {
+    if (status != SKB_OK) {
+        char msg[1024];
+        skb_last_error(msg, sizeof(msg));
+        fail(status, msg);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int skb_flux_plan_create(const skb_workload* workload, const skb_retained_mode* modes, uint64_t mode_count,
+                         uint64_t nj, const skb_mode_group* groups, uint64_t group_count, const double* forward,
+                         const double* inverse, int precision, skb_flux_plan** plan) {
+    return guarded([&] {
+        if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan pointer");
+        *plan = nullptr;
+        check_precision(precision);
+        validate_workload(workload);
+        if (workload->nx != workload->ny)
+            fail(SKB_INVALID_ARGUMENT, "production-lifetime-flux requires a square horizontal grid.");
+        if (!modes || mode_count == 0) fail(SKB_INVALID_ARGUMENT, "the flux graph requires retained modes");
+        const uint64_t targets = workload->fields;
+        if (3 + 3 * targets > 48) fail(SKB_UNSUPPORTED, "at most 15 flux targets");
+        auto p = std::make_unique<skb_flux_plan>();
+        p->w = *workload;
+        p->precision = precision;
+        p->targets = targets;
+        p->modes = mode_count;
+        p->nj = nj;
+        skb_workload win = *workload;
+        win.fields = 3 + 3 * targets;
+        check_status(skb_vertical_plan_create(&win, nj, groups, group_count, forward, inverse, precision, &p->vin));
+        check_status(skb_vertical_plan_create(workload, nj, groups, group_count, forward, inverse, precision,
+                                              &p->vout));
+        if (skb_vertical_plan_items(p->vin) == 0) fail(SKB_INVALID_ARGUMENT, "empty vertical family");
+        check_status(skb_pruned_plan_create(&win, modes, mode_count, precision, &p->pin));
+        check_status(skb_pruned_plan_create(workload, modes, mode_count, precision, &p->pout));
+        const uint64_t cb = complex_bytes(precision), rb = real_bytes(precision);
+        const uint64_t volume = workload->nx * workload->ny * workload->nz;
+        p->phys_in.allocate(mode_count * workload->nz * win.fields * cb);
+        p->phys_out.allocate(mode_count * workload->nz * targets * cb);
+        p->real_in.allocate(volume * win.fields * rb);
+        p->real_out.allocate(volume * targets * rb);
+        *plan = p.release();
+    });
+}
+
+int skb_flux_plan_destroy(skb_flux_plan* plan) {
+    return guarded([&] { delete plan; });
+}
+
+int skb_flux_set_fused(skb_flux_plan* plan, int fused) {
+    return guarded([&] {
+        if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
+        plan->fused = fused != 0;
+    });
+}
+
+int skb_flux_execute(skb_flux_plan* plan, const void* input_modal, void* output_modal, void* stream) {
+    return guarded([&] {
+        if (!plan || !input_modal || !output_modal) fail(SKB_INVALID_ARGUMENT, "null argument");
+        auto& p = *plan;
+        const auto& w = p.w;
+        const double scale = 1.0 / (static_cast<double>(w.nx * w.ny) * static_cast<double>(w.nx * w.ny));
+        check_status(skb_vertical_inverse(p.vin, input_modal, p.phys_in.ptr, stream));
+        if (p.fused && flux_rows_supported(p.pin, p.pout)) {
+            check_status(skb_pruned_inverse_columns(p.pin, p.phys_in.ptr, stream));
+            flux_rows(p.pin, p.pout, static_cast<int>(p.targets), scale, stream);
+            check_status(skb_pruned_forward_columns(p.pout, p.phys_out.ptr, stream));
+        } else {
+            const uint64_t volume = w.nx * w.ny * w.nz;
+            const size_t rb = real_bytes(p.precision);
+            check_status(skb_pruned_inverse(p.pin, p.phys_in.ptr, p.real_in.ptr, stream));
+            for (uint64_t t = 0; t < p.targets; ++t) {
+                const char* base = static_cast<const char*>(p.real_in.ptr);
+                check_status(skb_pointwise_advection(volume, scale, base, base + (3 + 3 * t) * volume * rb,
+                                                     static_cast<char*>(p.real_out.ptr) + t * volume * rb,
+                                                     p.precision, stream));
+            }
+            check_status(skb_pruned_forward(p.pout, p.real_out.ptr, p.phys_out.ptr, stream));
+        }
+        check_status(skb_vertical_forward(p.vout, p.phys_out.ptr, output_modal, stream));
+    });
+}
+
+}  // extern "C"

@@ -137,6 +137,10 @@ def _load():
         "skb_vertical_matrix_bytes": (u64, [vp]),
         "skb_pointwise_advection": (i32, [u64, dbl, vp, vp, vp, i32, vp]),
         "skb_vertical_wvm_plan_create": (i32, [W, vp, u64, u64, vp, u64, vp, vp, i32, pp]),
+        "skb_flux_plan_create": (i32, [W, vp, u64, u64, vp, u64, vp, vp, i32, pp]),
+        "skb_flux_plan_destroy": (i32, [vp]),
+        "skb_flux_set_fused": (i32, [vp, i32]),
+        "skb_flux_execute": (i32, [vp, vp, vp, vp]),
         "skb_vertical_wvm_forward": (i32, [vp, vp, vp, vp]),
         "skb_vertical_wvm_inverse": (i32, [vp, vp, vp, vp]),
         "skb_vertical_wvm_modal_elements": (u64, [vp]),
@@ -164,7 +168,8 @@ EXPORTED = [
     "skb_vertical_plan_set_grid", "skb_vertical_plan_items", "skb_pruned_forward_split", "skb_pruned_inverse_split",
     "skb_pruned_forward_split_fields", "skb_pruned_inverse_split_fields", "skb_vertical_wvm_plan_create",
     "skb_vertical_wvm_forward", "skb_vertical_wvm_inverse", "skb_vertical_wvm_modal_elements",
-    "skb_vertical_wvm_initialize_modal", "skb_vertical_wvm_initialize_spectrum",
+    "skb_vertical_wvm_initialize_modal", "skb_vertical_wvm_initialize_spectrum", "skb_flux_plan_create",
+    "skb_flux_plan_destroy", "skb_flux_set_fused", "skb_flux_execute",
 ]
 
 
@@ -586,6 +591,50 @@ class WvmVerticalPlan:
     def close(self):
         if getattr(self, "h", None):
             lib.skb_vertical_plan_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+
+class FluxPlan:
+    """Production-lifetime flux graph (src/benchmark.cpp:3957-4234): 3 + 3T
+    modal input fields [U, V, W, q0x, q0y, q0z, ...] -> T modal outputs, with
+    T = w.fields.  `fused` selects the single-pass physical-space stage."""
+
+    def __init__(self, w: Workload, modes: np.ndarray, nj: int, groups: np.ndarray, forward: np.ndarray,
+                 inverse: np.ndarray, precision: int = SKB_F64):
+        self.w, self.nj, self.precision = w, nj, precision
+        self.modes = len(modes)
+        self.targets = w.fields
+        self._wc = w.c()
+        forward = np.ascontiguousarray(forward, np.float64)
+        inverse = np.ascontiguousarray(inverse, np.float64)
+        if forward.size != len(groups) * nj * w.nz or inverse.size != forward.size:
+            raise ValueError("matrix family size does not match groups x nj x nz")
+        h = ctypes.c_void_p()
+        check(lib.skb_flux_plan_create(ctypes.byref(self._wc), _np(modes), len(modes), nj, _np(groups), len(groups),
+                                       _np(forward), _np(inverse), precision, ctypes.byref(h)))
+        self.h = h
+
+    @property
+    def input_elements(self) -> int:
+        return self.modes * self.nj * (3 + 3 * self.targets)
+
+    @property
+    def output_elements(self) -> int:
+        return self.modes * self.nj * self.targets
+
+    def set_fused(self, fused: bool):
+        check(lib.skb_flux_set_fused(self.h, 1 if fused else 0))
+
+    def execute(self, input_modal, output_modal, stream=None):
+        cd = complex_dtype(self.precision)
+        check(lib.skb_flux_execute(self.h, _dev(input_modal, self.input_elements, cd, "input_modal"),
+                                   _dev(output_modal, self.output_elements, cd, "output_modal"), _stream(stream)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.skb_flux_plan_destroy(self.h)
             self.h = None
 
     __del__ = close

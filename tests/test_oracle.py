@@ -353,3 +353,47 @@ def test_wvm_direct_vertical_matches_compact_path_and_partner_symmetry():
     touched |= set(nxh * ((w.ny - int(m["stored_ky"])) % w.ny) for m in modes if m["stored_kx"] == 0)
     untouched = [f for f in range(nxh * w.ny) if f not in touched]
     assert untouched and np.all(blocks[untouched] == 0)
+
+
+def test_flux_oracle_direct_and_fft_compositions_agree():
+    w = O.Workload(8, 8, 7, 2)  # 2 targets: 9 input fields
+    modes = O.retained_modes(w)
+    keys, first, count = O.k2_groups(modes)
+    nj = 2 * (w.nz - 1) // 3
+    fwd, inv = O.k2_vertical_family(w.nz, nj, keys)
+    x = O.flux_modal_fixture(w, modes, nj, 3 + 3 * w.fields, 41)
+    a = O.production_flux(w, modes, nj, first, count, fwd, inv, x, w.fields, use_direct=True)
+    b = O.production_flux(w, modes, nj, first, count, fwd, inv, x, w.fields, use_direct=False)
+    assert O.max_relative_error(b, a) < 1e-12
+    assert np.abs(a).max() > 0
+
+
+def test_flux_oracle_dc_closed_form():
+    """DC-only input: every physical field is constant on each plane, so the
+    product is constant, the raw forward transform puts NxNy * product in the
+    DC bin and nothing elsewhere, and the DC modal output is
+    F_dc (NxNy * -(u qx + v qy + w qz) / (NxNy)^2) per target."""
+    w = O.Workload(16, 16, 9, 2)
+    modes = O.retained_modes(w)
+    keys, first, count = O.k2_groups(modes)
+    nj = 2 * (w.nz - 1) // 3
+    fwd, inv = O.k2_vertical_family(w.nz, nj, keys)
+    nin = 3 + 3 * w.fields
+    rng = np.random.default_rng(9)
+    x = np.zeros((len(modes), nin, nj), np.complex128)
+    dc = int(np.nonzero((modes["k"] == 0) & (modes["l"] == 0))[0][0])
+    x[dc] = rng.standard_normal((nin, nj))
+    got = O.production_flux(w, modes, nj, first, count, fwd, inv, x.reshape(-1), w.fields)
+    got = got.reshape(len(modes), w.fields, nj)
+    g = int(np.searchsorted(first.astype(np.int64), dc, side="right") - 1)
+    F = fwd.reshape(len(first), nj, w.nz)[g]
+    I = inv.reshape(len(first), w.nz, nj)[g]
+    phys = x[dc].real @ I.T  # (nin, nz): z profile of every field
+    n2 = float(w.nx * w.ny)
+    for t in range(w.fields):
+        d = 3 + 3 * t
+        prod = -(phys[0] * phys[d] + phys[1] * phys[d + 1] + phys[2] * phys[d + 2]) / n2 ** 2
+        expect = F @ (n2 * prod)
+        assert np.allclose(got[dc, t], expect, rtol=1e-12, atol=1e-14 * np.abs(expect).max())
+    others = np.delete(got, dc, axis=0)
+    assert np.abs(others).max() < 1e-12 * np.abs(got[dc]).max()
