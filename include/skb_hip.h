@@ -248,9 +248,10 @@ int skb_vertical_wvm_initialize_spectrum(skb_vertical_plan* plan, void* full_spe
  * inverse (K^2 family, Nj x Nz per group) -> pruned horizontal inverse ->
  * per target -(u*qx + v*qy + w*qz)/(Nx*Ny)^2 -> raw pruned forward ->
  * vertical forward.  The grid must be square (src/benchmark.cpp:3981-3984).
- * By default the physical-space stage is one fused row pass (inverse rows,
- * products and forward rows in registers; Nx >= 64); skb_flux_set_fused(plan, 0)
- * selects separate kernels with the real fields in HBM. */
+ * By default FP32 plans with Nx >= 64 run the physical-space stage as one
+ * fused row pass (inverse rows, products and forward rows in registers);
+ * skb_flux_set_fused(plan, 0), FP64 plans and smaller grids use separate
+ * kernels with the real fields in HBM. */
 typedef struct skb_flux_plan skb_flux_plan;
 int skb_flux_plan_create(const skb_workload* workload, const skb_retained_mode* modes, uint64_t mode_count,
                          uint64_t nj, const skb_mode_group* groups, uint64_t group_count, const double* forward,
@@ -258,6 +259,26 @@ int skb_flux_plan_create(const skb_workload* workload, const skb_retained_mode* 
 int skb_flux_plan_destroy(skb_flux_plan* plan);
 int skb_flux_set_fused(skb_flux_plan* plan, int fused);
 int skb_flux_execute(skb_flux_plan* plan, const void* input_modal, void* output_modal, void* stream);
+
+/* Advection flux of one tracer, the graph above with spectral inputs (no
+ * reference counterpart as a single call; it composes FFTWPrunedProvider
+ * inverse / forward, a vertical product and PointwiseAdvectionExecutor).
+ * workload->fields must be 4: the compact retained input holds u, v, w, q
+ * (plane z + Nz*field).  q_x = i*2pi*k/lx q and q_y = i*2pi*l/ly q are formed
+ * spectrally, q_z = D q with D the Nz x Nz row-major derivative matrix (on
+ * MFMA), and the output is the compact retained field
+ * F_xy[-(u q_x + v q_y + w q_z)] / (Nx*Ny)^2 of the unnormalised inverses,
+ * truncated to the retained modes. */
+typedef struct skb_advection_flux_plan skb_advection_flux_plan;
+int skb_advection_flux_plan_create(const skb_workload* workload, const skb_retained_mode* modes, uint64_t mode_count,
+                                   const double* dz, int precision, skb_advection_flux_plan** plan);
+int skb_advection_flux_plan_destroy(skb_advection_flux_plan* plan);
+int skb_advection_flux_set_fused(skb_advection_flux_plan* plan, int fused);
+int skb_advection_flux_execute(skb_advection_flux_plan* plan, const void* spectral, void* flux, void* stream);
+/* Second-order centred d/dz on Nz uniform levels over [0, lz], one-sided
+ * second order at both ends (rows sum to zero); Nz x Nz row-major.  No
+ * reference counterpart. */
+int skb_vertical_derivative_matrix(uint64_t nz, double lz, double* d);
 
 /* ---- pointwise advection -----------------------------------------------
  * Replaces PointwiseAdvectionExecutor::execute (src/pointwise_advection.hpp

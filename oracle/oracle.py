@@ -473,6 +473,46 @@ def production_flux(w: Workload, modes: np.ndarray, nj: int, first, count, forwa
     return vertical_forward(wt, nj, first, count, forward, compact)
 
 
+def vertical_derivative_matrix(nz: int, lz: float = 1.0) -> np.ndarray:
+    """Second-order centred d/dz on nz uniform levels over [0, lz], one-sided
+    second order at the ends (the matrix the provider builds; no reference
+    counterpart)."""
+    h = lz / (nz - 1)
+    d = np.zeros((nz, nz))
+    d[0, :3] = np.array([-1.5, 2.0, -0.5]) / h
+    for k in range(1, nz - 1):
+        d[k, k - 1], d[k, k + 1] = -0.5 / h, 0.5 / h
+    d[nz - 1, nz - 3:] = np.array([0.5, -2.0, 1.5]) / h
+    return d.reshape(-1)
+
+
+def advection_flux(w: Workload, modes: np.ndarray, dz: np.ndarray, spectral: np.ndarray,
+                   use_direct: bool = False) -> np.ndarray:
+    """Advection flux of one tracer from compact retained (u, v, w, q):
+    q_x = i 2pi k/lx q, q_y = i 2pi l/ly q (logical (k, l) of each mode),
+    q_z = D q along z; unnormalised inverses of the six fields; product
+    -(u q_x + v q_y + w q_z)/(Nx Ny)^2; raw forward truncated to the modes."""
+    n = len(modes)
+    sp = np.asarray(spectral, np.complex128).reshape(n, 4, w.nz)
+    u, v, ww, q = sp[:, 0], sp[:, 1], sp[:, 2], sp[:, 3]
+    k = modes["k"].astype(np.float64)[:, None]
+    l = modes["l"].astype(np.float64)[:, None]
+    D = np.asarray(dz, np.float64).reshape(w.nz, w.nz)
+    six = np.stack([u, v, ww, 1j * (2 * np.pi / w.lx) * k * q, 1j * (2 * np.pi / w.ly) * l * q, q @ D.T], axis=1)
+    w6 = Workload(w.nx, w.ny, w.nz, 6, w.lx, w.ly)
+    w1 = Workload(w.nx, w.ny, w.nz, 1, w.lx, w.ly)
+    if use_direct:
+        real = direct_c2r(w6, embed_retained(w6, modes, six.reshape(-1)))
+    else:
+        real = retained_inverse(w6, modes, six.reshape(-1))
+    vol = w.nx * w.ny * w.nz
+    r = real.reshape(6, vol)
+    prod = -(r[0] * r[3] + r[1] * r[4] + r[2] * r[5]) / float(w.nx * w.ny) ** 2
+    if use_direct:
+        return gather_retained(w1, modes, direct_r2c(w1, prod))
+    return retained_forward(w1, modes, prod)
+
+
 def vertical_forward_blas(w: Workload, nj: int, first, count, forward: np.ndarray, physical: np.ndarray) -> np.ndarray:
     """Same product as vertical_forward, through numpy's BLAS (multi-threaded):
     used only by the CPU baseline timing.  Equality with the C loop is

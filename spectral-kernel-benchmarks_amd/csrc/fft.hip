@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "flux_rows.hpp"
 
 namespace skb {
 
@@ -243,19 +244,25 @@ __global__ __launch_bounds__(kThreads) void row_c2r_kernel(const cx<R>* __restri
 // ---------------------------------------------------------------------------
 enum ColIO { kStrided = 0, kRetained = 1, kSplit = 2 };
 
-// Compact retained operand in split form (kSplit): separate real and
-// imaginary arrays, one view per field.  Plane p = z + nz * f of mode m lives
-// at re[f][z + stride[f] * m] and im[f][...].  The plain split layout
-// (plane + P * mode) is one view with nz = P and stride = P.  Forward outputs
-// are multiplied by `scale` (src/pruned_fftw_provider.cpp:1053-1083).
+// Compact retained operand given as one view per field (kSplit).  Plane
+// p = z + nz * f of mode m lives at re[f][zs*z + stride[f]*m] and
+// im[f][...] (R elements).  Split arrays have zs = 1 and im apart from re;
+// interleaved complex arrays have zs = 2 and im = re + 1.  The plain split
+// layout (plane + P * mode) is one view with nz = P and stride = P.  Forward
+// outputs are multiplied by `scale` (src/pruned_fftw_provider.cpp:1053-1083).
+// Inputs may carry a spectral derivative factor per view, applied to the
+// stored value at column kx, row ky: i*kx_scale*kx (kFactorDx) or
+// i*ky_scale*l with l the signed row (kFactorDy) -- the stored-coordinate form
+// of i*k*q at the logical (k, l), conjugation included.
 constexpr int kMaxSplitViews = 16;
 template <typename R>
 struct SplitViews {
     R* re[kMaxSplitViews];
     R* im[kMaxSplitViews];
     long long stride[kMaxSplitViews];
-    int nz;
-    R scale;
+    int factor[kMaxSplitViews];
+    int nz, zs, ny;
+    R scale, kx_scale, ky_scale;
 };
 
 // The view of plane p, with z folded into the pointers (lanes hold one plane).
@@ -264,19 +271,29 @@ struct ViewLane {
     R* re;
     R* im;
     size_t stride;
+    int factor;
 };
 template <typename R>
 __device__ __forceinline__ ViewLane<R> view_lane(const SplitViews<R>& v, int p) {
     const int f = p / v.nz, z = p - f * v.nz;
-    ViewLane<R> out{v.re[0], v.im[0], static_cast<size_t>(v.stride[0])};
+    ViewLane<R> out{v.re[0], v.im[0], static_cast<size_t>(v.stride[0]), v.factor[0]};
     // unrolled select: a lane-varying index into the argument array would go
     // through private memory
 #pragma unroll
     for (int i = 1; i < kMaxSplitViews; ++i)
-        if (f == i) out = ViewLane<R>{v.re[i], v.im[i], static_cast<size_t>(v.stride[i])};
-    out.re += z;
-    out.im += z;
+        if (f == i) out = ViewLane<R>{v.re[i], v.im[i], static_cast<size_t>(v.stride[i]), v.factor[i]};
+    out.re += static_cast<size_t>(v.zs) * z;
+    out.im += static_cast<size_t>(v.zs) * z;
     return out;
+}
+
+// x * i * a for the view's derivative factor at stored (kx, row).
+template <typename R>
+__device__ __forceinline__ cx<R> apply_factor(const SplitViews<R>& v, int factor, int kx, int row, cx<R> x) {
+    if (factor == kFactorNone) return x;
+    const R a = factor == kFactorDx ? v.kx_scale * static_cast<R>(kx)
+                                    : v.ky_scale * static_cast<R>(2 * row < v.ny ? row : row - v.ny);
+    return cx<R>{-a * x.y, a * x.x};
 }
 
 template <typename R, int NY, bool INV, int IN, int OUT>
@@ -311,10 +328,12 @@ __global__ __launch_bounds__(kThreads) void col_kernel(const cx<R>* in, cx<R>* o
             if constexpr (IN == kSplit) {
                 const ViewLane<R> vl = view_lane(views, p);
                 v = cx<R>{vl.re[vl.stride * ent.mode], vl.im[vl.stride * ent.mode]};
+                if (ent.conj) v.y = -v.y;
+                v = apply_factor(views, vl.factor, kx, ent.row, v);
             } else {
                 v = in[P * ent.mode + p];
+                if (ent.conj) v.y = -v.y;
             }
-            if (ent.conj) v.y = -v.y;
             buf[pit * SEQ + ent.row] = v;
         }
     }
@@ -761,9 +780,9 @@ struct FluxGeo {
     static constexpr int BUF = S * (M1 > ZS ? M1 : ZS);
 };
 
-// KEEP: hold the c2r rows of u, v, w in registers for all targets (FP32);
-// otherwise recompute them per target (FP64, register budget).
-template <typename R, int NX, bool KEEP>
+// The c2r rows of u, v, w stay in registers for all targets.  FP32 only: in
+// FP64 the register set does not fit, and the separate-kernel path is faster.
+template <typename R, int NX>
 __global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void flux_rows_kernel(const cx<R>* __restrict__ tin,
                                                                           cx<R>* __restrict__ tout,
                                                                           const cx<R>* __restrict__ tw, int nz,
@@ -780,19 +799,16 @@ __global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void flux_rows_kernel(const
     for (int i = threadIdx.x; i < NX; i += NT) twl[i] = tw[i];
     const size_t rin = static_cast<size_t>(ps_in) * static_cast<size_t>(kc) * y;
     const size_t rout = static_cast<size_t>(ps_out) * static_cast<size_t>(kc) * y;
-    cx<R> sh[KEEP ? 3 : 1][N2];
-    if constexpr (KEEP) {
+    cx<R> sh[3][N2];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) flux_c2r<R, NX>(sh[d], tin, buf, twl, rin, ps_in, kc, d * nz + z0, z0, nz);
-    }
+    for (int d = 0; d < 3; ++d) flux_c2r<R, NX>(sh[d], tin, buf, twl, rin, ps_in, kc, d * nz + z0, z0, nz);
     for (int tg = 0; tg < targets; ++tg) {
         cx<R> acc[N2];
 #pragma unroll
         for (int k = 0; k < N2; ++k) acc[k] = cx<R>{R(0), R(0)};
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            cx<R>* sd = sh[KEEP ? d : 0];
-            if constexpr (!KEEP) flux_c2r<R, NX>(sd, tin, buf, twl, rin, ps_in, kc, d * nz + z0, z0, nz);
+            const cx<R>* sd = sh[d];
             cx<R> q[N2];
             flux_c2r<R, NX>(q, tin, buf, twl, rin, ps_in, kc, (3 + 3 * tg + d) * nz + z0, z0, nz);
 #pragma unroll
@@ -850,10 +866,12 @@ __device__ __forceinline__ void col2_load(cx<R>* v, const cx<R>* in, int item, i
                 if constexpr (IN == kSplit) {
                     const size_t at = vl.stride * static_cast<size_t>(code[n1] >> 1);
                     x = cx<R>{vl.re[at], vl.im[at]};
+                    if (code[n1] & 1) x.y = -x.y;
+                    x = apply_factor(views, vl.factor, kx, N2 * n1 + n2, x);
                 } else {
                     x = in[P * (code[n1] >> 1) + p];
+                    if (code[n1] & 1) x.y = -x.y;
                 }
-                if (code[n1] & 1) x.y = -x.y;
             }
             v[n1] = x;
         }
@@ -1372,6 +1390,7 @@ SplitViews<R> plain_split(const void* re, const void* im, int planes, double sca
     v.im[0] = static_cast<R*>(const_cast<void*>(im));
     v.stride[0] = planes;
     v.nz = planes;
+    v.zs = 1;
     v.scale = static_cast<R>(scale);
     return v;
 }
@@ -1396,6 +1415,7 @@ SplitViews<R> field_split(const skb_pruned_plan* p, const skb_field_view* fields
         v.stride[f] = static_cast<long long>(fields[f].mode_stride);
     }
     v.nz = static_cast<int>(p->w.nz);
+    v.zs = 1;
     v.scale = static_cast<R>(scale);
     return v;
 }
@@ -1665,7 +1685,7 @@ bool flux_rows_supported(const skb_pruned_plan* pin, const skb_pruned_plan* pout
     if (!pin || !pout) return false;
     const auto &a = pin->w, &b = pout->w;
     return a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.nx >= 64 && pin->kc == pout->kc &&
-           pin->precision == pout->precision && a.fields == 3 + 3 * b.fields;
+           pin->precision == SKB_F32 && pout->precision == SKB_F32 && a.fields == 3 + 3 * b.fields;
 }
 
 namespace {
@@ -1676,7 +1696,7 @@ void launch_flux_rows(skb_pruned_plan* pin, skb_pruned_plan* pout, int targets, 
         constexpr int TP = 2 * G::PAIRS;
         const int nz = static_cast<int>(pin->w.nz);
         const int items = static_cast<int>(pin->w.ny) * ((nz + TP - 1) / TP);
-        hipLaunchKernelGGL((flux_rows_kernel<R, NX, sizeof(R) == 4>), dim3(xcd_grid(items)), dim3(G::THREADS), 0, s,
+        hipLaunchKernelGGL((flux_rows_kernel<R, NX>), dim3(xcd_grid(items)), dim3(G::THREADS), 0, s,
                            static_cast<const cx<R>*>(pin->scratch.ptr), static_cast<cx<R>*>(pout->scratch.ptr),
                            pin->tw.x.as<cx<R>>(), nz, pin->ps, pout->ps, pin->kc, targets, static_cast<R>(scale),
                            items);
@@ -1700,10 +1720,40 @@ void flux_rows_t(skb_pruned_plan* pin, skb_pruned_plan* pout, int targets, doubl
 }
 }  // namespace
 
+namespace {
+template <typename R>
+void flux_inverse_columns_t(skb_pruned_plan* pin, const FluxSource* src, int count, hipStream_t s) {
+    SplitViews<R> v{};
+    const int nz = static_cast<int>(pin->w.nz);
+    for (int f = 0; f < count; ++f) {
+        if (!src[f].base || src[f].field < 0 || src[f].field >= src[f].fields)
+            fail(SKB_INVALID_ARGUMENT, "bad flux source");
+        R* re = static_cast<R*>(const_cast<void*>(src[f].base)) + 2 * static_cast<size_t>(nz) * src[f].field;
+        v.re[f] = re;
+        v.im[f] = re + 1;
+        v.stride[f] = 2LL * nz * src[f].fields;
+        v.factor[f] = src[f].factor;
+    }
+    v.nz = nz;
+    v.zs = 2;
+    v.ny = static_cast<int>(pin->w.ny);
+    v.scale = R(1);
+    v.kx_scale = static_cast<R>(2.0 * M_PI / pin->w.lx);
+    v.ky_scale = static_cast<R>(2.0 * M_PI / pin->w.ly);
+    stage_inverse_columns_split<R>(pin, v, s);
+}
+}  // namespace
+
+void flux_inverse_columns(skb_pruned_plan* pin, const FluxSource* sources, int count, void* stream) {
+    if (!pin || !sources || count != static_cast<int>(pin->w.fields) || count > kMaxSplitViews)
+        fail(SKB_INVALID_ARGUMENT, "flux sources must match the plan's fields (at most 16)");
+    if (pin->precision == SKB_F64) flux_inverse_columns_t<double>(pin, sources, count, as_stream(stream));
+    else flux_inverse_columns_t<float>(pin, sources, count, as_stream(stream));
+}
+
 void flux_rows(skb_pruned_plan* pin, skb_pruned_plan* pout, int targets, double scale, void* stream) {
     if (!flux_rows_supported(pin, pout)) fail(SKB_RUNTIME_ERROR, "fused flux rows: mismatched plans");
-    if (pin->precision == SKB_F64) flux_rows_t<double>(pin, pout, targets, scale, as_stream(stream));
-    else flux_rows_t<float>(pin, pout, targets, scale, as_stream(stream));
+    flux_rows_t<float>(pin, pout, targets, scale, as_stream(stream));
 }
 
 }  // namespace skb

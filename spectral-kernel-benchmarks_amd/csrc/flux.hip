@@ -13,6 +13,14 @@
 // Steps 2-4 run either as separate kernels (real volumes in HBM) or fused
 // into one row pass that keeps the physical fields in registers
 // (skb_flux_set_fused).
+//
+// The advection flux of one tracer (skb_advection_flux_*) is the same graph
+// with spectral inputs: fields (u, v, w, q) in the compact retained layout;
+// d/dx q and d/dy q are spectral (i*2pi*k/lx, i*2pi*l/ly, applied inside the
+// inverse column pass), d/dz q = D q through an Nz x Nz matrix on the
+// vertical GEMM; one output field -(u q_x + v q_y + w q_z)/(Nx*Ny)^2,
+// forward-transformed and truncated to the retained modes.
+#include <algorithm>
 #include <memory>
 
 #include "common.hpp"
@@ -49,6 +57,22 @@ void check_status(int status) {
 }
 
 }  // namespace
+
+struct skb_advection_flux_plan {
+    skb_workload w{};  // fields = 4: u, v, w, q
+    int precision = SKB_F64;
+    uint64_t modes = 0;
+    skb_vertical_plan* dz = nullptr;  // one group holding D, fields = 1
+    skb_pruned_plan* pin = nullptr;   // 6 fields: u, v, w, q_x, q_y, q_z
+    skb_pruned_plan* pout = nullptr;  // 1 field
+    skb::DeviceBuffer qz, real_in, real_out;
+    bool fused = true;
+    ~skb_advection_flux_plan() {
+        if (dz) skb_vertical_plan_destroy(dz);
+        if (pin) skb_pruned_plan_destroy(pin);
+        if (pout) skb_pruned_plan_destroy(pout);
+    }
+};
 
 extern "C" {
 
@@ -124,6 +148,94 @@ int skb_flux_execute(skb_flux_plan* plan, const void* input_modal, void* output_
             check_status(skb_pruned_forward(p.pout, p.real_out.ptr, p.phys_out.ptr, stream));
         }
         check_status(skb_vertical_forward(p.vout, p.phys_out.ptr, output_modal, stream));
+    });
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int skb_vertical_derivative_matrix(uint64_t nz, double lz, double* d) {
+    return guarded([&] {
+        if (!d || nz < 3 || !(lz > 0.0)) fail(SKB_INVALID_ARGUMENT, "derivative matrix needs nz >= 3 and lz > 0");
+        const double h = lz / static_cast<double>(nz - 1);
+        std::fill_n(d, nz * nz, 0.0);
+        d[0] = -1.5 / h;
+        d[1] = 2.0 / h;
+        d[2] = -0.5 / h;
+        for (uint64_t k = 1; k + 1 < nz; ++k) {
+            d[k * nz + k - 1] = -0.5 / h;
+            d[k * nz + k + 1] = 0.5 / h;
+        }
+        d[(nz - 1) * nz + nz - 3] = 0.5 / h;
+        d[(nz - 1) * nz + nz - 2] = -2.0 / h;
+        d[(nz - 1) * nz + nz - 1] = 1.5 / h;
+    });
+}
+
+int skb_advection_flux_plan_create(const skb_workload* workload, const skb_retained_mode* modes, uint64_t mode_count,
+                                   const double* dz, int precision, skb_advection_flux_plan** plan) {
+    return guarded([&] {
+        if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan pointer");
+        *plan = nullptr;
+        check_precision(precision);
+        validate_workload(workload);
+        if (workload->fields != 4) fail(SKB_INVALID_ARGUMENT, "the advection flux takes four fields: u, v, w, q");
+        if (!modes || mode_count == 0) fail(SKB_INVALID_ARGUMENT, "the flux graph requires retained modes");
+        if (!dz) fail(SKB_INVALID_ARGUMENT, "null vertical derivative matrix");
+        auto p = std::make_unique<skb_advection_flux_plan>();
+        p->w = *workload;
+        p->precision = precision;
+        p->modes = mode_count;
+        skb_workload w1 = *workload, w6 = *workload;
+        w1.fields = 1;
+        w6.fields = 6;
+        const skb_mode_group all{0, 0, mode_count};
+        check_status(skb_vertical_plan_create(&w1, workload->nz, &all, 1, dz, dz, precision, &p->dz));
+        check_status(skb_pruned_plan_create(&w6, modes, mode_count, precision, &p->pin));
+        check_status(skb_pruned_plan_create(&w1, modes, mode_count, precision, &p->pout));
+        const uint64_t volume = workload->nx * workload->ny * workload->nz;
+        p->qz.allocate(mode_count * workload->nz * complex_bytes(precision));
+        p->real_in.allocate(6 * volume * real_bytes(precision));
+        p->real_out.allocate(volume * real_bytes(precision));
+        *plan = p.release();
+    });
+}
+
+int skb_advection_flux_plan_destroy(skb_advection_flux_plan* plan) {
+    return guarded([&] { delete plan; });
+}
+
+int skb_advection_flux_set_fused(skb_advection_flux_plan* plan, int fused) {
+    return guarded([&] {
+        if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
+        plan->fused = fused != 0;
+    });
+}
+
+int skb_advection_flux_execute(skb_advection_flux_plan* plan, const void* spectral, void* flux, void* stream) {
+    return guarded([&] {
+        if (!plan || !spectral || !flux) fail(SKB_INVALID_ARGUMENT, "null argument");
+        auto& p = *plan;
+        const auto& w = p.w;
+        const double scale = 1.0 / (static_cast<double>(w.nx * w.ny) * static_cast<double>(w.nx * w.ny));
+        vertical_forward_columns(p.dz, spectral, 4, 3, p.qz.ptr, stream);
+        const FluxSource sources[6] = {{spectral, 4, 0, kFactorNone}, {spectral, 4, 1, kFactorNone},
+                                       {spectral, 4, 2, kFactorNone}, {spectral, 4, 3, kFactorDx},
+                                       {spectral, 4, 3, kFactorDy},   {p.qz.ptr, 1, 0, kFactorNone}};
+        flux_inverse_columns(p.pin, sources, 6, stream);
+        if (p.fused && flux_rows_supported(p.pin, p.pout)) {
+            flux_rows(p.pin, p.pout, 1, scale, stream);
+            check_status(skb_pruned_forward_columns(p.pout, flux, stream));
+        } else {
+            const uint64_t volume = w.nx * w.ny * w.nz;
+            const size_t rb = real_bytes(p.precision);
+            check_status(skb_pruned_inverse_rows(p.pin, p.real_in.ptr, stream));
+            const char* base = static_cast<const char*>(p.real_in.ptr);
+            check_status(skb_pointwise_advection(volume, scale, base, base + 3 * volume * rb, p.real_out.ptr,
+                                                 p.precision, stream));
+            check_status(skb_pruned_forward(p.pout, p.real_out.ptr, flux, stream));
+        }
     });
 }
 

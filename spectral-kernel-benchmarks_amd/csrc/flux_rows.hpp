@@ -7,10 +7,28 @@
 namespace skb {
 
 // True when the two pruned plans (3 + 3T input fields, T output fields) share
-// a grid the fused kernel handles (Nx >= 64, same Ny, Nz, k_x band, precision).
+// a grid the fused kernel handles (FP32, Nx >= 64, same Ny, Nz, k_x band).
 bool flux_rows_supported(const skb_pruned_plan* pin, const skb_pruned_plan* pout);
 // pout's intermediate <- forward rows of -scale * (u*qx + v*qy + w*qz) per
 // target, from the inverse rows of pin's intermediate.
 void flux_rows(skb_pruned_plan* pin, skb_pruned_plan* pout, int targets, double scale, void* stream);
+
+// Vertical forward product of the plan's columns read from a wider compact
+// array: column c at in + K*(in_cs*c + in_co) (vertical.hip).
+void vertical_forward_columns(const skb_vertical_plan* p, const void* in, int in_cs, int in_co, void* out,
+                              void* stream);
+
+// One source field of the inverse column pass of a flux graph: compact
+// interleaved values of field `field` of an array with `fields` fields,
+// optionally multiplied by a spectral derivative factor.
+enum SpectralFactor { kFactorNone = 0, kFactorDx = 1, kFactorDy = 2 };
+struct FluxSource {
+    const void* base;  // compact array (plane + P*mode, P = nz*fields)
+    int fields, field;
+    int factor;        // SpectralFactor
+};
+// pin's intermediate <- inverse column transforms of the given sources, one
+// per field of pin (at most 16), with factors i*2pi*kx/lx and i*2pi*l/ly.
+void flux_inverse_columns(skb_pruned_plan* pin, const FluxSource* sources, int count, void* stream);
 
 }  // namespace skb

@@ -37,6 +37,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "flux_rows.hpp"
 
 namespace skb {
 
@@ -90,6 +91,7 @@ struct GemmArgs {
     const R* arow;  // [group][VR][4*KS]: rows 16*MB .. M-1 (zero for k >= K)
     const GemmTile* tiles;
     int ntiles, M, K, MB, VR, F;
+    int in_cs, in_co;    // compact: input column c starts at in + K*(in_cs*c + in_co)
     const int* qmap;     // WVM direct: frequency q of each mode
     const int* partner;  // WVM direct: frequency of the k_x = 0 conjugate partner, or -1
 };
@@ -141,8 +143,9 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmArgs<R> g) 
             const int e = tid + 256 * n, c = e / KP, z = e - c * KP;
             bv[n] = cx<R>{R(0), R(0)};
             if (e < 16 * KP && c < t.ncols && z < g.K) {
-                const size_t src = WVM ? static_cast<size_t>(cbase[c]) + z
-                                       : static_cast<size_t>(g.K) * (t.col_begin + c) + z;
+                const size_t src =
+                    WVM ? static_cast<size_t>(cbase[c]) + z
+                        : static_cast<size_t>(g.K) * (static_cast<size_t>(g.in_cs) * (t.col_begin + c) + g.in_co) + z;
                 bv[n] = g.in[src];
             }
         }
@@ -399,6 +402,8 @@ GemmArgs<R> args(const skb_vertical_plan* p, const GemmOperand& op, const void* 
     a.MB = op.split.mb;
     a.VR = op.split.vr;
     a.F = static_cast<int>(p->w.fields);
+    a.in_cs = 1;
+    a.in_co = 0;
     a.qmap = p->qmap.as<int>();
     a.partner = p->partner.as<int>();
     return a;
@@ -421,6 +426,35 @@ void run(const skb_vertical_plan* p, bool forward, const void* in, void* out, vo
         else launch_gemm<float, false>(a, p->grid, s);
     }
 }
+
+}  // namespace
+
+namespace skb {
+
+// Forward product over a column subset of a wider compact array: input
+// column c (c = field + fields*mode of the plan) is read at
+// in + K*(in_cs*c + in_co); the output is the plan's dense layout.
+void vertical_forward_columns(const skb_vertical_plan* p, const void* in, int in_cs, int in_co, void* out,
+                              void* stream) {
+    if (!p || !in || !out) fail(SKB_INVALID_ARGUMENT, "null argument");
+    if (p->wvm || in_cs < 1 || in_co < 0 || in_co >= in_cs) fail(SKB_INVALID_ARGUMENT, "bad column selection");
+    const hipStream_t s = as_stream(stream);
+    if (p->precision == SKB_F64) {
+        auto a = args<double>(p, p->fwd, in, out);
+        a.in_cs = in_cs;
+        a.in_co = in_co;
+        launch_gemm<double, false>(a, p->grid, s);
+    } else {
+        auto a = args<float>(p, p->fwd, in, out);
+        a.in_cs = in_cs;
+        a.in_co = in_co;
+        launch_gemm<float, false>(a, p->grid, s);
+    }
+}
+
+}  // namespace skb
+
+namespace {
 
 void fill_zero(const skb_vertical_plan* p, void* out, uint64_t elements, void* stream) {
     if (!p || !out) fail(SKB_INVALID_ARGUMENT, "null argument");

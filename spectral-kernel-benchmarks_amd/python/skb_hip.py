@@ -141,6 +141,11 @@ def _load():
         "skb_flux_plan_destroy": (i32, [vp]),
         "skb_flux_set_fused": (i32, [vp, i32]),
         "skb_flux_execute": (i32, [vp, vp, vp, vp]),
+        "skb_advection_flux_plan_create": (i32, [W, vp, u64, vp, i32, pp]),
+        "skb_advection_flux_plan_destroy": (i32, [vp]),
+        "skb_advection_flux_set_fused": (i32, [vp, i32]),
+        "skb_advection_flux_execute": (i32, [vp, vp, vp, vp]),
+        "skb_vertical_derivative_matrix": (i32, [u64, dbl, vp]),
         "skb_vertical_wvm_forward": (i32, [vp, vp, vp, vp]),
         "skb_vertical_wvm_inverse": (i32, [vp, vp, vp, vp]),
         "skb_vertical_wvm_modal_elements": (u64, [vp]),
@@ -169,7 +174,9 @@ EXPORTED = [
     "skb_pruned_forward_split_fields", "skb_pruned_inverse_split_fields", "skb_vertical_wvm_plan_create",
     "skb_vertical_wvm_forward", "skb_vertical_wvm_inverse", "skb_vertical_wvm_modal_elements",
     "skb_vertical_wvm_initialize_modal", "skb_vertical_wvm_initialize_spectrum", "skb_flux_plan_create",
-    "skb_flux_plan_destroy", "skb_flux_set_fused", "skb_flux_execute",
+    "skb_flux_plan_destroy", "skb_flux_set_fused", "skb_flux_execute", "skb_advection_flux_plan_create",
+    "skb_advection_flux_plan_destroy", "skb_advection_flux_set_fused", "skb_advection_flux_execute",
+    "skb_vertical_derivative_matrix",
 ]
 
 
@@ -635,6 +642,43 @@ class FluxPlan:
     def close(self):
         if getattr(self, "h", None):
             lib.skb_flux_plan_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+
+def vertical_derivative_matrix(nz: int, lz: float = 1.0) -> np.ndarray:
+    d = np.zeros(nz * nz)
+    check(lib.skb_vertical_derivative_matrix(nz, float(lz), _np(d)))
+    return d
+
+
+class AdvectionFluxPlan:
+    """Advection flux of one tracer: compact retained (u, v, w, q) ->
+    compact retained -(u q_x + v q_y + w q_z) / (Nx*Ny)^2 (w.fields == 4)."""
+
+    def __init__(self, w: Workload, modes: np.ndarray, dz: np.ndarray, precision: int = SKB_F64):
+        self.w, self.precision, self.modes = w, precision, len(modes)
+        self._wc = w.c()
+        dz = np.ascontiguousarray(dz, np.float64)
+        if dz.size != w.nz * w.nz:
+            raise ValueError("dz must be Nz x Nz")
+        h = ctypes.c_void_p()
+        check(lib.skb_advection_flux_plan_create(ctypes.byref(self._wc), _np(modes), len(modes), _np(dz), precision,
+                                                 ctypes.byref(h)))
+        self.h = h
+
+    def set_fused(self, fused: bool):
+        check(lib.skb_advection_flux_set_fused(self.h, 1 if fused else 0))
+
+    def execute(self, spectral, flux, stream=None):
+        cd = complex_dtype(self.precision)
+        check(lib.skb_advection_flux_execute(self.h, _dev(spectral, self.modes * self.w.planes, cd, "spectral"),
+                                             _dev(flux, self.modes * self.w.nz, cd, "flux"), _stream(stream)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.skb_advection_flux_plan_destroy(self.h)
             self.h = None
 
     __del__ = close

@@ -397,3 +397,46 @@ def test_flux_oracle_dc_closed_form():
         assert np.allclose(got[dc, t], expect, rtol=1e-12, atol=1e-14 * np.abs(expect).max())
     others = np.delete(got, dc, axis=0)
     assert np.abs(others).max() < 1e-12 * np.abs(got[dc]).max()
+
+
+def test_vertical_derivative_matrix_exact_on_quadratics():
+    nz, lz = 17, 3.0
+    D = O.vertical_derivative_matrix(nz, lz).reshape(nz, nz)
+    z = np.linspace(0.0, lz, nz)
+    assert np.allclose(D @ np.ones(nz), 0.0, atol=1e-12)
+    assert np.allclose(D @ z, np.ones(nz), atol=1e-12)
+    assert np.allclose(D @ z ** 2, 2 * z, atol=1e-11)
+
+
+def test_advection_flux_oracle_closed_forms_and_direct_dft():
+    w = O.Workload(16, 16, 9, 4, 2.0, 3.0)
+    modes = O.retained_modes(w)
+    n, nz = len(modes), w.nz
+    D = O.vertical_derivative_matrix(nz, 1.5)
+    dc = int(np.nonzero((modes["k"] == 0) & (modes["l"] == 0))[0][0])
+    # (-2, 3) is stored conjugated at (2, -3): exercises the conjugation
+    m = int(np.nonzero((modes["k"] == -2) & (modes["l"] == 3))[0][0])
+    n2 = float(w.nx * w.ny)
+    # constant horizontal flow, q one Fourier mode constant in z (q_z = 0)
+    U0, V0, A0 = 0.7, -1.3, 0.4 - 0.9j
+    sp = np.zeros((n, 4, nz), np.complex128)
+    sp[dc, 0, :], sp[dc, 1, :], sp[m, 3, :] = U0, V0, A0
+    got = O.advection_flux(w, modes, D, sp.reshape(-1)).reshape(n, nz)
+    kx, ly = 2 * np.pi * -2 / w.lx, 2 * np.pi * 3 / w.ly
+    expect = -(U0 * 1j * kx + V0 * 1j * ly) * A0 / n2
+    assert np.allclose(got[m], expect, rtol=1e-12)
+    assert np.abs(np.delete(got, m, axis=0)).max() < 1e-12 * abs(expect)
+    # vertical flow against q(z) = z^2 (the stencil is exact on quadratics)
+    W0 = 1.1
+    z = np.linspace(0.0, 1.5, nz)
+    sp = np.zeros((n, 4, nz), np.complex128)
+    sp[dc, 2, :] = W0
+    sp[m, 3, :] = A0 * z ** 2
+    got = O.advection_flux(w, modes, D, sp.reshape(-1)).reshape(n, nz)
+    assert np.allclose(got[m], -W0 * A0 * 2 * z / n2, rtol=1e-11, atol=1e-14)
+    # the FFT composition equals the direct DFT on random input
+    rng = np.random.default_rng(3)
+    sp = (rng.standard_normal((n, 4, nz)) + 1j * rng.standard_normal((n, 4, nz))).reshape(-1)
+    a = O.advection_flux(w, modes, D, sp, use_direct=True)
+    b = O.advection_flux(w, modes, D, sp)
+    assert O.max_relative_error(b, a) < 1e-12
