@@ -184,8 +184,11 @@ int skb_pruned_inverse_rows(skb_pruned_plan* plan, void* real, void* stream);
 /* Launch shape of one stage (0 forward rows, 1 forward columns, 2 inverse
  * columns, 3 inverse rows): the number of workgroups, 0 = one per work item
  * (the default).  Fewer workgroups than items makes them persistent, each
- * prefetching its next item.  Set by the launch autotuner; results do not
- * depend on it.  No reference counterpart (FFTW plans choose internally). */
+ * prefetching its next item.  Set by the launch autotuner.  The persistent
+ * kernels are separate instantiations and may round differently in the last
+ * bits (FMA contraction), so results agree with the default launch to
+ * rounding, not bit for bit.  No reference counterpart (FFTW plans choose
+ * internally). */
 int skb_pruned_plan_set_grid(skb_pruned_plan* plan, int stage, int workgroups);
 uint64_t skb_pruned_plan_items(const skb_pruned_plan* plan, int stage);
 /* FFTWPrunedProvider::activeKxCount / scratchBytes. */
@@ -279,6 +282,37 @@ int skb_advection_flux_execute(skb_advection_flux_plan* plan, const void* spectr
  * second order at both ends (rows sum to zero); Nz x Nz row-major.  No
  * reference counterpart. */
 int skb_vertical_derivative_matrix(uint64_t nz, double lz, double* d);
+
+/* ---- batched uneven workload --------------------------------------------
+ * Retained-operator evaluations (pruned inverse -> pruned forward -> grouped
+ * vertical GEMM) over size classes of uneven grids, recorded once into a
+ * hipGraph and replayed with one launch call.  Class c stacks `instances`
+ * copies of `workload` as one workload with fields * instances fields
+ * (instance i = fields [F*i, F*i + F)), so each stage is one launch over all
+ * instances of the class.  No reference counterpart (the reference runs one
+ * workload per process). */
+typedef struct skb_batch skb_batch;
+int skb_batch_create(int precision, skb_batch** batch);
+int skb_batch_destroy(skb_batch* batch);
+int skb_batch_add_class(skb_batch* batch, const skb_workload* workload, uint64_t instances,
+                        const skb_retained_mode* modes, uint64_t mode_count, uint64_t nj,
+                        const skb_mode_group* groups, uint64_t group_count, const double* forward,
+                        const double* inverse, int* class_index);
+/* Binds the class's caller-owned device arrays in the stacked layout: input
+ * compact retained (plane + P*mode, P = Nz*F*instances) and output modal
+ * (j + Nj*(field + F*instances*mode)).  Binding drops a captured graph. */
+int skb_batch_bind(skb_batch* batch, int cls, const void* input, void* output);
+/* One evaluation of every class, launched eagerly. */
+int skb_batch_run(skb_batch* batch, void* stream);
+/* Records one evaluation into a hipGraph (stream must not be the default
+ * stream); skb_batch_launch replays it. */
+int skb_batch_capture(skb_batch* batch, void* stream);
+int skb_batch_launch(skb_batch* batch, void* stream);
+/* Times every launch-shape candidate of every stage of every class (reps
+ * launches each) and keeps the fastest; writes a JSON report.  Invalidates a
+ * captured graph.  Results agree with the default launch shapes to rounding
+ * (see skb_pruned_plan_set_grid). */
+int skb_batch_autotune(skb_batch* batch, int reps, void* stream, char* report, size_t capacity);
 
 /* ---- pointwise advection -----------------------------------------------
  * Replaces PointwiseAdvectionExecutor::execute (src/pointwise_advection.hpp

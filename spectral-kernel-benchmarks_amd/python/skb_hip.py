@@ -146,6 +146,14 @@ def _load():
         "skb_advection_flux_set_fused": (i32, [vp, i32]),
         "skb_advection_flux_execute": (i32, [vp, vp, vp, vp]),
         "skb_vertical_derivative_matrix": (i32, [u64, dbl, vp]),
+        "skb_batch_create": (i32, [i32, pp]),
+        "skb_batch_destroy": (i32, [vp]),
+        "skb_batch_add_class": (i32, [vp, W, u64, vp, u64, u64, vp, u64, vp, vp, ctypes.POINTER(i32)]),
+        "skb_batch_bind": (i32, [vp, i32, vp, vp]),
+        "skb_batch_run": (i32, [vp, vp]),
+        "skb_batch_capture": (i32, [vp, vp]),
+        "skb_batch_launch": (i32, [vp, vp]),
+        "skb_batch_autotune": (i32, [vp, i32, vp, ctypes.c_char_p, ctypes.c_size_t]),
         "skb_vertical_wvm_forward": (i32, [vp, vp, vp, vp]),
         "skb_vertical_wvm_inverse": (i32, [vp, vp, vp, vp]),
         "skb_vertical_wvm_modal_elements": (u64, [vp]),
@@ -176,7 +184,8 @@ EXPORTED = [
     "skb_vertical_wvm_initialize_modal", "skb_vertical_wvm_initialize_spectrum", "skb_flux_plan_create",
     "skb_flux_plan_destroy", "skb_flux_set_fused", "skb_flux_execute", "skb_advection_flux_plan_create",
     "skb_advection_flux_plan_destroy", "skb_advection_flux_set_fused", "skb_advection_flux_execute",
-    "skb_vertical_derivative_matrix",
+    "skb_vertical_derivative_matrix", "skb_batch_create", "skb_batch_destroy", "skb_batch_add_class",
+    "skb_batch_bind", "skb_batch_run", "skb_batch_capture", "skb_batch_launch", "skb_batch_autotune",
 ]
 
 
@@ -390,7 +399,7 @@ class FftPlan:
             _dev(wvm, w.spectrum_elements, complex_dtype(self.precision), "wvm"), _stream(stream)))
 
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and lib is not None:
             lib.skb_fft_plan_destroy(self.h)
             self.h = None
 
@@ -501,7 +510,7 @@ class PrunedPlan:
         return int(lib.skb_pruned_scratch_bytes(self.h))
 
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and lib is not None:
             lib.skb_pruned_plan_destroy(self.h)
             self.h = None
 
@@ -546,7 +555,7 @@ class VerticalPlan:
         return int(lib.skb_vertical_plan_items(self.h))
 
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and lib is not None:
             lib.skb_vertical_plan_destroy(self.h)
             self.h = None
 
@@ -596,7 +605,7 @@ class WvmVerticalPlan:
             _stream(stream)))
 
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and lib is not None:
             lib.skb_vertical_plan_destroy(self.h)
             self.h = None
 
@@ -640,7 +649,7 @@ class FluxPlan:
                                    _dev(output_modal, self.output_elements, cd, "output_modal"), _stream(stream)))
 
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and lib is not None:
             lib.skb_flux_plan_destroy(self.h)
             self.h = None
 
@@ -677,8 +686,70 @@ class AdvectionFluxPlan:
                                              _dev(flux, self.modes * self.w.nz, cd, "flux"), _stream(stream)))
 
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and lib is not None:
             lib.skb_advection_flux_plan_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+
+class Batch:
+    """Retained-operator evaluations over size classes of uneven grids,
+    replayed from one hipGraph.  Class arrays use the stacked layout of a
+    workload with fields * instances fields."""
+
+    def __init__(self, precision: int = SKB_F32):
+        self.precision = precision
+        h = ctypes.c_void_p()
+        check(lib.skb_batch_create(precision, ctypes.byref(h)))
+        self.h = h
+        self.classes = []
+        self._keep = []
+
+    def add_class(self, w: Workload, instances: int, modes: np.ndarray, nj: int, groups: np.ndarray,
+                  forward: np.ndarray, inverse: np.ndarray) -> int:
+        wc = w.c()
+        forward = np.ascontiguousarray(forward, np.float64)
+        inverse = np.ascontiguousarray(inverse, np.float64)
+        idx = ctypes.c_int()
+        check(lib.skb_batch_add_class(self.h, ctypes.byref(wc), instances, _np(modes), len(modes), nj, _np(groups),
+                                      len(groups), _np(forward), _np(inverse), ctypes.byref(idx)))
+        self.classes.append({"w": w, "instances": instances, "modes": len(modes), "nj": nj})
+        return idx.value
+
+    def input_elements(self, cls: int) -> int:
+        c = self.classes[cls]
+        return c["modes"] * c["w"].planes * c["instances"]
+
+    def output_elements(self, cls: int) -> int:
+        c = self.classes[cls]
+        return c["modes"] * c["w"].fields * c["instances"] * c["nj"]
+
+    def bind(self, cls: int, input_tensor, output_tensor):
+        cd = complex_dtype(self.precision)
+        check(lib.skb_batch_bind(self.h, cls, _dev(input_tensor, self.input_elements(cls), cd, "input"),
+                                 _dev(output_tensor, self.output_elements(cls), cd, "output")))
+        self._keep.append((input_tensor, output_tensor))
+
+    def run(self, stream=None):
+        check(lib.skb_batch_run(self.h, _stream(stream)))
+
+    def capture(self, stream):
+        check(lib.skb_batch_capture(self.h, _stream(stream)))
+
+    def launch(self, stream):
+        check(lib.skb_batch_launch(self.h, _stream(stream)))
+
+    def autotune(self, reps: int = 20, stream=None):
+        import json
+
+        buf = ctypes.create_string_buffer(1 << 16)
+        check(lib.skb_batch_autotune(self.h, reps, _stream(stream), buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+    def close(self):
+        if getattr(self, "h", None) and lib is not None:
+            lib.skb_batch_destroy(self.h)
             self.h = None
 
     __del__ = close
