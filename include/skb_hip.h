@@ -278,6 +278,9 @@ int skb_advection_flux_plan_create(const skb_workload* workload, const skb_retai
 int skb_advection_flux_plan_destroy(skb_advection_flux_plan* plan);
 int skb_advection_flux_set_fused(skb_advection_flux_plan* plan, int fused);
 int skb_advection_flux_execute(skb_advection_flux_plan* plan, const void* spectral, void* flux, void* stream);
+/* Device memory the plan holds (intermediates and scratch), for admission
+ * control by an HBM budget. */
+uint64_t skb_advection_flux_device_bytes(const skb_advection_flux_plan* plan);
 /* Second-order centred d/dz on Nz uniform levels over [0, lz], one-sided
  * second order at both ends (rows sum to zero); Nz x Nz row-major.  No
  * reference counterpart. */

@@ -36,6 +36,14 @@ struct skb_flux_plan {
     skb_pruned_plan* pout = nullptr;    // T fields
     skb::DeviceBuffer phys_in, phys_out, real_in, real_out;
     bool fused = true;
+    // Real volumes of the separate-kernel path, allocated only when that path
+    // can run (at creation, or when skb_flux_set_fused selects it).
+    void allocate_real() {
+        if (real_in.ptr) return;
+        const uint64_t volume = w.nx * w.ny * w.nz, rb = skb::real_bytes(precision);
+        real_in.allocate(volume * (3 + 3 * targets) * rb);
+        real_out.allocate(volume * targets * rb);
+    }
     ~skb_flux_plan() {
         if (vin) skb_vertical_plan_destroy(vin);
         if (vout) skb_vertical_plan_destroy(vout);
@@ -67,6 +75,12 @@ struct skb_advection_flux_plan {
     skb_pruned_plan* pout = nullptr;  // 1 field
     skb::DeviceBuffer qz, real_in, real_out;
     bool fused = true;
+    void allocate_real() {
+        if (real_in.ptr) return;
+        const uint64_t volume = w.nx * w.ny * w.nz, rb = skb::real_bytes(precision);
+        real_in.allocate(6 * volume * rb);
+        real_out.allocate(volume * rb);
+    }
     ~skb_advection_flux_plan() {
         if (dz) skb_vertical_plan_destroy(dz);
         if (pin) skb_pruned_plan_destroy(pin);
@@ -103,12 +117,10 @@ int skb_flux_plan_create(const skb_workload* workload, const skb_retained_mode* 
         if (skb_vertical_plan_items(p->vin) == 0) fail(SKB_INVALID_ARGUMENT, "empty vertical family");
         check_status(skb_pruned_plan_create(&win, modes, mode_count, precision, &p->pin));
         check_status(skb_pruned_plan_create(workload, modes, mode_count, precision, &p->pout));
-        const uint64_t cb = complex_bytes(precision), rb = real_bytes(precision);
-        const uint64_t volume = workload->nx * workload->ny * workload->nz;
+        const uint64_t cb = complex_bytes(precision);
         p->phys_in.allocate(mode_count * workload->nz * win.fields * cb);
         p->phys_out.allocate(mode_count * workload->nz * targets * cb);
-        p->real_in.allocate(volume * win.fields * rb);
-        p->real_out.allocate(volume * targets * rb);
+        if (!flux_rows_supported(p->pin, p->pout)) p->allocate_real();
         *plan = p.release();
     });
 }
@@ -120,6 +132,7 @@ int skb_flux_plan_destroy(skb_flux_plan* plan) {
 int skb_flux_set_fused(skb_flux_plan* plan, int fused) {
     return guarded([&] {
         if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
+        if (!fused) plan->allocate_real();
         plan->fused = fused != 0;
     });
 }
@@ -194,10 +207,8 @@ int skb_advection_flux_plan_create(const skb_workload* workload, const skb_retai
         check_status(skb_vertical_plan_create(&w1, workload->nz, &all, 1, dz, dz, precision, &p->dz));
         check_status(skb_pruned_plan_create(&w6, modes, mode_count, precision, &p->pin));
         check_status(skb_pruned_plan_create(&w1, modes, mode_count, precision, &p->pout));
-        const uint64_t volume = workload->nx * workload->ny * workload->nz;
         p->qz.allocate(mode_count * workload->nz * complex_bytes(precision));
-        p->real_in.allocate(6 * volume * real_bytes(precision));
-        p->real_out.allocate(volume * real_bytes(precision));
+        if (!flux_rows_supported(p->pin, p->pout)) p->allocate_real();
         *plan = p.release();
     });
 }
@@ -209,8 +220,16 @@ int skb_advection_flux_plan_destroy(skb_advection_flux_plan* plan) {
 int skb_advection_flux_set_fused(skb_advection_flux_plan* plan, int fused) {
     return guarded([&] {
         if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
+        if (!fused) plan->allocate_real();
         plan->fused = fused != 0;
     });
+}
+
+// Device bytes owned by the plan (its sub-plans' scratch included).
+uint64_t skb_advection_flux_device_bytes(const skb_advection_flux_plan* plan) {
+    if (!plan) return 0;
+    return plan->qz.bytes + plan->real_in.bytes + plan->real_out.bytes + skb_pruned_scratch_bytes(plan->pin) +
+           skb_pruned_scratch_bytes(plan->pout);
 }
 
 int skb_advection_flux_execute(skb_advection_flux_plan* plan, const void* spectral, void* flux, void* stream) {

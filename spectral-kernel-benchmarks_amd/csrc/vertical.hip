@@ -34,6 +34,8 @@
 //            f32: i = 4 * (l / 16) + r      f64: i = l / 16 + 4 * r
 #include <algorithm>
 #include <memory>
+#include <string>
+#include <utility>
 #include <vector>
 
 #include "common.hpp"
@@ -71,8 +73,16 @@ struct GemmTile {
     int pad;
 };
 
-// Row blocks per wavefront held at once: M <= 132 gives at most 9 blocks.
-constexpr int kMaxSlots = 3;
+// Row blocks per wavefront held at once (accumulator slots): 3 covers
+// M <= 196; instantiations with KS >= 22 (K > 84, i.e. large Nz) hold 4,
+// covering M <= 260.
+template <int KS>
+constexpr int gemm_slots() {
+    return KS >= 22 ? 4 : 3;
+}
+constexpr bool gemm_ks_instantiated(int ks, bool fp64) {
+    return (ks >= 1 && ks <= 17) || ks == 22 || ks == 33 || (!fp64 && (ks == 43 || ks == 65));
+}
 
 // LDS column stride of the B block, in complex elements.  With KP = 4*KS
 // the MFMA B reads of one instruction (16 columns x 4 k) hit distinct banks:
@@ -101,6 +111,7 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmArgs<R> g) 
     using A4 = typename Acc<R>::type;
     constexpr int KP = 4 * KS;
     constexpr int CS = gemm_cs<R, KS>();
+    constexpr int kMaxSlots = gemm_slots<KS>();
     constexpr int NL = (16 * KP + 255) / 256;  // B elements loaded per thread
     extern __shared__ __align__(16) unsigned char gemm_lds[];
     cx<R>* sb = reinterpret_cast<cx<R>*>(gemm_lds);  // B block, then the output stage
@@ -290,7 +301,13 @@ void launch_gemm(const GemmArgs<R>& a, int req, hipStream_t s) {
         SKB_KS(11) SKB_KS(12) SKB_KS(13) SKB_KS(14) SKB_KS(15) SKB_KS(16) SKB_KS(17) SKB_KS(22) SKB_KS(33)
 #undef SKB_KS
     }
-    fail(SKB_UNSUPPORTED, "vertical GEMM inner dimension: ceil(K/4) must be <= 17, 22 or 33");
+    if constexpr (sizeof(R) == 4) {
+        switch ((a.K + 3) / 4) {
+            case 43: launch_gemm_ks<R, 43, WVM>(a, req, s); return;
+            case 65: launch_gemm_ks<R, 65, WVM>(a, req, s); return;
+        }
+    }
+    fail(SKB_UNSUPPORTED, "vertical GEMM inner dimension not instantiated");
 }
 
 }  // namespace
@@ -360,7 +377,21 @@ std::unique_ptr<skb_vertical_plan> make_plan(const skb_workload* workload, uint6
     if (!forward || !inverse) fail(SKB_INVALID_ARGUMENT, "vertical GEMM requires forward and inverse matrices");
     const uint64_t nz = workload->nz;
     if (nj == 0 || nz == 0) fail(SKB_INVALID_ARGUMENT, "Vertical operator dimensions must be positive.");
-    if (nz > 132 || nj > 132) fail(SKB_UNSUPPORTED, "vertical dimensions above 132 (Nz, Nj) are not instantiated");
+    const bool fp64 = precision == SKB_F64;
+    const uint64_t limit = fp64 ? 132 : 260;
+    if (nz > limit || nj > limit)
+        fail(SKB_UNSUPPORTED, fp64 ? "FP64 vertical dimensions (Nz, Nj) above 132 are not instantiated"
+                                   : "FP32 vertical dimensions (Nz, Nj) above 260 are not instantiated");
+    // both directions: K in the instantiated set, row blocks within the slots
+    for (const auto& mk : {std::pair<uint64_t, uint64_t>{nj, nz}, std::pair<uint64_t, uint64_t>{nz, nj}}) {
+        const int M = static_cast<int>(mk.first), K = static_cast<int>(mk.second), ks = (K + 3) / 4;
+        if (!gemm_ks_instantiated(ks, fp64))
+            fail(SKB_UNSUPPORTED, "vertical GEMM inner dimension " + std::to_string(K) + " is not instantiated");
+        const int slots = ks >= 22 ? 4 : 3;
+        if ((row_split(M).mb + 3) / 4 > slots)
+            fail(SKB_UNSUPPORTED, "vertical GEMM output dimension " + std::to_string(M) + " with inner dimension " +
+                                      std::to_string(K) + " is not instantiated");
+    }
     uint64_t modes = 0;
     validate_groups(groups, group_count, &modes);
     const uint64_t fields = workload->fields;

@@ -146,6 +146,7 @@ def _load():
         "skb_advection_flux_set_fused": (i32, [vp, i32]),
         "skb_advection_flux_execute": (i32, [vp, vp, vp, vp]),
         "skb_vertical_derivative_matrix": (i32, [u64, dbl, vp]),
+        "skb_advection_flux_device_bytes": (u64, [vp]),
         "skb_batch_create": (i32, [i32, pp]),
         "skb_batch_destroy": (i32, [vp]),
         "skb_batch_add_class": (i32, [vp, W, u64, vp, u64, u64, vp, u64, vp, vp, ctypes.POINTER(i32)]),
@@ -186,6 +187,7 @@ EXPORTED = [
     "skb_advection_flux_plan_destroy", "skb_advection_flux_set_fused", "skb_advection_flux_execute",
     "skb_vertical_derivative_matrix", "skb_batch_create", "skb_batch_destroy", "skb_batch_add_class",
     "skb_batch_bind", "skb_batch_run", "skb_batch_capture", "skb_batch_launch", "skb_batch_autotune",
+    "skb_advection_flux_device_bytes",
 ]
 
 
@@ -679,6 +681,9 @@ class AdvectionFluxPlan:
 
     def set_fused(self, fused: bool):
         check(lib.skb_advection_flux_set_fused(self.h, 1 if fused else 0))
+
+    def device_bytes(self) -> int:
+        return int(lib.skb_advection_flux_device_bytes(self.h))
 
     def execute(self, spectral, flux, stream=None):
         cd = complex_dtype(self.precision)
