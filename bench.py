@@ -39,6 +39,10 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--quiet", action="store_true", help="no progress lines on stderr")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the secondary measurements (advection flux 512^2 x Nz129, batched uneven workload, "
+                         "serving loop)")
+    ap.add_argument("--serving-requests", type=int, default=1500)
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for rocprofv3: warmup, --steps evals, then one 256 MiB device copy "
                          "(the calibration kernel for FETCH_SIZE / WRITE_SIZE)")
@@ -217,6 +221,148 @@ def cpu_baseline(args, w, x64, nj, groups, fwd):
     return rec, out, O
 
 
+def _timed_events(torch, fn, reps, stream=None):
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s = stream or torch.cuda.current_stream()
+    a.record(s)
+    for _ in range(reps):
+        fn()
+    b.record(s)
+    b.synchronize()
+    return a.elapsed_time(b) / reps  # ms
+
+
+def device_spectrum(torch, S, w, modes, seed, device, dtype):
+    """Seeded complex Gaussian x (1 + k^2 + l^2)^(-5/6) per mode (DC real),
+    generated on the device."""
+    import numpy as np
+
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    n = len(modes)
+    amp = torch.from_numpy((1.0 + modes["k"].astype(np.float64) ** 2 + modes["l"].astype(np.float64) ** 2)
+                           ** (-5.0 / 6.0)).to(device)
+    re = torch.randn((n, w.planes), generator=g, device=device, dtype=torch.float64)
+    im = torch.randn((n, w.planes), generator=g, device=device, dtype=torch.float64)
+    x = torch.complex(re, im) * amp.view(n, 1)
+    dc = int(np.nonzero((modes["k"] == 0) & (modes["l"] == 0))[0][0])
+    x[dc] = x[dc].real.to(torch.complex128)
+    return x.reshape(-1).to(dtype)
+
+
+def bench_flux(args, S, torch, device):
+    """Advection flux of one tracer at 512^2 x Nz129 (u, v, w, q): FP32 fused
+    physical stage, beside the FP64 form; error of FP32 vs FP64 on the GPU."""
+    w = S.Workload(512, 512, 129, 4)
+    modes = S.retained_modes(w)
+    dz = S.vertical_derivative_matrix(w.nz, 1.0)
+    x64 = device_spectrum(torch, S, w, modes, 129, device, torch.complex128)
+    out = {}
+    res = {}
+    for prec, name in ((S.SKB_F32, "fp32"), (S.SKB_F64, "fp64")):
+        plan = S.AdvectionFluxPlan(w, modes, dz, prec)
+        cd = S.complex_dtype(prec)
+        x = x64.to(cd)
+        f = torch.empty(len(modes) * w.nz, dtype=cd, device=device)
+        plan.execute(x, f)
+        ms = _timed_events(torch, lambda: plan.execute(x, f), 20)
+        res[name] = f.to(torch.complex128)
+        out[name] = {"evals_per_s": 1e3 / ms, "ms_per_eval": ms}
+        del plan
+    d = (res["fp32"] - res["fp64"]).abs().max().item()
+    out["fp32_vs_fp64_max_rel"] = d / res["fp64"].abs().max().item()
+    out["workload"] = "advection flux 512x512 Nz129, fields u,v,w,q; spectral d/dx, d/dy; 129x129 d/dz on MFMA"
+    return out
+
+
+def bench_batched(args, S, torch, device):
+    """The uneven batch: 40 x 128^2xNz33x3, 40 x 256^2xNz65x3, 16 x 512^2xNz129x4
+    retained-operator instances in one hipGraph, launch shapes autotuned."""
+    import numpy as np
+
+    mix = [((128, 128, 33, 3), 40), ((256, 256, 65, 3), 40), ((512, 512, 129, 4), 16)]
+    # a class stacks at most 2^31 real elements (the kernels' per-array
+    # limit): larger groups of instances become several classes
+    chunks = []
+    for shape, inst in mix:
+        per = max(1, (1 << 31) // (shape[0] * shape[1] * shape[2] * shape[3]))
+        while inst > 0:
+            chunks.append((shape, min(per, inst)))
+            inst -= min(per, inst)
+    out = {"classes": [f"{n}x {s[0]}^2xNz{s[2]}x{s[3]}" for s, n in chunks]}
+    for prec, name in ((S.SKB_F32, "fp32"), (S.SKB_F64, "fp64")):
+        batch = S.Batch(prec)
+        cd = S.complex_dtype(prec)
+        keep = []
+        for shape, inst in chunks:
+            w = S.Workload(*shape)
+            modes = S.retained_modes(w)
+            groups = S.k2_binned_groups(modes, 32)
+            fwd, inv = S.qr_gaussian_family(w.nz, len(groups), 129)
+            cls = batch.add_class(w, inst, modes, w.nz, groups, fwd, inv)
+            ws = S.Workload(shape[0], shape[1], shape[2], shape[3] * inst)
+            x = device_spectrum(torch, S, ws, modes, 7 + cls, device, cd)
+            y = torch.empty(batch.output_elements(cls), dtype=cd, device=device)
+            batch.bind(cls, x, y)
+            keep.append((x, y))
+        side = torch.cuda.Stream()
+        tune = batch.autotune(5, side)
+        batch.capture(side)
+        batch.launch(side)
+        ms = _timed_events(torch, lambda: batch.launch(side), 10, stream=side)
+        out[name] = {"batches_per_s": 1e3 / ms, "ms_per_batch": ms, "instance_evals_per_s": 96e3 / ms,
+                     "autotuned_grids": [[st["grid"] for st in c["stages"]] for c in tune]}
+        del batch, keep
+        torch.cuda.synchronize()
+    out["graph"] = "one hipGraph replay per batch (5 kernel launches per class, %d classes)" % len(chunks)
+    return out
+
+
+def bench_serving(args, S, torch, device):
+    """Serving mix: 64 clients (50% 256^2xNz65x3, 30% 512^2xNz129x4,
+    15% 128^2xNz33x3, 5% 1024^2xNz257x4), Poisson arrivals at 70% of the
+    measured single-request capacity, FP32 advection-flux steps; sampled
+    requests checked against the FP64 path (sizes up to Nz129)."""
+    import numpy as np
+
+    import serving as V
+
+    classes = [V.make_class(shape, f, device) for shape, f, _ in V.DEFAULT_MIX]
+    labels = V.assign_clients(V.DEFAULT_MIX, 64, 5)
+    clients = [V.Client(i, classes[labels[i]], V.initial_state(classes[labels[i]], 500 + i, device))
+               for i in range(64)]
+    server = V.Server(classes, budget_bytes=64 << 30, dt=1e-3, device=device, max_slots=4)
+    # capacity: mix-weighted single-request service time
+    svc = []
+    for c, (_, _, share) in zip(classes, V.DEFAULT_MIX):
+        cl = next(x for x in clients if x.cls is c)
+        server.step_alone(cl)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            server.step_alone(cl)
+        svc.append(share * (time.perf_counter() - t0) / 3)
+        cl.steps = 0
+    rate = 0.7 / sum(svc)
+    arrivals = V.poisson_arrivals(rate, args.serving_requests, 64, 9)
+    samples = []
+    done = server.run(clients, arrivals, sample_every=50,
+                      on_sample=lambda req, cl, x, f: samples.append((cl.cls, x, f)) if cl.cls.w.nz <= 132 else None)
+    summ = V.latency_summary(done, clients)
+    worst = 0.0
+    plans64 = {}
+    for cls, x, f in samples:
+        if cls.key not in plans64:
+            plans64[cls.key] = S.AdvectionFluxPlan(cls.w, cls.modes, cls.dz, S.SKB_F64)
+        ref = torch.empty(len(cls.modes) * cls.w.nz, dtype=torch.complex128, device=device)
+        plans64[cls.key].execute(x.to(torch.complex128), ref)
+        worst = max(worst, (f.to(torch.complex128) - ref).abs().max().item() / ref.abs().max().item())
+    summ.update({"arrival_rate_per_s": rate, "clients": 64, "slots": server.slots(),
+                 "max_in_flight": server.max_inflight, "sampled_checks": len(samples),
+                 "sampled_fp32_vs_fp64_max_rel": worst, "hbm_budget_bytes": 64 << 30,
+                 "executor_bytes": server.used})
+    return summ
+
+
 def main():
     args = parse()
     if args.summarize_pmc:
@@ -383,6 +529,14 @@ def main():
         "device": torch.cuda.get_device_name(0),
         "library": S.version(),
     }
+    if not args.no_extras:
+        for key, fn in (("advection_flux", bench_flux), ("batched", bench_batched), ("serving", bench_serving)):
+            log(args, "extra: %s" % key)
+            try:
+                line[key] = fn(args, S, torch, device)
+            except Exception as exc:  # a failing extra must not hide the headline
+                line[key] = {"error": "%s: %s" % (type(exc).__name__, exc)}
+            torch.cuda.synchronize()
     print(json.dumps(line), flush=True)
     return 0
 

@@ -492,12 +492,16 @@ __device__ __forceinline__ void r2c_load(cx<R>* v, const R* real, int item, int 
     const int y = item / ntiles, p0 = (item % ntiles) * TP;
     const int pa = p0 + 2 * s1, pb = pa + 1;
     const bool act = s1 < S;
-    const R* ra = real + (static_cast<size_t>(pa) * ny + y) * NX + n2;
-    const R* rb = ra + static_cast<size_t>(ny) * NX;
+    // Branch-free: a missing plane reads row 0 of plane 0 (in bounds) and
+    // selects zero.
+    const bool has_a = act && pa < planes, has_b = act && pb < planes;
+    const R* ra = has_a ? real + (static_cast<size_t>(pa) * ny + y) * NX + n2 : real;
+    const R* rb = has_b ? real + (static_cast<size_t>(pb) * ny + y) * NX + n2 : real;
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1) {
-        v[n1].x = (act && pa < planes) ? ra[N2 * n1] : R(0);
-        v[n1].y = (act && pb < planes) ? rb[N2 * n1] : R(0);
+        const R a = ra[N2 * n1], b = rb[N2 * n1];
+        v[n1].x = has_a ? a : R(0);
+        v[n1].y = has_b ? b : R(0);
     }
 }
 
@@ -579,37 +583,37 @@ __global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void row2_r2c_kernel(const 
 // read).  Thread (pair s fastest, n2).
 // Core: the pair (pa, pa + 1) of row y (row_base = ps*kc*y); has_a / has_b
 // say which of the two planes exist.  Thread (s1 = t % S, n2 = t / S).
-template <typename R, int NX>
-__device__ __forceinline__ void c2r_gather(cx<R>* v, const cx<R>* in, size_t row_base, int ps, int kc, int pa,
-                                           bool has_a, bool has_b) {
+// Branch-free: a lane with nothing to read loads the row's first element
+// (always in bounds) and selects zero, so no exec-mask branches surround the
+// 16 loads.  WIDE (FP32, even ps and pa): each pair is one 16-byte load; it
+// stays in bounds when has_b is false, because then ps > planes and plane
+// pa + 1 is padding.  Callers pick WIDE from wave-uniform values.
+template <typename R, int NX, bool WIDE>
+__device__ __forceinline__ void c2r_gather_t(cx<R>* v, const cx<R>* in, size_t row_base, int ps, int kc, int pa,
+                                             bool has_a, bool has_b) {
     using G = Geo2<R, NX>;
     constexpr int N1 = G::N1, N2 = G::N2, S = G::PAIRS;
     const int n2 = static_cast<int>(threadIdx.x) / S;
-    const bool act = n2 < N2;
-    // FP32 with an even plane stride and an even pa: the pair is one aligned
-    // 16-byte load.
-    const bool wide = sizeof(R) == 4 && ((ps | pa) & 1) == 0 && has_b;
+    const bool act = n2 < N2 && has_a;
+    const cx<R>* row = in + row_base;
+    constexpr bool wide = WIDE && sizeof(R) == 4;
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1) {
         const int k = N2 * n1 + n2;
         const bool upper = k > NX / 2;
         const int kx = upper ? NX - k : k;
-        cx<R> xa{R(0), R(0)}, xb{R(0), R(0)};
-        if (act && kx < kc) {
-            const size_t base = row_base + static_cast<size_t>(ps) * kx + pa;
-            if constexpr (sizeof(R) == 4) {
-                if (wide) {
-                    const float4 q = *reinterpret_cast<const float4*>(in + base);
-                    xa = cx<R>{q.x, q.y};
-                    xb = cx<R>{q.z, q.w};
-                } else {
-                    if (has_a) xa = in[base];
-                    if (has_b) xb = in[base + 1];
-                }
-            } else {
-                if (has_a) xa = in[base];
-                if (has_b) xb = in[base + 1];
-            }
+        const bool ok = act && kx < kc;
+        const unsigned off = ok ? static_cast<unsigned>(ps * kx + pa) : 0u;
+        cx<R> xa, xb;
+        if constexpr (wide) {
+            const float4 q = *reinterpret_cast<const float4*>(row + off);
+            xa = ok ? cx<R>{q.x, q.y} : cx<R>{R(0), R(0)};
+            xb = (ok && has_b) ? cx<R>{q.z, q.w} : cx<R>{R(0), R(0)};
+        } else {
+            const cx<R> a = row[off];
+            const cx<R> b = row[(ok && has_b) ? off + 1 : 0u];
+            xa = ok ? a : cx<R>{R(0), R(0)};
+            xb = (ok && has_b) ? b : cx<R>{R(0), R(0)};
         }
         if (kx == 0 || kx == NX / 2) {
             xa.y = R(0);
@@ -617,6 +621,14 @@ __device__ __forceinline__ void c2r_gather(cx<R>* v, const cx<R>* in, size_t row
         }
         v[n1] = upper ? cx<R>{xa.x + xb.y, xb.x - xa.y} : cx<R>{xa.x - xb.y, xa.y + xb.x};
     }
+}
+
+// wide: FP32 with ps and the tile's first plane both even (wave-uniform).
+template <typename R, int NX>
+__device__ __forceinline__ void c2r_gather(cx<R>* v, const cx<R>* in, size_t row_base, int ps, int kc, int pa,
+                                           bool has_a, bool has_b, bool wide) {
+    if (sizeof(R) == 4 && wide) c2r_gather_t<R, NX, true>(v, in, row_base, ps, kc, pa, has_a, has_b);
+    else c2r_gather_t<R, NX, false>(v, in, row_base, ps, kc, pa, has_a, has_b);
 }
 
 template <typename R, int NX>
@@ -628,7 +640,7 @@ __device__ __forceinline__ void c2r_load(cx<R>* v, const cx<R>* in, int item, in
     const int y = item / ntiles, p0 = (item % ntiles) * TP;
     const int pa = p0 + 2 * s1;
     c2r_gather<R, NX>(v, in, static_cast<size_t>(ps) * static_cast<size_t>(kc) * y, ps, kc, pa, pa < planes,
-                      pa + 1 < planes);
+                      pa + 1 < planes, (ps & 1) == 0);
 }
 
 template <typename R, int NX, bool LOOP>
@@ -714,7 +726,7 @@ __device__ __forceinline__ void flux_c2r(cx<R>* out, const cx<R>* tin, cx<R>* bu
     const int k1 = t % N1, s2 = t / N1;
     const int za = z0 + 2 * s1;
     cx<R> v[N1];
-    c2r_gather<R, NX>(v, tin, row_base, ps, kc, pbase + 2 * s1, za < nz, za + 1 < nz);
+    c2r_gather<R, NX>(v, tin, row_base, ps, kc, pbase + 2 * s1, za < nz, za + 1 < nz, ((ps | pbase) & 1) == 0);
     __syncthreads();  // buf and twl ready / free
     if (n2 < N2) {
         rdft<N1, true>(v);
@@ -843,12 +855,16 @@ __device__ __forceinline__ void col2_load(cx<R>* v, const cx<R>* in, int item, i
     const int ntiles = (planes + TP - 1) / TP;
     const int kx = item / ntiles, p = (item % ntiles) * TP + pp;
     const size_t P = static_cast<size_t>(planes);
+    // Branch-free loads: a lane with nothing to read loads element 0 (in
+    // bounds) and selects zero.
     if constexpr (IN == kStrided) {
+        const bool act = n2 < N2 && p < planes;
 #pragma unroll
         for (int n1 = 0; n1 < N1; ++n1) {
             const int yy = N2 * n1 + n2;
-            v[n1] = (n2 < N2 && p < planes) ? in[static_cast<size_t>(ps) * (kx + static_cast<size_t>(in_kc) * yy) + p]
-                                            : cx<R>{R(0), R(0)};
+            const size_t off = act ? static_cast<size_t>(ps) * (kx + static_cast<size_t>(in_kc) * yy) + p : 0;
+            const cx<R> x = in[off];
+            v[n1] = act ? x : cx<R>{R(0), R(0)};
         }
     } else {
         // all codes first, then all values: two batched round trips
@@ -862,16 +878,17 @@ __device__ __forceinline__ void col2_load(cx<R>* v, const cx<R>* in, int item, i
 #pragma unroll
         for (int n1 = 0; n1 < N1; ++n1) {
             cx<R> x{R(0), R(0)};
-            if (code[n1] >= 0 && p < planes) {
-                if constexpr (IN == kSplit) {
+            if constexpr (IN == kSplit) {
+                if (code[n1] >= 0 && p < planes) {
                     const size_t at = vl.stride * static_cast<size_t>(code[n1] >> 1);
                     x = cx<R>{vl.re[at], vl.im[at]};
                     if (code[n1] & 1) x.y = -x.y;
                     x = apply_factor(views, vl.factor, kx, N2 * n1 + n2, x);
-                } else {
-                    x = in[P * (code[n1] >> 1) + p];
-                    if (code[n1] & 1) x.y = -x.y;
                 }
+            } else {
+                const bool ok = code[n1] >= 0 && p < planes;
+                const cx<R> y = in[ok ? P * static_cast<size_t>(code[n1] >> 1) + p : 0];
+                x = ok ? cx<R>{y.x, (code[n1] & 1) ? -y.y : y.y} : cx<R>{R(0), R(0)};
             }
             v[n1] = x;
         }

@@ -116,6 +116,7 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmArgs<R> g) 
     extern __shared__ __align__(16) unsigned char gemm_lds[];
     cx<R>* sb = reinterpret_cast<cx<R>*>(gemm_lds);  // B block, then the output stage
     __shared__ int cbase[16], obase[16], pbase[16];
+    __shared__ R sar[4 * KP];  // vector-row coefficients (VR <= 4)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int j = lane & 15, q = lane >> 4;
     const bool one = gridDim.x >= static_cast<unsigned>(g.ntiles);
@@ -130,6 +131,11 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmArgs<R> g) 
             const R* a_ptr = g.ap + (static_cast<size_t>(t.group) * g.MB + wave) * KS * 64 + lane;
 #pragma unroll
             for (int s = 0; s < KS; ++s) a0[s] = a_ptr[s * 64];
+        }
+        // vector-row coefficients of the group -> LDS with the B block
+        {
+            const R* ap = g.arow + static_cast<size_t>(t.group) * g.VR * KP;
+            for (int e = tid; e < g.VR * KP; e += 256) sar[e] = ap[e];
         }
         if constexpr (WVM) {
             if (tid < 16) {
@@ -197,10 +203,9 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmArgs<R> g) 
         // across lanes j, j+16, j+32, j+48.
         cx<R> vrow{R(0), R(0)};
         if (wave < g.VR) {
-            const R* ar = g.arow + (static_cast<size_t>(t.group) * g.VR + wave) * KP + q * KS;
 #pragma unroll
             for (int s = 0; s < KS; ++s) {
-                const R a = ar[s];
+                const R a = sar[wave * KP + q * KS + s];
                 const cx<R> b = sb[j * CS + q * KS + s];
                 vrow.x += a * b.x;
                 vrow.y += a * b.y;

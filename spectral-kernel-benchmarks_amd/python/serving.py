@@ -12,8 +12,9 @@ Nz x Nz matrix; fused physical-space stage; FP32).
 Scheduling:
   * Requests arrive as a Poisson process; each names a client.
   * Executors (a plan and its own HIP stream) are pooled per size class.
-    The pool is sized by an HBM budget: executor device memory comes from the
-    plan, and client state is counted too.
+    The pools are sized by an HBM budget on executor device memory (the
+    plans' intermediates and scratch); client states are resident on top of
+    it.
   * A request is admitted as soon as an executor of its class is free and the
     client has nothing in flight (a client's steps are sequential).
     Requests that arrive while others run are admitted as executors free up,
@@ -254,8 +255,15 @@ def assign_clients(mix, clients: int, seed: int):
     return labels.tolist()
 
 
-def latency_summary(done):
+def latency_summary(done, clients=None):
     lat = np.array([r.finish - r.arrival for r in done]) * 1e3
     span = max(r.finish for r in done) - min(r.arrival for r in done)
-    return {"requests": len(done), "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "mean_ms": float(lat.mean()), "evals_per_s": len(done) / span}
+    out = {"requests": len(done), "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "mean_ms": float(lat.mean()), "evals_per_s": len(done) / span}
+    if clients is not None:
+        per = {}
+        for r in done:
+            per.setdefault(str(clients[r.client].cls.key), []).append(1e3 * (r.finish - r.arrival))
+        out["per_class"] = {k: {"requests": len(v), "p50_ms": float(np.percentile(v, 50)),
+                                "p99_ms": float(np.percentile(v, 99))} for k, v in per.items()}
+    return out

@@ -4,9 +4,11 @@ apart, optionally one strided view per field), through the C ABI.
 Mirrors the reference's checks (tests/skbench_tests.cpp:1341-1460): the
 split forward against the direct-DFT oracle gathered by mode, scaling by 0.25
 against a separate scale, the split inverse against the direct c2r of the
-embedded spectrum, and field views with mode stride Nz + 3.  The split and
-interleaved paths share every FFT instruction, so their outputs are compared
-bit for bit; the gaps between field-view columns must stay untouched.
+embedded spectrum, and field views with mode stride Nz + 3; the gaps between
+field-view columns must stay untouched.  The split and interleaved forms run
+the same FFT source in separately compiled kernel instantiations, so they are
+compared to rounding (1e-12 FP64, 1e-6 FP32 on the max-scale-normalised
+error), not bit for bit.
 """
 import sys
 
@@ -80,6 +82,11 @@ def test_split_forward_and_inverse_match_direct_dft(device, kind):
     assert np.array_equal(host(d_re), oracle_ret.real) and np.array_equal(host(d_im), oracle_ret.imag)
 
 
+def _close(a, b, precision):
+    tol = 1e-12 if precision == S.SKB_F64 else 1e-6
+    return O.max_relative_error(np.asarray(a, np.complex128), np.asarray(b, np.complex128)) <= tol
+
+
 def _field_views(torch, w, nmodes, stride, dtype, device):
     per = w.nz + stride * (nmodes - 1)
     views = []
@@ -124,16 +131,16 @@ def test_split_field_views_equal_interleaved(device, precision, shape):
     views = _field_views(torch, w, nm, stride, rd, device)
     plan.forward_split_fields(x, views)
     got, gaps = _gather_views(views, w, nm)
-    assert np.array_equal(got, np.transpose(ref_h, (1, 2, 0)))
+    assert _close(got, np.transpose(ref_h, (1, 2, 0)), precision)
     assert np.all(gaps == SENTINEL)
 
     # plain split layout equals the interleaved one
     re = torch.empty(nm * w.planes, dtype=rd, device=device)
     im = torch.empty_like(re)
     plan.forward_split(x, re, im)
-    assert np.array_equal(host(re) + 1j * host(im).astype(np.complex128), host(ref).astype(np.complex128))
+    assert _close(host(re) + 1j * host(im).astype(np.complex128), host(ref), precision)
 
-    # inverse: views holding the spectrum give the interleaved inverse exactly
+    # inverse: views holding the spectrum give the interleaved inverse
     spec = O.spectral_gaussian(ow(w), O.retained_modes(ow(w)), 129)
     d_spec = dev(spec, cd, device)
     real_ref = torch.empty(w.real_elements, dtype=rd, device=device)
@@ -150,12 +157,12 @@ def test_split_field_views_equal_interleaved(device, precision, shape):
         vi.copy_(torch.from_numpy(hi).to(rd))
     real = torch.empty_like(real_ref)
     plan.inverse_split_fields(in_views, real)
-    assert np.array_equal(host(real), host(real_ref))
+    assert _close(host(real), host(real_ref), precision)
     re_in = dev(spec.real, rd, device)
     im_in = dev(spec.imag, rd, device)
     real2 = torch.empty_like(real_ref)
     plan.inverse_split(re_in, im_in, real2)
-    assert np.array_equal(host(real2), host(real_ref))
+    assert _close(host(real2), host(real_ref), precision)
 
 
 def test_split_field_view_validation(device):
