@@ -130,6 +130,12 @@ int skb_fft_gather_retained(skb_fft_plan* plan, const skb_retained_mode* modes, 
                             const void* wvm, void* retained, void* stream);
 int skb_fft_embed_retained(skb_fft_plan* plan, const skb_retained_mode* modes, uint64_t mode_count,
                            const void* retained, void* wvm, void* stream);
+/* gatherRetainedModal (src/model.cpp:341-355): full WVM-order modal array
+ * (j + Nj*field + Nj*fields*(kx + (Nx/2+1)*ky)) -> compact modal array
+ * (j + Nj*field + Nj*fields*mode), conjugated where the mode conjugates its
+ * stored value. */
+int skb_fft_gather_retained_modal(skb_fft_plan* plan, const skb_retained_mode* modes, uint64_t mode_count,
+                                  uint64_t nj, const void* full_modal, void* modal, void* stream);
 
 /* ---- pruned retained-mode transforms -----------------------------------
  * Replaces FFTWPrunedProvider (include/skbench/skbench.hpp:467-520;
@@ -241,6 +247,68 @@ int skb_vertical_wvm_inverse(skb_vertical_plan* plan, const void* full_modal, vo
 uint64_t skb_vertical_wvm_modal_elements(const skb_vertical_plan* plan);
 int skb_vertical_wvm_initialize_modal(skb_vertical_plan* plan, void* full_modal, void* stream);
 int skb_vertical_wvm_initialize_spectrum(skb_vertical_plan* plan, void* full_spectrum, void* stream);
+
+/* ---- synthetic modal work and the spectral pipeline ---------------------
+ * syntheticModalWorkWeights (src/model.cpp:246-267): host, one real weight
+ * per modal element at j + Nj*(field + fields*mode):
+ * 0.75 + 0.5 * ((17|k| + 29|l| + 7j + 11 field) mod 97) / 96. */
+int skb_synthetic_modal_weights(const skb_workload* workload, const skb_retained_mode* modes, uint64_t mode_count,
+                                double* weights);
+/* applySyntheticModalWorkInterleaved / Split (src/model.cpp:269-284) on the
+ * device: output[e] = weights[e] * input[e] over `count` complex elements;
+ * weights, input and output in `precision`.  In place is allowed. */
+int skb_modal_work(int precision, uint64_t count, const void* weights, const void* input, void* output,
+                   void* stream);
+int skb_modal_work_split(int precision, uint64_t count, const void* weights, const void* input_real,
+                         const void* input_imag, void* output_real, void* output_imag, void* stream);
+/* applySyntheticModalWorkWvm (src/model.cpp:286-308) on a WVM-direct plan's
+ * full modal arrays: the retained entries of the input scaled by the
+ * weights (compact modal layout) into the output, and for k_x = 0,
+ * k_y != 0, 2k_y != Ny the conjugate written at (0, Ny - k_y).  Out of
+ * place; other output entries are left as they are. */
+int skb_vertical_wvm_modal_work(skb_vertical_plan* plan, const void* weights, const void* full_modal_input,
+                                void* full_modal_output, void* stream);
+
+/* Synthetic antialiased spectral pipeline: the spectral-pipeline kernel of
+ * runSpectralPipelineBenchmark (src/benchmark.cpp:2978-3547; timed sequence
+ * 3389-3403).  One evaluation maps real planes to real planes:
+ * unnormalised r2c -> retained modes -> vertical forward (K^2 family,
+ * Nj x Nz per group, Nj = 2(Nz-1)/3) -> diagonal modal weights -> vertical
+ * inverse -> zero-padded half spectrum -> unnormalised c2r.
+ * policy SKB_PIPELINE_WVM_DIRECT follows the wvm-direct boundary
+ * (src/benchmark.cpp:3204-3403): full WVM spectrum and modal arrays.
+ * policy SKB_PIPELINE_COMPACT follows the compact split boundaries
+ * (plane-major-fused-split, streaming-pruned-compact-split,
+ * src/benchmark.cpp:3405-3547): pruned transforms and the compact modal
+ * round trip, by default one kernel per tile (forward product, weights and
+ * inverse product with the modal block in LDS) where it is instantiated
+ * (Nz 5-8, 9-12, 17-20, 33-36, 65-68, 129-132; FP32 also 257-260).
+ * weights: host array in the modal layout, or NULL for the synthetic
+ * weights above.  Inputs are preserved; real_input may equal real_output. */
+enum skb_pipeline_policy { SKB_PIPELINE_WVM_DIRECT = 0, SKB_PIPELINE_COMPACT = 1 };
+typedef struct skb_spectral_pipeline skb_spectral_pipeline;
+int skb_spectral_pipeline_create(const skb_workload* workload, const skb_retained_mode* modes, uint64_t mode_count,
+                                 const skb_mode_group* groups, uint64_t group_count, const double* forward,
+                                 const double* inverse, const double* weights, int policy, int precision,
+                                 skb_spectral_pipeline** plan);
+int skb_spectral_pipeline_destroy(skb_spectral_pipeline* plan);
+int skb_spectral_pipeline_execute(skb_spectral_pipeline* plan, const void* real_input, void* real_output,
+                                  void* stream);
+/* The same evaluation stage by stage, writing the compact intermediates the
+ * reference's correctness metrics read (src/benchmark.cpp:3175-3198):
+ * retained horizontal coefficients and the inverse vertical output
+ * (retained layout), the forward modal and the post-work modal (modal
+ * layout).  Separate launches throughout (no fused round trip). */
+int skb_spectral_pipeline_stages(skb_spectral_pipeline* plan, const void* real_input, void* real_output,
+                                 void* retained, void* forward_modal, void* post_modal, void* inverse_retained,
+                                 void* stream);
+/* Compact policy: 1 = one-kernel modal round trip, 0 = three launches
+ * through a modal array (allocated on first use).  No reference
+ * counterpart. */
+int skb_spectral_pipeline_set_fused(skb_spectral_pipeline* plan, int fused);
+int skb_spectral_pipeline_fused(const skb_spectral_pipeline* plan);
+/* Device memory the plan holds (arrays, matrices, scratch). */
+uint64_t skb_spectral_pipeline_device_bytes(const skb_spectral_pipeline* plan);
 
 /* ---- antialiased nonlinear flux graph ------------------------------------
  * Replaces the production-lifetime flux graph of runProductionLifetimeFluxBenchmark

@@ -532,3 +532,98 @@ def retained_operator_fast(w: Workload, modes, retained, nj, first, count, forwa
     real = retained_inverse(w, modes, retained, workers)
     back = retained_forward(w, modes, real, workers)
     return vertical_forward_blas(w, nj, first, count, forward, back)
+
+
+def vertical_inverse_blas(w: Workload, nj: int, first, count, inverse: np.ndarray, modal: np.ndarray) -> np.ndarray:
+    """Same product as vertical_inverse, through numpy's BLAS; equality with the
+    C loop is checked in tests/test_oracle.py."""
+    nmodes = int(first[-1] + count[-1])
+    M = np.asarray(modal, np.complex128).reshape(nmodes, w.fields, nj)
+    inv = np.asarray(inverse, np.float64).reshape(len(first), w.nz, nj)
+    out = np.empty((nmodes, w.fields, w.nz), np.complex128)
+    for g in range(len(first)):
+        a, b = int(first[g]), int(first[g] + count[g])
+        out[a:b] = M[a:b] @ inv[g].T
+    return out.reshape(-1)
+
+
+# ---- synthetic modal work and the spectral pipeline ---------------------------
+def synthetic_modal_weights(w: Workload, modes: np.ndarray) -> np.ndarray:
+    """syntheticModalWorkWeights (src/model.cpp:246-267): weight of (mode, j,
+    field) = 0.75 + 0.5 * key / 96 with key = (17|k| + 29|l| + 7j + 11 field)
+    mod 97, stored at modalSpectrumIndex j + Nj*(field + F*mode)
+    (src/model.cpp:233-238).  Integer arithmetic in uint64."""
+    nj, F = w.nj, w.fields
+    k = np.abs(modes["k"].astype(np.int64)).astype(np.uint64)
+    l = np.abs(modes["l"].astype(np.int64)).astype(np.uint64)
+    j = np.arange(nj, dtype=np.uint64)
+    f = np.arange(F, dtype=np.uint64)
+    key = (np.uint64(17) * k[:, None, None] + np.uint64(29) * l[:, None, None] + np.uint64(11) * f[None, :, None]
+           + np.uint64(7) * j[None, None, :]) % np.uint64(97)
+    return (0.75 + 0.5 * key.astype(np.float64) / 96.0).reshape(-1)
+
+
+def _stored_frequencies(w: Workload, modes: np.ndarray):
+    kx = modes["stored_kx"].astype(np.int64)
+    ky = modes["stored_ky"].astype(np.int64)
+    return kx, ky, kx + w.nx_half * ky
+
+
+def apply_modal_work_wvm(w: Workload, modes: np.ndarray, weights: np.ndarray, full_in: np.ndarray,
+                         full_out: np.ndarray) -> np.ndarray:
+    """applySyntheticModalWorkWvm (src/model.cpp:286-308): a copy of full_out
+    whose retained entries are weight * full_in at the STORED frequency, and
+    whose k_x = 0 partner (0, Ny - ky) entries (ky != 0, 2ky != Ny) hold the
+    conjugate of that product."""
+    nj, F = w.nj, w.fields
+    kx, ky, q = _stored_frequencies(w, modes)
+    src = np.asarray(full_in, np.complex128).reshape(w.nx_half * w.ny, F * nj)
+    out = np.array(full_out, np.complex128).reshape(w.nx_half * w.ny, F * nj)
+    out[q] = src[q] * np.asarray(weights, np.float64).reshape(len(modes), F * nj)
+    part = np.nonzero((kx == 0) & (ky != 0) & (2 * ky != w.ny))[0]
+    out[w.nx_half * ((w.ny - ky[part]) % w.ny)] = np.conj(out[q[part]])
+    return out.reshape(-1)
+
+
+def gather_retained_modal(w: Workload, modes: np.ndarray, full_modal: np.ndarray) -> np.ndarray:
+    """gatherRetainedModal (src/model.cpp:341-355)."""
+    nj, F = w.nj, w.fields
+    _, _, q = _stored_frequencies(w, modes)
+    out = np.asarray(full_modal, np.complex128).reshape(w.nx_half * w.ny, F * nj)[q].copy()
+    cj = modes["conj"] != 0
+    out[cj] = np.conj(out[cj])
+    return out.reshape(-1)
+
+
+def embed_retained_modal(w: Workload, modes: np.ndarray, modal: np.ndarray) -> np.ndarray:
+    """embedRetainedModal (src/model.cpp:357-379): zero fill, stored value
+    (conjugated for conjugating modes) at the stored frequency, and its
+    conjugate at the k_x = 0 partner."""
+    nj, F = w.nj, w.fields
+    kx, ky, q = _stored_frequencies(w, modes)
+    stored = np.array(modal, np.complex128).reshape(len(modes), F * nj)
+    cj = modes["conj"] != 0
+    stored[cj] = np.conj(stored[cj])
+    out = np.zeros((w.nx_half * w.ny, F * nj), np.complex128)
+    out[q] = stored
+    part = np.nonzero((kx == 0) & (ky != 0) & (2 * ky != w.ny))[0]
+    out[w.nx_half * ((w.ny - ky[part]) % w.ny)] = np.conj(stored[part])
+    return out.reshape(-1)
+
+
+def spectral_pipeline(w: Workload, modes: np.ndarray, first, count, forward: np.ndarray, inverse: np.ndarray,
+                      weights: np.ndarray, real: np.ndarray, workers=None) -> dict:
+    """The reference's canonical spectral-pipeline oracle
+    (src/benchmark.cpp:3085-3114): unnormalised r2c in WVM order, gather of
+    the retained modes, vertical forward, modal work (weights in the modal
+    layout, applySyntheticModalWorkInterleaved, src/model.cpp:269-274),
+    vertical inverse, embed into the zero-padded half spectrum, unnormalised
+    c2r.  Returns the five arrays the reference's correctness metrics read
+    (src/benchmark.cpp:3175-3198)."""
+    nj = w.nj
+    retained = retained_forward(w, modes, real, workers)
+    fwd = vertical_forward_blas(w, nj, first, count, forward, retained)
+    post = np.asarray(weights, np.float64) * fwd
+    inv = vertical_inverse_blas(w, nj, first, count, inverse, post)
+    out = retained_inverse(w, modes, inv, workers)
+    return {"retained": retained, "forward": fwd, "post": post, "inverse": inv, "output": out}

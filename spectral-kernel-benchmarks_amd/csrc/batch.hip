@@ -57,14 +57,6 @@ using namespace skb;
 
 namespace {
 
-void check_status(int status) {
-    if (status != SKB_OK) {
-        char msg[1024];
-        skb_last_error(msg, sizeof(msg));
-        fail(status, msg);
-    }
-}
-
 void run_class(skb_batch_class& c, hipStream_t s) {
     check_status(skb_pruned_inverse(c.pruned, c.x, c.real.ptr, s));
     check_status(skb_pruned_forward(c.pruned, c.real.ptr, c.back.ptr, s));

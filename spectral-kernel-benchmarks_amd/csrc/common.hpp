@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -99,6 +100,21 @@ __device__ __forceinline__ int xcd_item(int b, int items) {
 // Sizes the hand-written FFT kernels are instantiated for.
 constexpr uint64_t kMinFft = 4;
 constexpr uint64_t kMaxFft = 1024;
+
+// Rethrows a failed status of the library's own C entry points.
+inline void check_status(int status) {
+    if (status != SKB_OK) {
+        char msg[1024];
+        skb_last_error(msg, sizeof(msg));
+        fail(status, msg);
+    }
+}
+
+// Grid of a grid-stride elementwise kernel of 256-thread workgroups.
+inline dim3 stream_grid(size_t total) {
+    const size_t blocks = (total + 255) / 256;
+    return dim3(static_cast<unsigned>(std::max<size_t>(1, std::min<size_t>(blocks, 8192))));
+}
 
 inline size_t complex_bytes(int precision) { return precision == SKB_F64 ? 16 : 8; }
 inline size_t real_bytes(int precision) { return precision == SKB_F64 ? 8 : 4; }

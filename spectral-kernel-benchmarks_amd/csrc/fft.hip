@@ -29,7 +29,7 @@
 #include <vector>
 
 #include "common.hpp"
-#include "flux_rows.hpp"
+#include "internal.hpp"
 
 namespace skb {
 
@@ -1450,11 +1450,6 @@ void require_ptrs(const void* a, const void* b) {
     if (!a || !b) fail(SKB_INVALID_ARGUMENT, "null device pointer");
 }
 
-dim3 stream_grid(size_t total) {
-    const size_t blocks = (total + 255) / 256;
-    return dim3(static_cast<unsigned>(std::min<size_t>(blocks, 8192)));
-}
-
 }  // namespace
 
 extern "C" {
@@ -1514,6 +1509,27 @@ int skb_fft_gather_retained(skb_fft_plan* plan, const skb_retained_mode* modes, 
                                static_cast<const cx<float>*>(wvm), static_cast<cx<float>*>(retained),
                                plan->gather_table.as<ColEntry>(), planes, nxh, total);
         check_launch("gather launch");
+    });
+}
+
+int skb_fft_gather_retained_modal(skb_fft_plan* plan, const skb_retained_mode* modes, uint64_t mode_count,
+                                  uint64_t nj, const void* full_modal, void* modal, void* stream) {
+    return guarded([&] {
+        if (!plan || nj == 0) fail(SKB_INVALID_ARGUMENT, "null plan or zero vertical modes");
+        require_ptrs(full_modal, modal);
+        refresh_mode_tables(plan, modes, mode_count);
+        const int block = static_cast<int>(nj * plan->w.fields);
+        const int nxh = static_cast<int>(plan->w.nx / 2 + 1);
+        const size_t total = mode_count * static_cast<size_t>(block);
+        if (plan->precision == SKB_F64)
+            hipLaunchKernelGGL(gather_kernel<double>, stream_grid(total), dim3(256), 0, as_stream(stream),
+                               static_cast<const cx<double>*>(full_modal), static_cast<cx<double>*>(modal),
+                               plan->gather_table.as<ColEntry>(), block, nxh, total);
+        else
+            hipLaunchKernelGGL(gather_kernel<float>, stream_grid(total), dim3(256), 0, as_stream(stream),
+                               static_cast<const cx<float>*>(full_modal), static_cast<cx<float>*>(modal),
+                               plan->gather_table.as<ColEntry>(), block, nxh, total);
+        check_launch("modal gather launch");
     });
 }
 

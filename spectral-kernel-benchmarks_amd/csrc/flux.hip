@@ -24,7 +24,7 @@
 #include <memory>
 
 #include "common.hpp"
-#include "flux_rows.hpp"
+#include "internal.hpp"
 
 struct skb_flux_plan {
     skb_workload w{};  // fields = targets (the output workload)
@@ -53,18 +53,6 @@ struct skb_flux_plan {
 };
 
 using namespace skb;
-
-namespace {
-
-void check_status(int status) {
-    if (status != SKB_OK) {
-        char msg[1024];
-        skb_last_error(msg, sizeof(msg));
-        fail(status, msg);
-    }
-}
-
-}  // namespace
 
 struct skb_advection_flux_plan {
     skb_workload w{};  // fields = 4: u, v, w, q

@@ -450,4 +450,26 @@ int skb_spectral_fixture(const skb_workload* w, const skb_retained_mode* modes, 
     });
 }
 
+// syntheticModalWorkWeights (src/model.cpp:246-267): the real diagonal
+// modal weight of (mode, j, field), at the modal index j + Nj*(field +
+// F*mode); |k| and |l| are taken in unsigned arithmetic.
+int skb_synthetic_modal_weights(const skb_workload* w, const skb_retained_mode* modes, uint64_t mode_count,
+                                double* weights) {
+    return guarded([&] {
+        validate_workload(w);
+        if (!modes || !weights) fail(SKB_INVALID_ARGUMENT, "modal weights need modes and an output array");
+        const uint64_t nj = w->nz < 2 ? 0 : 2 * (w->nz - 1) / 3, F = w->fields;
+        const auto magnitude = [](int64_t v) {
+            return v < 0 ? uint64_t{0} - static_cast<uint64_t>(v) : static_cast<uint64_t>(v);
+        };
+        double* out = weights;
+        for (uint64_t m = 0; m < mode_count; ++m) {
+            const uint64_t base = 17 * magnitude(modes[m].k) + 29 * magnitude(modes[m].l);
+            for (uint64_t f = 0; f < F; ++f)
+                for (uint64_t j = 0; j < nj; ++j)
+                    *out++ = 0.75 + 0.5 * static_cast<double>((base + 7 * j + 11 * f) % 97) / 96.0;
+        }
+    });
+}
+
 }  // extern "C"

@@ -1,5 +1,7 @@
-// Fused physical-space row stage of the flux graph, implemented in fft.hip
-// next to the pruned plans whose intermediates it reads and writes.
+// Entry points shared between the library's source files (not exported):
+// the fused physical-space row stage of the flux graph (fft.hip, next to the
+// pruned plans whose intermediates it reads and writes) and the vertical
+// products the composed graphs call (vertical.hip).
 #pragma once
 
 #include "skb_hip.h"
@@ -17,6 +19,15 @@ void flux_rows(skb_pruned_plan* pin, skb_pruned_plan* pout, int targets, double 
 // array: column c at in + K*(in_cs*c + in_co) (vertical.hip).
 void vertical_forward_columns(const skb_vertical_plan* p, const void* in, int in_cs, int in_co, void* out,
                               void* stream);
+
+// Modal round trip in one kernel: out[:, c] = I_g (w[:, c] .* (F_g in[:, c]))
+// over a compact plan's columns (weights in the plan's precision, modal
+// layout j + Nj*c); out may equal in.  Supported when the kernel is
+// instantiated for (Nz, Nj) (vertical.hip).
+bool vertical_round_trip_supported(const skb_vertical_plan* p);
+void vertical_round_trip(const skb_vertical_plan* p, const void* weights, const void* in, void* out, void* stream);
+// out[e] = w[e] * in[e] over n complex elements (in place allowed).
+void modal_work(int precision, uint64_t n, const void* weights, const void* in, void* out, void* stream);
 
 // One source field of the inverse column pass of a flux graph: compact
 // interleaved values of field `field` of an array with `fields` fields,

@@ -39,7 +39,7 @@
 #include <vector>
 
 #include "common.hpp"
-#include "flux_rows.hpp"
+#include "internal.hpp"
 
 namespace skb {
 
@@ -246,6 +246,238 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmArgs<R> g) 
     }
 }
 
+// Modal round trip of a K^2 group's columns in one kernel:
+//     out[:, c] = I_g * (w[:, c] .* (F_g * in[:, c]))
+// with F_g (Nj x Nz) the forward and I_g (Nz x Nj) the inverse matrix of the
+// column's group and w the real diagonal modal weights (layout j + Nj*c, the
+// modal layout).  The tile's 16 x Nj modal block never leaves the
+// workgroup: the forward product is scaled in registers and written to LDS
+// as the B block of the inverse product, so HBM sees the physical columns
+// once in and once out.  `out` may equal `in`: a tile reads all of its
+// columns into LDS before it writes any of them, and tiles own disjoint
+// columns.
+template <typename R>
+struct RoundTripArgs {
+    const cx<R>* in;
+    cx<R>* out;
+    const R* w;
+    const R *ap1, *arow1;  // forward operand (M = Nj, K = Nz)
+    const R *ap2, *arow2;  // inverse operand (M = Nz, K = Nj)
+    const GemmTile* tiles;
+    int ntiles, nz, nj, mb1, vr1, mb2, vr2;
+};
+
+template <typename R, int KS1, int KS2>
+__global__ __launch_bounds__(256) void modal_round_trip_kernel(const RoundTripArgs<R> g) {
+    using A4 = typename Acc<R>::type;
+    constexpr int KP1 = 4 * KS1, KP2 = 4 * KS2;
+    constexpr int CS1 = gemm_cs<R, KS1>(), CS2 = gemm_cs<R, KS2>();
+    constexpr int S1 = gemm_slots<KS1>(), S2 = gemm_slots<KS2>();
+    constexpr int NL = (16 * KP1 + 255) / 256;
+    __shared__ __align__(16) cx<R> sb1[16 * CS1];  // forward B block, then the 16 x Nz output stage
+    __shared__ __align__(16) cx<R> sb2[16 * CS2];  // weighted modal tile = inverse B block
+    __shared__ R sar1[4 * KP1], sar2[4 * KP2];     // vector-row coefficients (VR <= 4)
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int j = lane & 15, q = lane >> 4;
+    const bool one = gridDim.x >= static_cast<unsigned>(g.ntiles);
+    const int step = one ? g.ntiles : static_cast<int>(gridDim.x);
+    for (int tile = one ? xcd_item(blockIdx.x, g.ntiles) : static_cast<int>(blockIdx.x); tile < g.ntiles;
+         tile += step) {
+        const GemmTile t = g.tiles[__builtin_amdgcn_readfirstlane(tile)];
+        const size_t grp = static_cast<size_t>(t.group);
+        R a0[KS1];
+        if (wave < g.mb1) {
+            const R* a_ptr = g.ap1 + (grp * g.mb1 + wave) * KS1 * 64 + lane;
+#pragma unroll
+            for (int s = 0; s < KS1; ++s) a0[s] = a_ptr[s * 64];
+        }
+        // weights of this lane's forward rows; zero outside Nj and the tile
+        const bool col_ok = j < t.ncols;
+        const R* w_col = g.w + static_cast<size_t>(g.nj) * (t.col_begin + (col_ok ? j : 0));
+        R wt[S1][4];
+#pragma unroll
+        for (int sl = 0; sl < S1; ++sl) {
+            const int mb = wave + 4 * sl;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * mb + Acc<R>::row(lane, r);
+                const bool ok = col_ok && mb < g.mb1 && row < g.nj;
+                const R v = w_col[ok ? row : 0];
+                wt[sl][r] = ok ? v : R(0);
+            }
+        }
+        R wv = R(0);
+        if (wave < g.vr1 && col_ok) wv = w_col[16 * g.mb1 + wave];
+        for (int e = tid; e < g.vr1 * KP1; e += 256) sar1[e] = g.arow1[grp * g.vr1 * KP1 + e];
+        for (int e = tid; e < g.vr2 * KP2; e += 256) sar2[e] = g.arow2[grp * g.vr2 * KP2 + e];
+        // forward B block -> LDS, zero outside the tile's columns and for k >= Nz
+        cx<R> bv[NL];
+#pragma unroll
+        for (int n = 0; n < NL; ++n) {
+            const int e = tid + 256 * n, c = e / KP1, z = e - c * KP1;
+            bv[n] = cx<R>{R(0), R(0)};
+            if (e < 16 * KP1 && c < t.ncols && z < g.nz)
+                bv[n] = g.in[static_cast<size_t>(g.nz) * (t.col_begin + c) + z];
+        }
+        // inverse A fragments of this wavefront's first row block
+        R b0[KS2];
+        if (wave < g.mb2) {
+            const R* a_ptr = g.ap2 + (grp * g.mb2 + wave) * KS2 * 64 + lane;
+#pragma unroll
+            for (int s = 0; s < KS2; ++s) b0[s] = a_ptr[s * 64];
+        }
+#pragma unroll
+        for (int n = 0; n < NL; ++n) {
+            const int e = tid + 256 * n, c = e / KP1, z = e - c * KP1;
+            if (e < 16 * KP1) sb1[c * CS1 + z] = bv[n];
+        }
+        __syncthreads();
+        // forward product, scaled by the weights into the inverse B block
+#pragma unroll
+        for (int sl = 0; sl < S1; ++sl) {
+            const int mb = wave + 4 * sl;
+            if (mb < g.mb1) {
+                R a[KS1];
+                if (sl == 0) {
+#pragma unroll
+                    for (int s = 0; s < KS1; ++s) a[s] = a0[s];
+                } else {
+                    const R* a_ptr = g.ap1 + (grp * g.mb1 + mb) * KS1 * 64 + lane;
+#pragma unroll
+                    for (int s = 0; s < KS1; ++s) a[s] = a_ptr[s * 64];
+                }
+                A4 re = {0, 0, 0, 0}, im = {0, 0, 0, 0};
+#pragma unroll
+                for (int s = 0; s < KS1; ++s) {
+                    const cx<R> b = sb1[j * CS1 + 4 * s + q];
+                    re = Acc<R>::mfma(a[s], b.x, re);
+                    im = Acc<R>::mfma(a[s], b.y, im);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * mb + Acc<R>::row(lane, r);
+                    if (row < KP2) sb2[j * CS2 + row] = cx<R>{re[r] * wt[sl][r], im[r] * wt[sl][r]};
+                }
+            }
+        }
+        if (wave < g.vr1) {
+            cx<R> v{R(0), R(0)};
+#pragma unroll
+            for (int s = 0; s < KS1; ++s) {
+                const R a = sar1[wave * KP1 + q * KS1 + s];
+                const cx<R> b = sb1[j * CS1 + q * KS1 + s];
+                v.x += a * b.x;
+                v.y += a * b.y;
+            }
+            v.x += __shfl_xor(v.x, 16);
+            v.y += __shfl_xor(v.y, 16);
+            v.x += __shfl_xor(v.x, 32);
+            v.y += __shfl_xor(v.y, 32);
+            if (q == 0) sb2[j * CS2 + 16 * g.mb1 + wave] = cx<R>{v.x * wv, v.y * wv};
+        }
+        {
+            // rows the forward product does not cover, up to the inverse K padding
+            const int covered = g.vr1 > 0 ? g.nj : 16 * g.mb1;
+            const int gap = KP2 > covered ? KP2 - covered : 0;
+            for (int e = tid; e < 16 * gap; e += 256) sb2[(e / gap) * CS2 + covered + e % gap] = cx<R>{R(0), R(0)};
+        }
+        __syncthreads();
+        // inverse product; the result is staged in the forward B block's LDS
+        cx<R>* st = sb1;
+#pragma unroll
+        for (int sl = 0; sl < S2; ++sl) {
+            const int mb = wave + 4 * sl;
+            if (mb < g.mb2) {
+                R a[KS2];
+                if (sl == 0) {
+#pragma unroll
+                    for (int s = 0; s < KS2; ++s) a[s] = b0[s];
+                } else {
+                    const R* a_ptr = g.ap2 + (grp * g.mb2 + mb) * KS2 * 64 + lane;
+#pragma unroll
+                    for (int s = 0; s < KS2; ++s) a[s] = a_ptr[s * 64];
+                }
+                A4 re = {0, 0, 0, 0}, im = {0, 0, 0, 0};
+#pragma unroll
+                for (int s = 0; s < KS2; ++s) {
+                    const cx<R> b = sb2[j * CS2 + 4 * s + q];
+                    re = Acc<R>::mfma(a[s], b.x, re);
+                    im = Acc<R>::mfma(a[s], b.y, im);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * mb + Acc<R>::row(lane, r);
+                    if (row < g.nz) st[j * g.nz + row] = cx<R>{re[r], im[r]};
+                }
+            }
+        }
+        if (wave < g.vr2) {
+            cx<R> v{R(0), R(0)};
+#pragma unroll
+            for (int s = 0; s < KS2; ++s) {
+                const R a = sar2[wave * KP2 + q * KS2 + s];
+                const cx<R> b = sb2[j * CS2 + q * KS2 + s];
+                v.x += a * b.x;
+                v.y += a * b.y;
+            }
+            v.x += __shfl_xor(v.x, 16);
+            v.y += __shfl_xor(v.y, 16);
+            v.x += __shfl_xor(v.x, 32);
+            v.y += __shfl_xor(v.y, 32);
+            if (q == 0) st[j * g.nz + 16 * g.mb2 + wave] = v;
+        }
+        __syncthreads();
+        const int total = g.nz * t.ncols;
+        for (int e = tid; e < total; e += 256) g.out[static_cast<size_t>(g.nz) * t.col_begin + e] = st[e];
+        __syncthreads();  // LDS is reused by the next tile
+    }
+}
+
+// applySyntheticModalWork{Interleaved,Split,Wvm} (src/model.cpp:269-308):
+// out = w * in, one real weight per complex modal element.
+template <typename R>
+__global__ void modal_work_kernel(const R* __restrict__ w, const cx<R>* __restrict__ in, cx<R>* __restrict__ out,
+                                  size_t n) {
+    for (size_t e = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; e < n;
+         e += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const R s = w[e];
+        const cx<R> v = in[e];
+        out[e] = cx<R>{s * v.x, s * v.y};
+    }
+}
+
+template <typename R>
+__global__ void modal_work_split_kernel(const R* __restrict__ w, const R* __restrict__ in_re,
+                                        const R* __restrict__ in_im, R* __restrict__ out_re, R* __restrict__ out_im,
+                                        size_t n) {
+    for (size_t e = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; e < n;
+         e += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const R s = w[e];
+        out_re[e] = s * in_re[e];
+        out_im[e] = s * in_im[e];
+    }
+}
+
+// Full WVM modal arrays: element e = r + block*m (r = j + Nj*field, block =
+// Nj*F) of the logical modal order is read and written at frequency q[m];
+// the k_x = 0 partner frequency receives the conjugate.
+template <typename R>
+__global__ void modal_work_wvm_kernel(const R* __restrict__ w, const cx<R>* __restrict__ in, cx<R>* __restrict__ out,
+                                      const int* __restrict__ qmap, const int* __restrict__ partner, int block,
+                                      size_t n) {
+    for (size_t e = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; e < n;
+         e += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const size_t m = e / block, r = e - m * block;
+        const size_t at = static_cast<size_t>(block) * qmap[m] + r;
+        const R s = w[e];
+        const cx<R> v = in[at];
+        const cx<R> o{s * v.x, s * v.y};
+        out[at] = o;
+        const int p = partner[m];
+        if (p >= 0) out[static_cast<size_t>(block) * p + r] = conjg(o);
+    }
+}
+
 template <typename R>
 __global__ void fill_zero_kernel(cx<R>* __restrict__ out, size_t total) {
     for (size_t e = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; e < total;
@@ -313,6 +545,33 @@ void launch_gemm(const GemmArgs<R>& a, int req, hipStream_t s) {
         }
     }
     fail(SKB_UNSUPPORTED, "vertical GEMM inner dimension not instantiated");
+}
+
+// Round-trip kernel instantiations, (KS1, KS2) = (ceil(Nz/4), ceil(Nj/4)) for
+// Nj = 2(Nz-1)/3: Nz 5-8, 9-12, 17-20, 33-36, 65-68, 129-132 and (FP32)
+// 257-260.  Other sizes run the three-launch form (forward GEMM, weights,
+// inverse GEMM).
+template <typename R, int KS1, int KS2>
+void launch_round_trip_ks(const RoundTripArgs<R>& a, int req, hipStream_t s) {
+    const int grid = req > 0 && req < a.ntiles ? req : xcd_grid(a.ntiles);
+    hipLaunchKernelGGL((modal_round_trip_kernel<R, KS1, KS2>), dim3(grid), dim3(256), 0, s, a);
+    check_hip(hipGetLastError(), "modal round-trip launch");
+}
+
+template <typename R>
+bool launch_round_trip(const RoundTripArgs<R>& a, int req, hipStream_t s, bool dry_run) {
+    const int ks1 = (a.nz + 3) / 4, ks2 = (a.nj + 3) / 4;
+#define SKB_RT(A, B)                                                  \
+    if (ks1 == A && ks2 == B) {                                       \
+        if (!dry_run && a.ntiles > 0) launch_round_trip_ks<R, A, B>(a, req, s); \
+        return true;                                                  \
+    }
+    SKB_RT(2, 1) SKB_RT(3, 2) SKB_RT(5, 3) SKB_RT(9, 6) SKB_RT(17, 11) SKB_RT(33, 22)
+    if constexpr (sizeof(R) == 4) {
+        SKB_RT(65, 43)
+    }
+#undef SKB_RT
+    return false;
 }
 
 }  // namespace
@@ -492,6 +751,71 @@ void vertical_forward_columns(const skb_vertical_plan* p, const void* in, int in
 
 namespace {
 
+template <typename R>
+RoundTripArgs<R> round_trip_args(const skb_vertical_plan* p, const void* weights, const void* in, void* out) {
+    RoundTripArgs<R> a{};
+    a.in = static_cast<const cx<R>*>(in);
+    a.out = static_cast<cx<R>*>(out);
+    a.w = static_cast<const R*>(weights);
+    a.ap1 = p->fwd.ap.as<R>();
+    a.arow1 = p->fwd.arow.as<R>();
+    a.ap2 = p->inv.ap.as<R>();
+    a.arow2 = p->inv.arow.as<R>();
+    a.tiles = p->tiles.as<GemmTile>();
+    a.ntiles = p->ntiles;
+    a.nz = static_cast<int>(p->w.nz);
+    a.nj = static_cast<int>(p->nj);
+    a.mb1 = p->fwd.split.mb;
+    a.vr1 = p->fwd.split.vr;
+    a.mb2 = p->inv.split.mb;
+    a.vr2 = p->inv.split.vr;
+    return a;
+}
+
+bool round_trip(const skb_vertical_plan* p, const void* weights, const void* in, void* out, hipStream_t s,
+                bool dry_run) {
+    // row blocks must fit the accumulator slots of each phase
+    const int ks1 = static_cast<int>((p->w.nz + 3) / 4), ks2 = static_cast<int>((p->nj + 3) / 4);
+    if ((p->fwd.split.mb + 3) / 4 > (ks1 >= 22 ? 4 : 3) || (p->inv.split.mb + 3) / 4 > (ks2 >= 22 ? 4 : 3))
+        return false;
+    if (p->precision == SKB_F64)
+        return launch_round_trip(round_trip_args<double>(p, weights, in, out), p->grid, s, dry_run);
+    return launch_round_trip(round_trip_args<float>(p, weights, in, out), p->grid, s, dry_run);
+}
+
+template <typename R>
+void launch_modal_work(uint64_t n, const void* w, const void* in, void* out, hipStream_t s) {
+    hipLaunchKernelGGL(modal_work_kernel<R>, stream_grid(n), dim3(256), 0, s, static_cast<const R*>(w),
+                       static_cast<const cx<R>*>(in), static_cast<cx<R>*>(out), static_cast<size_t>(n));
+    check_hip(hipGetLastError(), "modal work launch");
+}
+
+}  // namespace
+
+namespace skb {
+
+bool vertical_round_trip_supported(const skb_vertical_plan* p) {
+    return p && !p->wvm && round_trip(p, nullptr, nullptr, nullptr, nullptr, true);
+}
+
+void vertical_round_trip(const skb_vertical_plan* p, const void* weights, const void* in, void* out, void* stream) {
+    if (!p || !weights || !in || !out) fail(SKB_INVALID_ARGUMENT, "null argument");
+    if (p->wvm) fail(SKB_LOGIC_ERROR, "the modal round trip runs on compact vertical plans");
+    if (!round_trip(p, weights, in, out, as_stream(stream), false))
+        fail(SKB_UNSUPPORTED, "modal round trip not instantiated for this vertical size");
+}
+
+void modal_work(int precision, uint64_t n, const void* weights, const void* in, void* out, void* stream) {
+    if (n == 0) return;
+    if (!weights || !in || !out) fail(SKB_INVALID_ARGUMENT, "null argument");
+    if (precision == SKB_F64) launch_modal_work<double>(n, weights, in, out, as_stream(stream));
+    else launch_modal_work<float>(n, weights, in, out, as_stream(stream));
+}
+
+}  // namespace skb
+
+namespace {
+
 void fill_zero(const skb_vertical_plan* p, void* out, uint64_t elements, void* stream) {
     if (!p || !out) fail(SKB_INVALID_ARGUMENT, "null argument");
     if (!p->wvm) fail(SKB_LOGIC_ERROR, "output initialisation belongs to WVM-direct vertical plans");
@@ -596,6 +920,60 @@ int skb_vertical_wvm_initialize_spectrum(skb_vertical_plan* plan, void* full_spe
         if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
         const auto& w = plan->w;
         fill_zero(plan, full_spectrum, (w.nx / 2 + 1) * w.ny * w.nz * w.fields, stream);
+    });
+}
+
+int skb_modal_work(int precision, uint64_t count, const void* weights, const void* input, void* output,
+                   void* stream) {
+    return guarded([&] {
+        check_precision(precision);
+        modal_work(precision, count, weights, input, output, stream);
+    });
+}
+
+int skb_modal_work_split(int precision, uint64_t count, const void* weights, const void* input_real,
+                         const void* input_imag, void* output_real, void* output_imag, void* stream) {
+    return guarded([&] {
+        check_precision(precision);
+        if (count == 0) return;
+        if (!weights || !input_real || !input_imag || !output_real || !output_imag)
+            fail(SKB_INVALID_ARGUMENT, "null argument");
+        const hipStream_t s = as_stream(stream);
+        if (precision == SKB_F64)
+            hipLaunchKernelGGL(modal_work_split_kernel<double>, stream_grid(count), dim3(256), 0, s,
+                               static_cast<const double*>(weights), static_cast<const double*>(input_real),
+                               static_cast<const double*>(input_imag), static_cast<double*>(output_real),
+                               static_cast<double*>(output_imag), static_cast<size_t>(count));
+        else
+            hipLaunchKernelGGL(modal_work_split_kernel<float>, stream_grid(count), dim3(256), 0, s,
+                               static_cast<const float*>(weights), static_cast<const float*>(input_real),
+                               static_cast<const float*>(input_imag), static_cast<float*>(output_real),
+                               static_cast<float*>(output_imag), static_cast<size_t>(count));
+        check_hip(hipGetLastError(), "modal work launch");
+    });
+}
+
+int skb_vertical_wvm_modal_work(skb_vertical_plan* plan, const void* weights, const void* full_modal_input,
+                                void* full_modal_output, void* stream) {
+    return guarded([&] {
+        if (!plan || !weights || !full_modal_input || !full_modal_output) fail(SKB_INVALID_ARGUMENT, "null argument");
+        if (!plan->wvm) fail(SKB_LOGIC_ERROR, "WVM modal work needs a WVM-direct vertical plan");
+        if (full_modal_input == full_modal_output)
+            fail(SKB_INVALID_ARGUMENT, "WVM modal work is out of place (partner entries are rewritten)");
+        const int block = static_cast<int>(plan->nj * plan->w.fields);
+        const uint64_t n = plan->modes * block;
+        const hipStream_t s = as_stream(stream);
+        if (plan->precision == SKB_F64)
+            hipLaunchKernelGGL(modal_work_wvm_kernel<double>, stream_grid(n), dim3(256), 0, s,
+                               static_cast<const double*>(weights), static_cast<const cx<double>*>(full_modal_input),
+                               static_cast<cx<double>*>(full_modal_output), plan->qmap.as<int>(),
+                               plan->partner.as<int>(), block, static_cast<size_t>(n));
+        else
+            hipLaunchKernelGGL(modal_work_wvm_kernel<float>, stream_grid(n), dim3(256), 0, s,
+                               static_cast<const float*>(weights), static_cast<const cx<float>*>(full_modal_input),
+                               static_cast<cx<float>*>(full_modal_output), plan->qmap.as<int>(),
+                               plan->partner.as<int>(), block, static_cast<size_t>(n));
+        check_hip(hipGetLastError(), "WVM modal work launch");
     });
 }
 

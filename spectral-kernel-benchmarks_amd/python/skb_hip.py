@@ -160,6 +160,18 @@ def _load():
         "skb_vertical_wvm_modal_elements": (u64, [vp]),
         "skb_vertical_wvm_initialize_modal": (i32, [vp, vp, vp]),
         "skb_vertical_wvm_initialize_spectrum": (i32, [vp, vp, vp]),
+        "skb_fft_gather_retained_modal": (i32, [vp, vp, u64, u64, vp, vp, vp]),
+        "skb_synthetic_modal_weights": (i32, [W, vp, u64, vp]),
+        "skb_modal_work": (i32, [i32, u64, vp, vp, vp, vp]),
+        "skb_modal_work_split": (i32, [i32, u64, vp, vp, vp, vp, vp, vp]),
+        "skb_vertical_wvm_modal_work": (i32, [vp, vp, vp, vp, vp]),
+        "skb_spectral_pipeline_create": (i32, [W, vp, u64, vp, u64, vp, vp, vp, i32, i32, pp]),
+        "skb_spectral_pipeline_destroy": (i32, [vp]),
+        "skb_spectral_pipeline_execute": (i32, [vp, vp, vp, vp]),
+        "skb_spectral_pipeline_stages": (i32, [vp, vp, vp, vp, vp, vp, vp, vp]),
+        "skb_spectral_pipeline_set_fused": (i32, [vp, i32]),
+        "skb_spectral_pipeline_fused": (i32, [vp]),
+        "skb_spectral_pipeline_device_bytes": (u64, [vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -187,7 +199,10 @@ EXPORTED = [
     "skb_advection_flux_plan_destroy", "skb_advection_flux_set_fused", "skb_advection_flux_execute",
     "skb_vertical_derivative_matrix", "skb_batch_create", "skb_batch_destroy", "skb_batch_add_class",
     "skb_batch_bind", "skb_batch_run", "skb_batch_capture", "skb_batch_launch", "skb_batch_autotune",
-    "skb_advection_flux_device_bytes",
+    "skb_advection_flux_device_bytes", "skb_fft_gather_retained_modal", "skb_synthetic_modal_weights",
+    "skb_modal_work", "skb_modal_work_split", "skb_vertical_wvm_modal_work", "skb_spectral_pipeline_create",
+    "skb_spectral_pipeline_destroy", "skb_spectral_pipeline_execute", "skb_spectral_pipeline_stages",
+    "skb_spectral_pipeline_set_fused", "skb_spectral_pipeline_fused", "skb_spectral_pipeline_device_bytes",
 ]
 
 
@@ -256,6 +271,18 @@ def _np(a):
 
 
 # ---- host model ---------------------------------------------------------------
+SKB_PIPELINE_WVM_DIRECT = 0
+SKB_PIPELINE_COMPACT = 1
+
+
+def synthetic_modal_weights(w: "Workload", modes: np.ndarray) -> np.ndarray:
+    """syntheticModalWorkWeights (src/model.cpp:246-267), modal layout."""
+    out = np.zeros(len(modes) * w.fields * w.nj)
+    wc = w.c()
+    check(lib.skb_synthetic_modal_weights(ctypes.byref(wc), _np(modes), len(modes), _np(out)))
+    return out
+
+
 def retained_modes(w: Workload) -> np.ndarray:
     wc = w.c()
     n = ctypes.c_uint64()
@@ -392,6 +419,14 @@ class FftPlan:
         check(lib.skb_fft_gather_retained(
             self.h, _np(modes), len(modes), _dev(wvm, w.spectrum_elements, complex_dtype(self.precision), "wvm"),
             _dev(retained, len(modes) * w.planes, complex_dtype(self.precision), "retained"), _stream(stream)))
+
+    def gather_retained_modal(self, modes, full_modal, modal, stream=None):
+        """gatherRetainedModal: full WVM-order modal -> compact modal."""
+        w, cd = self.w, complex_dtype(self.precision)
+        nj = w.nj
+        check(lib.skb_fft_gather_retained_modal(
+            self.h, _np(modes), len(modes), nj, _dev(full_modal, w.nx_half * w.ny * w.fields * nj, cd, "full_modal"),
+            _dev(modal, len(modes) * w.fields * nj, cd, "modal"), _stream(stream)))
 
     def embed_retained(self, modes, retained, wvm, stream=None):
         w = self.w
@@ -572,6 +607,7 @@ class WvmVerticalPlan:
     def __init__(self, w: Workload, modes: np.ndarray, nj: int, groups: np.ndarray, forward: np.ndarray,
                  inverse: np.ndarray, precision: int = SKB_F64):
         self.w, self.nj, self.precision = w, nj, precision
+        self.modes = len(modes)
         self._wc = w.c()
         forward = np.ascontiguousarray(forward, np.float64)
         inverse = np.ascontiguousarray(inverse, np.float64)
@@ -693,6 +729,93 @@ class AdvectionFluxPlan:
     def close(self):
         if getattr(self, "h", None) and lib is not None:
             lib.skb_advection_flux_plan_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+
+def modal_work(weights, inp, out, precision: int, stream=None):
+    """applySyntheticModalWorkInterleaved on the device: out = weights * inp."""
+    n = weights.numel()
+    check(lib.skb_modal_work(precision, n, _dev(weights, n, real_dtype(precision), "weights"),
+                             _dev(inp, n, complex_dtype(precision), "input"),
+                             _dev(out, n, complex_dtype(precision), "output"), _stream(stream)))
+
+
+def modal_work_split(weights, in_re, in_im, out_re, out_im, precision: int, stream=None):
+    """applySyntheticModalWorkSplit on the device."""
+    n = weights.numel()
+    rd = real_dtype(precision)
+    check(lib.skb_modal_work_split(precision, n, _dev(weights, n, rd, "weights"), _dev(in_re, n, rd, "input_real"),
+                                   _dev(in_im, n, rd, "input_imag"), _dev(out_re, n, rd, "output_real"),
+                                   _dev(out_im, n, rd, "output_imag"), _stream(stream)))
+
+
+def wvm_modal_work(plan: "WvmVerticalPlan", weights, full_in, full_out, stream=None):
+    """applySyntheticModalWorkWvm on a WVM-direct plan's full modal arrays."""
+    n = plan.modes * plan.w.fields * plan.nj
+    cd = complex_dtype(plan.precision)
+    check(lib.skb_vertical_wvm_modal_work(plan.h, _dev(weights, n, real_dtype(plan.precision), "weights"),
+                                          _dev(full_in, plan.modal_elements, cd, "full_modal_input"),
+                                          _dev(full_out, plan.modal_elements, cd, "full_modal_output"),
+                                          _stream(stream)))
+
+
+class SpectralPipeline:
+    """Synthetic antialiased spectral pipeline (src/benchmark.cpp:2978-3547):
+    real planes -> r2c -> retained -> vertical forward -> modal weights ->
+    vertical inverse -> c2r.  policy SKB_PIPELINE_WVM_DIRECT or
+    SKB_PIPELINE_COMPACT; weights None = the synthetic weights."""
+
+    def __init__(self, w: Workload, modes: np.ndarray, groups: np.ndarray, forward: np.ndarray, inverse: np.ndarray,
+                 policy: int = SKB_PIPELINE_COMPACT, precision: int = SKB_F64, weights: np.ndarray = None):
+        self.w, self.policy, self.precision = w, policy, precision
+        self.modes, self.nj = len(modes), w.nj
+        self._wc = w.c()
+        forward = np.ascontiguousarray(forward, np.float64)
+        inverse = np.ascontiguousarray(inverse, np.float64)
+        if forward.size != len(groups) * self.nj * w.nz or inverse.size != forward.size:
+            raise ValueError("matrix family size does not match groups x nj x nz")
+        wp = None
+        if weights is not None:
+            weights = np.ascontiguousarray(weights, np.float64)
+            if weights.size != self.modes * w.fields * self.nj:
+                raise ValueError("weights must hold one value per modal element")
+            wp = _np(weights)
+        h = ctypes.c_void_p()
+        check(lib.skb_spectral_pipeline_create(ctypes.byref(self._wc), _np(modes), len(modes), _np(groups),
+                                               len(groups), _np(forward), _np(inverse), wp, policy, precision,
+                                               ctypes.byref(h)))
+        self.h = h
+
+    @property
+    def fused(self) -> bool:
+        return bool(lib.skb_spectral_pipeline_fused(self.h))
+
+    def set_fused(self, fused: bool):
+        check(lib.skb_spectral_pipeline_set_fused(self.h, 1 if fused else 0))
+
+    def device_bytes(self) -> int:
+        return int(lib.skb_spectral_pipeline_device_bytes(self.h))
+
+    def execute(self, real_in, real_out, stream=None):
+        rd = real_dtype(self.precision)
+        n = self.w.real_elements
+        check(lib.skb_spectral_pipeline_execute(self.h, _dev(real_in, n, rd, "real_input"),
+                                                _dev(real_out, n, rd, "real_output"), _stream(stream)))
+
+    def stages(self, real_in, real_out, retained, forward_modal, post_modal, inverse_retained, stream=None):
+        rd, cd = real_dtype(self.precision), complex_dtype(self.precision)
+        n, nr, nm = self.w.real_elements, self.modes * self.w.planes, self.modes * self.w.fields * self.nj
+        check(lib.skb_spectral_pipeline_stages(
+            self.h, _dev(real_in, n, rd, "real_input"), _dev(real_out, n, rd, "real_output"),
+            _dev(retained, nr, cd, "retained"), _dev(forward_modal, nm, cd, "forward_modal"),
+            _dev(post_modal, nm, cd, "post_modal"), _dev(inverse_retained, nr, cd, "inverse_retained"),
+            _stream(stream)))
+
+    def close(self):
+        if getattr(self, "h", None) and lib is not None:
+            lib.skb_spectral_pipeline_destroy(self.h)
             self.h = None
 
     __del__ = close
