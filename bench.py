@@ -275,6 +275,53 @@ def bench_flux(args, S, torch, device):
     return out
 
 
+def bench_pipeline(args, S, torch, device):
+    """Synthetic spectral pipeline (r2c -> retained -> vertical forward ->
+    modal weights -> vertical inverse -> c2r) at 256^2 x Nz65 x 3 and
+    512^2 x Nz129 x 4, per boundary policy and precision.  The reference's
+    published M4 Max figures for the same round trips are quoted beside."""
+    import numpy as np
+
+    reference_ms = {(256, 65): {"wvm_direct": 13.258, "fused_split": 9.091},
+                    (512, 129): {"wvm_direct": 134.531, "fused_split": 131.357}}
+    out = {}
+    for nx, nz, f in ((256, 65, 3), (512, 129, 4)):
+        w = S.Workload(nx, nx, nz, f)
+        modes = S.retained_modes(w)
+        groups = S.k2_groups(modes)
+        fwd, inv = S.k2_vertical_family(w.nz, w.nj, groups)
+        g = torch.Generator(device=device)
+        g.manual_seed(129)
+        x64 = 2.0 * torch.rand(w.real_elements, generator=g, device=device, dtype=torch.float64) - 1.0
+        res, outputs = {}, {}
+        for policy, pname in ((S.SKB_PIPELINE_COMPACT, "compact"), (S.SKB_PIPELINE_WVM_DIRECT, "wvm_direct")):
+            for prec, name in ((S.SKB_F32, "fp32"), (S.SKB_F64, "fp64")):
+                pipe = S.SpectralPipeline(w, modes, groups, fwd, inv, policy, prec)
+                rd = S.real_dtype(prec)
+                x = x64.to(rd)
+                y = torch.empty_like(x)
+                variants = [("", None)] + ([("_unfused", False)] if policy == S.SKB_PIPELINE_COMPACT else [])
+                for suffix, fused in variants:
+                    if fused is not None:
+                        pipe.set_fused(fused)
+                    pipe.execute(x, y)
+                    ms = _timed_events(torch, lambda: pipe.execute(x, y), 10)
+                    res["%s_%s%s" % (pname, name, suffix)] = {"ms_per_eval": ms, "evals_per_s": 1e3 / ms}
+                outputs[(pname, name)] = y.to(torch.float64)
+                del pipe
+        ref = outputs[("compact", "fp64")]
+        scale = ref.abs().max().item()
+        res["fp32_vs_fp64_max_rel"] = max((outputs[(p, "fp32")] - ref).abs().max().item() / scale
+                                          for p in ("compact", "wvm_direct"))
+        res["wvm_vs_compact_fp64_max_rel"] = (outputs[("wvm_direct", "fp64")] - ref).abs().max().item() / scale
+        res["reference_m4max_ms"] = reference_ms[(nx, nz)]
+        out["%dx%d_nz%d_f%d" % (nx, nx, nz, f)] = res
+        del x64, outputs, ref
+        torch.cuda.empty_cache()
+    out["workload"] = "synthetic spectral pipeline, K^2 family, synthetic modal weights; uniform[-1,1) real input"
+    return out
+
+
 def bench_batched(args, S, torch, device):
     """The uneven batch: 40 x 128^2xNz33x3, 40 x 256^2xNz65x3, 16 x 512^2xNz129x4
     retained-operator instances in one hipGraph, launch shapes autotuned."""
@@ -530,7 +577,8 @@ def main():
         "library": S.version(),
     }
     if not args.no_extras:
-        for key, fn in (("advection_flux", bench_flux), ("batched", bench_batched), ("serving", bench_serving)):
+        for key, fn in (("advection_flux", bench_flux), ("spectral_pipeline", bench_pipeline),
+                        ("batched", bench_batched), ("serving", bench_serving)):
             log(args, "extra: %s" % key)
             try:
                 line[key] = fn(args, S, torch, device)

@@ -157,4 +157,43 @@ __device__ __forceinline__ cx<R> scal(cx<R> a, R s) {
     return {a.x * s, a.y * s};
 }
 
+// ---- bounds-checked buffer access -----------------------------------------
+// A raw buffer resource (stride 0) over `bytes` bytes: a load at an offset at
+// or past `bytes` returns zero and such a store is dropped, so a kernel can
+// issue a fixed number of memory operations whatever its edge cases.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, uint32_t bytes) {
+    // raw buffer, stride 0: offsets at or past `bytes` read zero / are dropped
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), static_cast<short>(0), bytes, 0x00020000);
+}
+
+// The LLVM buffer intrinsics, bound directly: this compiler lowers
+// __builtin_amdgcn_raw_buffer_load_b64 to a 32-bit load.
+typedef float bf2 __attribute__((ext_vector_type(2)));
+typedef double bd2 __attribute__((ext_vector_type(2)));
+__device__ bf2 raw_buffer_load_f2(__amdgpu_buffer_rsrc_t r, int voffset, int soffset, int aux) __asm(
+    "llvm.amdgcn.raw.ptr.buffer.load.v2f32");
+__device__ bd2 raw_buffer_load_d2(__amdgpu_buffer_rsrc_t r, int voffset, int soffset, int aux) __asm(
+    "llvm.amdgcn.raw.ptr.buffer.load.v2f64");
+__device__ void raw_buffer_store_f2(bf2 v, __amdgpu_buffer_rsrc_t r, int voffset, int soffset, int aux) __asm(
+    "llvm.amdgcn.raw.ptr.buffer.store.v2f32");
+__device__ void raw_buffer_store_d2(bd2 v, __amdgpu_buffer_rsrc_t r, int voffset, int soffset, int aux) __asm(
+    "llvm.amdgcn.raw.ptr.buffer.store.v2f64");
+
+template <typename R>
+__device__ __forceinline__ cx<R> buffer_load_cx(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    if constexpr (sizeof(R) == 4) {
+        const bf2 v = raw_buffer_load_f2(r, static_cast<int>(off), 0, 0);
+        return cx<R>{v.x, v.y};
+    } else {
+        const bd2 v = raw_buffer_load_d2(r, static_cast<int>(off), 0, 0);
+        return cx<R>{v.x, v.y};
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ void buffer_store_cx(cx<R> x, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    if constexpr (sizeof(R) == 4) raw_buffer_store_f2(bf2{x.x, x.y}, r, static_cast<int>(off), 0, 0);
+    else raw_buffer_store_d2(bd2{x.x, x.y}, r, static_cast<int>(off), 0, 0);
+}
+
 }  // namespace skb

@@ -246,6 +246,160 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmArgs<R> g) 
     }
 }
 
+// Streaming form for the compact layout with the output height M fixed at
+// compile time.  Persistent workgroups each own a contiguous run of tiles
+// (tiles are ordered by group, so a run mostly stays in one group).  The A
+// fragments of every row block stay in registers and the vector-row
+// coefficients in LDS until the group changes; the B block of the tile after
+// next is loaded into registers while the current tile is multiplied.
+// Loads and stores go through buffer resources sized to the tile, so a
+// column past the tile reads zero and an element past it is not written:
+// every thread issues the same, compile-time number of memory operations per
+// tile.  On gfx9 one in-order counter tracks loads and stores, so a fixed
+// store count lets the wait for the prefetched B block skip the current
+// tile's stores.
+constexpr int rows_mb(int M) { return (M >= 16 && M % 16 <= 4) ? M / 16 : (M + 15) / 16; }
+constexpr int rows_vr(int M) { return (M >= 16 && M % 16 <= 4) ? M % 16 : 0; }
+
+template <typename R, int KS, int M>
+__global__ __launch_bounds__(256) void grouped_gemm_fixed_kernel(const GemmArgs<R> g) {
+    using A4 = typename Acc<R>::type;
+    constexpr int KP = 4 * KS;
+    constexpr int CS = gemm_cs<R, KS>();
+    constexpr int MB = rows_mb(M), VR = rows_vr(M), SL = (MB + 3) / 4;
+    constexpr int NL = (16 * KP + 255) / 256;  // B elements per thread
+    constexpr int NS = (16 * M + 255) / 256;   // output elements per thread
+    constexpr uint32_t CB = sizeof(cx<R>);
+    __shared__ __align__(16) cx<R> sb[16 * CS];  // B block
+    __shared__ __align__(16) cx<R> st[16 * M];   // output stage, column-major
+    __shared__ R sar[(VR > 0 ? VR : 1) * KP];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int j = lane & 15, q = lane >> 4;
+    const int nwg = static_cast<int>(gridDim.x), wg = static_cast<int>(blockIdx.x);
+    const int t_begin = static_cast<int>((static_cast<long long>(g.ntiles) * wg) / nwg);
+    const int t_end = static_cast<int>((static_cast<long long>(g.ntiles) * (wg + 1)) / nwg);
+    if (t_begin >= t_end) return;  // uniform across the workgroup
+    const int in_step = g.K * g.in_cs;  // input column stride
+    auto load_b = [&](cx<R>* bv, const GemmTile& t) {
+        const cx<R>* base = g.in + static_cast<size_t>(g.K) * (static_cast<size_t>(g.in_cs) * t.col_begin + g.in_co);
+        const uint32_t bytes = static_cast<uint32_t>(in_step * (t.ncols - 1) + g.K) * CB;
+        const __amdgpu_buffer_rsrc_t r = buffer_rsrc(base, bytes);
+#pragma unroll
+        for (int n = 0; n < NL; ++n) {
+            const int e = tid + 256 * n, c = e / KP, z = e - c * KP;
+            bv[n] = buffer_load_cx<R>(r, static_cast<uint32_t>(in_step * c + z) * CB);
+        }
+    };
+    // Padding rows z >= K are zeroed here rather than at load time, so the
+    // prefetch is not waited for until its block is needed.
+    auto put_b = [&](const cx<R>* bv) {
+#pragma unroll
+        for (int n = 0; n < NL; ++n) {
+            const int e = tid + 256 * n, c = e / KP, z = e - c * KP;
+            if (16 * KP % 256 == 0 || e < 16 * KP) sb[c * CS + z] = z < g.K ? bv[n] : cx<R>{R(0), R(0)};
+        }
+    };
+    R a[SL][KS];
+    auto load_group = [&](int group) {
+#pragma unroll
+        for (int sl = 0; sl < SL; ++sl) {
+            const int mb = wave + 4 * sl;
+            if (mb < MB) {
+                const R* a_ptr = g.ap + (static_cast<size_t>(group) * MB + mb) * KS * 64 + lane;
+#pragma unroll
+                for (int s = 0; s < KS; ++s) a[sl][s] = a_ptr[s * 64];
+            }
+        }
+        if constexpr (VR > 0) {
+            const R* ap = g.arow + static_cast<size_t>(group) * VR * KP;
+            for (int e = tid; e < VR * KP; e += 256) sar[e] = ap[e];
+        }
+        // Group changes are rare: complete the fragment loads here, so the
+        // multiply of the following tiles carries no wait for them.
+        __builtin_amdgcn_s_waitcnt(0);
+    };
+    GemmTile cur = g.tiles[__builtin_amdgcn_readfirstlane(t_begin)];
+    int group = cur.group;
+    load_group(group);
+    cx<R> bv[NL];
+    load_b(bv, cur);
+    put_b(bv);
+    GemmTile nxt = cur;
+    if (t_begin + 1 < t_end) {
+        nxt = g.tiles[__builtin_amdgcn_readfirstlane(t_begin + 1)];
+        load_b(bv, nxt);
+    }
+    __syncthreads();
+    for (int tile = t_begin; tile < t_end; ++tile) {
+        A4 re[SL], im[SL];
+#pragma unroll
+        for (int sl = 0; sl < SL; ++sl) {
+            const int mb = wave + 4 * sl;
+            if (mb < MB) {
+                A4 r = {0, 0, 0, 0}, i = {0, 0, 0, 0};
+#pragma unroll
+                for (int s = 0; s < KS; ++s) {
+                    const cx<R> b = sb[j * CS + 4 * s + q];
+                    r = Acc<R>::mfma(a[sl][s], b.x, r);
+                    i = Acc<R>::mfma(a[sl][s], b.y, i);
+                }
+                re[sl] = r;
+                im[sl] = i;
+            }
+        }
+        cx<R> vrow{R(0), R(0)};
+        if (VR > 0 && wave < VR) {
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const R av = sar[wave * KP + q * KS + s];
+                const cx<R> b = sb[j * CS + q * KS + s];
+                vrow.x += av * b.x;
+                vrow.y += av * b.y;
+            }
+            vrow.x += __shfl_xor(vrow.x, 16);
+            vrow.y += __shfl_xor(vrow.y, 16);
+            vrow.x += __shfl_xor(vrow.x, 32);
+            vrow.y += __shfl_xor(vrow.y, 32);
+        }
+#pragma unroll
+        for (int sl = 0; sl < SL; ++sl) {
+            const int mb = wave + 4 * sl;
+            if (mb < MB) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * mb + Acc<R>::row(lane, r);
+                    if (row < M) st[j * M + row] = cx<R>{re[sl][r], im[sl][r]};
+                }
+            }
+        }
+        if (VR > 0 && wave < VR && q == 0) st[j * M + 16 * MB + wave] = vrow;
+        __syncthreads();  // stage complete; B block and coefficients are free
+        {
+            const __amdgpu_buffer_rsrc_t r =
+                buffer_rsrc(g.out + static_cast<size_t>(M) * cur.col_begin, static_cast<uint32_t>(M * cur.ncols) * CB);
+#pragma unroll
+            for (int n = 0; n < NS; ++n) {
+                const int e = tid + 256 * n;
+                const cx<R> v = st[e < 16 * M ? e : 0];
+                buffer_store_cx<R>(v, r, static_cast<uint32_t>(e) * CB);
+            }
+        }
+        if (tile + 1 < t_end) {
+            put_b(bv);  // the B block prefetched for tile + 1
+            cur = nxt;
+            if (cur.group != group) {
+                group = cur.group;
+                load_group(group);
+            }
+            if (tile + 2 < t_end) {
+                nxt = g.tiles[__builtin_amdgcn_readfirstlane(tile + 2)];
+                load_b(bv, nxt);
+            }
+            __syncthreads();  // B block (and coefficients) of tile + 1 visible; the stage is drained
+        }
+    }
+}
+
 // Modal round trip of a K^2 group's columns in one kernel:
 //     out[:, c] = I_g * (w[:, c] .* (F_g * in[:, c]))
 // with F_g (Nj x Nz) the forward and I_g (Nz x Nj) the inverse matrix of the
@@ -519,9 +673,40 @@ void operand_order(const double* mats, uint64_t groups, int M, int K, std::vecto
 
 namespace {
 
+// Workgroups of the fixed-height streaming kernel when the plan does not set
+// a count: three per CU of the 256-CU part.
+constexpr int kStreamGrid = 768;
+
+template <typename R, int KS, int M>
+void launch_gemm_fixed(const GemmArgs<R>& a, int req, hipStream_t s) {
+    const int grid = std::min(a.ntiles, req > 0 ? req : kStreamGrid);
+    hipLaunchKernelGGL((grouped_gemm_fixed_kernel<R, KS, M>), dim3(grid), dim3(256), 0, s, a);
+    check_hip(hipGetLastError(), "vertical GEMM launch");
+}
+
+// Output heights of the fixed-height kernel: the (K, M) pairs of the vertical
+// families at Nz = 33, 65, 129 (M = K = Nz for square families, Nj x Nz
+// forward and Nz x Nj inverse for the truncated ones); FP64 up to Nz = 65.
+template <typename R>
+bool launch_gemm_fixed_any(const GemmArgs<R>& a, int req, hipStream_t s) {
+    const int ks = (a.K + 3) / 4;
+#define SKB_FX(KSV, MV)                                  \
+    if (ks == KSV && a.M == MV) {                        \
+        launch_gemm_fixed<R, KSV, MV>(a, req, s);        \
+        return true;                                     \
+    }
+    SKB_FX(9, 33) SKB_FX(9, 21) SKB_FX(6, 33) SKB_FX(17, 65) SKB_FX(17, 42) SKB_FX(11, 65)
+    if constexpr (sizeof(R) == 4) {
+        SKB_FX(33, 129) SKB_FX(33, 85) SKB_FX(22, 129)
+    }
+#undef SKB_FX
+    return false;
+}
+
 template <typename R, int KS, bool WVM>
 void launch_gemm_ks(const GemmArgs<R>& a, int req, hipStream_t s) {
     constexpr int CS = gemm_cs<R, KS>();
+    if (!WVM && launch_gemm_fixed_any<R>(a, req, s)) return;
     const size_t lds = static_cast<size_t>(std::max(16 * CS, 16 * a.M)) * sizeof(cx<R>);
     const int grid = req > 0 && req < a.ntiles ? req : xcd_grid(a.ntiles);
     hipLaunchKernelGGL((grouped_gemm_kernel<R, KS, WVM>), dim3(grid), dim3(256), lds, s, a);
