@@ -196,4 +196,15 @@ __device__ __forceinline__ void buffer_store_cx(cx<R> x, __amdgpu_buffer_rsrc_t 
     else raw_buffer_store_d2(bd2{x.x, x.y}, r, static_cast<int>(off), 0, 0);
 }
 
+__device__ void raw_buffer_store_f1(float v, __amdgpu_buffer_rsrc_t r, int voffset, int soffset, int aux) __asm(
+    "llvm.amdgcn.raw.ptr.buffer.store.f32");
+__device__ void raw_buffer_store_d1(double v, __amdgpu_buffer_rsrc_t r, int voffset, int soffset, int aux) __asm(
+    "llvm.amdgcn.raw.ptr.buffer.store.f64");
+
+template <typename R>
+__device__ __forceinline__ void buffer_store_real(R x, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    if constexpr (sizeof(R) == 4) raw_buffer_store_f1(x, r, static_cast<int>(off), 0, 0);
+    else raw_buffer_store_d1(x, r, static_cast<int>(off), 0, 0);
+}
+
 }  // namespace skb

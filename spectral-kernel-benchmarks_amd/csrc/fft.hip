@@ -682,16 +682,22 @@ __global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void row2_c2r_kernel(const 
 #pragma unroll
         for (int m = 0; m < N2; ++m) w[m] = buf[s2 * SS + k1 * YS + m];
         rdft<N2, true>(w);
-        const int pa = p0 + 2 * s2;
-        R* ra = real + (static_cast<size_t>(pa) * ny + y) * NX + k1;
-        R* rb = ra + static_cast<size_t>(ny) * NX;
-        if (pa < planes) {
+        // Bounds-checked stores over the tile's rows (planes past the last
+        // are dropped), so every thread issues 2*N2 stores: the wait for the
+        // prefetched item then does not include them.
+        {
+            const int pa = p0 + 2 * s2;
+            const R* base = real + (static_cast<size_t>(p0) * ny + y) * NX;
+            const uint32_t plane_bytes = static_cast<uint32_t>(ny) * NX * sizeof(R);
+            const int valid = planes - p0 < TP ? planes - p0 : TP;  // planes of this tile that exist (>= 1)
+            const __amdgpu_buffer_rsrc_t r =
+                buffer_rsrc(base, static_cast<uint32_t>(valid - 1) * plane_bytes + NX * sizeof(R));
+            const uint32_t oa = static_cast<uint32_t>(pa - p0) * plane_bytes + k1 * sizeof(R);
 #pragma unroll
-            for (int k2 = 0; k2 < N2; ++k2) ra[N1 * k2] = w[k2].x;
-        }
-        if (pa + 1 < planes) {
-#pragma unroll
-            for (int k2 = 0; k2 < N2; ++k2) rb[N1 * k2] = w[k2].y;
+            for (int k2 = 0; k2 < N2; ++k2) {
+                buffer_store_real<R>(w[k2].x, r, oa + N1 * k2 * sizeof(R));
+                buffer_store_real<R>(w[k2].y, r, oa + plane_bytes + N1 * k2 * sizeof(R));
+            }
         }
         if (!LOOP || next >= items) break;
         __syncthreads();  // buf is rewritten by the next item
