@@ -310,6 +310,44 @@ int skb_spectral_pipeline_fused(const skb_spectral_pipeline* plan);
 /* Device memory the plan holds (arrays, matrices, scratch). */
 uint64_t skb_spectral_pipeline_device_bytes(const skb_spectral_pipeline* plan);
 
+/* ---- constant-stratification type-I vertical transforms -----------------
+ * Replaces the six FFTW guru64 REDFT00/RODFT00 plans and the forward
+ * normalisation of the constant-stratification vertical benchmark
+ * (src/constant_stratification.cpp:95-147 R2RPlan, 196-235
+ * normalizeForwardDct / normalizeForwardDst, 271-292
+ * executeCompleteVerticalGraph).  Arrays are complex [rows][channels][nz],
+ * z adjacent (element (row, channel, z) at z + nz*(channel + channels*row));
+ * a transform applies, in place, to channels [first, first + count) of every
+ * row, real and imaginary parts alike.  Kinds:
+ *   SKB_TYPE1_DCT          REDFT00 of length nz (unnormalised)
+ *   SKB_TYPE1_DST          RODFT00 of the nz - 2 interior points; the two
+ *                          endpoints are left as they are
+ *   SKB_TYPE1_DCT_FORWARD  DCT-I, then x 1/(nz-1) and the last point x 0.5
+ *   SKB_TYPE1_DST_FORWARD  DST-I, then x 1/(nz-1), endpoints set to zero
+ * Each is an nz x nz real matrix on the matrix cores (nz <= 132 FP64, 260
+ * FP32). */
+enum skb_type1_kind { SKB_TYPE1_DCT = 0, SKB_TYPE1_DST = 1, SKB_TYPE1_DCT_FORWARD = 2, SKB_TYPE1_DST_FORWARD = 3 };
+typedef struct skb_type1_plan skb_type1_plan;
+/* The transform of `kind` as a row-major nz x nz matrix (host). */
+int skb_type1_matrix(uint64_t nz, int kind, double* matrix);
+int skb_type1_plan_create(uint64_t rows, uint64_t nz, int precision, skb_type1_plan** plan);
+int skb_type1_plan_destroy(skb_type1_plan* plan);
+int skb_type1_execute(skb_type1_plan* plan, int kind, uint64_t channels, uint64_t first_channel, uint64_t count,
+                      void* data, void* stream);
+/* executeCompleteVerticalGraph: on a three-channel arena, 3 x (DCT of
+ * channels 0-1, DST of channel 2) and 2 x (DST of channels 0-1, DCT of
+ * channel 2); on a one-channel arena, 2 x forward DCT and 2 x forward DST
+ * (15 inverse and 4 forward complex channel transforms). */
+int skb_type1_vertical_schedule(skb_type1_plan* plan, void* arena3, void* arena1, void* stream);
+/* The benchmark's deterministic fill (src/constant_stratification.cpp:237-269):
+ * value(row, channel, z) = (sin a, cos 1.7a), a = 0.017(1 + row mod 97) +
+ * 0.113(1 + channel) + 0.071(1 + z).  modes == NULL fills `rows` full-half
+ * rows; otherwise row m holds the retained mode m's stored frequency
+ * stored_kx + (Nx/2+1) stored_ky (nx_half = Nx/2+1), conjugated where the
+ * mode conjugates its stored value.  Interleaved FP64 output. */
+int skb_type1_fill(uint64_t rows, uint64_t channels, uint64_t nz, const skb_retained_mode* modes, uint64_t nx_half,
+                   double* data);
+
 /* ---- antialiased nonlinear flux graph ------------------------------------
  * Replaces the production-lifetime flux graph of runProductionLifetimeFluxBenchmark
  * (src/benchmark.cpp:3957-4234; serial oracle 4182-4234).  workload->fields

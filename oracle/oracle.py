@@ -627,3 +627,104 @@ def spectral_pipeline(w: Workload, modes: np.ndarray, first, count, forward: np.
     inv = vertical_inverse_blas(w, nj, first, count, inverse, post)
     out = retained_inverse(w, modes, inv, workers)
     return {"retained": retained, "forward": fwd, "post": post, "inverse": inv, "output": out}
+
+
+# ---- constant-stratification type-I vertical transforms ----------------------
+def direct_forward_dct(x: np.ndarray) -> np.ndarray:
+    """directForwardDct (src/constant_stratification.cpp:237-256): the WVM
+    normalised DCT-I of one complex column, as an explicit sum."""
+    x = np.asarray(x, np.complex128)
+    nz = x.size
+    out = np.zeros(nz, np.complex128)
+    for j in range(nz):
+        v = x[0] + (1.0 if j % 2 == 0 else -1.0) * x[-1]
+        for z in range(1, nz - 1):
+            v += 2.0 * np.cos(np.pi * (j * z) / (nz - 1)) * x[z]
+        v /= nz - 1
+        if j == nz - 1:
+            v *= 0.5
+        out[j] = v
+    return out
+
+
+def direct_forward_dst(x: np.ndarray) -> np.ndarray:
+    """directForwardDst (src/constant_stratification.cpp:258-269): the WVM
+    normalised DST-I of the interior, zero endpoints."""
+    x = np.asarray(x, np.complex128)
+    nz = x.size
+    out = np.zeros(nz, np.complex128)
+    for j in range(1, nz - 1):
+        v = 0.0
+        for z in range(1, nz - 1):
+            v += 2.0 * np.sin(np.pi * (j * z) / (nz - 1)) * x[z]
+        out[j] = v / (nz - 1)
+    return out
+
+
+TYPE1_DCT, TYPE1_DST, TYPE1_DCT_FORWARD, TYPE1_DST_FORWARD = 0, 1, 2, 3
+
+
+def type1(kind: int, data: np.ndarray, first: int = 0, count: int | None = None) -> np.ndarray:
+    """One FFTW R2RPlan execution (src/constant_stratification.cpp:95-147) on
+    complex data (rows, channels, nz), channels [first, first + count):
+    REDFT00 over all nz points or RODFT00 over the interior (scipy's
+    unnormalised DCT/DST type 1 are those transforms), real and imaginary
+    parts alike; the forward kinds add normalizeForwardDct / Dst
+    (196-235).  Returns a new array."""
+    import scipy.fft
+
+    out = np.array(data, np.complex128)
+    count = out.shape[1] - first if count is None else count
+    sl = out[:, first:first + count, :]
+    nz = out.shape[2]
+    if kind in (TYPE1_DCT, TYPE1_DCT_FORWARD):
+        t = scipy.fft.dct(sl.real, type=1, axis=-1) + 1j * scipy.fft.dct(sl.imag, type=1, axis=-1)
+        if kind == TYPE1_DCT_FORWARD:
+            t /= nz - 1
+            t[..., -1] *= 0.5
+        sl[...] = t
+    else:
+        inner = sl[..., 1:-1]
+        t = scipy.fft.dst(inner.real, type=1, axis=-1) + 1j * scipy.fft.dst(inner.imag, type=1, axis=-1)
+        if kind == TYPE1_DST_FORWARD:
+            sl[..., 1:-1] = t / (nz - 1)
+            sl[..., 0] = 0.0
+            sl[..., -1] = 0.0
+        else:
+            sl[..., 1:-1] = t
+    return out
+
+
+def type1_schedule(arena3: np.ndarray, arena1: np.ndarray):
+    """executeCompleteVerticalGraph (src/constant_stratification.cpp:271-292)."""
+    a3, a1 = np.array(arena3, np.complex128), np.array(arena1, np.complex128)
+    for _ in range(3):
+        a3 = type1(TYPE1_DCT, a3, 0, 2)
+        a3 = type1(TYPE1_DST, a3, 2, 1)
+    for _ in range(2):
+        a3 = type1(TYPE1_DST, a3, 0, 2)
+        a3 = type1(TYPE1_DCT, a3, 2, 1)
+    for _ in range(2):
+        a1 = type1(TYPE1_DCT_FORWARD, a1)
+    for _ in range(2):
+        a1 = type1(TYPE1_DST_FORWARD, a1)
+    return a3, a1
+
+
+def type1_fill(rows: int, channels: int, nz: int, modes: np.ndarray | None = None, nx_half: int = 0) -> np.ndarray:
+    """fillFull / fillCompact (src/constant_stratification.cpp:237-269):
+    (sin a, cos 1.7a), a = 0.017(1 + row mod 97) + 0.113(1 + channel) +
+    0.071(1 + z); compact rows use the mode's stored frequency and conjugate
+    conjugating modes."""
+    if modes is None:
+        row = np.arange(rows, dtype=np.int64)
+        cj = np.zeros(rows, bool)
+    else:
+        row = modes["stored_kx"].astype(np.int64) + nx_half * modes["stored_ky"].astype(np.int64)
+        cj = modes["conj"] != 0
+    c = np.arange(channels)
+    z = np.arange(nz)
+    a = (0.017 * (1 + row % 97))[:, None, None] + (0.113 * (1 + c))[None, :, None] + (0.071 * (1 + z))[None, None, :]
+    out = np.sin(a) + 1j * np.cos(1.7 * a)
+    out[cj] = np.conj(out[cj])
+    return out

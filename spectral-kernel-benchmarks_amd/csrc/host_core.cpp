@@ -472,4 +472,29 @@ int skb_synthetic_modal_weights(const skb_workload* w, const skb_retained_mode* 
     });
 }
 
+// fillFull / fillCompact of the constant-stratification vertical benchmark
+// (src/constant_stratification.cpp:237-269).
+int skb_type1_fill(uint64_t rows, uint64_t channels, uint64_t nz, const skb_retained_mode* modes, uint64_t nx_half,
+                   double* data) {
+    return guarded([&] {
+        if (!data || rows == 0 || channels == 0 || nz == 0) fail(SKB_INVALID_ARGUMENT, "bad fill arguments");
+        for (uint64_t r = 0; r < rows; ++r) {
+            uint64_t row = r;
+            bool conj = false;
+            if (modes) {
+                row = modes[r].stored_kx + nx_half * modes[r].stored_ky;
+                conj = modes[r].conjugates_stored_value != 0;
+            }
+            for (uint64_t c = 0; c < channels; ++c)
+                for (uint64_t z = 0; z < nz; ++z) {
+                    const double a = 0.017 * static_cast<double>(1 + row % 97) + 0.113 * static_cast<double>(1 + c) +
+                                     0.071 * static_cast<double>(1 + z);
+                    const uint64_t at = 2 * (z + nz * (c + channels * r));
+                    data[at] = std::sin(a);
+                    data[at + 1] = conj ? -std::cos(1.7 * a) : std::cos(1.7 * a);
+                }
+        }
+    });
+}
+
 }  // extern "C"

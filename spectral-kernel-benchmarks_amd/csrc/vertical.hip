@@ -33,6 +33,7 @@
 //   D[i][j]: j = l % 16 and, for accumulator element r,
 //            f32: i = 4 * (l / 16) + r      f64: i = l / 16 + 4 * r
 #include <algorithm>
+#include <cmath>
 #include <memory>
 #include <string>
 #include <utility>
@@ -101,7 +102,7 @@ struct GemmArgs {
     const R* arow;  // [group][VR][4*KS]: rows 16*MB .. M-1 (zero for k >= K)
     const GemmTile* tiles;
     int ntiles, M, K, MB, VR, F;
-    int in_cs, in_co;    // compact: input column c starts at in + K*(in_cs*c + in_co)
+    int in_ld, out_ld;   // compact: column c at in + in_ld*c and out + out_ld*c (complex elements)
     const int* qmap;     // WVM direct: frequency q of each mode
     const int* partner;  // WVM direct: frequency of the k_x = 0 conjugate partner, or -1
 };
@@ -162,7 +163,7 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmArgs<R> g) 
             if (e < 16 * KP && c < t.ncols && z < g.K) {
                 const size_t src =
                     WVM ? static_cast<size_t>(cbase[c]) + z
-                        : static_cast<size_t>(g.K) * (static_cast<size_t>(g.in_cs) * (t.col_begin + c) + g.in_co) + z;
+                        : static_cast<size_t>(g.in_ld) * (t.col_begin + c) + z;
                 bv[n] = g.in[src];
             }
         }
@@ -238,8 +239,11 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmArgs<R> g) 
                 const int c = e / g.M, row = e - c * g.M;
                 g.out[static_cast<size_t>(obase[c]) + row] = v;
                 if (pbase[c] >= 0) g.out[static_cast<size_t>(pbase[c]) + row] = conjg(v);
-            } else {
+            } else if (g.out_ld == g.M) {
                 g.out[static_cast<size_t>(g.M) * t.col_begin + e] = v;
+            } else {
+                const int c = e / g.M, row = e - c * g.M;
+                g.out[static_cast<size_t>(g.out_ld) * (t.col_begin + c) + row] = v;
             }
         }
         __syncthreads();  // the stage and the column bases are reused by the next tile
@@ -279,9 +283,9 @@ __global__ __launch_bounds__(256) void grouped_gemm_fixed_kernel(const GemmArgs<
     const int t_begin = static_cast<int>((static_cast<long long>(g.ntiles) * wg) / nwg);
     const int t_end = static_cast<int>((static_cast<long long>(g.ntiles) * (wg + 1)) / nwg);
     if (t_begin >= t_end) return;  // uniform across the workgroup
-    const int in_step = g.K * g.in_cs;  // input column stride
+    const int in_step = g.in_ld;  // input column stride
     auto load_b = [&](cx<R>* bv, const GemmTile& t) {
-        const cx<R>* base = g.in + static_cast<size_t>(g.K) * (static_cast<size_t>(g.in_cs) * t.col_begin + g.in_co);
+        const cx<R>* base = g.in + static_cast<size_t>(g.in_ld) * t.col_begin;
         const uint32_t bytes = static_cast<uint32_t>(in_step * (t.ncols - 1) + g.K) * CB;
         const __amdgpu_buffer_rsrc_t r = buffer_rsrc(base, bytes);
 #pragma unroll
@@ -375,13 +379,17 @@ __global__ __launch_bounds__(256) void grouped_gemm_fixed_kernel(const GemmArgs<
         if (VR > 0 && wave < VR && q == 0) st[j * M + 16 * MB + wave] = vrow;
         __syncthreads();  // stage complete; B block and coefficients are free
         {
+            // column c of the tile at out + out_ld*(col_begin + c); elements of
+            // columns past the tile fall outside the resource and are dropped
+            const int ld = g.out_ld;
             const __amdgpu_buffer_rsrc_t r =
-                buffer_rsrc(g.out + static_cast<size_t>(M) * cur.col_begin, static_cast<uint32_t>(M * cur.ncols) * CB);
+                buffer_rsrc(g.out + static_cast<size_t>(ld) * cur.col_begin,
+                            static_cast<uint32_t>(ld * (cur.ncols - 1) + M) * CB);
 #pragma unroll
             for (int n = 0; n < NS; ++n) {
-                const int e = tid + 256 * n;
+                const int e = tid + 256 * n, c = e / M, row = e - c * M;
                 const cx<R> v = st[e < 16 * M ? e : 0];
-                buffer_store_cx<R>(v, r, static_cast<uint32_t>(e) * CB);
+                buffer_store_cx<R>(v, r, c < 16 ? static_cast<uint32_t>(ld * c + row) * CB : 0xFFFFFFF0u);
             }
         }
         if (tile + 1 < t_end) {
@@ -882,8 +890,8 @@ GemmArgs<R> args(const skb_vertical_plan* p, const GemmOperand& op, const void* 
     a.MB = op.split.mb;
     a.VR = op.split.vr;
     a.F = static_cast<int>(p->w.fields);
-    a.in_cs = 1;
-    a.in_co = 0;
+    a.in_ld = op.K;
+    a.out_ld = op.M;
     a.qmap = p->qmap.as<int>();
     a.partner = p->partner.as<int>();
     return a;
@@ -914,20 +922,21 @@ namespace skb {
 // Forward product over a column subset of a wider compact array: input
 // column c (c = field + fields*mode of the plan) is read at
 // in + K*(in_cs*c + in_co); the output is the plan's dense layout.
+// (GemmArgs carries the column stride in_ld = K*in_cs and an input pointer
+// advanced by K*in_co.)
 void vertical_forward_columns(const skb_vertical_plan* p, const void* in, int in_cs, int in_co, void* out,
                               void* stream) {
     if (!p || !in || !out) fail(SKB_INVALID_ARGUMENT, "null argument");
     if (p->wvm || in_cs < 1 || in_co < 0 || in_co >= in_cs) fail(SKB_INVALID_ARGUMENT, "bad column selection");
+    const size_t shift = static_cast<size_t>(p->fwd.K) * in_co;
     const hipStream_t s = as_stream(stream);
     if (p->precision == SKB_F64) {
-        auto a = args<double>(p, p->fwd, in, out);
-        a.in_cs = in_cs;
-        a.in_co = in_co;
+        auto a = args<double>(p, p->fwd, static_cast<const cx<double>*>(in) + shift, out);
+        a.in_ld = p->fwd.K * in_cs;
         launch_gemm<double, false>(a, p->grid, s);
     } else {
-        auto a = args<float>(p, p->fwd, in, out);
-        a.in_cs = in_cs;
-        a.in_co = in_co;
+        auto a = args<float>(p, p->fwd, static_cast<const cx<float>*>(in) + shift, out);
+        a.in_ld = p->fwd.K * in_cs;
         launch_gemm<float, false>(a, p->grid, s);
     }
 }
@@ -1012,6 +1021,117 @@ void fill_zero(const skb_vertical_plan* p, void* out, uint64_t elements, void* s
         hipLaunchKernelGGL(fill_zero_kernel<float>, grid, dim3(256), 0, as_stream(stream),
                            static_cast<cx<float>*>(out), static_cast<size_t>(elements));
     check_hip(hipGetLastError(), "fill launch");
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Type-I vertical transforms of the constant-stratification operator
+// (src/constant_stratification.cpp:95-147 plans, 196-235 normalisation,
+// 271-292 schedule).  Each transform is an Nz x Nz real matrix applied to
+// the complex z-columns of one channel of a [rows][channels][Nz] array, in
+// place, on the same MFMA GEMM as the K^2 family (one group, column stride
+// Nz*channels).  The FFTW kinds map to matrices:
+//   DCT          REDFT00 of length Nz:  Y_k = x_0 + (-1)^k x_{Nz-1}
+//                                           + 2 sum_{j=1}^{Nz-2} x_j cos(pi j k/(Nz-1))
+//   DST          RODFT00 of the Nz-2 interior points (endpoint rows identity,
+//                as an in-place plan on the interior leaves them untouched):
+//                Y_k = 2 sum_{j=1}^{Nz-2} x_j sin(pi j k/(Nz-1))
+//   DCT_FORWARD  DCT, then x 1/(Nz-1) and the last row x 0.5
+//                (normalizeForwardDct)
+//   DST_FORWARD  DST, then x 1/(Nz-1) and zero endpoint rows
+//                (normalizeForwardDst)
+// A dense matrix on the matrix cores rather than a length-2(Nz-1) FFT: at
+// Nz <= 260 the product is far below the matrix-core roofline and the stage
+// stays bound by HBM, with no bit-reversal or twiddle passes.
+// ---------------------------------------------------------------------------
+namespace {
+
+// cos / sin of pi*m/n with m reduced modulo 2n in integers first.
+double cos_pi(long long m, long long n) {
+    m %= 2 * n;
+    return std::cos(3.14159265358979323846264338327950288 * static_cast<double>(m) / static_cast<double>(n));
+}
+double sin_pi(long long m, long long n) {
+    m %= 2 * n;
+    return std::sin(3.14159265358979323846264338327950288 * static_cast<double>(m) / static_cast<double>(n));
+}
+
+void type1_matrix(int kind, uint64_t nz_, std::vector<double>& a) {
+    const long long nz = static_cast<long long>(nz_), n1 = nz - 1;
+    a.assign(static_cast<size_t>(nz * nz), 0.0);
+    const bool forward = kind == SKB_TYPE1_DCT_FORWARD || kind == SKB_TYPE1_DST_FORWARD;
+    const double scale = forward ? 1.0 / static_cast<double>(n1) : 1.0;
+    if (kind == SKB_TYPE1_DCT || kind == SKB_TYPE1_DCT_FORWARD) {
+        for (long long k = 0; k < nz; ++k) {
+            const double rs = scale * (forward && k == n1 ? 0.5 : 1.0);
+            for (long long j = 0; j < nz; ++j) {
+                double v;
+                if (j == 0) v = 1.0;
+                else if (j == n1) v = (k % 2 == 0) ? 1.0 : -1.0;
+                else v = 2.0 * cos_pi(j * k, n1);
+                a[static_cast<size_t>(k * nz + j)] = rs * v;
+            }
+        }
+    } else {
+        for (long long k = 1; k < n1; ++k)
+            for (long long j = 1; j < n1; ++j) a[static_cast<size_t>(k * nz + j)] = scale * 2.0 * sin_pi(j * k, n1);
+        if (!forward) {
+            a[0] = 1.0;
+            a[static_cast<size_t>(n1 * nz + n1)] = 1.0;
+        }
+    }
+}
+
+void check_kind(int kind) {
+    if (kind < SKB_TYPE1_DCT || kind > SKB_TYPE1_DST_FORWARD) fail(SKB_INVALID_ARGUMENT, "unknown type-I transform kind");
+}
+
+}  // namespace
+
+struct skb_type1_plan {
+    uint64_t rows = 0, nz = 0;
+    int precision = SKB_F64;
+    int ntiles = 0;
+    skb::GemmOperand op[4];  // indexed by kind
+    skb::DeviceBuffer tiles;
+};
+
+namespace {
+
+template <typename R>
+void type1_run(const skb_type1_plan* p, int kind, uint64_t channels, uint64_t channel, void* data, hipStream_t s) {
+    const GemmOperand& op = p->op[kind];
+    GemmArgs<R> a{};
+    cx<R>* col0 = static_cast<cx<R>*>(data) + p->nz * channel;
+    a.in = col0;
+    a.out = col0;
+    a.ap = op.ap.as<R>();
+    a.arow = op.arow.as<R>();
+    a.tiles = p->tiles.as<GemmTile>();
+    a.ntiles = p->ntiles;
+    a.M = op.M;
+    a.K = op.K;
+    a.MB = op.split.mb;
+    a.VR = op.split.vr;
+    a.F = 1;
+    a.in_ld = static_cast<int>(p->nz * channels);
+    a.out_ld = a.in_ld;
+    launch_gemm<R, false>(a, 0, s);
+}
+
+void type1_execute(const skb_type1_plan* p, int kind, uint64_t channels, uint64_t first, uint64_t count, void* data,
+                   void* stream) {
+    if (!p || !data) fail(SKB_INVALID_ARGUMENT, "null argument");
+    check_kind(kind);
+    if (channels == 0 || count == 0 || first + count > channels)
+        fail(SKB_INVALID_ARGUMENT, "Invalid constant-stratification R2R plan shape.");
+    if (p->rows * channels * p->nz > (1ull << 31)) fail(SKB_UNSUPPORTED, "type-I arrays above 2^31 elements");
+    const hipStream_t s = as_stream(stream);
+    for (uint64_t c = first; c < first + count; ++c) {
+        if (p->precision == SKB_F64) type1_run<double>(p, kind, channels, c, data, s);
+        else type1_run<float>(p, kind, channels, c, data, s);
+    }
 }
 
 }  // namespace
@@ -1172,5 +1292,74 @@ int skb_vertical_plan_set_grid(skb_vertical_plan* plan, int workgroups) {
 }
 
 uint64_t skb_vertical_plan_items(const skb_vertical_plan* plan) { return plan ? static_cast<uint64_t>(plan->ntiles) : 0; }
+
+int skb_type1_matrix(uint64_t nz, int kind, double* matrix) {
+    return guarded([&] {
+        if (!matrix || nz < 3) fail(SKB_INVALID_ARGUMENT, "type-I transforms need Nz >= 3 and an output matrix");
+        check_kind(kind);
+        std::vector<double> a;
+        type1_matrix(kind, nz, a);
+        std::copy(a.begin(), a.end(), matrix);
+    });
+}
+
+int skb_type1_plan_create(uint64_t rows, uint64_t nz, int precision, skb_type1_plan** plan) {
+    return guarded([&] {
+        if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan pointer");
+        *plan = nullptr;
+        check_precision(precision);
+        if (rows == 0 || nz < 3) fail(SKB_INVALID_ARGUMENT, "Invalid constant-stratification R2R plan shape.");
+        const bool fp64 = precision == SKB_F64;
+        const int ks = static_cast<int>((nz + 3) / 4);
+        if (nz > (fp64 ? 132u : 260u) || !gemm_ks_instantiated(ks, fp64) ||
+            (row_split(static_cast<int>(nz)).mb + 3) / 4 > (ks >= 22 ? 4 : 3))
+            fail(SKB_UNSUPPORTED, "type-I transform length " + std::to_string(nz) + " is not instantiated");
+        if (rows > (1ull << 26)) fail(SKB_UNSUPPORTED, "too many type-I rows");
+        auto p = std::make_unique<skb_type1_plan>();
+        p->rows = rows;
+        p->nz = nz;
+        p->precision = precision;
+        std::vector<double> a;
+        for (int kind = 0; kind < 4; ++kind) {
+            type1_matrix(kind, nz, a);
+            p->op[kind].build(a.data(), 1, static_cast<int>(nz), static_cast<int>(nz), precision);
+        }
+        std::vector<GemmTile> tiles;
+        for (uint64_t c = 0; c < rows; c += 16)
+            tiles.push_back(GemmTile{0, static_cast<int>(c), static_cast<int>(std::min<uint64_t>(16, rows - c)), 0});
+        p->ntiles = static_cast<int>(tiles.size());
+        p->tiles.upload(tiles.data(), tiles.size() * sizeof(GemmTile));
+        *plan = p.release();
+    });
+}
+
+int skb_type1_plan_destroy(skb_type1_plan* plan) {
+    return guarded([&] { delete plan; });
+}
+
+int skb_type1_execute(skb_type1_plan* plan, int kind, uint64_t channels, uint64_t first_channel, uint64_t count,
+                      void* data, void* stream) {
+    return guarded([&] { type1_execute(plan, kind, channels, first_channel, count, data, stream); });
+}
+
+int skb_type1_vertical_schedule(skb_type1_plan* plan, void* arena3, void* arena1, void* stream) {
+    return guarded([&] {
+        if (!plan || !arena3 || !arena1) fail(SKB_INVALID_ARGUMENT, "null argument");
+        // executeCompleteVerticalGraph (src/constant_stratification.cpp:271-292): one
+        // velocity reconstruction and two cosine- and two sine-family derivative
+        // reconstructions on the three-channel arena, then four projected flux
+        // targets on the one-channel arena.
+        for (int g = 0; g < 3; ++g) {
+            type1_execute(plan, SKB_TYPE1_DCT, 3, 0, 2, arena3, stream);
+            type1_execute(plan, SKB_TYPE1_DST, 3, 2, 1, arena3, stream);
+        }
+        for (int g = 0; g < 2; ++g) {
+            type1_execute(plan, SKB_TYPE1_DST, 3, 0, 2, arena3, stream);
+            type1_execute(plan, SKB_TYPE1_DCT, 3, 2, 1, arena3, stream);
+        }
+        for (int g = 0; g < 2; ++g) type1_execute(plan, SKB_TYPE1_DCT_FORWARD, 1, 0, 1, arena1, stream);
+        for (int g = 0; g < 2; ++g) type1_execute(plan, SKB_TYPE1_DST_FORWARD, 1, 0, 1, arena1, stream);
+    });
+}
 
 }  // extern "C"

@@ -172,6 +172,12 @@ def _load():
         "skb_spectral_pipeline_set_fused": (i32, [vp, i32]),
         "skb_spectral_pipeline_fused": (i32, [vp]),
         "skb_spectral_pipeline_device_bytes": (u64, [vp]),
+        "skb_type1_matrix": (i32, [u64, i32, vp]),
+        "skb_type1_plan_create": (i32, [u64, u64, i32, pp]),
+        "skb_type1_plan_destroy": (i32, [vp]),
+        "skb_type1_execute": (i32, [vp, i32, u64, u64, u64, vp, vp]),
+        "skb_type1_vertical_schedule": (i32, [vp, vp, vp, vp]),
+        "skb_type1_fill": (i32, [u64, u64, u64, vp, u64, vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -203,6 +209,8 @@ EXPORTED = [
     "skb_modal_work", "skb_modal_work_split", "skb_vertical_wvm_modal_work", "skb_spectral_pipeline_create",
     "skb_spectral_pipeline_destroy", "skb_spectral_pipeline_execute", "skb_spectral_pipeline_stages",
     "skb_spectral_pipeline_set_fused", "skb_spectral_pipeline_fused", "skb_spectral_pipeline_device_bytes",
+    "skb_type1_matrix", "skb_type1_plan_create", "skb_type1_plan_destroy", "skb_type1_execute",
+    "skb_type1_vertical_schedule", "skb_type1_fill",
 ]
 
 
@@ -816,6 +824,53 @@ class SpectralPipeline:
     def close(self):
         if getattr(self, "h", None) and lib is not None:
             lib.skb_spectral_pipeline_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+
+SKB_TYPE1_DCT, SKB_TYPE1_DST, SKB_TYPE1_DCT_FORWARD, SKB_TYPE1_DST_FORWARD = 0, 1, 2, 3
+
+
+def type1_matrix(nz: int, kind: int) -> np.ndarray:
+    """The type-I transform of `kind` as a row-major nz x nz matrix."""
+    m = np.zeros(nz * nz)
+    check(lib.skb_type1_matrix(nz, kind, _np(m)))
+    return m.reshape(nz, nz)
+
+
+def type1_fill(rows: int, channels: int, nz: int, modes: np.ndarray = None, nx_half: int = 0) -> np.ndarray:
+    """The constant-stratification benchmark's deterministic fill, complex
+    (rows, channels, nz)."""
+    out = np.zeros(rows * channels * nz * 2)
+    check(lib.skb_type1_fill(rows, channels, nz, None if modes is None else _np(modes), nx_half, _np(out)))
+    return out.view(np.complex128).reshape(rows, channels, nz)
+
+
+class Type1Plan:
+    """Constant-stratification type-I vertical transforms on [rows][channels][nz]
+    complex device arrays, in place."""
+
+    def __init__(self, rows: int, nz: int, precision: int = SKB_F64):
+        self.rows, self.nz, self.precision = rows, nz, precision
+        h = ctypes.c_void_p()
+        check(lib.skb_type1_plan_create(rows, nz, precision, ctypes.byref(h)))
+        self.h = h
+
+    def execute(self, kind: int, data, channels: int, first: int = 0, count: int = None, stream=None):
+        count = channels - first if count is None else count
+        check(lib.skb_type1_execute(self.h, kind, channels, first, count,
+                                    _dev(data, self.rows * channels * self.nz, complex_dtype(self.precision), "data"),
+                                    _stream(stream)))
+
+    def schedule(self, arena3, arena1, stream=None):
+        cd = complex_dtype(self.precision)
+        check(lib.skb_type1_vertical_schedule(self.h, _dev(arena3, self.rows * 3 * self.nz, cd, "arena3"),
+                                              _dev(arena1, self.rows * self.nz, cd, "arena1"), _stream(stream)))
+
+    def close(self):
+        if getattr(self, "h", None) and lib is not None:
+            lib.skb_type1_plan_destroy(self.h)
             self.h = None
 
     __del__ = close
