@@ -322,6 +322,35 @@ def bench_pipeline(args, S, torch, device):
     return out
 
 
+def bench_type1(args, S, torch, device):
+    """Constant-stratification vertical schedule (15 inverse + 4 forward
+    complex channel transforms of type I) at 256^2 x Nz65, on full-half rows
+    and on retained rows, FP32 and FP64."""
+    import numpy as np
+
+    w = S.Workload(256, 256, 65, 4)
+    modes = S.retained_modes(w)
+    out = {}
+    for name, rows, m in (("full_half", w.nx_half * w.ny, None), ("retained", len(modes), modes)):
+        a3 = S.type1_fill(rows, 3, w.nz, m, w.nx_half)
+        a1 = S.type1_fill(rows, 1, w.nz, m, w.nx_half)
+        res = {"rows": rows}
+        for prec, pname in ((S.SKB_F32, "fp32"), (S.SKB_F64, "fp64")):
+            cd = S.complex_dtype(prec)
+            d3 = torch.from_numpy(a3.reshape(-1)).to(device=device, dtype=cd)
+            d1 = torch.from_numpy(a1.reshape(-1)).to(device=device, dtype=cd)
+            plan = S.Type1Plan(rows, w.nz, prec)
+            plan.schedule(d3, d1)
+            ms = _timed_events(torch, lambda: plan.schedule(d3, d1), 20)
+            one = rows * w.nz * (8 if prec == S.SKB_F32 else 16)
+            res[pname] = {"ms_per_schedule": ms, "schedules_per_s": 1e3 / ms,
+                          "GBps_38_channel_passes": 38 * one / (ms * 1e-3) / 1e9}
+            del plan, d3, d1
+        out[name] = res
+    out["workload"] = "constant-stratification vertical schedule 256x256 Nz65: 19 type-I channel transforms on MFMA"
+    return out
+
+
 def bench_batched(args, S, torch, device):
     """The uneven batch: 40 x 128^2xNz33x3, 40 x 256^2xNz65x3, 16 x 512^2xNz129x4
     retained-operator instances in one hipGraph, launch shapes autotuned."""
@@ -578,7 +607,8 @@ def main():
     }
     if not args.no_extras:
         for key, fn in (("advection_flux", bench_flux), ("spectral_pipeline", bench_pipeline),
-                        ("batched", bench_batched), ("serving", bench_serving)):
+                        ("constant_stratification", bench_type1), ("batched", bench_batched),
+                        ("serving", bench_serving)):
             log(args, "extra: %s" % key)
             try:
                 line[key] = fn(args, S, torch, device)
