@@ -670,7 +670,12 @@ __global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void row2_c2r_kernel(const 
         if constexpr (LOOP) {
             if (next < items) c2r_load<R, NX>(vn, in, next, ny, planes, ps, kc);
         }
-        const int y = item / ntiles, p0 = (item % ntiles) * TP;
+        // y and p0 are wave-uniform, but the division leaves them in vector
+        // registers; without readfirstlane the buffer resource built from
+        // them below is taken as divergent and every store is wrapped in a
+        // waterfall loop.
+        const int y = __builtin_amdgcn_readfirstlane(item / ntiles);
+        const int p0 = __builtin_amdgcn_readfirstlane((item % ntiles) * TP);
         if (n2 < N2) {
             rdft<N1, true>(v);
 #pragma unroll

@@ -74,6 +74,16 @@ struct GemmTile {
     int pad;
 };
 
+// Record i (wave-uniform) of the tile table.  The table is read-only while a
+// kernel runs; reading it through the constant address space makes the fetch
+// a scalar load, counted by lgkmcnt instead of the vector-memory counter, so
+// waiting for a record does not also wait for the stores in flight.
+__device__ __forceinline__ GemmTile tile_record(const GemmTile* tiles, int i) {
+    using ConstTile = const __attribute__((address_space(4))) GemmTile;
+    const ConstTile* t = (ConstTile*)tiles + __builtin_amdgcn_readfirstlane(i);
+    return GemmTile{t->group, t->col_begin, t->ncols, t->pad};
+}
+
 // Row blocks per wavefront held at once (accumulator slots): 3 covers
 // M <= 196; instantiations with KS >= 22 (K > 84, i.e. large Nz) hold 4,
 // covering M <= 260.
@@ -124,7 +134,7 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmArgs<R> g) 
     const int step = one ? g.ntiles : static_cast<int>(gridDim.x);
     for (int tile = one ? xcd_item(blockIdx.x, g.ntiles) : static_cast<int>(blockIdx.x); tile < g.ntiles;
          tile += step) {
-        const GemmTile t = g.tiles[__builtin_amdgcn_readfirstlane(tile)];
+        const GemmTile t = tile_record(g.tiles, tile);
         // A fragments of this wavefront's first row block: issued before the
         // B block so both loads are in flight together.
         R a0[KS];
@@ -322,7 +332,7 @@ __global__ __launch_bounds__(256) void grouped_gemm_fixed_kernel(const GemmArgs<
         // multiply of the following tiles carries no wait for them.
         __builtin_amdgcn_s_waitcnt(0);
     };
-    GemmTile cur = g.tiles[__builtin_amdgcn_readfirstlane(t_begin)];
+    GemmTile cur = tile_record(g.tiles, t_begin);
     int group = cur.group;
     load_group(group);
     cx<R> bv[NL];
@@ -330,7 +340,7 @@ __global__ __launch_bounds__(256) void grouped_gemm_fixed_kernel(const GemmArgs<
     put_b(bv);
     GemmTile nxt = cur;
     if (t_begin + 1 < t_end) {
-        nxt = g.tiles[__builtin_amdgcn_readfirstlane(t_begin + 1)];
+        nxt = tile_record(g.tiles, t_begin + 1);
         load_b(bv, nxt);
     }
     __syncthreads();
@@ -400,7 +410,7 @@ __global__ __launch_bounds__(256) void grouped_gemm_fixed_kernel(const GemmArgs<
                 load_group(group);
             }
             if (tile + 2 < t_end) {
-                nxt = g.tiles[__builtin_amdgcn_readfirstlane(tile + 2)];
+                nxt = tile_record(g.tiles, tile + 2);
                 load_b(bv, nxt);
             }
             __syncthreads();  // B block (and coefficients) of tile + 1 visible; the stage is drained
@@ -445,7 +455,7 @@ __global__ __launch_bounds__(256) void modal_round_trip_kernel(const RoundTripAr
     const int step = one ? g.ntiles : static_cast<int>(gridDim.x);
     for (int tile = one ? xcd_item(blockIdx.x, g.ntiles) : static_cast<int>(blockIdx.x); tile < g.ntiles;
          tile += step) {
-        const GemmTile t = g.tiles[__builtin_amdgcn_readfirstlane(tile)];
+        const GemmTile t = tile_record(g.tiles, tile);
         const size_t grp = static_cast<size_t>(t.group);
         R a0[KS1];
         if (wave < g.mb1) {
