@@ -161,6 +161,10 @@ __device__ __forceinline__ cx<R> scal(cx<R> a, R s) {
 // A raw buffer resource (stride 0) over `bytes` bytes: a load at an offset at
 // or past `bytes` returns zero and such a store is dropped, so a kernel can
 // issue a fixed number of memory operations whatever its edge cases.
+// An offset past every resource this library builds (all are under 2 GiB);
+// adding an element size to it cannot wrap to a valid offset.
+constexpr uint32_t kBufferPast = 0x80000000u;
+
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, uint32_t bytes) {
     // raw buffer, stride 0: offsets at or past `bytes` read zero / are dropped
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), static_cast<short>(0), bytes, 0x00020000);
@@ -169,7 +173,10 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, 
 // The LLVM buffer intrinsics, bound directly: this compiler lowers
 // __builtin_amdgcn_raw_buffer_load_b64 to a 32-bit load.
 typedef float bf2 __attribute__((ext_vector_type(2)));
+typedef float bf4 __attribute__((ext_vector_type(4)));
 typedef double bd2 __attribute__((ext_vector_type(2)));
+__device__ bf4 raw_buffer_load_f4(__amdgpu_buffer_rsrc_t r, int voffset, int soffset, int aux) __asm(
+    "llvm.amdgcn.raw.ptr.buffer.load.v4f32");
 __device__ bf2 raw_buffer_load_f2(__amdgpu_buffer_rsrc_t r, int voffset, int soffset, int aux) __asm(
     "llvm.amdgcn.raw.ptr.buffer.load.v2f32");
 __device__ bd2 raw_buffer_load_d2(__amdgpu_buffer_rsrc_t r, int voffset, int soffset, int aux) __asm(

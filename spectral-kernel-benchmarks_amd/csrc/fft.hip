@@ -581,41 +581,46 @@ __global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void row2_r2c_kernel(const 
 // c2r pass-1 input: Z[k] = X_a[k] + i X_b[k] with the Hermitian extension;
 // Im of the kx = 0 and kx = Nx/2 bins is ignored and kx >= kc is zero (never
 // read).  Thread (pair s fastest, n2).
-// Core: the pair (pa, pa + 1) of row y (row_base = ps*kc*y); has_a / has_b
-// say which of the two planes exist.  Thread (s1 = t % S, n2 = t / S).
-// Branch-free: a lane with nothing to read loads the row's first element
-// (always in bounds) and selects zero, so no exec-mask branches surround the
-// 16 loads.  WIDE (FP32, even ps and pa): each pair is one 16-byte load; it
-// stays in bounds when has_b is false, because then ps > planes and plane
-// pa + 1 is padding.  Callers pick WIDE from wave-uniform values.
+// Core: the pair (pa, pa + 1) of row y (row_base = ps*kc*y, wave-uniform);
+// has_a / has_b say which of the two planes exist.  Thread (s1 = t % S,
+// n2 = t / S).  Loads go through a buffer resource over the row's ps*kc
+// elements: a lane with nothing to read passes an offset past the end and
+// gets zero, so the 16 loads carry neither branches nor selects.  WIDE
+// (FP32, even ps and pa): each pair is one 16-byte load; plane pa + 1 is then
+// padding (ps > planes) when has_b is false, and is selected away.  Callers
+// pick WIDE from wave-uniform values.
 template <typename R, int NX, bool WIDE>
 __device__ __forceinline__ void c2r_gather_t(cx<R>* v, const cx<R>* in, size_t row_base, int ps, int kc, int pa,
                                              bool has_a, bool has_b) {
     using G = Geo2<R, NX>;
     constexpr int N1 = G::N1, N2 = G::N2, S = G::PAIRS;
+    constexpr uint32_t CB = sizeof(cx<R>);
     const int n2 = static_cast<int>(threadIdx.x) / S;
     const bool act = n2 < N2 && has_a;
-    const cx<R>* row = in + row_base;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(row_base));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(row_base >> 32));
+    const __amdgpu_buffer_rsrc_t rs =
+        buffer_rsrc(in + ((static_cast<size_t>(hi) << 32) | lo), static_cast<uint32_t>(ps) * kc * CB);
     constexpr bool wide = WIDE && sizeof(R) == 4;
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1) {
         const int k = N2 * n1 + n2;
-        const bool upper = k > NX / 2;
+        // k = N2*n1 + n2 with n2 < N2: only n1 = N1/2 straddles Nx/2, and only
+        // n1 = 0 (k = 0) and n1 = N1/2 (k = Nx/2) can hold a self-conjugate bin
+        const bool upper = 2 * n1 > N1 || (2 * n1 == N1 && n2 > 0);
         const int kx = upper ? NX - k : k;
         const bool ok = act && kx < kc;
-        const unsigned off = ok ? static_cast<unsigned>(ps * kx + pa) : 0u;
+        const uint32_t off = ok ? static_cast<uint32_t>(ps * kx + pa) * CB : kBufferPast;
         cx<R> xa, xb;
         if constexpr (wide) {
-            const float4 q = *reinterpret_cast<const float4*>(row + off);
-            xa = ok ? cx<R>{q.x, q.y} : cx<R>{R(0), R(0)};
-            xb = (ok && has_b) ? cx<R>{q.z, q.w} : cx<R>{R(0), R(0)};
+            const bf4 q = raw_buffer_load_f4(rs, static_cast<int>(off), 0, 0);
+            xa = cx<R>{q.x, q.y};
+            xb = has_b ? cx<R>{q.z, q.w} : cx<R>{R(0), R(0)};
         } else {
-            const cx<R> a = row[off];
-            const cx<R> b = row[(ok && has_b) ? off + 1 : 0u];
-            xa = ok ? a : cx<R>{R(0), R(0)};
-            xb = (ok && has_b) ? b : cx<R>{R(0), R(0)};
+            xa = buffer_load_cx<R>(rs, off);
+            xb = buffer_load_cx<R>(rs, ok && has_b ? off + CB : kBufferPast);
         }
-        if (kx == 0 || kx == NX / 2) {
+        if ((n1 == 0 || 2 * n1 == N1) && (kx == 0 || kx == NX / 2)) {
             xa.y = R(0);
             xb.y = R(0);
         }

@@ -314,7 +314,11 @@ __global__ __launch_bounds__(256) void grouped_gemm_fixed_kernel(const GemmArgs<
         }
     };
     R a[SL][KS];
-    auto load_group = [&](int group) {
+    // The vector-ALU row's coefficients pass through registers on their way
+    // to LDS, so issuing them does not wait for the load.
+    constexpr int NR = (VR * KP + 255) / 256;
+    R rv[NR > 0 ? NR : 1];
+    auto issue_group = [&](int group) {
 #pragma unroll
         for (int sl = 0; sl < SL; ++sl) {
             const int mb = wave + 4 * sl;
@@ -326,23 +330,44 @@ __global__ __launch_bounds__(256) void grouped_gemm_fixed_kernel(const GemmArgs<
         }
         if constexpr (VR > 0) {
             const R* ap = g.arow + static_cast<size_t>(group) * VR * KP;
-            for (int e = tid; e < VR * KP; e += 256) sar[e] = ap[e];
+#pragma unroll
+            for (int n = 0; n < NR; ++n) {
+                const int e = tid + 256 * n;
+                rv[n] = ap[e < VR * KP ? e : 0];
+            }
         }
+    };
+    auto put_row = [&]() {
+        if constexpr (VR > 0) {
+#pragma unroll
+            for (int n = 0; n < NR; ++n) {
+                const int e = tid + 256 * n;
+                if (e < VR * KP) sar[e] = rv[n];
+            }
+        }
+    };
+    auto load_group = [&](int group) {
+        issue_group(group);
+        put_row();
         // Group changes are rare: complete the fragment loads here, so the
         // multiply of the following tiles carries no wait for them.
         __builtin_amdgcn_s_waitcnt(0);
     };
+    // Prologue: the group's fragments and the first two B blocks are all in
+    // flight before the first wait.  A workgroup with one tile loads its
+    // block twice rather than branch around the second load.
     GemmTile cur = tile_record(g.tiles, t_begin);
     int group = cur.group;
-    load_group(group);
+    issue_group(group);
     cx<R> bv[NL];
     load_b(bv, cur);
+    GemmTile nxt = tile_record(g.tiles, t_begin + 1 < t_end ? t_begin + 1 : t_begin);
+    cx<R> bn[NL];
+    load_b(bn, nxt);
     put_b(bv);
-    GemmTile nxt = cur;
-    if (t_begin + 1 < t_end) {
-        nxt = tile_record(g.tiles, t_begin + 1);
-        load_b(bv, nxt);
-    }
+    put_row();
+#pragma unroll
+    for (int n = 0; n < NL; ++n) bv[n] = bn[n];
     __syncthreads();
     for (int tile = t_begin; tile < t_end; ++tile) {
         A4 re[SL], im[SL];
@@ -399,7 +424,7 @@ __global__ __launch_bounds__(256) void grouped_gemm_fixed_kernel(const GemmArgs<
             for (int n = 0; n < NS; ++n) {
                 const int e = tid + 256 * n, c = e / M, row = e - c * M;
                 const cx<R> v = st[e < 16 * M ? e : 0];
-                buffer_store_cx<R>(v, r, c < 16 ? static_cast<uint32_t>(ld * c + row) * CB : 0xFFFFFFF0u);
+                buffer_store_cx<R>(v, r, c < 16 ? static_cast<uint32_t>(ld * c + row) * CB : kBufferPast);
             }
         }
         if (tile + 1 < t_end) {
