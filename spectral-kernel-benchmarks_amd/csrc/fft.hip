@@ -458,15 +458,23 @@ struct Split2 {
     static constexpr int N2 = N / N1;
 };
 
-// Planes (column kernels) or plane pairs (row kernels) per workgroup, and
-// the workgroup size: keep each 16-byte-or-wider global access run at >= 128 B.
+// Column kernels: planes per workgroup and the workgroup size; each global
+// access run stays at >= 128 B.
 template <typename R, int N>
 struct Geo2 {
     static constexpr int N1 = Split2<N>::N1, N2 = Split2<N>::N2;
     static constexpr int LANES = (sizeof(R) == 4 ? 256 : 128);
-    static constexpr int TP = LANES / N1;       // column kernel: planes per workgroup
-    static constexpr int PAIRS = LANES / N1;    // row kernels: plane pairs per workgroup
+    static constexpr int TP = LANES / N1;       // planes per workgroup
     static constexpr int THREADS = LANES;       // TP * N1 >= TP * N2
+};
+
+// Row kernels: plane pairs per workgroup and the workgroup size.
+constexpr int kRowLanesF32 = 256;
+template <typename R, int N>
+struct RowGeo {
+    static constexpr int N1 = Split2<N>::N1, N2 = Split2<N>::N2;
+    static constexpr int THREADS = sizeof(R) == 4 ? kRowLanesF32 : 128;
+    static constexpr int PAIRS = THREADS / N1;  // THREADS >= PAIRS * N2, as N2 <= N1
 };
 
 // First work item of a workgroup.  One workgroup per item (LOOP = false):
@@ -485,7 +493,7 @@ __device__ __forceinline__ int first_item(int items) {
 // the next item's inputs into registers while transforming the current one.
 template <typename R, int NX>
 __device__ __forceinline__ void r2c_load(cx<R>* v, const R* real, int item, int ny, int planes) {
-    using G = Geo2<R, NX>;
+    using G = RowGeo<R, NX>;
     constexpr int N1 = G::N1, N2 = G::N2, S = G::PAIRS, TP = 2 * S;
     const int t = threadIdx.x, n2 = t % N2, s1 = t / N2;
     const int ntiles = (planes + TP - 1) / TP;
@@ -509,11 +517,11 @@ __device__ __forceinline__ void r2c_load(cx<R>* v, const R* real, int item, int 
 // stride (planes for WVM order, padded to whole cache lines for the pruned
 // intermediate).
 template <typename R, int NX, bool LOOP>
-__global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void row2_r2c_kernel(const R* __restrict__ real,
+__global__ __launch_bounds__((RowGeo<R, NX>::THREADS)) void row2_r2c_kernel(const R* __restrict__ real,
                                                                          cx<R>* __restrict__ out,
                                                                          const cx<R>* __restrict__ tw, int ny,
                                                                          int planes, int ps, int kc, int items) {
-    using G = Geo2<R, NX>;
+    using G = RowGeo<R, NX>;
     constexpr int N1 = G::N1, N2 = G::N2, S = G::PAIRS, TP = 2 * S, NT = G::THREADS;
     // LDS strides in complex elements are odd, so the 16 (FP32) or 8 (FP64)
     // plane pairs touched by one instruction fall on distinct banks.
@@ -556,17 +564,36 @@ __global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void row2_r2c_kernel(const 
 #pragma unroll
         for (int k2 = 0; k2 < N2; ++k2) buf[s2 * ZS + k1 + N1 * k2] = w[k2];
         __syncthreads();
-        // Hermitian split of each pair and plane-fastest store of kx < kc
-        const size_t row_base = static_cast<size_t>(ps) * static_cast<size_t>(kc) * y;
-        for (int e = t; e < kc * TP; e += NT) {
-            const int pit = e % TP, kx = e / TP;
-            const int p = p0 + pit;
-            if (p >= planes) continue;
-            const cx<R> zk = buf[(pit >> 1) * ZS + kx];
-            const cx<R> zn = buf[(pit >> 1) * ZS + ((NX - kx) & (NX - 1))];
-            const cx<R> x = (pit & 1) == 0 ? cx<R>{R(0.5) * (zk.x + zn.x), R(0.5) * (zk.y - zn.y)}
+        // Hermitian split of each pair and plane-fastest store of kx < kc,
+        // four elements per thread at a time: the LDS reads of a group are
+        // issued together, and the stores go through a buffer resource over
+        // the row (an element past kc or past the last plane is sent past it
+        // and dropped), so the loop carries no divergent branches.
+        {
+            constexpr uint32_t CB = sizeof(cx<R>);
+            const int yu = __builtin_amdgcn_readfirstlane(y), p0u = __builtin_amdgcn_readfirstlane(p0);
+            const __amdgpu_buffer_rsrc_t rs =
+                buffer_rsrc(out + static_cast<size_t>(ps) * static_cast<size_t>(kc) * yu,
+                            static_cast<uint32_t>(ps) * kc * CB);
+            const int total = kc * TP;
+            for (int e0 = 0; e0 < total; e0 += 4 * NT) {
+                cx<R> xs[4];
+                uint32_t os[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int e = e0 + t + i * NT;
+                    const int pit = e % TP, kx = e / TP;
+                    const int kxl = kx < NX / 2 ? kx : NX / 2;  // in-range LDS index past the end
+                    const cx<R> zk = buf[(pit >> 1) * ZS + kxl];
+                    const cx<R> zn = buf[(pit >> 1) * ZS + ((NX - kxl) & (NX - 1))];
+                    xs[i] = (pit & 1) == 0 ? cx<R>{R(0.5) * (zk.x + zn.x), R(0.5) * (zk.y - zn.y)}
                                            : cx<R>{R(0.5) * (zk.y + zn.y), R(0.5) * (zn.x - zk.x)};
-            out[row_base + static_cast<size_t>(ps) * kx + p] = x;
+                    os[i] = e < total && p0u + pit < planes ? static_cast<uint32_t>(ps * kx + p0u + pit) * CB
+                                                            : kBufferPast;
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) buffer_store_cx<R>(xs[i], rs, os[i]);
+            }
         }
         if (!LOOP || next >= items) break;
         __syncthreads();  // buf is rewritten by the next item
@@ -592,7 +619,7 @@ __global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void row2_r2c_kernel(const 
 template <typename R, int NX, bool WIDE>
 __device__ __forceinline__ void c2r_gather_t(cx<R>* v, const cx<R>* in, size_t row_base, int ps, int kc, int pa,
                                              bool has_a, bool has_b) {
-    using G = Geo2<R, NX>;
+    using G = RowGeo<R, NX>;
     constexpr int N1 = G::N1, N2 = G::N2, S = G::PAIRS;
     constexpr uint32_t CB = sizeof(cx<R>);
     const int n2 = static_cast<int>(threadIdx.x) / S;
@@ -638,7 +665,7 @@ __device__ __forceinline__ void c2r_gather(cx<R>* v, const cx<R>* in, size_t row
 
 template <typename R, int NX>
 __device__ __forceinline__ void c2r_load(cx<R>* v, const cx<R>* in, int item, int ny, int planes, int ps, int kc) {
-    using G = Geo2<R, NX>;
+    using G = RowGeo<R, NX>;
     constexpr int S = G::PAIRS, TP = 2 * S;
     const int s1 = static_cast<int>(threadIdx.x) % S;
     const int ntiles = (planes + TP - 1) / TP;
@@ -649,11 +676,11 @@ __device__ __forceinline__ void c2r_load(cx<R>* v, const cx<R>* in, int item, in
 }
 
 template <typename R, int NX, bool LOOP>
-__global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void row2_c2r_kernel(const cx<R>* __restrict__ in,
+__global__ __launch_bounds__((RowGeo<R, NX>::THREADS)) void row2_c2r_kernel(const cx<R>* __restrict__ in,
                                                                          R* __restrict__ real,
                                                                          const cx<R>* __restrict__ tw, int ny,
                                                                          int planes, int ps, int kc, int items) {
-    using G = Geo2<R, NX>;
+    using G = RowGeo<R, NX>;
     constexpr int N1 = G::N1, N2 = G::N2, S = G::PAIRS, TP = 2 * S, NT = G::THREADS;
     constexpr int YS = N2 + 1;
     constexpr int SS = N1 * YS + 1;  // odd: conflict-free across pairs
@@ -735,7 +762,7 @@ __global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void row2_c2r_kernel(const 
 template <typename R, int NX>
 __device__ __forceinline__ void flux_c2r(cx<R>* out, const cx<R>* tin, cx<R>* buf, const cx<R>* twl, size_t row_base,
                                          int ps, int kc, int pbase, int z0, int nz) {
-    using G = Geo2<R, NX>;
+    using G = RowGeo<R, NX>;
     constexpr int N1 = G::N1, N2 = G::N2, S = G::PAIRS;
     constexpr int YS = N2 + 1, SS = N1 * YS + 1;
     const int t = threadIdx.x, s1 = t % S, n2 = t / S;
@@ -762,7 +789,7 @@ __device__ __forceinline__ void flux_c2r(cx<R>* out, const cx<R>* tin, cx<R>* bu
 template <typename R, int NX>
 __device__ __forceinline__ void flux_r2c_store(cx<R>* acc, cx<R>* tout, cx<R>* buf, const cx<R>* twl,
                                                size_t row_base, int ps, int kc, int pbase, int z0, int nz) {
-    using G = Geo2<R, NX>;
+    using G = RowGeo<R, NX>;
     constexpr int N1 = G::N1, N2 = G::N2, S = G::PAIRS, TP = 2 * S, NT = G::THREADS;
     constexpr int YS2 = N1 + 1, SS2 = N2 * YS2 + 1, ZS = NX + 1;
     const int t = threadIdx.x;
@@ -801,7 +828,7 @@ __device__ __forceinline__ void flux_r2c_store(cx<R>* acc, cx<R>* tout, cx<R>* b
 
 template <typename R, int NX>
 struct FluxGeo {
-    using G = Geo2<R, NX>;
+    using G = RowGeo<R, NX>;
     static constexpr int S = G::PAIRS;
     static constexpr int SS = G::N1 * (G::N2 + 1) + 1, SS2 = G::N2 * (G::N1 + 1) + 1, ZS = NX + 1;
     static constexpr int M1 = SS > SS2 ? SS : SS2;
@@ -811,12 +838,12 @@ struct FluxGeo {
 // The c2r rows of u, v, w stay in registers for all targets.  FP32 only: in
 // FP64 the register set does not fit, and the separate-kernel path is faster.
 template <typename R, int NX>
-__global__ __launch_bounds__((Geo2<R, NX>::THREADS)) void flux_rows_kernel(const cx<R>* __restrict__ tin,
+__global__ __launch_bounds__((RowGeo<R, NX>::THREADS)) void flux_rows_kernel(const cx<R>* __restrict__ tin,
                                                                           cx<R>* __restrict__ tout,
                                                                           const cx<R>* __restrict__ tw, int nz,
                                                                           int ps_in, int ps_out, int kc, int targets,
                                                                           R scale, int items) {
-    using G = Geo2<R, NX>;
+    using G = RowGeo<R, NX>;
     constexpr int N2 = G::N2, TP = 2 * G::PAIRS, NT = G::THREADS;
     __shared__ cx<R> buf[FluxGeo<R, NX>::BUF];
     __shared__ cx<R> twl[NX];
@@ -1209,7 +1236,7 @@ int launch_grid(int items, int requested) {
 template <typename R, int NX, bool LOOP>
 void launch_row2(bool inverse, const void* in, void* out, const void* tw, int ny, int planes, int ps, int kc,
                  int items, int grid, hipStream_t stream) {
-    using G = Geo2<R, NX>;
+    using G = RowGeo<R, NX>;
     if (inverse)
         hipLaunchKernelGGL((row2_c2r_kernel<R, NX, LOOP>), dim3(grid), dim3(G::THREADS), 0, stream,
                            static_cast<const cx<R>*>(in), static_cast<R*>(out), static_cast<const cx<R>*>(tw), ny,
@@ -1226,7 +1253,7 @@ template <typename R, int NX>
 void launch_row(bool inverse, const void* in, void* out, const void* tw, int ny, int planes, int ps, int kc, int req,
                 hipStream_t stream) {
     if constexpr (NX >= 64) {
-        using G = Geo2<R, NX>;
+        using G = RowGeo<R, NX>;
         constexpr int TP = 2 * G::PAIRS;
         const int items = ny * ((planes + TP - 1) / TP);
         const int grid = launch_grid(items, req);
@@ -1932,7 +1959,7 @@ namespace {
 template <typename R, int NX>
 void launch_flux_rows(skb_pruned_plan* pin, skb_pruned_plan* pout, int targets, double scale, hipStream_t s) {
     if constexpr (NX >= 64) {
-        using G = Geo2<R, NX>;
+        using G = RowGeo<R, NX>;
         constexpr int TP = 2 * G::PAIRS;
         const int nz = static_cast<int>(pin->w.nz);
         const int items = static_cast<int>(pin->w.ny) * ((nz + TP - 1) / TP);
