@@ -905,12 +905,29 @@ __device__ __forceinline__ void col2_load(cx<R>* v, const cx<R>* in, int item, i
     // bounds) and selects zero.
     if constexpr (IN == kStrided) {
         const bool act = n2 < N2 && p < planes;
+        constexpr uint32_t CB = sizeof(cx<R>);
+        const uint64_t extent = static_cast<uint64_t>(ps) * static_cast<uint64_t>(in_kc) * NY * CB;
+        if (extent < kBufferPast) {
+            // The strided operand fits one buffer resource: 32-bit offsets,
+            // and a lane with nothing to read gets zero from an offset past
+            // the end (no branch, no select).
+            const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(in, static_cast<uint32_t>(extent));
 #pragma unroll
-        for (int n1 = 0; n1 < N1; ++n1) {
-            const int yy = N2 * n1 + n2;
-            const size_t off = act ? static_cast<size_t>(ps) * (kx + static_cast<size_t>(in_kc) * yy) + p : 0;
-            const cx<R> x = in[off];
-            v[n1] = act ? x : cx<R>{R(0), R(0)};
+            for (int n1 = 0; n1 < N1; ++n1) {
+                const int yy = N2 * n1 + n2;
+                const uint32_t off =
+                    act ? static_cast<uint32_t>(ps * (kx + in_kc * yy) + p) * CB : kBufferPast;
+                v[n1] = buffer_load_cx<R>(rs, off);
+            }
+        } else {
+#pragma unroll
+            for (int n1 = 0; n1 < N1; ++n1) {
+                const int yy = N2 * n1 + n2;
+                const size_t off =
+                    act ? static_cast<size_t>(ps) * (kx + static_cast<size_t>(in_kc) * yy) + p : 0;
+                const cx<R> x = in[off];
+                v[n1] = act ? x : cx<R>{R(0), R(0)};
+            }
         }
     } else {
         // all codes first, then all values: two batched round trips
