@@ -124,31 +124,67 @@ class Operator:
         self.real = torch.empty(w.real_elements, dtype=rd, device=device)
         self.back = torch.empty(len(modes) * w.planes, dtype=cd, device=device)
         self.modal = torch.empty(len(modes) * w.fields * nj, dtype=cd, device=device)
+        # per-call form (stream taken at each call, so it can be captured)
+        self.calls = [
+            lambda: self.pruned.inverse_columns(self.x),
+            lambda: self.pruned.inverse_rows(self.real),
+            lambda: self.pruned.forward_rows(self.real),
+            lambda: self.pruned.forward_columns(self.back),
+            lambda: self.vert.forward(self.back, self.modal),
+        ]
+        # prepared launches: arguments checked once, one C-ABI call per stage
         self.stages = [
-            ("inverse_columns", lambda: self.pruned.inverse_columns(self.x)),
-            ("inverse_rows", lambda: self.pruned.inverse_rows(self.real)),
-            ("forward_rows", lambda: self.pruned.forward_rows(self.real)),
-            ("forward_columns", lambda: self.pruned.forward_columns(self.back)),
-            ("vertical_gemm", lambda: self.vert.forward(self.back, self.modal)),
+            ("inverse_columns", self.pruned.bind_stage("inverse_columns", self.x)),
+            ("inverse_rows", self.pruned.bind_stage("inverse_rows", self.real)),
+            ("forward_rows", self.pruned.bind_stage("forward_rows", self.real)),
+            ("forward_columns", self.pruned.bind_stage("forward_columns", self.back)),
+            ("vertical_gemm", self.vert.bind_forward(self.back, self.modal)),
         ]
 
     def step(self):
         for _, fn in self.stages:
             fn()
 
+    def capture(self):
+        """Records one evaluation (the five kernel launches) into a hipGraph
+        (one launch call per evaluation).  The graph is checked against the
+        eager launches before it is timed: three replays, each after eager
+        work, must reproduce the eager output bit for bit."""
+        torch = self.torch
+        torch.cuda.synchronize()
+        self.step()
+        torch.cuda.synchronize()
+        ref = self.modal.clone()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            for fn in self.calls:
+                fn()
+        for _ in range(3):
+            self.modal.zero_()
+            self.step()
+            self.modal.zero_()
+            torch.cuda.synchronize()
+            self.graph.replay()
+            torch.cuda.synchronize()
+            if not torch.equal(self.modal, ref):
+                raise RuntimeError("graph replay differs from the eager evaluation")
 
-def timed(torch, op, steps):
+    def replay(self):
+        self.graph.replay()
+
+
+def timed(torch, fn, steps):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        op.step()
+        fn()
     torch.cuda.synchronize()
     return time.perf_counter() - t0
 
 
 def instrumented(torch, op, steps, reps=50):
     """Device times from HIP events recorded on the stream the kernels run on
-    (torch's current stream, which the plans are given).
+    (torch's current stream, which the prepared launches are bound to).
 
     per_eval: one event pair around each evaluation (latency percentiles).
     per_stage: for each kernel, `reps` back-to-back launches between one event
@@ -156,7 +192,6 @@ def instrumented(torch, op, steps, reps=50):
     per-kernel pairs inside an evaluation would add.  Every stage is
     idempotent (it rewrites its output from its input), so repeating it alone
     does the same work as inside the evaluation."""
-    names = [n for n, _ in op.stages]
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     torch.cuda.synchronize()
     for s in range(steps):
@@ -457,6 +492,13 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         return 2
     device = torch.device("cuda:0")
+    # Everything runs on one created stream (default priority): launches on
+    # the legacy null stream carry extra synchronisation per kernel.
+    with torch.cuda.stream(torch.cuda.Stream(device=device)):
+        return run(args, np, torch, S, device)
+
+
+def run(args, np, torch, S, device):
     w = S.Workload(256, 256, 65, 3)
     log(args, "setup %s family=%s" % (w, args.family))
     modes = S.retained_modes(w)
@@ -502,15 +544,20 @@ def main():
     for _ in range(args.warmup):
         f32.step()
     log(args, "timed %d steps (fp32)" % args.steps)
-    wall = timed(torch, f32, args.steps)
+    wall = timed(torch, f32.step, args.steps)
     value = args.steps / wall
     log(args, "fp32 %.1f evals/s (%.2f us/eval)" % (value, 1e6 * wall / args.steps))
     per_stage, per_eval = instrumented(torch, f32, args.steps)
+    log(args, "graph replay (one hipGraph per evaluation, checked against eager launches)")
+    f32.capture()
+    graph_steps = max(10, args.steps // 4)
+    wall_graph = timed(torch, f32.replay, graph_steps)
+    log(args, "fp32 graph replay %.1f evals/s" % (graph_steps / wall_graph))
     log(args, "fp64 form")
     for _ in range(max(2, args.warmup // 4)):
         f64.step()
     f64_steps = max(10, args.steps // 4)
-    wall64 = timed(torch, f64, f64_steps)
+    wall64 = timed(torch, f64.step, f64_steps)
     _, per_eval64 = instrumented(torch, f64, f64_steps)
 
     # algorithmic (stage-boundary) bytes attributed to each kernel, FP32 storage
@@ -589,6 +636,7 @@ def main():
                         "K^2-grouped vertical GEMM",
             "nx": 256, "ny": 256, "nz": 65, "fields": 3, "retained_modes": nkl, "vertical_family": args.family,
             "vertical_groups": len(groups), "nj": nj,
+            "launch": "five prepared C-ABI launches per evaluation on one stream",
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -599,6 +647,8 @@ def main():
         "error_vs_fp64_gpu": {"max_scale_normalized": err_max, "relative_l2": err_l2, "tolerance": 1e-6,
                               "passed": err_max <= 1e-6 and err_l2 <= 1e-6},
         "error_vs_fp64_oracle": err_oracle,
+        "graph_replay": {"evals_per_s": graph_steps / wall_graph, "ms_per_step": 1e3 * wall_graph / graph_steps,
+                         "note": "the same five kernels recorded into one hipGraph, one replay per evaluation"},
         "fp64": {"value": f64_steps / wall64, "unit": "evals/s", "ms_per_step": 1e3 * wall64 / f64_steps,
                  "p50_ms": percentile(per_eval64, 0.5), "p99_ms": percentile(per_eval64, 0.99),
                  "bytes_per_eval": 2 * eval_bytes},

@@ -401,6 +401,23 @@ def _stream(stream):
     return ctypes.c_void_p(s.cuda_stream)
 
 
+class Bound:
+    """One C-ABI call with its arguments validated and converted once: calling
+    the object launches it again on the same buffers and stream, with no
+    per-call checks on the host (the serving loop's form of a stage).  The
+    tensors are kept alive with the object."""
+
+    __slots__ = ("fn", "args", "keep")
+
+    def __init__(self, fn, args, keep):
+        self.fn, self.args, self.keep = fn, args, keep
+
+    def __call__(self):
+        status = self.fn(*self.args)
+        if status:
+            check(status)
+
+
 class FftPlan:
     """Full batched 2-D r2c / c2r in WVM order (FFTWProvider)."""
 
@@ -541,6 +558,18 @@ class PrunedPlan:
 
     STAGES = ("forward_rows", "forward_columns", "inverse_columns", "inverse_rows")
 
+    def bind_stage(self, name: str, tensor, stream=None) -> Bound:
+        """A prepared launch of one stage (see Bound) on `tensor`: the real
+        planes for the row stages, the compact retained array for the column
+        stages."""
+        if name not in self.STAGES:
+            raise InvalidArgument(1, "unknown stage %r" % name)
+        if name.endswith("rows"):
+            ptr = _dev(tensor, self.w.real_elements, real_dtype(self.precision), "real")
+        else:
+            ptr = _dev(tensor, self.retained_elements, complex_dtype(self.precision), "retained")
+        return Bound(getattr(lib, "skb_pruned_" + name), (self.h, ptr, _stream(stream)), (tensor, self))
+
     def set_grid(self, stage: int, workgroups: int):
         """Launch shape of one stage (0 = one workgroup per work item)."""
         check(lib.skb_pruned_plan_set_grid(self.h, stage, workgroups))
@@ -583,6 +612,13 @@ class VerticalPlan:
         cd = complex_dtype(self.precision)
         check(lib.skb_vertical_forward(self.h, _dev(physical, self.modes * self.w.planes, cd, "physical"),
                                        _dev(modal, self.modes * self.w.fields * self.nj, cd, "modal"), _stream(stream)))
+
+    def bind_forward(self, physical, modal, stream=None) -> Bound:
+        """A prepared forward launch (see Bound)."""
+        cd = complex_dtype(self.precision)
+        args = (self.h, _dev(physical, self.modes * self.w.planes, cd, "physical"),
+                _dev(modal, self.modes * self.w.fields * self.nj, cd, "modal"), _stream(stream))
+        return Bound(lib.skb_vertical_forward, args, (physical, modal, self))
 
     def inverse(self, modal, physical, stream=None):
         cd = complex_dtype(self.precision)

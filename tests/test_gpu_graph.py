@@ -1,0 +1,71 @@
+"""One retained-operator evaluation (pruned inverse columns and rows, pruned
+forward rows and columns, grouped vertical GEMM), launched stage by stage
+through the C ABI from torch's stream, recorded into a torch CUDA graph (a
+hipGraph on ROCm) the way bench.py times it.  The graph is replayed several
+times with eager work on other inputs and synchronisations between replays;
+every replay must equal the eager evaluation bit for bit, and the eager
+evaluation must match the FP64 oracle (FP32 within 1e-6, FP64 within 1e-12,
+max-scale-normalised)."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import skb_hip as S
+
+pytestmark = pytest.mark.gpu
+
+
+def note(msg):
+    print("[case] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("shape", [(64, 64, 17, 2), (256, 256, 65, 3)])
+@pytest.mark.parametrize("precision", [S.SKB_F32, S.SKB_F64])
+def test_graph_replays_equal_eager(device, shape, precision):
+    import torch
+
+    note("graph replay %s p=%d" % (shape, precision))
+    w = S.Workload(*shape)
+    modes = S.retained_modes(w)
+    groups = S.k2_binned_groups(modes, 32)
+    fwd, inv = S.qr_gaussian_family(w.nz, len(groups), 129)
+    cd, rd = S.complex_dtype(precision), S.real_dtype(precision)
+    pr = S.PrunedPlan(w, modes, precision)
+    vp = S.VerticalPlan(w, w.nz, groups, fwd, inv, precision)
+    x64 = S.spectral_fixture(w, modes, 129, -1.0)
+    x = torch.from_numpy(x64).to(device=device, dtype=cd)
+    other = torch.from_numpy(S.spectral_fixture(w, modes, 7, -1.0)).to(device=device, dtype=cd)
+    real = torch.empty(w.real_elements, dtype=rd, device=device)
+    back = torch.empty(len(modes) * w.planes, dtype=cd, device=device)
+    modal = torch.empty(len(modes) * w.fields * w.nz, dtype=cd, device=device)
+
+    def evaluate(src):
+        pr.inverse_columns(src)
+        pr.inverse_rows(real)
+        pr.forward_rows(real)
+        pr.forward_columns(back)
+        vp.forward(back, modal)
+
+    evaluate(x)
+    torch.cuda.synchronize()
+    ref = modal.clone()
+    got = ref.cpu().numpy().astype(np.complex128)
+    ow = O.Workload(w.nx, w.ny, w.nz, w.fields)
+    want = O.retained_operator(ow, O.retained_modes(ow), x64, w.nz, groups["first_mode"].copy(),
+                               groups["mode_count"].copy(), fwd)
+    tol = 1e-6 if precision == S.SKB_F32 else 1e-12
+    assert O.max_relative_error(got, want) <= tol
+
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        evaluate(x)
+    for rep in range(4):
+        evaluate(other)  # eager work on another input between replays
+        modal.zero_()
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(modal, ref), "replay %d" % rep
