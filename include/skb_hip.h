@@ -222,6 +222,19 @@ uint64_t skb_vertical_matrix_bytes(const skb_vertical_plan* plan);
  * reference counterpart. */
 int skb_vertical_plan_set_grid(skb_vertical_plan* plan, int workgroups);
 uint64_t skb_vertical_plan_items(const skb_vertical_plan* plan);
+/* Products of an FP32 plan's compact GEMMs.  SKB_PRODUCTS_SPLIT_BF16 forms
+ * each product on the bf16 matrix cores as the six leading terms of
+ * (A0 + A1 + A2)(b0 + b1 + b2), every factor split into three bf16 terms,
+ * with FP32 accumulation (the dropped terms are below 2^-24 of the product);
+ * SKB_PRODUCTS_FP32 uses the f32 16x16x4 matrix cores.  Split products need
+ * K = 32*n + 1 and M = 16*m + r with r <= 4 (Nz = 33, 65, 129 square
+ * families); other shapes, FP64 plans and the WVM-direct GEMM always use
+ * FP32 (FP64) products.  The default is SKB_PRODUCTS_SPLIT_BF16.  No
+ * reference counterpart (the reference's GEMMs are FP64 BLAS). */
+enum skb_gemm_products { SKB_PRODUCTS_FP32 = 0, SKB_PRODUCTS_SPLIT_BF16 = 1 };
+int skb_vertical_plan_set_products(skb_vertical_plan* plan, int products);
+/* The products the plan's forward (direction 0) or inverse (1) GEMM uses. */
+int skb_vertical_plan_products(const skb_vertical_plan* plan, int direction);
 
 /* ---- WVM-direct vertical GEMM ------------------------------------------
  * Replaces WvmDirectVerticalGemmProvider (include/skbench/skbench.hpp:220-257;

@@ -655,6 +655,45 @@ __device__ __forceinline__ void c2r_gather_t(cx<R>* v, const cx<R>* in, size_t r
     }
 }
 
+// The same gather through flat global addresses (absent elements read row
+// element 0 and are selected to zero), for the fused flux row kernel: at one
+// wavefront per SIMD and a full register file there, this form measured
+// faster than the buffer form above (512^2 x Nz129 flux 1.52 against 1.63 ms).
+template <typename R, int NX, bool WIDE>
+__device__ __forceinline__ void c2r_gather_flat_t(cx<R>* v, const cx<R>* in, size_t row_base, int ps, int kc,
+                                                  int pa, bool has_a, bool has_b) {
+    using G = RowGeo<R, NX>;
+    constexpr int N1 = G::N1, N2 = G::N2, S = G::PAIRS;
+    const int n2 = static_cast<int>(threadIdx.x) / S;
+    const bool act = n2 < N2 && has_a;
+    const cx<R>* row = in + row_base;
+    constexpr bool wide = WIDE && sizeof(R) == 4;
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) {
+        const int k = N2 * n1 + n2;
+        const bool upper = k > NX / 2;
+        const int kx = upper ? NX - k : k;
+        const bool ok = act && kx < kc;
+        const unsigned off = ok ? static_cast<unsigned>(ps * kx + pa) : 0u;
+        cx<R> xa, xb;
+        if constexpr (wide) {
+            const float4 q = *reinterpret_cast<const float4*>(row + off);
+            xa = ok ? cx<R>{q.x, q.y} : cx<R>{R(0), R(0)};
+            xb = (ok && has_b) ? cx<R>{q.z, q.w} : cx<R>{R(0), R(0)};
+        } else {
+            const cx<R> a = row[off];
+            const cx<R> b = row[(ok && has_b) ? off + 1 : 0u];
+            xa = ok ? a : cx<R>{R(0), R(0)};
+            xb = (ok && has_b) ? b : cx<R>{R(0), R(0)};
+        }
+        if (kx == 0 || kx == NX / 2) {
+            xa.y = R(0);
+            xb.y = R(0);
+        }
+        v[n1] = upper ? cx<R>{xa.x + xb.y, xb.x - xa.y} : cx<R>{xa.x - xb.y, xa.y + xb.x};
+    }
+}
+
 // wide: FP32 with ps and the tile's first plane both even (wave-uniform).
 template <typename R, int NX>
 __device__ __forceinline__ void c2r_gather(cx<R>* v, const cx<R>* in, size_t row_base, int ps, int kc, int pa,
@@ -769,7 +808,10 @@ __device__ __forceinline__ void flux_c2r(cx<R>* out, const cx<R>* tin, cx<R>* bu
     const int k1 = t % N1, s2 = t / N1;
     const int za = z0 + 2 * s1;
     cx<R> v[N1];
-    c2r_gather<R, NX>(v, tin, row_base, ps, kc, pbase + 2 * s1, za < nz, za + 1 < nz, ((ps | pbase) & 1) == 0);
+    if (((ps | pbase) & 1) == 0)
+        c2r_gather_flat_t<R, NX, true>(v, tin, row_base, ps, kc, pbase + 2 * s1, za < nz, za + 1 < nz);
+    else
+        c2r_gather_flat_t<R, NX, false>(v, tin, row_base, ps, kc, pbase + 2 * s1, za < nz, za + 1 < nz);
     __syncthreads();  // buf and twl ready / free
     if (n2 < N2) {
         rdft<N1, true>(v);

@@ -34,6 +34,7 @@
 //            f32: i = 4 * (l / 16) + r      f64: i = l / 16 + 4 * r
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <memory>
 #include <string>
 #include <utility>
@@ -115,6 +116,12 @@ struct GemmArgs {
     int in_ld, out_ld;   // compact: column c at in + in_ld*c and out + out_ld*c (complex elements)
     const int* qmap;     // WVM direct: frequency q of each mode
     const int* partner;  // WVM direct: frequency of the k_x = 0 conjugate partner, or -1
+    // split-BF16 products (FP32 plans with K = 32*NK + 1): A = A0 + A1 + A2
+    // in bf16, [group][MB][NK][3][64 lanes][8], lane (i, q) holding
+    // A[16*mb + i][32*t + 8*q + e]; and the last column A[row][K - 1] in FP32,
+    // [group][16*MB].
+    const uint16_t* a3;
+    const float* atail;
 };
 
 template <typename R, int KS, bool WVM>
@@ -443,6 +450,232 @@ __global__ __launch_bounds__(256) void grouped_gemm_fixed_kernel(const GemmArgs<
     }
 }
 
+// Split-BF16 form of the fixed-height kernel (FP32 plans, K = 32*NK + 1):
+// each product A*b is formed as the six leading terms of
+// (A0 + A1 + A2)(b0 + b1 + b2) on the bf16 matrix cores
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation), smallest terms first; the
+// three dropped terms are below 2^-24 of the product.  A is split once on
+// the host (from the FP64 matrix); each B element is split once per tile, by
+// the thread that moves it into LDS, into six bf16 planes (hi, mid, lo of Re
+// and of Im).  The last k (K - 1 = 32*NK) and the vector-ALU rows use the
+// FP32 copy of the B block.  Tile schedule, prefetch and stores are those of
+// grouped_gemm_fixed_kernel.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ uint32_t bf16_pair(float lo, float hi) {
+    // round to nearest even (v_cvt_pk_bf16_f32)
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    const bf16x2 v{static_cast<__bf16>(lo), static_cast<__bf16>(hi)};
+    return __builtin_bit_cast(uint32_t, v);
+}
+
+template <int NK, int M>
+__global__ __launch_bounds__(256) void grouped_gemm_split_kernel(const GemmArgs<float> g) {
+    using A4 = floatx4;
+    constexpr int KS = (32 * NK + 1 + 3) / 4;  // FP32 copy: K rounded up to 4
+    constexpr int KP = 4 * KS;
+    constexpr int CS = gemm_cs<float, KS>();
+    constexpr int MB = rows_mb(M), VR = rows_vr(M), SL = (MB + 3) / 4;
+    constexpr int NL = (16 * KP + 255) / 256;
+    constexpr int NS = (16 * M + 255) / 256;
+    constexpr int KB = 32 * NK;      // k handled on the bf16 matrix cores
+    constexpr int BS = KB + 8;       // bf16 plane column stride (16-byte reads spread over banks)
+    constexpr uint32_t CB = sizeof(cx<float>);
+    static_assert(16 * MB <= M, "every MFMA row exists");
+    __shared__ __align__(16) cx<float> sb[16 * CS];       // FP32 B block
+    __shared__ __align__(16) uint16_t sh[6][16 * BS];     // bf16 planes: Re hi, mid, lo, Im hi, mid, lo
+    __shared__ __align__(16) cx<float> st[16 * M];        // output stage, column-major
+    __shared__ float sar[(VR > 0 ? VR : 1) * KP];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int j = lane & 15, q = lane >> 4;
+    const int nwg = static_cast<int>(gridDim.x), wg = static_cast<int>(blockIdx.x);
+    const int t_begin = static_cast<int>((static_cast<long long>(g.ntiles) * wg) / nwg);
+    const int t_end = static_cast<int>((static_cast<long long>(g.ntiles) * (wg + 1)) / nwg);
+    if (t_begin >= t_end) return;  // uniform across the workgroup
+    const int in_step = g.in_ld;
+    auto load_b = [&](cx<float>* bv, const GemmTile& t) {
+        const cx<float>* base = g.in + static_cast<size_t>(g.in_ld) * t.col_begin;
+        const uint32_t bytes = static_cast<uint32_t>(in_step * (t.ncols - 1) + g.K) * CB;
+        const __amdgpu_buffer_rsrc_t r = buffer_rsrc(base, bytes);
+#pragma unroll
+        for (int n = 0; n < NL; ++n) {
+            const int e = tid + 256 * n, c = e / KP, z = e - c * KP;
+            bv[n] = buffer_load_cx<float>(r, static_cast<uint32_t>(in_step * c + z) * CB);
+        }
+    };
+    auto put_b = [&](const cx<float>* bv) {
+#pragma unroll
+        for (int n = 0; n < NL; ++n) {
+            const int e = tid + 256 * n, c = e / KP, z = e - c * KP;
+            if (16 * KP % 256 == 0 || e < 16 * KP) {
+                const cx<float> x = z < g.K ? bv[n] : cx<float>{0.f, 0.f};
+                sb[c * CS + z] = x;
+                if (z < KB) {
+                    const uint32_t h = bf16_pair(x.x, x.y);
+                    const float hx = __uint_as_float(h << 16), hy = __uint_as_float(h & 0xFFFF0000u);
+                    const float r1x = x.x - hx, r1y = x.y - hy;
+                    const uint32_t m = bf16_pair(r1x, r1y);
+                    const float mx = __uint_as_float(m << 16), my = __uint_as_float(m & 0xFFFF0000u);
+                    const uint32_t l = bf16_pair(r1x - mx, r1y - my);
+                    const int at = c * BS + z;
+                    sh[0][at] = static_cast<uint16_t>(h);
+                    sh[1][at] = static_cast<uint16_t>(m);
+                    sh[2][at] = static_cast<uint16_t>(l);
+                    sh[3][at] = static_cast<uint16_t>(h >> 16);
+                    sh[4][at] = static_cast<uint16_t>(m >> 16);
+                    sh[5][at] = static_cast<uint16_t>(l >> 16);
+                }
+            }
+        }
+    };
+    bf16x8 a[SL][NK][3];
+    floatx4 at4[SL];
+    constexpr int NR = (VR * KP + 255) / 256;
+    float rv[NR > 0 ? NR : 1];
+    auto issue_group = [&](int group) {
+#pragma unroll
+        for (int sl = 0; sl < SL; ++sl) {
+            const int mb = wave + 4 * sl;
+            if (mb < MB) {
+#pragma unroll
+                for (int t = 0; t < NK; ++t)
+#pragma unroll
+                    for (int p = 0; p < 3; ++p)
+                        a[sl][t][p] = *reinterpret_cast<const bf16x8*>(
+                            g.a3 + ((((static_cast<size_t>(group) * MB + mb) * NK + t) * 3 + p) * 64 + lane) * 8);
+                at4[sl] = *reinterpret_cast<const floatx4*>(g.atail + static_cast<size_t>(group) * 16 * MB + 16 * mb +
+                                                            4 * q);
+            }
+        }
+        if constexpr (VR > 0) {
+            const float* ap = g.arow + static_cast<size_t>(group) * VR * KP;
+#pragma unroll
+            for (int n = 0; n < NR; ++n) {
+                const int e = tid + 256 * n;
+                rv[n] = ap[e < VR * KP ? e : 0];
+            }
+        }
+    };
+    auto put_row = [&]() {
+        if constexpr (VR > 0) {
+#pragma unroll
+            for (int n = 0; n < NR; ++n) {
+                const int e = tid + 256 * n;
+                if (e < VR * KP) sar[e] = rv[n];
+            }
+        }
+    };
+    GemmTile cur = tile_record(g.tiles, t_begin);
+    int group = cur.group;
+    issue_group(group);
+    cx<float> bv[NL];
+    load_b(bv, cur);
+    GemmTile nxt = tile_record(g.tiles, t_begin + 1 < t_end ? t_begin + 1 : t_begin);
+    cx<float> bn[NL];
+    load_b(bn, nxt);
+    put_b(bv);
+    put_row();
+#pragma unroll
+    for (int n = 0; n < NL; ++n) bv[n] = bn[n];
+    __syncthreads();
+    for (int tile = t_begin; tile < t_end; ++tile) {
+        A4 re[SL], im[SL];
+#pragma unroll
+        for (int sl = 0; sl < SL; ++sl) {
+            const int mb = wave + 4 * sl;
+            if (mb < MB) {
+                A4 r = {0, 0, 0, 0}, i = {0, 0, 0, 0};
+#pragma unroll
+                for (int t = 0; t < NK; ++t) {
+                    const int off = j * BS + 32 * t + 8 * q;
+                    const bf16x8 r0 = *reinterpret_cast<const bf16x8*>(&sh[0][off]);
+                    const bf16x8 r1 = *reinterpret_cast<const bf16x8*>(&sh[1][off]);
+                    const bf16x8 r2 = *reinterpret_cast<const bf16x8*>(&sh[2][off]);
+                    const bf16x8 i0 = *reinterpret_cast<const bf16x8*>(&sh[3][off]);
+                    const bf16x8 i1 = *reinterpret_cast<const bf16x8*>(&sh[4][off]);
+                    const bf16x8 i2 = *reinterpret_cast<const bf16x8*>(&sh[5][off]);
+                    const bf16x8* A = a[sl][t];
+                    r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[2], r0, r, 0, 0, 0);
+                    i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[2], i0, i, 0, 0, 0);
+                    r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], r1, r, 0, 0, 0);
+                    i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], i1, i, 0, 0, 0);
+                    r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], r2, r, 0, 0, 0);
+                    i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], i2, i, 0, 0, 0);
+                    r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], r0, r, 0, 0, 0);
+                    i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], i0, i, 0, 0, 0);
+                    r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], r1, r, 0, 0, 0);
+                    i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], i1, i, 0, 0, 0);
+                    r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], r0, r, 0, 0, 0);
+                    i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], i0, i, 0, 0, 0);
+                }
+                // k = K - 1 in FP32: rows 4q + e of the block, column j
+                const cx<float> bt = sb[j * CS + KB];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    r[e] += at4[sl][e] * bt.x;
+                    i[e] += at4[sl][e] * bt.y;
+                }
+                re[sl] = r;
+                im[sl] = i;
+            }
+        }
+        cx<float> vrow{0.f, 0.f};
+        if (VR > 0 && wave < VR) {
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const float av = sar[wave * KP + q * KS + s];
+                const cx<float> b = sb[j * CS + q * KS + s];
+                vrow.x += av * b.x;
+                vrow.y += av * b.y;
+            }
+            vrow.x += __shfl_xor(vrow.x, 16);
+            vrow.y += __shfl_xor(vrow.y, 16);
+            vrow.x += __shfl_xor(vrow.x, 32);
+            vrow.y += __shfl_xor(vrow.y, 32);
+        }
+#pragma unroll
+        for (int sl = 0; sl < SL; ++sl) {
+            const int mb = wave + 4 * sl;
+            if (mb < MB) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * mb + Acc<float>::row(lane, r);
+                    st[j * M + row] = cx<float>{re[sl][r], im[sl][r]};
+                }
+            }
+        }
+        if (VR > 0 && wave < VR && q == 0) st[j * M + 16 * MB + wave] = vrow;
+        __syncthreads();  // stage complete; B block and coefficients are free
+        {
+            const int ld = g.out_ld;
+            const __amdgpu_buffer_rsrc_t r =
+                buffer_rsrc(g.out + static_cast<size_t>(ld) * cur.col_begin,
+                            static_cast<uint32_t>(ld * (cur.ncols - 1) + M) * CB);
+#pragma unroll
+            for (int n = 0; n < NS; ++n) {
+                const int e = tid + 256 * n, c = e / M, row = e - c * M;
+                const cx<float> v = st[e < 16 * M ? e : 0];
+                buffer_store_cx<float>(v, r, c < 16 ? static_cast<uint32_t>(ld * c + row) * CB : kBufferPast);
+            }
+        }
+        if (tile + 1 < t_end) {
+            put_b(bv);  // the B block prefetched for tile + 1
+            cur = nxt;
+            if (cur.group != group) {
+                group = cur.group;
+                issue_group(group);
+                put_row();
+                __builtin_amdgcn_s_waitcnt(0);
+            }
+            if (tile + 2 < t_end) {
+                nxt = tile_record(g.tiles, tile + 2);
+                load_b(bv, nxt);
+            }
+            __syncthreads();  // B block (and coefficients) of tile + 1 visible; the stage is drained
+        }
+    }
+}
+
 // Modal round trip of a K^2 group's columns in one kernel:
 //     out[:, c] = I_g * (w[:, c] .* (F_g * in[:, c]))
 // with F_g (Nj x Nz) the forward and I_g (Nz x Nj) the inverse matrix of the
@@ -730,9 +963,24 @@ void launch_gemm_fixed(const GemmArgs<R>& a, int req, hipStream_t s) {
 // Output heights of the fixed-height kernel: the (K, M) pairs of the vertical
 // families at Nz = 33, 65, 129 (M = K = Nz for square families, Nj x Nz
 // forward and Nz x Nj inverse for the truncated ones); FP64 up to Nz = 65.
+template <int NK, int M>
+void launch_gemm_split(const GemmArgs<float>& a, int req, hipStream_t s) {
+    const int grid = std::min(a.ntiles, req > 0 ? req : kStreamGrid);
+    hipLaunchKernelGGL((grouped_gemm_split_kernel<NK, M>), dim3(grid), dim3(256), 0, s, a);
+    check_hip(hipGetLastError(), "vertical GEMM launch");
+}
+
 template <typename R>
 bool launch_gemm_fixed_any(const GemmArgs<R>& a, int req, hipStream_t s) {
     const int ks = (a.K + 3) / 4;
+    if constexpr (sizeof(R) == 4) {
+        if (a.a3) {
+            // split-BF16 products (the plan built the split operand)
+            if (a.K == 65 && a.M == 65) return launch_gemm_split<2, 65>(a, req, s), true;
+            if (a.K == 33 && a.M == 33) return launch_gemm_split<1, 33>(a, req, s), true;
+            if (a.K == 129 && a.M == 129) return launch_gemm_split<4, 129>(a, req, s), true;
+        }
+    }
 #define SKB_FX(KSV, MV)                                  \
     if (ks == KSV && a.M == MV) {                        \
         launch_gemm_fixed<R, KSV, MV>(a, req, s);        \
@@ -805,14 +1053,69 @@ bool launch_round_trip(const RoundTripArgs<R>& a, int req, hipStream_t s, bool d
 }  // namespace
 
 // Matrices of one direction on the device.
+// bf16 with round-to-nearest-even, and back.
+inline uint16_t bf16_bits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return static_cast<uint16_t>((u >> 16) | 0x40u);  // NaN stays NaN
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+inline double bf16_value(uint16_t h) {
+    const uint32_t u = static_cast<uint32_t>(h) << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// Split-BF16 operand: A = A0 + A1 + A2 (each bf16; the residuals are taken
+// in FP64, so A0 + A1 + A2 agrees with A to about 2^-27 relative) in the
+// v_mfma_f32_16x16x32_bf16 fragment order, and the last column A[:, K-1] in
+// FP32.  Only for K = 32*NK + 1 and M = 16*MB + r with r <= 4.
+bool split_bf16_shape(int M, int K) { return K % 32 == 1 && K > 1 && M >= 16 && M % 16 <= 4; }
+void split_order(const double* mats, uint64_t groups, int M, int K, std::vector<uint16_t>& a3,
+                 std::vector<float>& atail) {
+    const int MB = M / 16, NK = K / 32;
+    a3.assign(static_cast<size_t>(groups) * MB * NK * 3 * 64 * 8, 0);
+    atail.assign(static_cast<size_t>(groups) * 16 * MB, 0.f);
+    for (uint64_t g = 0; g < groups; ++g)
+        for (int mb = 0; mb < MB; ++mb) {
+            for (int t = 0; t < NK; ++t)
+                for (int l = 0; l < 64; ++l)
+                    for (int e = 0; e < 8; ++e) {
+                        const int i = 16 * mb + (l & 15), k = 32 * t + 8 * (l >> 4) + e;
+                        const double v = mats[(g * M + i) * static_cast<uint64_t>(K) + k];
+                        const uint16_t h = bf16_bits(static_cast<float>(v));
+                        const double r1 = v - bf16_value(h);
+                        const uint16_t m = bf16_bits(static_cast<float>(r1));
+                        const uint16_t lo = bf16_bits(static_cast<float>(r1 - bf16_value(m)));
+                        const size_t at = ((((g * MB + mb) * NK + t) * 3) * 64 + l) * 8 + e;
+                        a3[at] = h;
+                        a3[at + 64 * 8] = m;
+                        a3[at + 2 * 64 * 8] = lo;
+                    }
+            for (int r = 0; r < 16; ++r)
+                atail[g * 16 * MB + 16 * mb + r] =
+                    static_cast<float>(mats[(g * M + 16 * mb + r) * static_cast<uint64_t>(K) + K - 1]);
+        }
+}
+
 struct GemmOperand {
     DeviceBuffer ap, arow;
+    DeviceBuffer a3, atail;  // split-BF16 form (FP32 plans of a split shape)
     int M = 0, K = 0;
     RowSplit split{0, 0};
     void build(const double* mats, uint64_t groups, int M_, int K_, int precision) {
         M = M_;
         K = K_;
         split = row_split(M);
+        if (precision == SKB_F32 && split_bf16_shape(M, K)) {
+            std::vector<uint16_t> h;
+            std::vector<float> t;
+            split_order(mats, groups, M, K, h, t);
+            a3.upload(h.data(), h.size() * sizeof(uint16_t));
+            atail.upload(t.data(), t.size() * sizeof(float));
+        }
         if (precision == SKB_F64) {
             std::vector<double> a, r;
             operand_order<double>(mats, groups, M, K, a, r);
@@ -836,6 +1139,7 @@ struct skb_vertical_plan {
     uint64_t groups = 0;
     int ntiles = 0;
     int grid = 0;  // workgroups per launch (0 = one per tile)
+    int products = SKB_PRODUCTS_SPLIT_BF16;
     uint64_t matrix_bytes = 0;  // logical matrix bytes per direction
     skb::GemmOperand fwd, inv;
     skb::DeviceBuffer tiles;
@@ -929,6 +1233,9 @@ GemmArgs<R> args(const skb_vertical_plan* p, const GemmOperand& op, const void* 
     a.out_ld = op.M;
     a.qmap = p->qmap.as<int>();
     a.partner = p->partner.as<int>();
+    const bool split = p->products == SKB_PRODUCTS_SPLIT_BF16 && op.a3.ptr;
+    a.a3 = split ? op.a3.as<uint16_t>() : nullptr;
+    a.atail = split ? op.atail.as<float>() : nullptr;
     return a;
 }
 
@@ -1327,6 +1634,22 @@ int skb_vertical_plan_set_grid(skb_vertical_plan* plan, int workgroups) {
 }
 
 uint64_t skb_vertical_plan_items(const skb_vertical_plan* plan) { return plan ? static_cast<uint64_t>(plan->ntiles) : 0; }
+
+int skb_vertical_plan_set_products(skb_vertical_plan* plan, int products) {
+    return guarded([&] {
+        if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
+        if (products != SKB_PRODUCTS_FP32 && products != SKB_PRODUCTS_SPLIT_BF16)
+            fail(SKB_INVALID_ARGUMENT, "products must be SKB_PRODUCTS_FP32 or SKB_PRODUCTS_SPLIT_BF16");
+        plan->products = products;
+    });
+}
+
+int skb_vertical_plan_products(const skb_vertical_plan* plan, int direction) {
+    if (!plan || (direction != 0 && direction != 1)) return -1;
+    const GemmOperand& op = direction == 0 ? plan->fwd : plan->inv;
+    return plan->products == SKB_PRODUCTS_SPLIT_BF16 && op.a3.ptr && !plan->wvm ? SKB_PRODUCTS_SPLIT_BF16
+                                                                                 : SKB_PRODUCTS_FP32;
+}
 
 int skb_type1_matrix(uint64_t nz, int kind, double* matrix) {
     return guarded([&] {

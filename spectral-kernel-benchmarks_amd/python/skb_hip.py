@@ -129,6 +129,8 @@ def _load():
         "skb_pruned_plan_items": (u64, [vp, i32]),
         "skb_vertical_plan_set_grid": (i32, [vp, i32]),
         "skb_vertical_plan_items": (u64, [vp]),
+        "skb_vertical_plan_set_products": (i32, [vp, i32]),
+        "skb_vertical_plan_products": (i32, [vp, i32]),
         "skb_pruned_scratch_bytes": (u64, [vp]),
         "skb_vertical_plan_create": (i32, [W, u64, vp, u64, vp, vp, i32, pp]),
         "skb_vertical_plan_destroy": (i32, [vp]),
@@ -197,6 +199,7 @@ EXPORTED = [
     "skb_pruned_forward_rows", "skb_pruned_forward_columns", "skb_pruned_inverse_columns", "skb_pruned_inverse_rows",
     "skb_vertical_plan_create", "skb_vertical_plan_destroy", "skb_vertical_forward", "skb_vertical_inverse",
     "skb_vertical_matrix_bytes", "skb_pointwise_advection", "skb_pruned_plan_set_grid", "skb_pruned_plan_items",
+    "skb_vertical_plan_set_products", "skb_vertical_plan_products",
     "skb_vertical_plan_set_grid", "skb_vertical_plan_items", "skb_pruned_forward_split", "skb_pruned_inverse_split",
     "skb_pruned_forward_split_fields", "skb_pruned_inverse_split_fields", "skb_vertical_wvm_plan_create",
     "skb_vertical_wvm_forward", "skb_vertical_wvm_inverse", "skb_vertical_wvm_modal_elements",
@@ -281,6 +284,8 @@ def _np(a):
 # ---- host model ---------------------------------------------------------------
 SKB_PIPELINE_WVM_DIRECT = 0
 SKB_PIPELINE_COMPACT = 1
+SKB_PRODUCTS_FP32 = 0
+SKB_PRODUCTS_SPLIT_BF16 = 1
 
 
 def synthetic_modal_weights(w: "Workload", modes: np.ndarray) -> np.ndarray:
@@ -634,6 +639,14 @@ class VerticalPlan:
 
     def items(self) -> int:
         return int(lib.skb_vertical_plan_items(self.h))
+
+    def set_products(self, products: int):
+        """SKB_PRODUCTS_SPLIT_BF16 (default) or SKB_PRODUCTS_FP32 for FP32 plans."""
+        check(lib.skb_vertical_plan_set_products(self.h, products))
+
+    def products(self, direction: int = 0) -> int:
+        """Products the forward (0) or inverse (1) GEMM uses."""
+        return int(lib.skb_vertical_plan_products(self.h, direction))
 
     def close(self):
         if getattr(self, "h", None) and lib is not None:
