@@ -58,7 +58,7 @@ KERNEL_NAMES = {  # substring of the device kernel name -> stage
     "row2_c2r_kernel<float, 256, false>": "inverse_rows",
     "row2_r2c_kernel<float, 256, false>": "forward_rows",
     "col2_kernel<float, 256, false, 0, 1, false>": "forward_columns",
-    "grouped_gemm_fixed_kernel<float": "vertical_gemm",
+    "grouped_gemm_split_kernel<2, 65>": "vertical_gemm",
 }
 
 
@@ -553,28 +553,36 @@ def run(args, np, torch, S, device):
     graph_steps = max(10, args.steps // 4)
     wall_graph = timed(torch, f32.replay, graph_steps)
     log(args, "fp32 graph replay %.1f evals/s" % (graph_steps / wall_graph))
-    # Two evaluations in flight: a second operator (own plans and buffers) on
-    # a second stream, launches alternating between the two.  Kernels of one
-    # evaluation then fill the gaps the other leaves (the GEMM's multiply
-    # phases against the transforms' memory phases).
-    side = torch.cuda.Stream(device=device)
-    with torch.cuda.stream(side):
-        f32b = Operator(S, torch, w, modes, groups, nj, fwd, inv, x64, S.SKB_F32, device)
-    pair_steps = 2 * (args.steps // 2)
-    for _ in range(4):
-        f32.step()
-        f32b.step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(pair_steps // 2):
-        f32.step()
-        f32b.step()
-    torch.cuda.synchronize()
-    wall_pair = time.perf_counter() - t0
-    log(args, "fp32 two streams %.1f evals/s" % (pair_steps / wall_pair))
-    if not torch.equal(f32b.modal, f32.modal):
-        raise RuntimeError("the second stream's evaluation differs from the first")
-    del f32b
+    two = None
+    if not args.no_extras:
+        # Two evaluations in flight: a second operator (own plans and buffers)
+        # on a second stream, launches alternating between the two.  Kernels of
+        # one evaluation then fill the gaps the other leaves (the GEMM's
+        # multiply phases against the transforms' memory phases).  An extra:
+        # kernels that overlap stretch each other's durations, so the kernel
+        # trace is taken without it.
+        log(args, "extra: two streams")
+        side = torch.cuda.Stream(device=device)
+        with torch.cuda.stream(side):
+            f32b = Operator(S, torch, w, modes, groups, nj, fwd, inv, x64, S.SKB_F32, device)
+        pair_steps = 2 * (args.steps // 2)
+        for _ in range(4):
+            f32.step()
+            f32b.step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(pair_steps // 2):
+            f32.step()
+            f32b.step()
+        torch.cuda.synchronize()
+        wall_pair = time.perf_counter() - t0
+        log(args, "fp32 two streams %.1f evals/s" % (pair_steps / wall_pair))
+        if not torch.equal(f32b.modal, f32.modal):
+            raise RuntimeError("the second stream's evaluation differs from the first")
+        del f32b
+        two = {"evals_per_s": pair_steps / wall_pair, "ms_per_eval": 1e3 * wall_pair / pair_steps,
+               "note": "two evaluations in flight on two streams (own plans and buffers), same inputs; "
+                       "outputs checked equal"}
     log(args, "fp64 form")
     for _ in range(max(2, args.warmup // 4)):
         f64.step()
@@ -669,9 +677,7 @@ def run(args, np, torch, S, device):
         "error_vs_fp64_gpu": {"max_scale_normalized": err_max, "relative_l2": err_l2, "tolerance": 1e-6,
                               "passed": err_max <= 1e-6 and err_l2 <= 1e-6},
         "error_vs_fp64_oracle": err_oracle,
-        "two_streams": {"evals_per_s": pair_steps / wall_pair, "ms_per_eval": 1e3 * wall_pair / pair_steps,
-                        "note": "two evaluations in flight on two streams (own plans and buffers), same inputs; "
-                                "outputs checked equal"},
+        "two_streams": two,
         "graph_replay": {"evals_per_s": graph_steps / wall_graph, "ms_per_step": 1e3 * wall_graph / graph_steps,
                          "note": "the same five kernels recorded into one hipGraph, one replay per evaluation"},
         "fp64": {"value": f64_steps / wall64, "unit": "evals/s", "ms_per_step": 1e3 * wall64 / f64_steps,
