@@ -183,6 +183,8 @@ __device__ bd2 raw_buffer_load_d2(__amdgpu_buffer_rsrc_t r, int voffset, int sof
     "llvm.amdgcn.raw.ptr.buffer.load.v2f64");
 __device__ void raw_buffer_store_f2(bf2 v, __amdgpu_buffer_rsrc_t r, int voffset, int soffset, int aux) __asm(
     "llvm.amdgcn.raw.ptr.buffer.store.v2f32");
+__device__ void raw_buffer_store_f4(bf4 v, __amdgpu_buffer_rsrc_t r, int voffset, int soffset, int aux) __asm(
+    "llvm.amdgcn.raw.ptr.buffer.store.v4f32");
 __device__ void raw_buffer_store_d2(bd2 v, __amdgpu_buffer_rsrc_t r, int voffset, int soffset, int aux) __asm(
     "llvm.amdgcn.raw.ptr.buffer.store.v2f64");
 

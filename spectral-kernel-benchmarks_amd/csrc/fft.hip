@@ -527,9 +527,7 @@ __global__ __launch_bounds__((RowGeo<R, NX>::THREADS)) void row2_r2c_kernel(cons
     // plane pairs touched by one instruction fall on distinct banks.
     constexpr int YS = N2 + 1;             // row of the transpose buffer
     constexpr int SS = N1 * YS + 1;        // one pair's transpose buffer
-    constexpr int ZS = NX + 1;             // one pair's spectrum
-    constexpr int BUF = S * (SS > ZS ? SS : ZS);
-    __shared__ cx<R> buf[BUF];
+    __shared__ cx<R> buf[S * SS];
     __shared__ cx<R> twl[NX];
     const int t = threadIdx.x;
     int item = first_item<LOOP>(items);
@@ -559,40 +557,48 @@ __global__ __launch_bounds__((RowGeo<R, NX>::THREADS)) void row2_r2c_kernel(cons
         cx<R> w[N2];
 #pragma unroll
         for (int m = 0; m < N2; ++m) w[m] = buf[s2 * SS + k1 * YS + m];
-        __syncthreads();  // buf becomes Z[s][k]
         rdft<N2, false>(w);
-#pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) buf[s2 * ZS + k1 + N1 * k2] = w[k2];
-        __syncthreads();
-        // Hermitian split of each pair and plane-fastest store of kx < kc,
-        // four elements per thread at a time: the LDS reads of a group are
-        // issued together, and the stores go through a buffer resource over
-        // the row (an element past kc or past the last plane is sent past it
-        // and dropped), so the loop carries no divergent branches.
+        // Thread (k1, s2) now holds Z[k1 + N1*k2] of pair s2.  The Hermitian
+        // split X_a = (Z_k + conj Z_{N-k})/2, X_b = (Z_k - conj Z_{N-k})/(2i)
+        // takes Z_{N-k} from the lane of k1' = (N1 - k1) mod N1 in the same
+        // pair (its k2' = N2-1-k2; for k1 = 0 the thread's own N2-k2), so
+        // the split needs no second pass through LDS.  Each thread stores its
+        // kx = k1 + N1*k2 < kc as one 16-byte plane pair (FP32, even ps: the
+        // pair's second plane is padding when it does not exist) or two
+        // 8-byte elements, through a buffer resource over the row: an
+        // element past kc or past the last plane goes past the row and is
+        // dropped.
         {
             constexpr uint32_t CB = sizeof(cx<R>);
             const int yu = __builtin_amdgcn_readfirstlane(y), p0u = __builtin_amdgcn_readfirstlane(p0);
             const __amdgpu_buffer_rsrc_t rs =
                 buffer_rsrc(out + static_cast<size_t>(ps) * static_cast<size_t>(kc) * yu,
                             static_cast<uint32_t>(ps) * kc * CB);
-            const int total = kc * TP;
-            for (int e0 = 0; e0 < total; e0 += 4 * NT) {
-                cx<R> xs[4];
-                uint32_t os[4];
+            const int pa = p0u + 2 * s2;
+            const bool has_a = pa < planes, has_b = pa + 1 < planes;
+            const bool wide = sizeof(R) == 4 && (ps & 1) == 0;  // uniform
+            const int lane = t & 63;
+            const int src = (lane & ~(N1 - 1)) | ((N1 - k1) & (N1 - 1));
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int e = e0 + t + i * NT;
-                    const int pit = e % TP, kx = e / TP;
-                    const int kxl = kx < NX / 2 ? kx : NX / 2;  // in-range LDS index past the end
-                    const cx<R> zk = buf[(pit >> 1) * ZS + kxl];
-                    const cx<R> zn = buf[(pit >> 1) * ZS + ((NX - kxl) & (NX - 1))];
-                    xs[i] = (pit & 1) == 0 ? cx<R>{R(0.5) * (zk.x + zn.x), R(0.5) * (zk.y - zn.y)}
-                                           : cx<R>{R(0.5) * (zk.y + zn.y), R(0.5) * (zn.x - zk.x)};
-                    os[i] = e < total && p0u + pit < planes ? static_cast<uint32_t>(ps * kx + p0u + pit) * CB
-                                                            : kBufferPast;
+            for (int k2 = 0; k2 <= N2 / 2; ++k2) {
+                if (N1 * k2 >= kc) break;  // uniform
+                const int kx = k1 + N1 * k2;
+                const cx<R> zk = w[k2];
+                const cx<R> pz = w[N2 - 1 - k2];
+                cx<R> zn{__shfl(pz.x, src), __shfl(pz.y, src)};
+                if (k1 == 0) zn = w[(N2 - k2) & (N2 - 1)];
+                const cx<R> xa{R(0.5) * (zk.x + zn.x), R(0.5) * (zk.y - zn.y)};
+                const cx<R> xb{R(0.5) * (zk.y + zn.y), R(0.5) * (zn.x - zk.x)};
+                const bool ok = kx < kc && has_a;
+                const uint32_t off = ok ? static_cast<uint32_t>(ps * kx + pa) * CB : kBufferPast;
+                if constexpr (sizeof(R) == 4) {
+                    if (wide) {
+                        raw_buffer_store_f4(bf4{xa.x, xa.y, xb.x, xb.y}, rs, static_cast<int>(off), 0, 0);
+                        continue;
+                    }
                 }
-#pragma unroll
-                for (int i = 0; i < 4; ++i) buffer_store_cx<R>(xs[i], rs, os[i]);
+                buffer_store_cx<R>(xa, rs, off);
+                buffer_store_cx<R>(xb, rs, ok && has_b ? off + CB : kBufferPast);
             }
         }
         if (!LOOP || next >= items) break;
