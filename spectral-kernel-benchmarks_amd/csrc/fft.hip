@@ -516,6 +516,23 @@ __device__ __forceinline__ void r2c_load(cx<R>* v, const R* real, int item, int 
 // Spectrum side: out[ps * (kx + kc * y) + p]; ps >= planes is the plane
 // stride (planes for WVM order, padded to whole cache lines for the pruned
 // intermediate).
+// The value held by lane (16 - i) mod 16 of each row of 16 lanes: DPP
+// row_mirror (lane 15 - i) followed by row_ror:1 (lane i - 1), two vector
+// moves with no LDS traffic.
+__device__ __forceinline__ float row_conj_lane(float v) {
+    const int m = __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xF, 0xF, false);
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, m, 0x121, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double row_conj_lane(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = static_cast<int>(b), hi = static_cast<int>(b >> 32);
+    const int mlo = __builtin_amdgcn_update_dpp(0, lo, 0x140, 0xF, 0xF, false);
+    const int mhi = __builtin_amdgcn_update_dpp(0, hi, 0x140, 0xF, 0xF, false);
+    const unsigned rlo = static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, mlo, 0x121, 0xF, 0xF, false));
+    const long long rhi = __builtin_amdgcn_update_dpp(0, mhi, 0x121, 0xF, 0xF, false);
+    return __longlong_as_double((rhi << 32) | rlo);
+}
+
 template <typename R, int NX, bool LOOP>
 __global__ __launch_bounds__((RowGeo<R, NX>::THREADS)) void row2_r2c_kernel(const R* __restrict__ real,
                                                                          cx<R>* __restrict__ out,
@@ -585,7 +602,12 @@ __global__ __launch_bounds__((RowGeo<R, NX>::THREADS)) void row2_r2c_kernel(cons
                 const int kx = k1 + N1 * k2;
                 const cx<R> zk = w[k2];
                 const cx<R> pz = w[N2 - 1 - k2];
-                cx<R> zn{__shfl(pz.x, src), __shfl(pz.y, src)};
+                cx<R> zn;
+                if constexpr (N1 == 16) {
+                    zn = cx<R>{row_conj_lane(pz.x), row_conj_lane(pz.y)};
+                } else {
+                    zn = cx<R>{__shfl(pz.x, src), __shfl(pz.y, src)};
+                }
                 if (k1 == 0) zn = w[(N2 - k2) & (N2 - 1)];
                 const cx<R> xa{R(0.5) * (zk.x + zn.x), R(0.5) * (zk.y - zn.y)};
                 const cx<R> xb{R(0.5) * (zk.y + zn.y), R(0.5) * (zn.x - zk.x)};
