@@ -476,7 +476,6 @@ __global__ __launch_bounds__(256) void grouped_gemm_split_kernel(const GemmArgs<
     constexpr int KP = 4 * KS;
     constexpr int CS = gemm_cs<float, KS>();
     constexpr int MB = rows_mb(M), VR = rows_vr(M), SL = (MB + 3) / 4;
-    constexpr int NL = (16 * KP + 255) / 256;
     constexpr int NS = (16 * M + 255) / 256;
     constexpr int KB = 32 * NK;      // k handled on the bf16 matrix cores
     constexpr int BS = KB + 8;       // bf16 plane column stride (16-byte reads spread over banks)
@@ -493,40 +492,46 @@ __global__ __launch_bounds__(256) void grouped_gemm_split_kernel(const GemmArgs<
     const int t_end = static_cast<int>((static_cast<long long>(g.ntiles) * (wg + 1)) / nwg);
     if (t_begin >= t_end) return;  // uniform across the workgroup
     const int in_step = g.in_ld;
-    auto load_b = [&](cx<float>* bv, const GemmTile& t) {
+    // B staging: the first KB rows of each column as pairs of complex
+    // elements (one 16-byte load, split into bf16 pairs written as 32-bit
+    // words), NK pairs per thread; the last row k = KB as one 8-byte load by
+    // the first 16 threads.  Columns past the tile read zero.
+    struct BRegs {
+        bf4 pair[NK];
+        cx<float> tail;
+    };
+    auto load_b = [&](BRegs& bv, const GemmTile& t) {
         const cx<float>* base = g.in + static_cast<size_t>(g.in_ld) * t.col_begin;
         const uint32_t bytes = static_cast<uint32_t>(in_step * (t.ncols - 1) + g.K) * CB;
         const __amdgpu_buffer_rsrc_t r = buffer_rsrc(base, bytes);
 #pragma unroll
-        for (int n = 0; n < NL; ++n) {
-            const int e = tid + 256 * n, c = e / KP, z = e - c * KP;
-            bv[n] = buffer_load_cx<float>(r, static_cast<uint32_t>(in_step * c + z) * CB);
+        for (int n = 0; n < NK; ++n) {
+            const int e = tid + 256 * n, c = e / (KB / 2), z = 2 * (e - c * (KB / 2));
+            bv.pair[n] = raw_buffer_load_f4(r, static_cast<int>(static_cast<uint32_t>(in_step * c + z) * CB), 0, 0);
         }
+        bv.tail = buffer_load_cx<float>(r, tid < 16 ? static_cast<uint32_t>(in_step * tid + KB) * CB : kBufferPast);
     };
-    auto put_b = [&](const cx<float>* bv) {
+    auto put_b = [&](const BRegs& bv) {
 #pragma unroll
-        for (int n = 0; n < NL; ++n) {
-            const int e = tid + 256 * n, c = e / KP, z = e - c * KP;
-            if (16 * KP % 256 == 0 || e < 16 * KP) {
-                const cx<float> x = z < g.K ? bv[n] : cx<float>{0.f, 0.f};
-                sb[c * CS + z] = x;
-                if (z < KB) {
-                    const uint32_t h = bf16_pair(x.x, x.y);
-                    const float hx = __uint_as_float(h << 16), hy = __uint_as_float(h & 0xFFFF0000u);
-                    const float r1x = x.x - hx, r1y = x.y - hy;
-                    const uint32_t m = bf16_pair(r1x, r1y);
-                    const float mx = __uint_as_float(m << 16), my = __uint_as_float(m & 0xFFFF0000u);
-                    const uint32_t l = bf16_pair(r1x - mx, r1y - my);
-                    const int at = c * BS + z;
-                    sh[0][at] = static_cast<uint16_t>(h);
-                    sh[1][at] = static_cast<uint16_t>(m);
-                    sh[2][at] = static_cast<uint16_t>(l);
-                    sh[3][at] = static_cast<uint16_t>(h >> 16);
-                    sh[4][at] = static_cast<uint16_t>(m >> 16);
-                    sh[5][at] = static_cast<uint16_t>(l >> 16);
-                }
-            }
+        for (int n = 0; n < NK; ++n) {
+            const int e = tid + 256 * n, c = e / (KB / 2), z = 2 * (e - c * (KB / 2));
+            const bf4 x = bv.pair[n];  // Re z, Im z, Re z+1, Im z+1
+            *reinterpret_cast<bf4*>(&sb[c * CS + z]) = x;
+            const uint32_t hr = bf16_pair(x.x, x.z), hi = bf16_pair(x.y, x.w);
+            const float r1[4] = {x.x - __uint_as_float(hr << 16), x.y - __uint_as_float(hi << 16),
+                                 x.z - __uint_as_float(hr & 0xFFFF0000u), x.w - __uint_as_float(hi & 0xFFFF0000u)};
+            const uint32_t mr = bf16_pair(r1[0], r1[2]), mi = bf16_pair(r1[1], r1[3]);
+            const uint32_t lr = bf16_pair(r1[0] - __uint_as_float(mr << 16), r1[2] - __uint_as_float(mr & 0xFFFF0000u));
+            const uint32_t li = bf16_pair(r1[1] - __uint_as_float(mi << 16), r1[3] - __uint_as_float(mi & 0xFFFF0000u));
+            const int at = (c * BS + z) / 2;  // 32-bit words: z is even
+            reinterpret_cast<uint32_t*>(sh[0])[at] = hr;
+            reinterpret_cast<uint32_t*>(sh[1])[at] = mr;
+            reinterpret_cast<uint32_t*>(sh[2])[at] = lr;
+            reinterpret_cast<uint32_t*>(sh[3])[at] = hi;
+            reinterpret_cast<uint32_t*>(sh[4])[at] = mi;
+            reinterpret_cast<uint32_t*>(sh[5])[at] = li;
         }
+        if (tid < 16) sb[tid * CS + KB] = bv.tail;
     };
     bf16x8 a[SL][NK][3];
     floatx4 at4[SL];
@@ -568,15 +573,19 @@ __global__ __launch_bounds__(256) void grouped_gemm_split_kernel(const GemmArgs<
     GemmTile cur = tile_record(g.tiles, t_begin);
     int group = cur.group;
     issue_group(group);
-    cx<float> bv[NL];
+    // rows K .. KP-1 of the FP32 copy stay zero (the vector-ALU rows read them)
+    for (int e = tid; e < 16 * (KP - 32 * NK - 1); e += 256) {
+        const int c = e / (KP - 32 * NK - 1);
+        sb[c * CS + 32 * NK + 1 + (e - c * (KP - 32 * NK - 1))] = cx<float>{0.f, 0.f};
+    }
+    BRegs bv;
     load_b(bv, cur);
     GemmTile nxt = tile_record(g.tiles, t_begin + 1 < t_end ? t_begin + 1 : t_begin);
-    cx<float> bn[NL];
+    BRegs bn;
     load_b(bn, nxt);
     put_b(bv);
     put_row();
-#pragma unroll
-    for (int n = 0; n < NL; ++n) bv[n] = bn[n];
+    bv = bn;
     __syncthreads();
     for (int tile = t_begin; tile < t_end; ++tile) {
         A4 re[SL], im[SL];
