@@ -476,14 +476,17 @@ __global__ __launch_bounds__(256) void grouped_gemm_split_kernel(const GemmArgs<
     constexpr int KP = 4 * KS;
     constexpr int CS = gemm_cs<float, KS>();
     constexpr int MB = rows_mb(M), VR = rows_vr(M), SL = (MB + 3) / 4;
-    constexpr int NS = (16 * M + 255) / 256;
     constexpr int KB = 32 * NK;      // k handled on the bf16 matrix cores
     constexpr int BS = KB + 8;       // bf16 plane column stride (16-byte reads spread over banks)
     constexpr uint32_t CB = sizeof(cx<float>);
     static_assert(16 * MB <= M, "every MFMA row exists");
-    __shared__ __align__(16) cx<float> sb[16 * CS];       // FP32 B block
-    __shared__ __align__(16) uint16_t sh[6][16 * BS];     // bf16 planes: Re hi, mid, lo, Im hi, mid, lo
-    __shared__ __align__(16) cx<float> st[16 * M];        // output stage, column-major
+    // B blocks, double-buffered where LDS allows three workgroups per CU (the
+    // next tile's block is staged while the current one is multiplied, one
+    // barrier per tile); the largest height keeps one buffer and a barrier
+    // before restaging it.
+    constexpr int NB = NK >= 4 ? 1 : 2;
+    __shared__ __align__(16) cx<float> sbb[NB][16 * CS];      // FP32 B block
+    __shared__ __align__(16) uint16_t shb[NB][6][16 * BS];    // bf16 planes: Re hi, mid, lo, Im hi, mid, lo
     __shared__ float sar[(VR > 0 ? VR : 1) * KP];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int j = lane & 15, q = lane >> 4;
@@ -511,7 +514,9 @@ __global__ __launch_bounds__(256) void grouped_gemm_split_kernel(const GemmArgs<
         }
         bv.tail = buffer_load_cx<float>(r, tid < 16 ? static_cast<uint32_t>(in_step * tid + KB) * CB : kBufferPast);
     };
-    auto put_b = [&](const BRegs& bv) {
+    auto put_b = [&](const BRegs& bv, int buf) {
+        cx<float>* sb = sbb[buf];
+        uint16_t(*sh)[16 * BS] = shb[buf];
 #pragma unroll
         for (int n = 0; n < NK; ++n) {
             const int e = tid + 256 * n, c = e / (KB / 2), z = 2 * (e - c * (KB / 2));
@@ -573,21 +578,25 @@ __global__ __launch_bounds__(256) void grouped_gemm_split_kernel(const GemmArgs<
     GemmTile cur = tile_record(g.tiles, t_begin);
     int group = cur.group;
     issue_group(group);
-    // rows K .. KP-1 of the FP32 copy stay zero (the vector-ALU rows read them)
-    for (int e = tid; e < 16 * (KP - 32 * NK - 1); e += 256) {
-        const int c = e / (KP - 32 * NK - 1);
-        sb[c * CS + 32 * NK + 1 + (e - c * (KP - 32 * NK - 1))] = cx<float>{0.f, 0.f};
+    // rows K .. KP-1 of the FP32 copies stay zero (the vector-ALU rows read them)
+    for (int e = tid; e < NB * 16 * (KP - 32 * NK - 1); e += 256) {
+        const int b = e / (16 * (KP - 32 * NK - 1)), f = e - b * 16 * (KP - 32 * NK - 1);
+        const int c = f / (KP - 32 * NK - 1);
+        sbb[b][c * CS + 32 * NK + 1 + (f - c * (KP - 32 * NK - 1))] = cx<float>{0.f, 0.f};
     }
     BRegs bv;
     load_b(bv, cur);
     GemmTile nxt = tile_record(g.tiles, t_begin + 1 < t_end ? t_begin + 1 : t_begin);
     BRegs bn;
     load_b(bn, nxt);
-    put_b(bv);
+    put_b(bv, 0);
     put_row();
     bv = bn;
     __syncthreads();
+    int cb = 0;  // buffer holding the current tile's block
     for (int tile = t_begin; tile < t_end; ++tile) {
+        const cx<float>* sb = sbb[cb];
+        const uint16_t(*sh)[16 * BS] = shb[cb];
         A4 re[SL], im[SL];
 #pragma unroll
         for (int sl = 0; sl < SL; ++sl) {
@@ -642,37 +651,35 @@ __global__ __launch_bounds__(256) void grouped_gemm_split_kernel(const GemmArgs<
             vrow.x += __shfl_xor(vrow.x, 32);
             vrow.y += __shfl_xor(vrow.y, 32);
         }
-#pragma unroll
-        for (int sl = 0; sl < SL; ++sl) {
-            const int mb = wave + 4 * sl;
-            if (mb < MB) {
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int row = 16 * mb + Acc<float>::row(lane, r);
-                    st[j * M + row] = cx<float>{re[sl][r], im[sl][r]};
-                }
-            }
-        }
-        if (VR > 0 && wave < VR && q == 0) st[j * M + 16 * MB + wave] = vrow;
-        __syncthreads();  // stage complete; B block and coefficients are free
         {
+            // results straight from the accumulators: lane (j, q) holds rows
+            // 4q .. 4q+3 of column j in each block, two 16-byte row pairs
             const int ld = g.out_ld;
             const __amdgpu_buffer_rsrc_t r =
                 buffer_rsrc(g.out + static_cast<size_t>(ld) * cur.col_begin,
                             static_cast<uint32_t>(ld * (cur.ncols - 1) + M) * CB);
+            const uint32_t col = static_cast<uint32_t>(ld * j) * CB;
 #pragma unroll
-            for (int n = 0; n < NS; ++n) {
-                const int e = tid + 256 * n, c = e / M, row = e - c * M;
-                const cx<float> v = st[e < 16 * M ? e : 0];
-                buffer_store_cx<float>(v, r, c < 16 ? static_cast<uint32_t>(ld * c + row) * CB : kBufferPast);
+            for (int sl = 0; sl < SL; ++sl) {
+                const int mb = wave + 4 * sl;
+                if (mb < MB) {
+                    const uint32_t at = col + static_cast<uint32_t>(16 * mb + 4 * q) * CB;
+                    raw_buffer_store_f4(bf4{re[sl][0], im[sl][0], re[sl][1], im[sl][1]}, r, static_cast<int>(at), 0, 0);
+                    raw_buffer_store_f4(bf4{re[sl][2], im[sl][2], re[sl][3], im[sl][3]}, r,
+                                        static_cast<int>(at + 2 * CB), 0, 0);
+                }
             }
+            if (VR > 0 && wave < VR)
+                buffer_store_cx<float>(vrow, r, q == 0 ? col + static_cast<uint32_t>(16 * MB + wave) * CB : kBufferPast);
         }
         if (tile + 1 < t_end) {
-            put_b(bv);  // the B block prefetched for tile + 1
+            if constexpr (NB == 1) __syncthreads();  // every wavefront is done with the block
+            put_b(bv, NB == 1 ? 0 : cb ^ 1);  // the B block prefetched for tile + 1
             cur = nxt;
             if (cur.group != group) {
                 group = cur.group;
                 issue_group(group);
+                __syncthreads();  // every vector-ALU row of this tile has read the old coefficients
                 put_row();
                 __builtin_amdgcn_s_waitcnt(0);
             }
@@ -680,7 +687,8 @@ __global__ __launch_bounds__(256) void grouped_gemm_split_kernel(const GemmArgs<
                 nxt = tile_record(g.tiles, tile + 2);
                 load_b(bv, nxt);
             }
-            __syncthreads();  // B block (and coefficients) of tile + 1 visible; the stage is drained
+            __syncthreads();  // block of tile + 1 complete; this tile's block is no longer read
+            if constexpr (NB == 2) cb ^= 1;
         }
     }
 }
