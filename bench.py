@@ -54,7 +54,7 @@ def parse():
 
 COPY_BYTES = 256 << 20
 KERNEL_NAMES = {  # substring of the device kernel name -> stage
-    "col2_stream_kernel<float, 256, true, 1, 0>": "inverse_columns",
+    "col2_kernel<float, 256, true, 1, 0, false>": "inverse_columns",
     "row2_c2r_kernel<float, 256, false>": "inverse_rows",
     "row2_r2c_kernel<float, 256, false>": "forward_rows",
     "col2_kernel<float, 256, false, 0, 1, false>": "forward_columns",

@@ -964,7 +964,7 @@ __global__ __launch_bounds__((RowGeo<R, NX>::THREADS)) void flux_rows_kernel(con
 template <typename R, int NY, int IN>
 __device__ __forceinline__ void col2_load(cx<R>* v, const cx<R>* in, int item, int kx_count, int planes, int ps,
                                           int in_kc, const int* __restrict__ map, const SplitViews<R>& views,
-                                          const int* lcode = nullptr) {
+                                          uint32_t in_bytes, const int* lcode = nullptr) {
     using G = Geo2<R, NY>;
     constexpr int N1 = G::N1, N2 = G::N2, TP = G::TP;
     const int t = threadIdx.x, pp = t % TP, n2 = t / TP;
@@ -1009,22 +1009,42 @@ __device__ __forceinline__ void col2_load(cx<R>* v, const cx<R>* in, int item, i
         if constexpr (IN == kSplit) {
             if (p < planes) vl = view_lane(views, p);
         }
+        if constexpr (IN == kSplit) {
 #pragma unroll
-        for (int n1 = 0; n1 < N1; ++n1) {
-            cx<R> x{R(0), R(0)};
-            if constexpr (IN == kSplit) {
+            for (int n1 = 0; n1 < N1; ++n1) {
+                cx<R> x{R(0), R(0)};
                 if (code[n1] >= 0 && p < planes) {
                     const size_t at = vl.stride * static_cast<size_t>(code[n1] >> 1);
                     x = cx<R>{vl.re[at], vl.im[at]};
                     if (code[n1] & 1) x.y = -x.y;
                     x = apply_factor(views, vl.factor, kx, N2 * n1 + n2, x);
                 }
-            } else {
+                v[n1] = x;
+            }
+        } else if (in_bytes) {
+            // Compact input within 32-bit buffer offsets: an absent mode reads
+            // zero from an offset past the end (no branch, no select).  The
+            // uniform choice of form sits outside the loop, so all N1 loads
+            // are in flight before the first is used.
+            const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(in, in_bytes);
+#pragma unroll
+            for (int n1 = 0; n1 < N1; ++n1) {
+                const bool ok = code[n1] >= 0 && p < planes;
+                const uint32_t off =
+                    ok ? static_cast<uint32_t>((P * static_cast<size_t>(code[n1] >> 1) + p) * sizeof(cx<R>))
+                       : kBufferPast;
+                v[n1] = buffer_load_cx<R>(rs, off);
+            }
+#pragma unroll
+            for (int n1 = 0; n1 < N1; ++n1)
+                if (code[n1] & 1) v[n1].y = -v[n1].y;
+        } else {
+#pragma unroll
+            for (int n1 = 0; n1 < N1; ++n1) {
                 const bool ok = code[n1] >= 0 && p < planes;
                 const cx<R> y = in[ok ? P * static_cast<size_t>(code[n1] >> 1) + p : 0];
-                x = ok ? cx<R>{y.x, (code[n1] & 1) ? -y.y : y.y} : cx<R>{R(0), R(0)};
+                v[n1] = ok ? cx<R>{y.x, (code[n1] & 1) ? -y.y : y.y} : cx<R>{R(0), R(0)};
             }
-            v[n1] = x;
         }
     }
 }
@@ -1034,7 +1054,7 @@ __global__ __launch_bounds__((Geo2<R, NY>::THREADS)) void col2_kernel(const cx<R
                                                                      const cx<R>* __restrict__ tw, int planes, int ps,
                                                                      int kx_count, int items, int in_kc, int out_kc,
                                                                      const int* __restrict__ map,
-                                                                     const SplitViews<R> views) {
+                                                                     const SplitViews<R> views, uint32_t in_bytes) {
     using G = Geo2<R, NY>;
     constexpr int N1 = G::N1, N2 = G::N2, TP = G::TP, NT = G::THREADS;
     __shared__ cx<R> buf[NY * TP];
@@ -1055,9 +1075,9 @@ __global__ __launch_bounds__((Geo2<R, NY>::THREADS)) void col2_kernel(const cx<R
         for (int i = t; i < NY; i += NT) scode[i] = map[kx0 * NY + i];
         for (int i = t; i < NY; i += NT) twl[i] = tw[i];
         __syncthreads();
-        col2_load<R, NY, IN>(v, in, item, kx_count, planes, ps, in_kc, map, views, scode);
+        col2_load<R, NY, IN>(v, in, item, kx_count, planes, ps, in_kc, map, views, in_bytes, scode);
     } else {
-        col2_load<R, NY, IN>(v, in, item, kx_count, planes, ps, in_kc, map, views);
+        col2_load<R, NY, IN>(v, in, item, kx_count, planes, ps, in_kc, map, views, in_bytes);
         for (int i = t; i < NY; i += NT) twl[i] = tw[i];
         __syncthreads();
     }
@@ -1066,7 +1086,8 @@ __global__ __launch_bounds__((Geo2<R, NY>::THREADS)) void col2_kernel(const cx<R
         const int next = LOOP ? item + static_cast<int>(gridDim.x) : items;
         cx<R> vn[LOOP ? N1 : 1];
         if constexpr (LOOP) {
-            if (next < items) col2_load<R, NY, IN>(vn, in, next, kx_count, planes, ps, in_kc, map, views);
+            if (next < items)
+                col2_load<R, NY, IN>(vn, in, next, kx_count, planes, ps, in_kc, map, views, in_bytes);
         }
         const int kx = item / ntiles, p = (item % ntiles) * TP + pp;
         // Output codes are loaded with (not after) the item's inputs: on gfx9
@@ -1125,136 +1146,6 @@ __global__ __launch_bounds__((Geo2<R, NY>::THREADS)) void col2_kernel(const cx<R
             for (int m = 0; m < N1; ++m) v[m] = vn[m];
         }
         item = next;
-    }
-}
-
-// Streaming column pass between T and the compact retained array (the two
-// column stages of the pruned plans: IN = kRetained / OUT = kStrided for the
-// inverse, IN = kStrided / OUT = kRetained for the forward).  A workgroup
-// owns one k_x and a run of plane tiles: the column's map codes and the
-// twiddles are staged in LDS once, and the next tile's inputs are loaded
-// while the current tile is transformed.  Compact-side accesses go through a
-// bounds-checked buffer resource (a missing mode is read as zero or not
-// written), so every thread issues the same number of loads and stores per
-// tile and the wait for the prefetched tile does not include the current
-// tile's stores (gfx9 counts loads and stores in one in-order counter).
-// T is written and read for every plane tile, padding planes included
-// (ps = ntiles * TP; padding planes are never read as data).
-// Byte offset of a missing mode; the launcher keeps compact arrays below it
-// (plane offsets of at most a few KiB are added to it).
-constexpr uint32_t kCompactMissing = 0xC0000000u;
-
-template <typename R, int NY, bool INV, int IN, int OUT>
-__global__ __launch_bounds__((Geo2<R, NY>::THREADS)) void col2_stream_kernel(const cx<R>* __restrict__ in,
-                                                                            cx<R>* __restrict__ out,
-                                                                            const cx<R>* __restrict__ tw, int planes,
-                                                                            int ps, int kc, int splits,
-                                                                            const int* __restrict__ map,
-                                                                            uint32_t compact_bytes) {
-    using G = Geo2<R, NY>;
-    constexpr int N1 = G::N1, N2 = G::N2, TP = G::TP, NT = G::THREADS;
-    static_assert((IN == kRetained && OUT == kStrided) || (IN == kStrided && OUT == kRetained), "T <-> compact");
-    constexpr uint32_t CB = sizeof(cx<R>);
-    constexpr uint32_t kMissing = kCompactMissing;  // past any compact array the launcher accepts
-    __shared__ cx<R> buf[NY * TP];
-    __shared__ cx<R> twl[NY];
-    __shared__ int scode[NY];
-    const int t = threadIdx.x;
-    const int kx = static_cast<int>(blockIdx.x) / splits, sp = static_cast<int>(blockIdx.x) % splits;
-    const int ntiles = (planes + TP - 1) / TP;
-    const int tb = sp * ntiles / splits, te = (sp + 1) * ntiles / splits;
-    if (kx >= kc || tb >= te) return;  // uniform across the workgroup
-    for (int i = t; i < NY; i += NT) {
-        scode[i] = map[kx * NY + i];
-        twl[i] = tw[i];
-    }
-    __syncthreads();
-    const int pp = t % TP, n2 = t / TP, k1 = t / TP;
-    const __amdgpu_buffer_rsrc_t compact = buffer_rsrc(IN == kRetained ? static_cast<const void*>(in) : out,
-                                                       compact_bytes);
-    const size_t P = static_cast<size_t>(planes);
-    // Tile-independent parts of this thread's addresses.
-    uint32_t coff[IN == kRetained ? N1 : N2];  // compact byte offsets at plane 0 (kMissing: no mode)
-    uint32_t cneg = 0;                         // bit i: conjugate element i on the way in / out
-    if constexpr (IN == kRetained) {
-#pragma unroll
-        for (int n1 = 0; n1 < N1; ++n1) {
-            const int code = n2 < N2 ? scode[N2 * n1 + n2] : -1;
-            coff[n1] = code >= 0 ? static_cast<uint32_t>(P * static_cast<size_t>(code >> 1) * CB) : kMissing;
-            cneg |= (code >= 0 && (code & 1)) ? (1u << n1) : 0u;
-        }
-    } else {
-#pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) {
-            const int code = scode[k1 + N1 * k2];
-            coff[k2] = code >= 0 ? static_cast<uint32_t>(P * static_cast<size_t>(code >> 1) * CB) : kMissing;
-            cneg |= (code >= 0 && (code & 1)) ? (1u << k2) : 0u;
-        }
-    }
-    auto load_tile = [&](cx<R>* v, int tile) {
-        const int p = tile * TP + pp;
-        if constexpr (IN == kRetained) {
-            const uint32_t pb = static_cast<uint32_t>(p) * CB;
-#pragma unroll
-            for (int n1 = 0; n1 < N1; ++n1) {
-                cx<R> x = buffer_load_cx<R>(compact, coff[n1] + pb);
-                v[n1] = x;
-            }
-        } else {
-#pragma unroll
-            for (int n1 = 0; n1 < N1; ++n1) {
-                const int yy = N2 * n1 + (n2 < N2 ? n2 : 0);
-                v[n1] = in[static_cast<size_t>(ps) * (kx + static_cast<size_t>(kc) * yy) + p];
-            }
-        }
-    };
-    cx<R> v[N1];
-    load_tile(v, tb);
-    for (int tile = tb; tile < te; ++tile) {
-        const bool more = tile + 1 < te;  // uniform
-        cx<R> vn[N1];
-        if (more) load_tile(vn, tile + 1);
-        const int p = tile * TP + pp;
-        // pass 1: thread (plane fastest, n2)
-        if (n2 < N2) {
-            if constexpr (IN == kRetained) {
-#pragma unroll
-                for (int n1 = 0; n1 < N1; ++n1)
-                    if ((cneg >> n1) & 1u) v[n1].y = -v[n1].y;
-            }
-            rdft<N1, INV>(v);
-#pragma unroll
-            for (int k = 0; k < N1; ++k) {
-                const cx<R> w = twl[n2 * k];
-                buf[(k * N2 + n2) * TP + pp] = INV ? cmulc(v[k], w) : cmul(v[k], w);
-            }
-        }
-        __syncthreads();
-        // pass 2: thread (plane fastest, k1)
-        cx<R> w[N2];
-#pragma unroll
-        for (int m = 0; m < N2; ++m) w[m] = buf[(k1 * N2 + m) * TP + pp];
-        rdft<N2, INV>(w);
-        if constexpr (OUT == kStrided) {
-#pragma unroll
-            for (int k2 = 0; k2 < N2; ++k2) {
-                const int row = k1 + N1 * k2;
-                out[static_cast<size_t>(ps) * (kx + static_cast<size_t>(kc) * row) + p] = w[k2];
-            }
-        } else {
-            const bool live = p < planes;
-            const uint32_t pb = static_cast<uint32_t>(p) * CB;
-#pragma unroll
-            for (int k2 = 0; k2 < N2; ++k2) {
-                cx<R> x = w[k2];
-                if ((cneg >> k2) & 1u) x.y = -x.y;
-                buffer_store_cx<R>(x, compact, live ? coff[k2] + pb : kMissing);
-            }
-        }
-        if (!more) break;
-        __syncthreads();  // buf is rewritten by the next tile
-#pragma unroll
-        for (int m = 0; m < N1; ++m) v[m] = vn[m];
     }
 }
 
@@ -1368,12 +1259,13 @@ constexpr int kStageForwardRows = 0, kStageForwardColumns = 1, kStageInverseColu
 
 template <typename R, int NY, bool LOOP>
 void launch_col2(ColVariant v, const cx<R>* i, cx<R>* o, const cx<R>* t, int planes, int ps, int kx_count, int items,
-                 int in_kc, int out_kc, const int* map, const SplitViews<R>& views, int grid_x, hipStream_t stream) {
+                 int in_kc, int out_kc, const int* map, const SplitViews<R>& views, uint32_t in_bytes, int grid_x,
+                 hipStream_t stream) {
     using G = Geo2<R, NY>;
     const dim3 grid(grid_x), block(G::THREADS);
 #define SKB_COL2(INV, IN, OUT)                                                                                    \
     hipLaunchKernelGGL((col2_kernel<R, NY, INV, IN, OUT, LOOP>), grid, block, 0, stream, i, o, t, planes, ps,     \
-                       kx_count, items, in_kc, out_kc, map, views)
+                       kx_count, items, in_kc, out_kc, map, views, in_bytes)
     switch (v) {
         case ColVariant::FwdStrided: SKB_COL2(false, kStrided, kStrided); break;
         case ColVariant::FwdToRetained: SKB_COL2(false, kStrided, kRetained); break;
@@ -1385,48 +1277,12 @@ void launch_col2(ColVariant v, const cx<R>* i, cx<R>* o, const cx<R>* t, int pla
 #undef SKB_COL2
 }
 
-// Streaming column stage of a pruned plan (col2_stream_kernel): returns
-// false when the shape needs the one-item-per-workgroup kernels (Ny < 64,
-// unpadded T, or a compact array past the buffer offset range).
-template <typename R, int NY>
-bool launch_col_stream_ny(bool inverse, const void* in, void* out, const void* tw, int planes, int ps, int kc,
-                          const int* map, uint64_t compact_bytes, int req, hipStream_t stream) {
-    if constexpr (NY < 64 || NY > 256) {
-        return false;  // larger columns need too many registers for the prefetch
-    } else {
-        using G = Geo2<R, NY>;
-        const int ntiles = (planes + G::TP - 1) / G::TP;
-        if (ps < ntiles * G::TP || compact_bytes > kCompactMissing) return false;
-        // workgroups: about `req` (default two per CU), whole k_x columns split into plane-tile runs
-        const int want = req > 0 ? req : 512;
-        const int splits = std::max(1, std::min(ntiles, (want + kc - 1) / kc));
-        const dim3 grid(static_cast<unsigned>(kc * splits)), block(G::THREADS);
-        if (!inverse) return false;  // the forward direction keeps one workgroup per item
-        hipLaunchKernelGGL((col2_stream_kernel<R, NY, true, kRetained, kStrided>), grid, block, 0, stream,
-                           static_cast<const cx<R>*>(in), static_cast<cx<R>*>(out), static_cast<const cx<R>*>(tw),
-                           planes, ps, kc, splits, map, static_cast<uint32_t>(compact_bytes));
-        check_launch("column launch");
-        return true;
-    }
-}
-
-template <typename R>
-bool launch_col_stream(bool inverse, int ny, const void* in, void* out, const void* tw, int planes, int ps, int kc,
-                       const int* map, uint64_t compact_bytes, int req, hipStream_t stream) {
-    switch (ny) {
-        case 64: return launch_col_stream_ny<R, 64>(inverse, in, out, tw, planes, ps, kc, map, compact_bytes, req, stream);
-        case 128: return launch_col_stream_ny<R, 128>(inverse, in, out, tw, planes, ps, kc, map, compact_bytes, req, stream);
-        case 256: return launch_col_stream_ny<R, 256>(inverse, in, out, tw, planes, ps, kc, map, compact_bytes, req, stream);
-        case 512: return launch_col_stream_ny<R, 512>(inverse, in, out, tw, planes, ps, kc, map, compact_bytes, req, stream);
-        case 1024: return launch_col_stream_ny<R, 1024>(inverse, in, out, tw, planes, ps, kc, map, compact_bytes, req, stream);
-        default: return false;
-    }
-}
-
 template <typename R, int NY>
 void launch_col(ColVariant v, const void* in, void* out, const void* tw, int planes, int ps, int kx_count, int in_kc,
                 int out_kc, const int* col_start, const ColEntry* entries, const int* map,
-                const SplitViews<R>& views, int req, hipStream_t stream) {
+                const SplitViews<R>& views, uint64_t in_bytes, int req, hipStream_t stream) {
+    // buffer addressing of a compact input needs every offset below 2 GiB
+    const uint32_t ib = in_bytes < kBufferPast ? static_cast<uint32_t>(in_bytes) : 0u;
     const auto* i = static_cast<const cx<R>*>(in);
     auto* o = static_cast<cx<R>*>(out);
     const auto* t = static_cast<const cx<R>*>(tw);
@@ -1435,9 +1291,11 @@ void launch_col(ColVariant v, const void* in, void* out, const void* tw, int pla
         const int items = kx_count * ((planes + G::TP - 1) / G::TP);
         const int grid = launch_grid(items, req);
         if (grid < items)
-            launch_col2<R, NY, true>(v, i, o, t, planes, ps, kx_count, items, in_kc, out_kc, map, views, grid, stream);
+            launch_col2<R, NY, true>(v, i, o, t, planes, ps, kx_count, items, in_kc, out_kc, map, views, ib, grid,
+                                     stream);
         else
-            launch_col2<R, NY, false>(v, i, o, t, planes, ps, kx_count, items, in_kc, out_kc, map, views, grid, stream);
+            launch_col2<R, NY, false>(v, i, o, t, planes, ps, kx_count, items, in_kc, out_kc, map, views, ib, grid,
+                                      stream);
     } else {
         if (ps != planes) fail(SKB_RUNTIME_ERROR, "padded plane stride needs Ny >= 64");
         constexpr int TP = col_tile<R, NY>();
@@ -1478,12 +1336,12 @@ void row_pass(bool inverse, int nx, const void* in, void* out, const void* tw, i
 template <typename R>
 void col_pass(ColVariant v, int ny, const void* in, void* out, const void* tw, int planes, int ps, int kx_count,
               int in_kc, int out_kc, const int* col_start, const ColEntry* entries, const int* map, int req,
-              hipStream_t s, const SplitViews<R>& views = SplitViews<R>{}) {
+              hipStream_t s, const SplitViews<R>& views = SplitViews<R>{}, uint64_t in_bytes = 0) {
     switch (ny) {
 #define SKB_COL_CASE(N)                                                                                        \
     case N:                                                                                                    \
         launch_col<R, N>(v, in, out, tw, planes, ps, kx_count, in_kc, out_kc, col_start, entries, map, views,  \
-                         req, s);                                                                              \
+                         in_bytes, req, s);                                                                    \
         return;
         SKB_POW2_CASES(SKB_COL_CASE)
 #undef SKB_COL_CASE
@@ -1688,13 +1546,10 @@ void stage_forward_columns(skb_pruned_plan* p, void* retained, hipStream_t s) {
 
 template <typename R>
 void stage_inverse_columns(skb_pruned_plan* p, const void* retained, hipStream_t s) {
-    if (launch_col_stream<R>(true, static_cast<int>(p->w.ny), retained, p->scratch.ptr, p->tw.y.ptr, p->planes, p->ps,
-                             p->kc, p->imap.as<int>(), p->modes * p->planes * sizeof(cx<R>),
-                             p->grid[kStageInverseColumns], s))
-        return;
     col_pass<R>(ColVariant::InvFromRetained, static_cast<int>(p->w.ny), retained, p->scratch.ptr, p->tw.y.ptr,
                 p->planes, p->ps, p->kc, p->kc, p->kc, p->inv_start.as<int>(), p->inv.as<ColEntry>(),
-                p->imap.as<int>(), p->grid[kStageInverseColumns], s);
+                p->imap.as<int>(), p->grid[kStageInverseColumns], s, SplitViews<R>{},
+                p->modes * p->planes * sizeof(cx<R>));
 }
 
 template <typename R>
