@@ -17,7 +17,7 @@ Pinning summary (details in tests/test_oracle.py):
   * retained_operator: closed form NxNy * vertical(x) on Hermitian-consistent
     inputs, and a direct-DFT composition on small grids.
   * k2_binned_groups / qr_gaussian_family: no reference counterpart (the
-    32-bin QR-Gaussian family is a BASELINE-only variant).  Pinned by QR
+    32-bin QR-Gaussian family is the benchmark's own variant).  Pinned by QR
     uniqueness: Q^T Q = I and Q^T G upper triangular with positive diagonal.
   * spectral_gaussian: no reference counterpart; "parity unpinned" beyond its
     statistical and Hermitian properties (it only generates inputs).
@@ -181,7 +181,7 @@ def squared_keys(modes: np.ndarray) -> np.ndarray:
 
 
 def k2_binned_groups(modes: np.ndarray, bins: int = 32):
-    """BASELINE-only grouping: bin b = floor(bins*K2/(K2max+1)) over the
+    """Benchmark grouping (no reference counterpart): bin b = floor(bins*K2/(K2max+1)) over the
     retained order, empty bins dropped.  Returns (bin_ids, first, count)."""
     k2 = squared_keys(modes).astype(np.int64)
     b = (bins * k2) // (int(k2.max()) + 1)
@@ -237,7 +237,7 @@ def gaussians(seed: int, count: int) -> np.ndarray:
 
 
 def qr_gaussian_family(n: int, groups: int, seed: int) -> np.ndarray:
-    """BASELINE-only vertical family: per group, Q of the QR factorisation of
+    """Benchmark vertical family (no reference counterpart): per group, Q of the QR factorisation of
     an n x n Gaussian matrix (row-major, drawn from gaussians(seed + group)),
     with R's diagonal made positive.  Returns forward matrices (groups*n*n,
     row-major Q^T so that modal = Q^T phys) -- the inverse is Q."""
