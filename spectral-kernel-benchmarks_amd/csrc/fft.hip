@@ -533,6 +533,33 @@ __device__ __forceinline__ double row_conj_lane(double v) {
     return __longlong_as_double((rhi << 32) | rlo);
 }
 
+// Row-pass transpose between the two register passes, through LDS in two
+// rounds (real parts, then imaginary parts) over one array of R: half the LDS
+// of a complex buffer, so the row kernels are limited by registers rather
+// than LDS (7 rather than 4 workgroups per CU at Nx = 256, FP32).  Thread
+// `put` writes z[k] at b[wo + k*YS]; every thread then reads w[m] from
+// b[ro + m].  YS = N2 + 1 and a pair stride of N1*YS + 4 keep both the
+// writes (16 pairs x 4 columns per wavefront) and the reads (16 rows x 4
+// pairs) within one extra cycle of the bank minimum.
+template <typename R, int NW, int NR, int YS>
+__device__ __forceinline__ void row_transpose(R* b, const cx<R>* z, bool put, int wo, int ro, cx<R>* w) {
+    if (put) {
+#pragma unroll
+        for (int k = 0; k < NW; ++k) b[wo + k * YS] = z[k].x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < NR; ++m) w[m].x = b[ro + m];
+    __syncthreads();
+    if (put) {
+#pragma unroll
+        for (int k = 0; k < NW; ++k) b[wo + k * YS] = z[k].y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < NR; ++m) w[m].y = b[ro + m];
+}
+
 template <typename R, int NX, bool LOOP>
 __global__ __launch_bounds__((RowGeo<R, NX>::THREADS)) void row2_r2c_kernel(const R* __restrict__ real,
                                                                          cx<R>* __restrict__ out,
@@ -540,11 +567,9 @@ __global__ __launch_bounds__((RowGeo<R, NX>::THREADS)) void row2_r2c_kernel(cons
                                                                          int planes, int ps, int kc, int items) {
     using G = RowGeo<R, NX>;
     constexpr int N1 = G::N1, N2 = G::N2, S = G::PAIRS, TP = 2 * S, NT = G::THREADS;
-    // LDS strides in complex elements are odd, so the 16 (FP32) or 8 (FP64)
-    // plane pairs touched by one instruction fall on distinct banks.
-    constexpr int YS = N2 + 1;             // row of the transpose buffer
-    constexpr int SS = N1 * YS + 1;        // one pair's transpose buffer
-    __shared__ cx<R> buf[S * SS];
+    constexpr int YS = N2 + 1;       // row of the transpose buffer
+    constexpr int SS = N1 * YS + 4;  // one pair's transpose buffer
+    __shared__ R buf[S * SS];
     __shared__ cx<R> twl[NX];
     const int t = threadIdx.x;
     int item = first_item<LOOP>(items);
@@ -567,13 +592,11 @@ __global__ __launch_bounds__((RowGeo<R, NX>::THREADS)) void row2_r2c_kernel(cons
         if (s1 < S) {
             rdft<N1, false>(v);
 #pragma unroll
-            for (int k = 0; k < N1; ++k) buf[s1 * SS + k * YS + n2] = cmul(v[k], twl[n2 * k]);
+            for (int k = 0; k < N1; ++k) v[k] = cmul(v[k], twl[n2 * k]);
         }
-        __syncthreads();
         // pass 2: thread (k1 fastest, pair s)
         cx<R> w[N2];
-#pragma unroll
-        for (int m = 0; m < N2; ++m) w[m] = buf[s2 * SS + k1 * YS + m];
+        row_transpose<R, N1, N2, YS>(buf, v, s1 < S, s1 * SS + n2, s2 * SS + k1 * YS, w);
         rdft<N2, false>(w);
         // Thread (k1, s2) now holds Z[k1 + N1*k2] of pair s2.  The Hermitian
         // split X_a = (Z_k + conj Z_{N-k})/2, X_b = (Z_k - conj Z_{N-k})/(2i)
@@ -750,8 +773,8 @@ __global__ __launch_bounds__((RowGeo<R, NX>::THREADS)) void row2_c2r_kernel(cons
     using G = RowGeo<R, NX>;
     constexpr int N1 = G::N1, N2 = G::N2, S = G::PAIRS, TP = 2 * S, NT = G::THREADS;
     constexpr int YS = N2 + 1;
-    constexpr int SS = N1 * YS + 1;  // odd: conflict-free across pairs
-    __shared__ cx<R> buf[S * SS];
+    constexpr int SS = N1 * YS + 4;
+    __shared__ R buf[S * SS];
     __shared__ cx<R> twl[NX];
     const int t = threadIdx.x;
     int item = first_item<LOOP>(items);
@@ -775,16 +798,15 @@ __global__ __launch_bounds__((RowGeo<R, NX>::THREADS)) void row2_c2r_kernel(cons
         // waterfall loop.
         const int y = __builtin_amdgcn_readfirstlane(item / ntiles);
         const int p0 = __builtin_amdgcn_readfirstlane((item % ntiles) * TP);
+        // pass 1: thread (pair s fastest, n2)
         if (n2 < N2) {
             rdft<N1, true>(v);
 #pragma unroll
-            for (int k = 0; k < N1; ++k) buf[s1 * SS + k * YS + n2] = cmulc(v[k], twl[n2 * k]);
+            for (int k = 0; k < N1; ++k) v[k] = cmulc(v[k], twl[n2 * k]);
         }
-        __syncthreads();
         // pass 2: thread (k1 fastest, pair s) -> real rows of planes 2s, 2s+1
         cx<R> w[N2];
-#pragma unroll
-        for (int m = 0; m < N2; ++m) w[m] = buf[s2 * SS + k1 * YS + m];
+        row_transpose<R, N1, N2, YS>(buf, v, n2 < N2, s1 * SS + n2, s2 * SS + k1 * YS, w);
         rdft<N2, true>(w);
         // Bounds-checked stores over the tile's rows (planes past the last
         // are dropped), so every thread issues 2*N2 stores: the wait for the
@@ -1049,6 +1071,33 @@ __device__ __forceinline__ void col2_load(cx<R>* v, const cx<R>* in, int item, i
     }
 }
 
+// Column-pass transpose between the two register passes, through LDS in two
+// rounds (real, then imaginary parts) over one array of R: half the LDS of a
+// complex buffer, so the column kernels are limited by registers rather than
+// LDS.  Entry (k, m) of the N1 x N2 exchange holds TP planes at
+// ((N2 + 1)*k + m)*TP: the extra row per block of N2 puts the rows that one
+// wavefront reads (consecutive k1 at the same m) on different banks.
+// Buffer size: N1*(N2 + 1)*TP elements.
+template <typename R, int N1, int N2, int TP>
+__device__ __forceinline__ void col_transpose(R* b, const cx<R>* z, bool put, int n2, int k1, int pp, cx<R>* w) {
+    auto at = [&](int k, int m) { return ((N2 + 1) * k + m) * TP + pp; };
+    if (put) {
+#pragma unroll
+        for (int k = 0; k < N1; ++k) b[at(k, n2)] = z[k].x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < N2; ++m) w[m].x = b[at(k1, m)];
+    __syncthreads();
+    if (put) {
+#pragma unroll
+        for (int k = 0; k < N1; ++k) b[at(k, n2)] = z[k].y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < N2; ++m) w[m].y = b[at(k1, m)];
+}
+
 template <typename R, int NY, bool INV, int IN, int OUT, bool LOOP>
 __global__ __launch_bounds__((Geo2<R, NY>::THREADS)) void col2_kernel(const cx<R>* in, cx<R>* out,
                                                                      const cx<R>* __restrict__ tw, int planes, int ps,
@@ -1057,7 +1106,7 @@ __global__ __launch_bounds__((Geo2<R, NY>::THREADS)) void col2_kernel(const cx<R
                                                                      const SplitViews<R> views, uint32_t in_bytes) {
     using G = Geo2<R, NY>;
     constexpr int N1 = G::N1, N2 = G::N2, TP = G::TP, NT = G::THREADS;
-    __shared__ cx<R> buf[NY * TP];
+    __shared__ R buf[N1 * (N2 + 1) * TP];
     __shared__ cx<R> twl[NY];
     // One workgroup per item: the item's column of map codes is staged in
     // LDS with one coalesced load per thread, instead of N1 (or N2) loads
@@ -1102,20 +1151,14 @@ __global__ __launch_bounds__((Geo2<R, NY>::THREADS)) void col2_kernel(const cx<R
         if constexpr (OUT == kSplit) {
             if (p < planes) ovl = view_lane(views, p);
         }
-        // pass 1: thread (plane fastest, n2)
+        // pass 1: thread (plane fastest, n2); pass 2: thread (plane fastest, k1)
+        cx<R> w[N2];
         if (n2 < N2) {
             rdft<N1, INV>(v);
 #pragma unroll
-            for (int k = 0; k < N1; ++k) {
-                const cx<R> w = twl[n2 * k];
-                buf[(k * N2 + n2) * TP + pp] = INV ? cmulc(v[k], w) : cmul(v[k], w);
-            }
+            for (int k = 0; k < N1; ++k) v[k] = INV ? cmulc(v[k], twl[n2 * k]) : cmul(v[k], twl[n2 * k]);
         }
-        __syncthreads();
-        // pass 2: thread (plane fastest, k1)
-        cx<R> w[N2];
-#pragma unroll
-        for (int m = 0; m < N2; ++m) w[m] = buf[(k1 * N2 + m) * TP + pp];
+        col_transpose<R, N1, N2, TP>(buf, v, n2 < N2, n2, k1, pp, w);
         rdft<N2, INV>(w);
         if (p < planes) {
 #pragma unroll
