@@ -575,6 +575,18 @@ class PrunedPlan:
             ptr = _dev(tensor, self.retained_elements, complex_dtype(self.precision), "retained")
         return Bound(getattr(lib, "skb_pruned_" + name), (self.h, ptr, _stream(stream)), (tensor, self))
 
+    def bind_forward(self, real, retained, stream=None) -> Bound:
+        """A prepared forward operator call (rows, then columns; see Bound)."""
+        args = (self.h, _dev(real, self.w.real_elements, real_dtype(self.precision), "real"),
+                _dev(retained, self.retained_elements, complex_dtype(self.precision), "retained"), _stream(stream))
+        return Bound(lib.skb_pruned_forward, args, (real, retained, self))
+
+    def bind_inverse(self, retained, real, stream=None) -> Bound:
+        """A prepared inverse operator call (columns, then rows; see Bound)."""
+        args = (self.h, _dev(retained, self.retained_elements, complex_dtype(self.precision), "retained"),
+                _dev(real, self.w.real_elements, real_dtype(self.precision), "real"), _stream(stream))
+        return Bound(lib.skb_pruned_inverse, args, (retained, real, self))
+
     def set_grid(self, stage: int, workgroups: int):
         """Launch shape of one stage (0 = one workgroup per work item)."""
         check(lib.skb_pruned_plan_set_grid(self.h, stage, workgroups))
