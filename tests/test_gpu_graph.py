@@ -5,7 +5,8 @@ hipGraph on ROCm) the way bench.py times it.  The graph is replayed several
 times with eager work on other inputs and synchronisations between replays;
 every replay must equal the eager evaluation bit for bit, and the eager
 evaluation must match the FP64 oracle (FP32 within 1e-6, FP64 within 1e-12,
-max-scale-normalised)."""
+max-scale-normalised).  The same evaluation as three bound operator calls
+must equal the five stage launches bit for bit."""
 import sys
 
 import numpy as np
@@ -69,3 +70,43 @@ def test_graph_replays_equal_eager(device, shape, precision):
         graph.replay()
         torch.cuda.synchronize()
         assert torch.equal(modal, ref), "replay %d" % rep
+
+
+@pytest.mark.parametrize("precision", [S.SKB_F32, S.SKB_F64])
+def test_operator_calls_equal_stage_launches(device, precision):
+    """The evaluation as three operator calls (pruned inverse, pruned forward,
+    vertical forward -- the provider calls of the reference's harness), bound
+    once and called again, equals the five single-stage launches bit for bit."""
+    import torch
+
+    shape = (256, 256, 65, 3)
+    note("operator calls vs stages %s p=%d" % (shape, precision))
+    w = S.Workload(*shape)
+    modes = S.retained_modes(w)
+    groups = S.k2_binned_groups(modes, 32)
+    fwd, inv = S.qr_gaussian_family(w.nz, len(groups), 129)
+    cd, rd = S.complex_dtype(precision), S.real_dtype(precision)
+    pr = S.PrunedPlan(w, modes, precision)
+    vp = S.VerticalPlan(w, w.nz, groups, fwd, inv, precision)
+    x = torch.from_numpy(S.spectral_fixture(w, modes, 129, -1.0)).to(device=device, dtype=cd)
+    real = torch.empty(w.real_elements, dtype=rd, device=device)
+    back = torch.empty(len(modes) * w.planes, dtype=cd, device=device)
+    modal = torch.empty(len(modes) * w.fields * w.nz, dtype=cd, device=device)
+    stages = [pr.bind_stage("inverse_columns", x), pr.bind_stage("inverse_rows", real),
+              pr.bind_stage("forward_rows", real), pr.bind_stage("forward_columns", back),
+              vp.bind_forward(back, modal)]
+    for fn in stages:
+        fn()
+    torch.cuda.synchronize()
+    ref_real, ref_back, ref_modal = real.clone(), back.clone(), modal.clone()
+    calls = [pr.bind_inverse(x, real), pr.bind_forward(real, back), vp.bind_forward(back, modal)]
+    for rep in range(3):
+        real.zero_()
+        back.zero_()
+        modal.zero_()
+        for fn in calls:
+            fn()
+        torch.cuda.synchronize()
+        assert torch.equal(real, ref_real), "real, call %d" % rep
+        assert torch.equal(back, ref_back), "compact, call %d" % rep
+        assert torch.equal(modal, ref_modal), "modal, call %d" % rep
