@@ -560,6 +560,35 @@ __device__ __forceinline__ void row_transpose(R* b, const cx<R>* z, bool put, in
     for (int m = 0; m < NR; ++m) w[m].y = b[ro + m];
 }
 
+// The value held by lane l ^ S (S = 16 or 32: rows of S lanes swapped in
+// pairs) through gfx950's v_permlane16/32_swap, a vector-ALU move: with both
+// operands the same register, the swap's first result holds the even row in
+// both rows of a pair and its second result the odd row, so a lane of an
+// odd row (`odd`) takes the first.  Other S go through ds_bpermute.
+template <int S>
+__device__ __forceinline__ uint32_t lane_xor_u32(uint32_t v, bool odd) {
+    if constexpr (S == 16) {
+        const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        return odd ? r[0] : r[1];
+    } else if constexpr (S == 32) {
+        const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        return odd ? r[0] : r[1];
+    } else {
+        return static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), S));
+    }
+}
+template <int S>
+__device__ __forceinline__ float lane_xor(float v, bool odd) {
+    return __uint_as_float(lane_xor_u32<S>(__float_as_uint(v), odd));
+}
+template <int S>
+__device__ __forceinline__ double lane_xor(double v, bool odd) {
+    const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
+    const uint64_t lo = lane_xor_u32<S>(static_cast<uint32_t>(b), odd);
+    const uint64_t hi = lane_xor_u32<S>(static_cast<uint32_t>(b >> 32), odd);
+    return __longlong_as_double(static_cast<long long>((hi << 32) | lo));
+}
+
 template <typename R, int NX, bool LOOP>
 __global__ __launch_bounds__((RowGeo<R, NX>::THREADS)) void row2_r2c_kernel(const R* __restrict__ real,
                                                                          cx<R>* __restrict__ out,
@@ -579,7 +608,13 @@ __global__ __launch_bounds__((RowGeo<R, NX>::THREADS)) void row2_r2c_kernel(cons
     for (int i = t; i < NX; i += NT) twl[i] = tw[i];
     __syncthreads();
     const int n2 = t % N2, s1 = t / N2;
-    const int k1 = t % N1, s2 = t / N1;
+    // Pass 2: thread (pair s2 fastest, k1).  Rows of S lanes take k1 in the
+    // order 0, N1/2, 1, N1-1, 2, N1-2, ..., so the threads of k1 and N1 - k1
+    // are S lanes apart in one wavefront (64/S >= 2 rows per wavefront).
+    const int s2 = t % S, kq = t / S;
+    const int kj = kq >> 1;
+    const int k1 = kj == 0 ? ((kq & 1) ? N1 / 2 : 0) : ((kq & 1) ? N1 - kj : kj);
+    static_assert(64 % S == 0 && 64 / S >= 2, "partner rows must share a wavefront");
     const int ntiles = (planes + TP - 1) / TP;
     for (;;) {
         const int next = LOOP ? item + static_cast<int>(gridDim.x) : items;
@@ -600,14 +635,16 @@ __global__ __launch_bounds__((RowGeo<R, NX>::THREADS)) void row2_r2c_kernel(cons
         rdft<N2, false>(w);
         // Thread (k1, s2) now holds Z[k1 + N1*k2] of pair s2.  The Hermitian
         // split X_a = (Z_k + conj Z_{N-k})/2, X_b = (Z_k - conj Z_{N-k})/(2i)
-        // takes Z_{N-k} from the lane of k1' = (N1 - k1) mod N1 in the same
-        // pair (its k2' = N2-1-k2; for k1 = 0 the thread's own N2-k2), so
-        // the split needs no second pass through LDS.  Each thread stores its
-        // kx = k1 + N1*k2 < kc as one 16-byte plane pair (FP32, even ps: the
-        // pair's second plane is padding when it does not exist) or two
-        // 8-byte elements, through a buffer resource over the row: an
-        // element past kc or past the last plane goes past the row and is
-        // dropped.
+        // takes Z_{N-k} from the thread of k1' = (N1 - k1) mod N1 and the same
+        // pair (its k2' = N2-1-k2), which sits S lanes away in the same
+        // wavefront (see the pass-2 mapping above); k1 = 0 and k1 = N1/2 are
+        // their own partners.  A wavefront holds S consecutive plane pairs of
+        // 64/S values of k1, so each store instruction writes runs of S
+        // plane pairs (256 bytes in FP32) per kx.  Each store is one 16-byte
+        // plane pair (FP32, even ps: the pair's second plane is padding when
+        // it does not exist) or two elements, through a buffer resource over
+        // the row: an element past kc or past the last plane goes past the
+        // row and is dropped.
         {
             constexpr uint32_t CB = sizeof(cx<R>);
             const int yu = __builtin_amdgcn_readfirstlane(y), p0u = __builtin_amdgcn_readfirstlane(p0);
@@ -617,20 +654,14 @@ __global__ __launch_bounds__((RowGeo<R, NX>::THREADS)) void row2_r2c_kernel(cons
             const int pa = p0u + 2 * s2;
             const bool has_a = pa < planes, has_b = pa + 1 < planes;
             const bool wide = sizeof(R) == 4 && (ps & 1) == 0;  // uniform
-            const int lane = t & 63;
-            const int src = (lane & ~(N1 - 1)) | ((N1 - k1) & (N1 - 1));
 #pragma unroll
             for (int k2 = 0; k2 <= N2 / 2; ++k2) {
                 if (N1 * k2 >= kc) break;  // uniform
                 const int kx = k1 + N1 * k2;
                 const cx<R> zk = w[k2];
                 const cx<R> pz = w[N2 - 1 - k2];
-                cx<R> zn;
-                if constexpr (N1 == 16) {
-                    zn = cx<R>{row_conj_lane(pz.x), row_conj_lane(pz.y)};
-                } else {
-                    zn = cx<R>{__shfl(pz.x, src), __shfl(pz.y, src)};
-                }
+                cx<R> zn{lane_xor<S>(pz.x, kq & 1), lane_xor<S>(pz.y, kq & 1)};
+                if (2 * k1 == N1) zn = pz;
                 if (k1 == 0) zn = w[(N2 - k2) & (N2 - 1)];
                 const cx<R> xa{R(0.5) * (zk.x + zn.x), R(0.5) * (zk.y - zn.y)};
                 const cx<R> xb{R(0.5) * (zk.y + zn.y), R(0.5) * (zn.x - zk.x)};
