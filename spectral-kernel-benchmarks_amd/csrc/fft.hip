@@ -538,9 +538,9 @@ __device__ __forceinline__ double row_conj_lane(double v) {
 // of a complex buffer, so the row kernels are limited by registers rather
 // than LDS (7 rather than 4 workgroups per CU at Nx = 256, FP32).  Thread
 // `put` writes z[k] at b[wo + k*YS]; every thread then reads w[m] from
-// b[ro + m].  YS = N2 + 1 and a pair stride of N1*YS + 4 keep both the
-// writes (16 pairs x 4 columns per wavefront) and the reads (16 rows x 4
-// pairs) within one extra cycle of the bank minimum.
+// b[ro + m].  YS = N2 + 1 and a pair stride of N1*YS + 2 keep the writes
+// and the reads of both row kernels' thread mappings within one or two
+// cycles of the bank minimum.
 template <typename R, int NW, int NR, int YS>
 __device__ __forceinline__ void row_transpose(R* b, const cx<R>* z, bool put, int wo, int ro, cx<R>* w) {
     if (put) {
@@ -597,7 +597,7 @@ __global__ __launch_bounds__((RowGeo<R, NX>::THREADS)) void row2_r2c_kernel(cons
     using G = RowGeo<R, NX>;
     constexpr int N1 = G::N1, N2 = G::N2, S = G::PAIRS, TP = 2 * S, NT = G::THREADS;
     constexpr int YS = N2 + 1;       // row of the transpose buffer
-    constexpr int SS = N1 * YS + 4;  // one pair's transpose buffer
+    constexpr int SS = N1 * YS + 2;  // one pair's transpose buffer
     __shared__ R buf[S * SS];
     __shared__ cx<R> twl[NX];
     const int t = threadIdx.x;
@@ -804,7 +804,7 @@ __global__ __launch_bounds__((RowGeo<R, NX>::THREADS)) void row2_c2r_kernel(cons
     using G = RowGeo<R, NX>;
     constexpr int N1 = G::N1, N2 = G::N2, S = G::PAIRS, TP = 2 * S, NT = G::THREADS;
     constexpr int YS = N2 + 1;
-    constexpr int SS = N1 * YS + 4;
+    constexpr int SS = N1 * YS + 2;
     __shared__ R buf[S * SS];
     __shared__ cx<R> twl[NX];
     const int t = threadIdx.x;
