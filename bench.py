@@ -344,11 +344,16 @@ def bench_pipeline(args, S, torch, device):
                 rd = S.real_dtype(prec)
                 x = x64.to(rd)
                 y = torch.empty_like(x)
-                variants = [("", None)] + ([("_unfused", False)] if policy == S.SKB_PIPELINE_COMPACT else [])
+                # compact: the default, then the other form of the modal round
+                # trip (one kernel or three launches) for comparison
+                variants = [("", None)]
+                if policy == S.SKB_PIPELINE_COMPACT:
+                    variants.append(("_unfused", False) if pipe.fused else ("_fused", True))
                 for suffix, fused in variants:
                     if fused is not None:
                         pipe.set_fused(fused)
-                    pipe.execute(x, y)
+                    for _ in range(10):  # warm: the first variant timed otherwise reads ~5% slow
+                        pipe.execute(x, y)
                     ms = _timed_events(torch, lambda: pipe.execute(x, y), 10)
                     res["%s_%s%s" % (pname, name, suffix)] = {"ms_per_eval": ms, "evals_per_s": 1e3 / ms}
                 outputs[(pname, name)] = y.to(torch.float64)
