@@ -516,22 +516,6 @@ __device__ __forceinline__ void r2c_load(cx<R>* v, const R* real, int item, int 
 // Spectrum side: out[ps * (kx + kc * y) + p]; ps >= planes is the plane
 // stride (planes for WVM order, padded to whole cache lines for the pruned
 // intermediate).
-// The value held by lane (16 - i) mod 16 of each row of 16 lanes: DPP
-// row_mirror (lane 15 - i) followed by row_ror:1 (lane i - 1), two vector
-// moves with no LDS traffic.
-__device__ __forceinline__ float row_conj_lane(float v) {
-    const int m = __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xF, 0xF, false);
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, m, 0x121, 0xF, 0xF, false));
-}
-__device__ __forceinline__ double row_conj_lane(double v) {
-    const long long b = __double_as_longlong(v);
-    const int lo = static_cast<int>(b), hi = static_cast<int>(b >> 32);
-    const int mlo = __builtin_amdgcn_update_dpp(0, lo, 0x140, 0xF, 0xF, false);
-    const int mhi = __builtin_amdgcn_update_dpp(0, hi, 0x140, 0xF, 0xF, false);
-    const unsigned rlo = static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, mlo, 0x121, 0xF, 0xF, false));
-    const long long rhi = __builtin_amdgcn_update_dpp(0, mhi, 0x121, 0xF, 0xF, false);
-    return __longlong_as_double((rhi << 32) | rlo);
-}
 
 // Row-pass transpose between the two register passes, through LDS in two
 // rounds (real parts, then imaginary parts) over one array of R: half the LDS
