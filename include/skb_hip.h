@@ -200,6 +200,74 @@ uint64_t skb_pruned_plan_items(const skb_pruned_plan* plan, int stage);
 /* FFTWPrunedProvider::activeKxCount / scratchBytes. */
 uint64_t skb_pruned_active_kx(const skb_pruned_plan* plan);
 uint64_t skb_pruned_scratch_bytes(const skb_pruned_plan* plan);
+/* Transform path of skb_pruned_forward and skb_pruned_inverse.
+ * SKB_PATH_AUTO (the default) takes SKB_PATH_PLANE where it is built (FP32,
+ * 256 x 256 planes, active k_x band 86: the 2/3-rule set) and
+ * SKB_PATH_TWO_PASS elsewhere.  SKB_PATH_PLANE: each plane's 2-D transform
+ * runs in one workgroup with the retained band kept on chip; the compact
+ * operand passes through a plane-major copy (mode + np*plane) in the plan's
+ * scratch.  SKB_PATH_TWO_PASS: row and column kernels through the
+ * kc x Ny x P intermediate (the only path of the split, field-view and
+ * stage entry points).  The paths agree to rounding.  Setting
+ * SKB_PATH_PLANE where it is not built fails with SKB_UNSUPPORTED.  No
+ * reference counterpart (FFTW plans choose their algorithm internally). */
+enum skb_transform_path { SKB_PATH_AUTO = 0, SKB_PATH_TWO_PASS = 1, SKB_PATH_PLANE = 2 };
+int skb_pruned_plan_set_path(skb_pruned_plan* plan, int path);
+/* The path forward / inverse take: SKB_PATH_PLANE or SKB_PATH_TWO_PASS. */
+int skb_pruned_plan_path(const skb_pruned_plan* plan);
+
+/* ---- retained-mode operator ---------------------------------------------
+ * One evaluation of the antialiased retained-mode operator enqueued from C++
+ * in one call: the pruned inverse c2r of the compact retained spectrum
+ * (z, field, mode) into real planes, the forward r2c truncated back to the
+ * retained modes, and the K^2-grouped vertical forward product into the
+ * modal layout j + Nj*(field + F*mode).  The reference's harnesses time the
+ * same composition as separate provider calls: FFTWPrunedProvider::inverse
+ * and forward (include/skbench/skbench.hpp:467-520;
+ * src/pruned_fftw_provider.cpp:230-276) and VerticalGemmProvider::
+ * executeForward (include/skbench/skbench.hpp:152-218;
+ * src/vertical_gemm.cpp:534-596).  The real planes are written and read in
+ * the plan's own buffer (skb_retained_operator_real).  groups, forward and
+ * inverse: the vertical family as for skb_vertical_plan_create.  The input
+ * is preserved; the plan owns every intermediate, allocated at creation. */
+typedef struct skb_retained_operator skb_retained_operator;
+int skb_retained_operator_create(const skb_workload* workload, const skb_retained_mode* modes, uint64_t mode_count,
+                                 uint64_t nj, const skb_mode_group* groups, uint64_t group_count,
+                                 const double* forward, const double* inverse, int precision,
+                                 skb_retained_operator** op);
+int skb_retained_operator_destroy(skb_retained_operator* op);
+int skb_retained_operator_execute(skb_retained_operator* op, const void* retained, void* modal, void* stream);
+/* One stage of an evaluation, for per-kernel timing: SKB_OP_STAGE_LAYOUT
+ * (the plane path's plane-major copy of `retained`; nothing on the two-pass
+ * path), SKB_OP_STAGE_INVERSE (into the real planes), SKB_OP_STAGE_FORWARD
+ * (real planes to the plan's retained intermediate) and SKB_OP_STAGE_VERTICAL
+ * (into `modal`).  Each reads what the stage before it left in the plan's
+ * buffers, so the four in order on one stream equal one execute. */
+enum skb_operator_stage {
+    SKB_OP_STAGE_LAYOUT = 0,
+    SKB_OP_STAGE_INVERSE = 1,
+    SKB_OP_STAGE_FORWARD = 2,
+    SKB_OP_STAGE_VERTICAL = 3
+};
+int skb_retained_operator_stage(skb_retained_operator* op, int stage, const void* retained, void* modal,
+                                void* stream);
+/* Transform path (enum skb_transform_path), as for the pruned plan; on the
+ * plane path the forward transform hands the vertical product its
+ * plane-major output directly when the vertical plan reads that layout
+ * (square FP32 split-BF16 families, Nz = 33, 65, 129). */
+int skb_retained_operator_set_path(skb_retained_operator* op, int path);
+int skb_retained_operator_path(const skb_retained_operator* op);
+/* Plane path: execute runs each plane's inverse and forward transform in one
+ * workgroup (1, the default): the real plane is written to memory and read
+ * back by the same workgroup, so the two passes' memory phases overlap.
+ * 0 launches the two transforms separately, as the stages do.  Results are
+ * identical. */
+int skb_retained_operator_set_fused(skb_retained_operator* op, int fused);
+int skb_retained_operator_fused(const skb_retained_operator* op);
+/* Device pointer to the real planes (x + Nx(y + Ny*plane)) of the last
+ * evaluation, and the plan's device allocation in bytes. */
+void* skb_retained_operator_real(const skb_retained_operator* op);
+uint64_t skb_retained_operator_device_bytes(const skb_retained_operator* op);
 
 /* ---- K^2-grouped vertical GEMM -----------------------------------------
  * Replaces VerticalGemmProvider (include/skbench/skbench.hpp:152-218;

@@ -157,6 +157,16 @@ __device__ __forceinline__ cx<R> scal(cx<R> a, R s) {
     return {a.x * s, a.y * s};
 }
 
+// Workgroup barrier that orders LDS only.  __syncthreads() also makes the
+// workgroup's global stores visible, which waits for every store in flight;
+// kernels whose barriers guard LDS alone use this one so stores keep draining
+// across it.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // ---- bounds-checked buffer access -----------------------------------------
 // A raw buffer resource (stride 0) over `bytes` bytes: a load at an offset at
 // or past `bytes` returns zero and such a store is dropped, so a kernel can
@@ -177,6 +187,8 @@ typedef float bf4 __attribute__((ext_vector_type(4)));
 typedef double bd2 __attribute__((ext_vector_type(2)));
 __device__ bf4 raw_buffer_load_f4(__amdgpu_buffer_rsrc_t r, int voffset, int soffset, int aux) __asm(
     "llvm.amdgcn.raw.ptr.buffer.load.v4f32");
+__device__ float raw_buffer_load_f1(__amdgpu_buffer_rsrc_t r, int voffset, int soffset, int aux) __asm(
+    "llvm.amdgcn.raw.ptr.buffer.load.f32");
 __device__ bf2 raw_buffer_load_f2(__amdgpu_buffer_rsrc_t r, int voffset, int soffset, int aux) __asm(
     "llvm.amdgcn.raw.ptr.buffer.load.v2f32");
 __device__ bd2 raw_buffer_load_d2(__amdgpu_buffer_rsrc_t r, int voffset, int soffset, int aux) __asm(

@@ -8,6 +8,13 @@
 
 namespace skb {
 
+struct PlaneTables;
+// The pruned plan's plane-resident tables and its plane-major scratch
+// (mode + np * plane), or null where that path is not built (fft.hip).
+const PlaneTables* pruned_plane_tables(const skb_pruned_plan* p);
+void* pruned_plane_scratch(const skb_pruned_plan* p);
+int pruned_planes(const skb_pruned_plan* p);
+
 // True when the two pruned plans (3 + 3T input fields, T output fields) share
 // a grid the fused kernel handles (FP32, Nx >= 64, same Ny, Nz, k_x band).
 bool flux_rows_supported(const skb_pruned_plan* pin, const skb_pruned_plan* pout);
@@ -19,6 +26,12 @@ void flux_rows(skb_pruned_plan* pin, skb_pruned_plan* pout, int targets, double 
 // array: column c at in + K*(in_cs*c + in_co) (vertical.hip).
 void vertical_forward_columns(const skb_vertical_plan* p, const void* in, int in_cs, int in_co, void* out,
                               void* stream);
+
+// Forward product read from the plane-major retained layout (element (mode,
+// z, field) at pm[mode + np*(z + Nz*field)]) into the plan's modal layout;
+// square FP32 split-BF16 plans only (vertical.hip).
+bool vertical_pm_supported(const skb_vertical_plan* p);
+void vertical_forward_pm(const skb_vertical_plan* p, const void* pm, int np, void* out, void* stream);
 
 // Modal round trip in one kernel: out[:, c] = I_g (w[:, c] .* (F_g in[:, c]))
 // over a compact plan's columns (weights in the plan's precision, modal

@@ -1,0 +1,588 @@
+// Plane-resident 2-D transforms of the retained-mode operator (FP32) and the
+// compact <-> plane-major layout transposes, for gfx950.
+//
+// The pruned inverse c2r and forward r2c of one N x N plane (N = 256, active
+// k_x band kc = 86, the 2/3-rule retained set) run in ONE workgroup of 1024
+// threads (16 wavefronts).  The plane's band of kc x N complex values never
+// leaves the CU between the column and the row transforms: the column
+// results stay in registers, move to rows through LDS, and the exchange
+// between the two register passes of every 256-point FFT stays inside a
+// wavefront.  HBM sees the retained values once and the real plane once per
+// direction; the kc x Ny x P intermediate of the two-kernel transforms is
+// gone.
+//
+// Retained values are read and written in the PLANE-MAJOR layout: mode m of
+// plane p at m + np*p, np = modes rounded up to 16, so one plane's modes are
+// one contiguous run.  The caller's compact layout (plane + P*mode) is
+// converted by pm_transpose_kernel, whose workgroups move 16 modes of every
+// plane (one contiguous 16*P run on the compact side, P runs of 128 bytes on
+// the plane-major side).
+//
+// Every barrier in these kernels orders LDS only (lds_barrier): the global
+// loads and stores of a plane are never read by another wavefront, so the
+// stores of one row half keep draining while the next half is transformed.
+//
+// Thread roles in a plane workgroup (tid = 64*wave + 16*g + n):
+//   columns  column c = 64*r + 4*wave + g (round r), lane n of its 16;
+//   rows     row pair (2f, 2f + 1) of a half of 128 rows, f = 4*wave + g,
+//            packed as one complex sequence (real rows as Re and Im).
+// A 256-point FFT is N1 x N2 = 16 x 16: lane n loads x[16 j + n] (j = 0..15),
+// does a 16-point DFT over j, multiplies by W^(n k1), exchanges with the 15
+// other lanes of its group through the wavefront's LDS scratch and does the
+// second 16-point DFT, ending with X[n + 16 k] (k = 0..15) in lane n.
+#include <algorithm>
+#include <vector>
+
+#include "common.hpp"
+#include "fft_core.hpp"
+#include "internal.hpp"
+#include "plane.hpp"
+
+namespace skb {
+
+namespace {
+
+constexpr int kPlaneN = 256;
+constexpr int kPlaneThreads = 1024;
+
+
+// Makes the LDS accesses of one wavefront before it visible to the lanes of the
+// same wavefront after it (LDS operations of a wavefront complete in order).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// In-place exchange between the two register passes of the four 256-point
+// FFTs of a wavefront: lane n of group g holds v[k] (k = 0..15) and ends with
+// v[m] = (value k = n of lane m).  Real parts, then imaginary parts, through
+// the wavefront's scratch of 4 x 272 floats: element (k, m) of group g sits at
+// 272 g + 17 k + m, so a lane addresses all its elements from one base with
+// immediate offsets, and the writes (fixed k) and reads (fixed m) of a
+// half-wavefront (two groups, 272 = 16 mod 32 apart) hit 32 distinct banks.
+constexpr int kExchangeFloats = 4 * 272;
+__device__ __forceinline__ void wave_exchange16(float* s, cx<float>* v, int g, int n) {
+    float* wr = s + 272 * g + n;
+    const float* rd = s + 272 * g + 17 * n;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) wr[17 * k] = v[k].x;
+    wave_sync();
+#pragma unroll
+    for (int m = 0; m < 16; ++m) v[m].x = rd[m];
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) wr[17 * k] = v[k].y;
+    wave_sync();
+#pragma unroll
+    for (int m = 0; m < 16; ++m) v[m].y = rd[m];
+    wave_sync();
+}
+
+template <int KC>
+struct PlaneGeo {
+    static constexpr int N = kPlaneN, H = N / 2;
+    static constexpr int ROUNDS = (KC + 63) / 64;  // column rounds of 64 columns
+    static constexpr int RP = KC | 1;              // row-buffer pitch (complex, odd: column writes spread over banks)
+    static constexpr int YP = N + 1;               // band-buffer pitch (floats, odd)
+    static constexpr int KH = (KC + 15) / 16;      // registers k of a row FFT holding kx = n + 16k < KC
+    static constexpr int SMAX = kPlaneMaxModes;    // retained values staged per plane
+    static constexpr int R0 = std::max(std::max(H * RP * 8, KC * YP * 4), SMAX * 8);
+    static constexpr int WT = 16 * kExchangeFloats * 4;  // one exchange scratch per wavefront
+    static constexpr int BYTES = R0 + WT + N * 8;
+    static_assert(KC <= 128 && BYTES <= 160 * 1024, "plane workgroup exceeds the CU's LDS");
+};
+
+// One 256-point FFT per 16-lane group: lane n holds x[16 j + n] in v[j]; on
+// return it holds X[n + 16 k] in v[k].  tw[i] = exp(-2 pi i i / 256) in LDS;
+// INV uses the conjugate (unnormalised inverse).
+template <bool INV>
+__device__ __forceinline__ void fft256(cx<float>* v, float* s, const cx<float>* tw, int g, int n) {
+    rdft<16, INV>(v);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = INV ? cmulc(v[k], tw[n * k]) : cmul(v[k], tw[n * k]);
+    wave_exchange16(s, v, g, n);
+    rdft<16, INV>(v);
+}
+
+// Lane i of each 16-lane row receives the value of lane (16 - i) mod 16
+// (row_mirror: lane i <- 15 - i, then row_ror:1: lane i <- i - 1).
+__device__ __forceinline__ float from_partner_lane(float v) {
+    int x = __float_as_int(v);
+    x = __builtin_amdgcn_update_dpp(0, x, 0x140, 0xF, 0xF, false);
+    x = __builtin_amdgcn_update_dpp(0, x, 0x121, 0xF, 0xF, false);
+    return __int_as_float(x);
+}
+
+// Code k (0..15) of a lane's 16 packed 16-bit column codes: 2*mode + conj, or
+// 0xFFFF where the (kx, ky) slot holds no retained value.
+__device__ __forceinline__ int unpack_code(const uint32_t* c, int k) {
+    const uint32_t w = c[k >> 1];
+    return static_cast<int>((k & 1) ? (w >> 16) : (w & 0xFFFFu));
+}
+
+__device__ __forceinline__ void load_codes(uint32_t* c, const uint32_t* __restrict__ table, int lane_slot) {
+    const bf4* src = reinterpret_cast<const bf4*>(table) + 2 * lane_slot;
+    const bf4 a = src[0], b = src[1];
+    c[0] = __float_as_uint(a.x);
+    c[1] = __float_as_uint(a.y);
+    c[2] = __float_as_uint(a.z);
+    c[3] = __float_as_uint(a.w);
+    c[4] = __float_as_uint(b.x);
+    c[5] = __float_as_uint(b.y);
+    c[6] = __float_as_uint(b.z);
+    c[7] = __float_as_uint(b.w);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Inverse: plane-major retained values of plane blockIdx.x -> its real plane.
+// icodes: per (column c, lane n) 16 codes of (kx = c, ky = 16 j + n), the
+// k_x = 0 Hermitian partners included (src/model.cpp:332-335).  The imaginary
+// part of the kx = 0 bin is ignored by the row c2r, as FFTW's c2r does.
+// ---------------------------------------------------------------------------
+template <int KC>
+__device__ __forceinline__ void inverse_plane(unsigned char* lds, const cx<float>* pm, float* real, const cx<float>* __restrict__ twg,
+                                              const uint32_t* __restrict__ icodes, int modes, int np, int p) {
+    using G = PlaneGeo<KC>;
+    constexpr int N = G::N, RP = G::RP;
+    cx<float>* r0 = reinterpret_cast<cx<float>*>(lds);  // staged modes, then the row buffer [128][RP]
+    const int tid = static_cast<int>(threadIdx.x), wave = tid >> 6, lane = tid & 63, g = lane >> 4, n = lane & 15;
+    float* wt = reinterpret_cast<float*>(lds + G::R0) + kExchangeFloats * wave;
+    cx<float>* tw = reinterpret_cast<cx<float>*>(lds + G::R0 + G::WT);
+
+    // 1. the plane's retained values (16-byte pairs; np is even and past an
+    //    odd count the pair reads the row's padding) and the twiddles
+    {
+        const __amdgpu_buffer_rsrc_t rs =
+            buffer_rsrc(pm + static_cast<size_t>(np) * p, static_cast<uint32_t>(np) * 8u);
+        const int pairs = (modes + 1) >> 1;
+        for (int i = tid; i < pairs; i += kPlaneThreads)
+            *reinterpret_cast<bf4*>(&r0[2 * i]) = raw_buffer_load_f4(rs, 16 * i, 0, 0);
+        if (tid < N) tw[tid] = twg[tid];
+    }
+    uint32_t code[G::ROUNDS][8];
+#pragma unroll
+    for (int r = 0; r < G::ROUNDS; ++r) {
+        const int c = 64 * r + 4 * wave + g;
+        if (64 * r + 4 * wave < KC) load_codes(code[r], icodes, (c < KC ? c : KC - 1) * 16 + n);
+    }
+    lds_barrier();
+
+    // 2. column inputs from the staged values
+    cx<float> y[G::ROUNDS][16];
+#pragma unroll
+    for (int r = 0; r < G::ROUNDS; ++r) {
+        const int c = 64 * r + 4 * wave + g;
+        if (64 * r + 4 * wave < KC) {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int e = unpack_code(code[r], j);
+                const bool ok = e != 0xFFFF && c < KC;
+                const cx<float> x = r0[ok ? (e >> 1) : 0];
+                y[r][j] = ok ? cx<float>{x.x, (e & 1) ? -x.y : x.y} : cx<float>{0.f, 0.f};
+            }
+        }
+    }
+    lds_barrier();  // the staging area becomes the row buffer
+
+    // 3. column transforms: lane n of column c ends with Y(c, n + 16 k) in
+    //    y[r][k]; rows 0..127 (k < 8) go to the row buffer at once, rows
+    //    128..255 stay in registers for the second half
+    const int f = 4 * wave + g;
+#pragma unroll
+    for (int r = 0; r < G::ROUNDS; ++r) {
+        const int c = 64 * r + 4 * wave + g;
+        if (64 * r + 4 * wave < KC) {
+            fft256<true>(y[r], wt, tw, g, n);
+            if (c < KC) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) r0[(n + 16 * k) * RP + c] = y[r][k];
+            }
+        }
+    }
+
+    // 4. rows, two halves of 128
+    const __amdgpu_buffer_rsrc_t out =
+        buffer_rsrc(real + static_cast<size_t>(p) * N * N, static_cast<uint32_t>(N * N * 4));
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        if (h) {
+            lds_barrier();  // every row load of the first half is done
+#pragma unroll
+            for (int r = 0; r < G::ROUNDS; ++r) {
+                const int c = 64 * r + 4 * wave + g;
+                if (64 * r + 4 * wave < KC && c < KC) {
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) r0[(n + 16 * k) * RP + c] = y[r][8 + k];
+                }
+            }
+        }
+        lds_barrier();
+        const cx<float>* ra = r0 + 2 * f * RP;
+        const cx<float>* rb = ra + RP;
+        cx<float> z[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            z[j] = cx<float>{0.f, 0.f};
+            if (16 * j < KC) {
+                // X_a[kx] + i X_b[kx] at kx = 16 j + n
+                const int kx = 16 * j + n;
+                const bool ok = kx < KC;
+                cx<float> a = ra[ok ? kx : 0], b = rb[ok ? kx : 0];
+                if (j == 0 && n == 0) a.y = b.y = 0.f;
+                z[j] = ok ? cx<float>{a.x - b.y, a.y + b.x} : cx<float>{0.f, 0.f};
+            } else if (j >= 8 && N - 16 * j - 15 < KC) {
+                // conj X_a[N - kx] + i conj X_b[N - kx] at position 16 j + n
+                const int kx = N - 16 * j - n;
+                const bool ok = kx < KC;
+                const cx<float> a = ra[ok ? kx : 0], b = rb[ok ? kx : 0];
+                z[j] = ok ? cx<float>{a.x + b.y, b.x - a.y} : cx<float>{0.f, 0.f};
+            }
+        }
+        fft256<true>(z, wt, tw, g, n);
+        // lane n holds x = n + 16 k of row 2f (real parts) and 2f + 1 (imaginary)
+        const uint32_t row = static_cast<uint32_t>((128 * h + 2 * f) * N + n) * 4u;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            buffer_store_real<float>(z[k].x, out, row + 64u * k);
+            buffer_store_real<float>(z[k].y, out, row + N * 4u + 64u * k);
+        }
+    }
+}
+
+template <int KC>
+__global__ __launch_bounds__(kPlaneThreads) void plane_inverse_kernel(const cx<float>* __restrict__ pm,
+                                                                     float* __restrict__ real,
+                                                                     const cx<float>* __restrict__ twg,
+                                                                     const uint32_t* __restrict__ icodes, int modes,
+                                                                     int np) {
+    __shared__ __align__(16) unsigned char lds[PlaneGeo<KC>::BYTES];
+    inverse_plane<KC>(lds, pm, real, twg, icodes, modes, np, static_cast<int>(blockIdx.x));
+}
+
+// ---------------------------------------------------------------------------
+// Forward: real plane blockIdx.x -> its plane-major retained values.
+// fcodes: per (column c, lane n) 16 codes of (kx = c, ky = n + 16 k).
+// ---------------------------------------------------------------------------
+// READBACK: the plane was written by this workgroup's inverse pass with the
+// same row decomposition, so each lane reads back the addresses it wrote
+// (program order suffices, no barrier); the loads are volatile so that the
+// compiler keeps them.  Otherwise the twiddles are staged first.
+template <int KC, bool READBACK>
+__device__ __forceinline__ void forward_plane(unsigned char* lds, const float* real, cx<float>* pm, const cx<float>* __restrict__ twg,
+                                              const uint32_t* __restrict__ fcodes, int modes, int np, int p) {
+    using G = PlaneGeo<KC>;
+    constexpr int N = G::N, YP = G::YP, KH = G::KH;
+    float* yb = reinterpret_cast<float*>(lds);           // band [KC][YP], real or imaginary parts
+    cx<float>* st = reinterpret_cast<cx<float>*>(lds);  // then the output staging [modes]
+    const int tid = static_cast<int>(threadIdx.x), wave = tid >> 6, lane = tid & 63, g = lane >> 4, n = lane & 15;
+    float* wt = reinterpret_cast<float*>(lds + G::R0) + kExchangeFloats * wave;
+    cx<float>* tw = reinterpret_cast<cx<float>*>(lds + G::R0 + G::WT);
+    const __amdgpu_buffer_rsrc_t in =
+        buffer_rsrc(real + static_cast<size_t>(p) * N * N, static_cast<uint32_t>(N * N * 4));
+    constexpr int kAux = READBACK ? static_cast<int>(0x80000000u) : 0;  // volatile
+    const int f = 4 * wave + g;
+    if (!READBACK && tid < N) tw[tid] = twg[tid];
+
+    // 1. rows: X_a, X_b (rows 2f, 2f + 1 of each half) for kx = n + 16 k < KC
+    cx<float> xa[2][KH], xb[2][KH];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t row = static_cast<uint32_t>((128 * h + 2 * f) * N + n) * 4u;
+        cx<float> z[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            z[j].x = raw_buffer_load_f1(in, static_cast<int>(row + 64u * j), 0, kAux);
+            z[j].y = raw_buffer_load_f1(in, static_cast<int>(row + N * 4u + 64u * j), 0, kAux);
+        }
+        // (READBACK: the barrier also ends the inverse pass's row-buffer reads
+        // before the band overwrites the region)
+        if (h == 0) lds_barrier();
+        fft256<false>(z, wt, tw, g, n);
+        // Z[n + 16 k]; its partner Z[N - n - 16 k] is register 15 - k of lane
+        // (16 - n) mod 16, or register (16 - k) mod 16 of lane 0 itself
+#pragma unroll
+        for (int k = 0; k < KH; ++k) {
+            const cx<float> pz = z[15 - k];
+            cx<float> zn{from_partner_lane(pz.x), from_partner_lane(pz.y)};
+            if (n == 0) zn = z[(16 - k) & 15];
+            xa[h][k] = cx<float>{0.5f * (z[k].x + zn.x), 0.5f * (z[k].y - zn.y)};
+            xb[h][k] = cx<float>{0.5f * (z[k].y + zn.y), 0.5f * (zn.x - z[k].x)};
+        }
+    }
+
+    // 2. the band through LDS, real parts then imaginary parts; the column
+    //    lanes gather their inputs after each
+    cx<float> cv[G::ROUNDS][16];
+#pragma unroll
+    for (int part = 0; part < 2; ++part) {
+        if (part) lds_barrier();  // every column lane has read the real parts
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int k = 0; k < KH; ++k) {
+                const int kx = n + 16 * k;
+                if (kx < KC) {
+                    float* col = yb + kx * YP + 128 * h + 2 * f;
+                    col[0] = part ? xa[h][k].y : xa[h][k].x;
+                    col[1] = part ? xb[h][k].y : xb[h][k].x;
+                }
+            }
+        lds_barrier();
+#pragma unroll
+        for (int r = 0; r < G::ROUNDS; ++r) {
+            const int c = 64 * r + 4 * wave + g;
+            if (64 * r + 4 * wave < KC) {
+                const float* col = yb + (c < KC ? c : KC - 1) * YP + n;
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    if (part) cv[r][j].y = col[16 * j];
+                    else cv[r][j].x = col[16 * j];
+                }
+            }
+        }
+    }
+    lds_barrier();  // the band is consumed; the region becomes the output staging
+
+    // 3. column transforms and the retained outputs
+#pragma unroll
+    for (int r = 0; r < G::ROUNDS; ++r) {
+        const int c = 64 * r + 4 * wave + g;
+        if (64 * r + 4 * wave < KC) {
+            fft256<false>(cv[r], wt, tw, g, n);
+            uint32_t code[8];
+            load_codes(code, fcodes, (c < KC ? c : KC - 1) * 16 + n);
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int e = unpack_code(code, k);
+                if (e != 0xFFFF && c < KC) {
+                    const cx<float> x = cv[r][k];
+                    st[e >> 1] = cx<float>{x.x, (e & 1) ? -x.y : x.y};
+                }
+            }
+        }
+    }
+    lds_barrier();
+
+    // 4. copy out (16-byte pairs; the pair past an odd count writes the row's padding)
+    {
+        const __amdgpu_buffer_rsrc_t rs =
+            buffer_rsrc(pm + static_cast<size_t>(np) * p, static_cast<uint32_t>(np) * 8u);
+        const int pairs = (modes + 1) >> 1;
+        for (int i = tid; i < pairs; i += kPlaneThreads)
+            raw_buffer_store_f4(*reinterpret_cast<const bf4*>(&st[2 * i]), rs, 16 * i, 0, 0);
+    }
+}
+
+template <int KC>
+__global__ __launch_bounds__(kPlaneThreads) void plane_forward_kernel(const float* __restrict__ real,
+                                                                     cx<float>* __restrict__ pm,
+                                                                     const cx<float>* __restrict__ twg,
+                                                                     const uint32_t* __restrict__ fcodes, int modes,
+                                                                     int np) {
+    __shared__ __align__(16) unsigned char lds[PlaneGeo<KC>::BYTES];
+    forward_plane<KC, false>(lds, real, pm, twg, fcodes, modes, np, static_cast<int>(blockIdx.x));
+}
+
+// ---------------------------------------------------------------------------
+// The retained operator's two transforms of plane blockIdx.x in one
+// workgroup: plane-major values -> inverse -> real plane (written to memory)
+// -> read back -> forward -> plane-major values (may alias the input: each
+// workgroup reads its plane's row of pm in full before it writes it).
+// ---------------------------------------------------------------------------
+template <int KC>
+__global__ __launch_bounds__(kPlaneThreads) void plane_operator_kernel(const cx<float>* pm_in,
+                                                                      cx<float>* pm_out,
+                                                                      float* __restrict__ real,
+                                                                      const cx<float>* __restrict__ twg,
+                                                                      const uint32_t* __restrict__ icodes,
+                                                                      const uint32_t* __restrict__ fcodes, int modes,
+                                                                      int np) {
+    __shared__ __align__(16) unsigned char lds[PlaneGeo<KC>::BYTES];
+    const int p = static_cast<int>(blockIdx.x);
+    inverse_plane<KC>(lds, pm_in, real, twg, icodes, modes, np, p);
+    forward_plane<KC, true>(lds, real, pm_out, twg, fcodes, modes, np, p);
+}
+
+// ---------------------------------------------------------------------------
+// Compact (plane + P*mode) <-> plane-major (mode + np*plane).  Workgroup b
+// moves modes [16b, 16b + 16) of every plane through an LDS tile [P][17].
+// ---------------------------------------------------------------------------
+template <bool TO_PM>
+__global__ __launch_bounds__(256) void pm_transpose_kernel(const cx<float>* __restrict__ in,
+                                                           cx<float>* __restrict__ out, int modes, int planes,
+                                                           int np) {
+    // 16-byte accesses (two complex values) on both sides: the compact run of
+    // a tile starts at 16 * P * 8 bytes (a multiple of 128) and its length
+    // nm * P is odd only for a last partial tile (one single tail element);
+    // a plane-major row segment of 16 values is 8 pairs.
+    extern __shared__ __align__(16) unsigned char pm_tile_lds[];
+    cx<float>* tile = reinterpret_cast<cx<float>*>(pm_tile_lds);
+    const int m0 = 16 * static_cast<int>(blockIdx.x);
+    const int nm = modes - m0 < 16 ? modes - m0 : 16;
+    const int tid = static_cast<int>(threadIdx.x);
+    const int run = nm * planes;  // compact elements of the tile
+    if constexpr (TO_PM) {
+        const bf4* src = reinterpret_cast<const bf4*>(in + static_cast<size_t>(m0) * planes);
+        for (int e2 = tid; 2 * e2 < run; e2 += 256) {
+            const int e = 2 * e2;
+            if (e + 1 < run) {
+                const bf4 q = src[e2];
+                const int m = e / planes, pl = e - m * planes;
+                tile[pl * 17 + m] = cx<float>{q.x, q.y};
+                const int m1 = pl + 1 == planes ? m + 1 : m, p1 = pl + 1 == planes ? 0 : pl + 1;
+                tile[p1 * 17 + m1] = cx<float>{q.z, q.w};
+            } else {
+                const int m = e / planes, pl = e - m * planes;
+                tile[pl * 17 + m] = in[static_cast<size_t>(m0) * planes + e];
+            }
+        }
+        lds_barrier();
+        for (int e = tid; e < 8 * planes; e += 256) {
+            const int mp = e & 7, pl = e >> 3, m = 2 * mp;
+            const cx<float> a = tile[pl * 17 + m], b = tile[pl * 17 + m + 1];
+            cx<float>* dst = out + static_cast<size_t>(np) * pl + m0 + m;
+            if (m + 1 < nm) *reinterpret_cast<bf4*>(dst) = bf4{a.x, a.y, b.x, b.y};
+            else if (m < nm) *dst = a;
+        }
+    } else {
+        for (int e = tid; e < 8 * planes; e += 256) {
+            const int mp = e & 7, pl = e >> 3, m = 2 * mp;
+            const cx<float>* srcp = in + static_cast<size_t>(np) * pl + m0 + m;
+            if (m + 1 < nm) {
+                const bf4 q = *reinterpret_cast<const bf4*>(srcp);
+                tile[pl * 17 + m] = cx<float>{q.x, q.y};
+                tile[pl * 17 + m + 1] = cx<float>{q.z, q.w};
+            } else if (m < nm) {
+                tile[pl * 17 + m] = *srcp;
+            }
+        }
+        lds_barrier();
+        bf4* dst = reinterpret_cast<bf4*>(out + static_cast<size_t>(m0) * planes);
+        for (int e2 = tid; 2 * e2 < run; e2 += 256) {
+            const int e = 2 * e2;
+            const int m = e / planes, pl = e - m * planes;
+            const cx<float> a = tile[pl * 17 + m];
+            if (e + 1 < run) {
+                const int m1 = pl + 1 == planes ? m + 1 : m, p1 = pl + 1 == planes ? 0 : pl + 1;
+                const cx<float> b = tile[p1 * 17 + m1];
+                dst[e2] = bf4{a.x, a.y, b.x, b.y};
+            } else {
+                out[static_cast<size_t>(m0) * planes + e] = a;
+            }
+        }
+    }
+}
+
+// The same move with 8-byte accesses, for compact arrays that are not 16-byte
+// aligned.
+template <bool TO_PM>
+__global__ __launch_bounds__(256) void pm_transpose_scalar_kernel(const cx<float>* __restrict__ in,
+                                                                  cx<float>* __restrict__ out, int modes,
+                                                                  int planes, int np) {
+    extern __shared__ __align__(16) unsigned char pm_tile_lds[];
+    cx<float>* tile = reinterpret_cast<cx<float>*>(pm_tile_lds);
+    const int m0 = 16 * static_cast<int>(blockIdx.x);
+    const int nm = modes - m0 < 16 ? modes - m0 : 16;
+    const int tid = static_cast<int>(threadIdx.x);
+    if constexpr (TO_PM) {
+        const cx<float>* src = in + static_cast<size_t>(m0) * planes;
+        for (int e = tid; e < nm * planes; e += 256) {
+            const int m = e / planes, pl = e - m * planes;
+            tile[pl * 17 + m] = src[e];
+        }
+        lds_barrier();
+        for (int e = tid; e < 16 * planes; e += 256) {
+            const int m = e & 15, pl = e >> 4;
+            if (m < nm) out[static_cast<size_t>(np) * pl + m0 + m] = tile[pl * 17 + m];
+        }
+    } else {
+        for (int e = tid; e < 16 * planes; e += 256) {
+            const int m = e & 15, pl = e >> 4;
+            if (m < nm) tile[pl * 17 + m] = in[static_cast<size_t>(np) * pl + m0 + m];
+        }
+        lds_barrier();
+        cx<float>* dst = out + static_cast<size_t>(m0) * planes;
+        for (int e = tid; e < nm * planes; e += 256) {
+            const int m = e / planes, pl = e - m * planes;
+            dst[e] = tile[pl * 17 + m];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+bool plane_supported(const skb_workload& w, uint64_t kc, uint64_t modes, int precision) {
+    const uint64_t planes = w.nz * w.fields;
+    return precision == SKB_F32 && w.nx == kPlaneN && w.ny == kPlaneN && kc == kPlaneKc &&
+           modes <= static_cast<uint64_t>(kPlaneMaxModes) && modes > 0 && planes > 0 &&
+           planes <= static_cast<uint64_t>(kPlaneMaxPlanes) &&
+           plane_np(modes) * planes * 8ull < (1ull << 31);
+}
+
+uint64_t plane_np(uint64_t modes) { return (modes + 15) / 16 * 16; }
+
+void PlaneTables::build(const skb_workload& w, uint64_t kc_, uint64_t modes_, const std::vector<int>& fmap,
+                        const std::vector<int>& imap) {
+    if (!plane_supported(w, kc_, modes_, SKB_F32)) fail(SKB_UNSUPPORTED, "plane-resident transforms: unsupported shape");
+    modes = static_cast<int>(modes_);
+    np = static_cast<int>(plane_np(modes_));
+    const int N = kPlaneN, KC = kPlaneKc;
+    // per (column c, lane n) 16 codes, two per 32-bit word
+    std::vector<uint16_t> f(static_cast<size_t>(KC) * 16 * 16), i(f.size());
+    auto pack = [](int code) { return static_cast<uint16_t>(code < 0 ? 0xFFFF : code); };
+    for (int c = 0; c < KC; ++c)
+        for (int n = 0; n < 16; ++n)
+            for (int k = 0; k < 16; ++k) {
+                f[(static_cast<size_t>(c) * 16 + n) * 16 + k] = pack(fmap[static_cast<size_t>(c) * N + n + 16 * k]);
+                i[(static_cast<size_t>(c) * 16 + n) * 16 + k] = pack(imap[static_cast<size_t>(c) * N + 16 * k + n]);
+            }
+    fcodes.upload(f.data(), f.size() * sizeof(uint16_t));
+    icodes.upload(i.data(), i.size() * sizeof(uint16_t));
+    std::vector<cx<float>> t(N);
+    for (int k = 0; k < N; ++k) {
+        const double a = 2.0 * M_PI * k / N;
+        t[k] = cx<float>{static_cast<float>(std::cos(a)), static_cast<float>(-std::sin(a))};
+    }
+    tw.upload(t.data(), t.size() * sizeof(t[0]));
+}
+
+void plane_inverse(const PlaneTables& t, const void* pm, void* real, int planes, hipStream_t s) {
+    hipLaunchKernelGGL((plane_inverse_kernel<kPlaneKc>), dim3(planes), dim3(kPlaneThreads), 0, s,
+                       static_cast<const cx<float>*>(pm), static_cast<float*>(real), t.tw.as<cx<float>>(),
+                       t.icodes.as<uint32_t>(), t.modes, t.np);
+    check_hip(hipGetLastError(), "plane inverse launch");
+}
+
+void plane_forward(const PlaneTables& t, const void* real, void* pm, int planes, hipStream_t s) {
+    hipLaunchKernelGGL((plane_forward_kernel<kPlaneKc>), dim3(planes), dim3(kPlaneThreads), 0, s,
+                       static_cast<const float*>(real), static_cast<cx<float>*>(pm), t.tw.as<cx<float>>(),
+                       t.fcodes.as<uint32_t>(), t.modes, t.np);
+    check_hip(hipGetLastError(), "plane forward launch");
+}
+
+void plane_operator(const PlaneTables& t, const void* pm_in, void* pm_out, void* real, int planes, hipStream_t s) {
+    hipLaunchKernelGGL((plane_operator_kernel<kPlaneKc>), dim3(planes), dim3(kPlaneThreads), 0, s,
+                       static_cast<const cx<float>*>(pm_in), static_cast<cx<float>*>(pm_out),
+                       static_cast<float*>(real), t.tw.as<cx<float>>(), t.icodes.as<uint32_t>(),
+                       t.fcodes.as<uint32_t>(), t.modes, t.np);
+    check_hip(hipGetLastError(), "plane operator launch");
+}
+
+void pm_transpose(bool to_pm, const void* in, void* out, int modes, int planes, int np, hipStream_t s) {
+    const dim3 grid((modes + 15) / 16);
+    const size_t lds = static_cast<size_t>(planes) * 17 * sizeof(cx<float>);
+    const auto* i = static_cast<const cx<float>*>(in);
+    auto* o = static_cast<cx<float>*>(out);
+    const bool wide = (reinterpret_cast<uintptr_t>(in) % 16 == 0) && (reinterpret_cast<uintptr_t>(out) % 16 == 0);
+    if (wide && to_pm) hipLaunchKernelGGL(pm_transpose_kernel<true>, grid, dim3(256), lds, s, i, o, modes, planes, np);
+    else if (wide) hipLaunchKernelGGL(pm_transpose_kernel<false>, grid, dim3(256), lds, s, i, o, modes, planes, np);
+    else if (to_pm) hipLaunchKernelGGL(pm_transpose_scalar_kernel<true>, grid, dim3(256), lds, s, i, o, modes, planes, np);
+    else hipLaunchKernelGGL(pm_transpose_scalar_kernel<false>, grid, dim3(256), lds, s, i, o, modes, planes, np);
+    check_hip(hipGetLastError(), "plane-major transpose launch");
+}
+
+}  // namespace skb

@@ -1,0 +1,42 @@
+// Plane-resident transforms of the retained operator (plane.hip): shapes,
+// per-plan device tables and launchers, shared with the plan code of fft.hip
+// and the operator of operator.hip.
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+
+namespace skb {
+
+// The one shape the plane-resident kernels are built for: 256 x 256 planes,
+// active k_x band 86 (the 2/3 rule), FP32.
+constexpr int kPlaneKc = 86;
+constexpr int kPlaneMaxModes = 11440;  // retained values staged in one workgroup's LDS
+constexpr int kPlaneMaxPlanes = 480;   // LDS tile of the layout transpose
+
+// Plane stride of the plane-major retained layout: modes rounded up to 16.
+uint64_t plane_np(uint64_t modes);
+bool plane_supported(const skb_workload& w, uint64_t kc, uint64_t modes, int precision);
+
+// Device tables of one mode set: twiddles and the packed 16-bit column codes
+// (2*mode + conj per (kx, ky) slot, 0xFFFF where empty) in the order each
+// kernel's lanes read them.  fmap / imap: dense [kx * Ny + ky] -> 2*mode +
+// conj or -1, forward (stored modes) and inverse (k_x = 0 partners added).
+struct PlaneTables {
+    DeviceBuffer tw, fcodes, icodes;
+    int modes = 0, np = 0;
+    void build(const skb_workload& w, uint64_t kc, uint64_t modes, const std::vector<int>& fmap,
+               const std::vector<int>& imap);
+};
+
+// pm: plane-major retained values (mode + np * plane), real: x + Nx(y + Ny*plane).
+void plane_inverse(const PlaneTables& t, const void* pm, void* real, int planes, hipStream_t s);
+void plane_forward(const PlaneTables& t, const void* real, void* pm, int planes, hipStream_t s);
+// Both transforms of each plane in one workgroup: pm_in -> real planes ->
+// pm_out (pm_out may equal pm_in).
+void plane_operator(const PlaneTables& t, const void* pm_in, void* pm_out, void* real, int planes, hipStream_t s);
+// compact (plane + P*mode) -> plane-major (to_pm) or back.
+void pm_transpose(bool to_pm, const void* in, void* out, int modes, int planes, int np, hipStream_t s);
+
+}  // namespace skb

@@ -122,6 +122,11 @@ struct GemmArgs {
     // [group][16*MB].
     const uint16_t* a3;
     const float* atail;
+    // plane-major input (split kernel, PM = true): element (mode, z, field) at
+    // in[mode + pm_np * (z + pm_nz * field)]; a tile is up to 16 consecutive
+    // modes of one group and one field (GemmTile::pad = field), written to the
+    // modal layout out + out_ld * (field + F * mode).
+    int pm_np, pm_nz;
 };
 
 template <typename R, int KS, bool WVM>
@@ -469,7 +474,7 @@ __device__ __forceinline__ uint32_t bf16_pair(float lo, float hi) {
     return __builtin_bit_cast(uint32_t, v);
 }
 
-template <int NK, int M>
+template <int NK, int M, bool PM>
 __global__ __launch_bounds__(256) void grouped_gemm_split_kernel(const GemmArgs<float> g) {
     using A4 = floatx4;
     constexpr int KS = (32 * NK + 1 + 3) / 4;  // FP32 copy: K rounded up to 4
@@ -504,22 +509,44 @@ __global__ __launch_bounds__(256) void grouped_gemm_split_kernel(const GemmArgs<
         cx<float> tail;
     };
     auto load_b = [&](BRegs& bv, const GemmTile& t) {
-        const cx<float>* base = g.in + static_cast<size_t>(g.in_ld) * t.col_begin;
-        const uint32_t bytes = static_cast<uint32_t>(in_step * (t.ncols - 1) + g.K) * CB;
-        const __amdgpu_buffer_rsrc_t r = buffer_rsrc(base, bytes);
+        if constexpr (PM) {
+            // column j = e % 16 of the tile, rows z, z + 1 with z = 2 (e / 16):
+            // 16 lanes read one 128-byte run of consecutive modes per row
+            const cx<float>* base =
+                g.in + static_cast<size_t>(g.pm_np) * static_cast<size_t>(g.pm_nz * t.pad) + t.col_begin;
+            const uint32_t np = static_cast<uint32_t>(g.pm_np);
+            const uint32_t bytes = (np * static_cast<uint32_t>(g.K - 1) + static_cast<uint32_t>(t.ncols)) * CB;
+            const __amdgpu_buffer_rsrc_t r = buffer_rsrc(base, bytes);
 #pragma unroll
-        for (int n = 0; n < NK; ++n) {
-            const int e = tid + 256 * n, c = e / (KB / 2), z = 2 * (e - c * (KB / 2));
-            bv.pair[n] = raw_buffer_load_f4(r, static_cast<int>(static_cast<uint32_t>(in_step * c + z) * CB), 0, 0);
+            for (int n = 0; n < NK; ++n) {
+                const int e = tid + 256 * n, c = e & 15, z = 2 * (e >> 4);
+                const bool ok = c < t.ncols;
+                const cx<float> lo = buffer_load_cx<float>(r, ok ? (np * z + c) * CB : kBufferPast);
+                const cx<float> hi = buffer_load_cx<float>(r, ok ? (np * (z + 1) + c) * CB : kBufferPast);
+                bv.pair[n] = bf4{lo.x, lo.y, hi.x, hi.y};
+            }
+            bv.tail = buffer_load_cx<float>(r, tid < t.ncols ? (np * KB + tid) * CB : kBufferPast);
+        } else {
+            const cx<float>* base = g.in + static_cast<size_t>(g.in_ld) * t.col_begin;
+            const uint32_t bytes = static_cast<uint32_t>(in_step * (t.ncols - 1) + g.K) * CB;
+            const __amdgpu_buffer_rsrc_t r = buffer_rsrc(base, bytes);
+#pragma unroll
+            for (int n = 0; n < NK; ++n) {
+                const int e = tid + 256 * n, c = e / (KB / 2), z = 2 * (e - c * (KB / 2));
+                bv.pair[n] =
+                    raw_buffer_load_f4(r, static_cast<int>(static_cast<uint32_t>(in_step * c + z) * CB), 0, 0);
+            }
+            bv.tail = buffer_load_cx<float>(r, tid < 16 ? static_cast<uint32_t>(in_step * tid + KB) * CB : kBufferPast);
         }
-        bv.tail = buffer_load_cx<float>(r, tid < 16 ? static_cast<uint32_t>(in_step * tid + KB) * CB : kBufferPast);
     };
     auto put_b = [&](const BRegs& bv, int buf) {
         cx<float>* sb = sbb[buf];
         uint16_t(*sh)[16 * BS] = shb[buf];
 #pragma unroll
         for (int n = 0; n < NK; ++n) {
-            const int e = tid + 256 * n, c = e / (KB / 2), z = 2 * (e - c * (KB / 2));
+            const int e = tid + 256 * n;
+            const int c = PM ? (e & 15) : e / (KB / 2);
+            const int z = PM ? 2 * (e >> 4) : 2 * (e - c * (KB / 2));
             const bf4 x = bv.pair[n];  // Re z, Im z, Re z+1, Im z+1
             *reinterpret_cast<bf4*>(&sb[c * CS + z]) = x;
             const uint32_t hr = bf16_pair(x.x, x.z), hi = bf16_pair(x.y, x.w);
@@ -654,10 +681,11 @@ __global__ __launch_bounds__(256) void grouped_gemm_split_kernel(const GemmArgs<
         {
             // results straight from the accumulators: lane (j, q) holds rows
             // 4q .. 4q+3 of column j in each block, two 16-byte row pairs
-            const int ld = g.out_ld;
+            const int ld = PM ? g.out_ld * g.F : g.out_ld;
+            cx<float>* obase = PM ? g.out + static_cast<size_t>(g.out_ld) * (cur.pad + static_cast<size_t>(g.F) * cur.col_begin)
+                                  : g.out + static_cast<size_t>(ld) * cur.col_begin;
             const __amdgpu_buffer_rsrc_t r =
-                buffer_rsrc(g.out + static_cast<size_t>(ld) * cur.col_begin,
-                            static_cast<uint32_t>(ld * (cur.ncols - 1) + M) * CB);
+                buffer_rsrc(obase, static_cast<uint32_t>(ld * (cur.ncols - 1) + M) * CB);
             const uint32_t col = static_cast<uint32_t>(ld * j) * CB;
 #pragma unroll
             for (int sl = 0; sl < SL; ++sl) {
@@ -980,10 +1008,10 @@ void launch_gemm_fixed(const GemmArgs<R>& a, int req, hipStream_t s) {
 // Output heights of the fixed-height kernel: the (K, M) pairs of the vertical
 // families at Nz = 33, 65, 129 (M = K = Nz for square families, Nj x Nz
 // forward and Nz x Nj inverse for the truncated ones); FP64 up to Nz = 65.
-template <int NK, int M>
+template <int NK, int M, bool PM = false>
 void launch_gemm_split(const GemmArgs<float>& a, int req, hipStream_t s) {
     const int grid = std::min(a.ntiles, req > 0 ? req : kStreamGrid);
-    hipLaunchKernelGGL((grouped_gemm_split_kernel<NK, M>), dim3(grid), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((grouped_gemm_split_kernel<NK, M, PM>), dim3(grid), dim3(256), 0, s, a);
     check_hip(hipGetLastError(), "vertical GEMM launch");
 }
 
@@ -1164,6 +1192,10 @@ struct skb_vertical_plan {
     bool wvm = false;
     uint64_t modes = 0;
     skb::DeviceBuffer qmap, partner;
+    // Tiles over a plane-major input: up to 16 consecutive modes of one group
+    // and one field (GemmTile::pad = field), group by group.
+    int pm_ntiles = 0;
+    skb::DeviceBuffer pm_tiles;
 };
 
 using namespace skb;
@@ -1228,6 +1260,15 @@ std::unique_ptr<skb_vertical_plan> make_plan(const skb_workload* workload, uint6
     p->fwd.build(forward, group_count, inj, inz, precision);
     p->inv.build(inverse, group_count, inz, inj, precision);
     p->tiles.upload(tiles.data(), tiles.size() * sizeof(GemmTile));
+    std::vector<GemmTile> pm;
+    for (uint64_t g = 0; g < group_count; ++g)
+        for (uint64_t f = 0; f < fields; ++f)
+            for (uint64_t m = 0; m < groups[g].mode_count; m += 16)
+                pm.push_back(GemmTile{static_cast<int>(g), static_cast<int>(groups[g].first_mode + m),
+                                      static_cast<int>(std::min<uint64_t>(16, groups[g].mode_count - m)),
+                                      static_cast<int>(f)});
+    p->pm_ntiles = static_cast<int>(pm.size());
+    p->pm_tiles.upload(pm.data(), pm.size() * sizeof(GemmTile));
     p->matrix_bytes = nj * nz * group_count * (precision == SKB_F64 ? 8 : 4);
     return p;
 }
@@ -1297,6 +1338,32 @@ void vertical_forward_columns(const skb_vertical_plan* p, const void* in, int in
         auto a = args<float>(p, p->fwd, static_cast<const cx<float>*>(in) + shift, out);
         a.in_ld = p->fwd.K * in_cs;
         launch_gemm<float, false>(a, p->grid, s);
+    }
+}
+
+// Forward product from the plane-major layout (element (mode, z, field) at
+// pm[mode + np * (z + Nz * field)]) into the plan's modal layout: the
+// split-BF16 kernel of square FP32 plans (Nz = 33, 65, 129).
+bool vertical_pm_supported(const skb_vertical_plan* p) {
+    if (!p || p->wvm || p->precision != SKB_F32 || p->products != SKB_PRODUCTS_SPLIT_BF16 || !p->fwd.a3.ptr)
+        return false;
+    const int M = p->fwd.M, K = p->fwd.K;
+    return M == K && (K == 33 || K == 65 || K == 129);
+}
+
+void vertical_forward_pm(const skb_vertical_plan* p, const void* pm, int np, void* out, void* stream) {
+    if (!vertical_pm_supported(p)) fail(SKB_UNSUPPORTED, "plane-major vertical product: unsupported plan");
+    if (!pm || !out || np < static_cast<int>(p->modes)) fail(SKB_INVALID_ARGUMENT, "bad plane-major operand");
+    auto a = args<float>(p, p->fwd, pm, out);
+    a.tiles = p->pm_tiles.as<GemmTile>();
+    a.ntiles = p->pm_ntiles;
+    a.pm_np = np;
+    a.pm_nz = static_cast<int>(p->w.nz);
+    const hipStream_t s = as_stream(stream);
+    switch (a.K) {
+        case 65: launch_gemm_split<2, 65, true>(a, p->grid, s); break;
+        case 33: launch_gemm_split<1, 33, true>(a, p->grid, s); break;
+        default: launch_gemm_split<4, 129, true>(a, p->grid, s); break;
     }
 }
 

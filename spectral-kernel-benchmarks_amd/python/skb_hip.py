@@ -9,6 +9,8 @@ Class names mirror the reference providers they replace:
   FftPlan       <- skbench::FFTWProvider      (include/skbench/skbench.hpp:364-421)
   PrunedPlan    <- skbench::FFTWPrunedProvider (include/skbench/skbench.hpp:467-520)
   VerticalPlan  <- skbench::VerticalGemmProvider (include/skbench/skbench.hpp:152-218)
+  OperatorPlan  <- the inverse -> forward -> vertical composition the
+                   reference's harness times as separate provider calls
   pointwise_advection <- PointwiseAdvectionExecutor::execute (src/pointwise_advection.hpp:18-43)
 """
 from __future__ import annotations
@@ -180,6 +182,18 @@ def _load():
         "skb_type1_execute": (i32, [vp, i32, u64, u64, u64, vp, vp]),
         "skb_type1_vertical_schedule": (i32, [vp, vp, vp, vp]),
         "skb_type1_fill": (i32, [u64, u64, u64, vp, u64, vp]),
+        "skb_pruned_plan_set_path": (i32, [vp, i32]),
+        "skb_pruned_plan_path": (i32, [vp]),
+        "skb_retained_operator_create": (i32, [W, vp, u64, u64, vp, u64, vp, vp, i32, pp]),
+        "skb_retained_operator_destroy": (i32, [vp]),
+        "skb_retained_operator_execute": (i32, [vp, vp, vp, vp]),
+        "skb_retained_operator_stage": (i32, [vp, i32, vp, vp, vp]),
+        "skb_retained_operator_set_path": (i32, [vp, i32]),
+        "skb_retained_operator_path": (i32, [vp]),
+        "skb_retained_operator_real": (vp, [vp]),
+        "skb_retained_operator_device_bytes": (u64, [vp]),
+        "skb_retained_operator_set_fused": (i32, [vp, i32]),
+        "skb_retained_operator_fused": (i32, [vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -213,7 +227,11 @@ EXPORTED = [
     "skb_spectral_pipeline_destroy", "skb_spectral_pipeline_execute", "skb_spectral_pipeline_stages",
     "skb_spectral_pipeline_set_fused", "skb_spectral_pipeline_fused", "skb_spectral_pipeline_device_bytes",
     "skb_type1_matrix", "skb_type1_plan_create", "skb_type1_plan_destroy", "skb_type1_execute",
-    "skb_type1_vertical_schedule", "skb_type1_fill",
+    "skb_type1_vertical_schedule", "skb_type1_fill", "skb_pruned_plan_set_path", "skb_pruned_plan_path",
+    "skb_retained_operator_create", "skb_retained_operator_destroy", "skb_retained_operator_execute",
+    "skb_retained_operator_stage", "skb_retained_operator_set_path", "skb_retained_operator_path",
+    "skb_retained_operator_real", "skb_retained_operator_device_bytes", "skb_retained_operator_set_fused",
+    "skb_retained_operator_fused",
 ]
 
 
@@ -286,6 +304,8 @@ SKB_PIPELINE_WVM_DIRECT = 0
 SKB_PIPELINE_COMPACT = 1
 SKB_PRODUCTS_FP32 = 0
 SKB_PRODUCTS_SPLIT_BF16 = 1
+SKB_PATH_AUTO, SKB_PATH_TWO_PASS, SKB_PATH_PLANE = 0, 1, 2
+SKB_OP_STAGE_LAYOUT, SKB_OP_STAGE_INVERSE, SKB_OP_STAGE_FORWARD, SKB_OP_STAGE_VERTICAL = 0, 1, 2, 3
 
 
 def synthetic_modal_weights(w: "Workload", modes: np.ndarray) -> np.ndarray:
@@ -400,6 +420,24 @@ def _dev(t, numel: int, dtype, name: str):
     return ctypes.c_void_p(t.data_ptr())
 
 
+class _DevicePointer:
+    """A device allocation owned by a plan, shown to torch through the
+    __cuda_array_interface__ protocol (no copy; `owner` keeps it alive)."""
+
+    def __init__(self, ptr: int, numel: int, typestr: str, owner):
+        self.owner = owner
+        self.__cuda_array_interface__ = {"shape": (numel,), "typestr": typestr, "data": (ptr, False),
+                                         "version": 2, "strides": None}
+
+
+def _device_view(ptr, numel: int, dtype, owner):
+    torch = _torch()
+    if not ptr:
+        raise LogicError(3, "null device pointer")
+    typestr = {torch.float32: "<f4", torch.float64: "<f8"}[dtype]
+    return torch.as_tensor(_DevicePointer(int(ptr), numel, typestr, owner), device="cuda")
+
+
 def _stream(stream):
     torch = _torch()
     s = stream if stream is not None else torch.cuda.current_stream()
@@ -410,14 +448,19 @@ class Bound:
     """One C-ABI call with its arguments validated and converted once: calling
     the object launches it again on the same buffers and stream, with no
     per-call checks on the host (the serving loop's form of a stage).  The
-    tensors are kept alive with the object."""
+    tensors are kept alive with the object; the last entry of `keep` is the
+    plan that owns the handle, and a call after that plan was closed raises
+    instead of passing a freed handle to the library."""
 
-    __slots__ = ("fn", "args", "keep")
+    __slots__ = ("fn", "args", "keep", "owner")
 
     def __init__(self, fn, args, keep):
         self.fn, self.args, self.keep = fn, args, keep
+        self.owner = keep[-1]
 
     def __call__(self):
+        if self.owner.h is None:
+            raise LogicError(3, "the plan of this bound call was closed")
         status = self.fn(*self.args)
         if status:
             check(status)
@@ -591,6 +634,14 @@ class PrunedPlan:
         """Launch shape of one stage (0 = one workgroup per work item)."""
         check(lib.skb_pruned_plan_set_grid(self.h, stage, workgroups))
 
+    def set_path(self, path: int):
+        """SKB_PATH_AUTO (default), SKB_PATH_TWO_PASS or SKB_PATH_PLANE."""
+        check(lib.skb_pruned_plan_set_path(self.h, path))
+
+    def path(self) -> int:
+        """The path forward / inverse take: SKB_PATH_PLANE or SKB_PATH_TWO_PASS."""
+        return int(lib.skb_pruned_plan_path(self.h))
+
     def items(self, stage: int) -> int:
         return int(lib.skb_pruned_plan_items(self.h, stage))
 
@@ -663,6 +714,85 @@ class VerticalPlan:
     def close(self):
         if getattr(self, "h", None) and lib is not None:
             lib.skb_vertical_plan_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+
+class OperatorPlan:
+    """One retained-operator evaluation per call (skb_retained_operator):
+    pruned inverse c2r of the compact retained spectrum into real planes,
+    forward r2c back to the retained modes, K^2-grouped vertical forward
+    product into the modal layout j + Nj*(field + F*mode)."""
+
+    STAGES = (SKB_OP_STAGE_LAYOUT, SKB_OP_STAGE_INVERSE, SKB_OP_STAGE_FORWARD, SKB_OP_STAGE_VERTICAL)
+
+    def __init__(self, w: Workload, modes: np.ndarray, nj: int, groups: np.ndarray, forward: np.ndarray,
+                 inverse: np.ndarray, precision: int = SKB_F32):
+        self.w, self.nj, self.precision, self.modes = w, nj, precision, modes
+        self._wc = w.c()
+        forward = np.ascontiguousarray(forward, np.float64)
+        inverse = np.ascontiguousarray(inverse, np.float64)
+        if forward.size != len(groups) * nj * w.nz or inverse.size != forward.size:
+            raise ValueError("matrix family size does not match groups x nj x nz")
+        h = ctypes.c_void_p()
+        check(lib.skb_retained_operator_create(ctypes.byref(self._wc), _np(modes), len(modes), nj, _np(groups),
+                                               len(groups), _np(forward), _np(inverse), precision, ctypes.byref(h)))
+        self.h = h
+
+    @property
+    def retained_elements(self):
+        return len(self.modes) * self.w.planes
+
+    @property
+    def modal_elements(self):
+        return len(self.modes) * self.w.fields * self.nj
+
+    def _args(self, retained, modal):
+        cd = complex_dtype(self.precision)
+        return (_dev(retained, self.retained_elements, cd, "retained"),
+                _dev(modal, self.modal_elements, cd, "modal"))
+
+    def execute(self, retained, modal, stream=None):
+        r, m = self._args(retained, modal)
+        check(lib.skb_retained_operator_execute(self.h, r, m, _stream(stream)))
+
+    def bind_execute(self, retained, modal, stream=None) -> Bound:
+        """A prepared evaluation (see Bound): one C-ABI call per evaluation."""
+        r, m = self._args(retained, modal)
+        return Bound(lib.skb_retained_operator_execute, (self.h, r, m, _stream(stream)), (retained, modal, self))
+
+    def stage(self, stage: int, retained, modal, stream=None):
+        r, m = self._args(retained, modal)
+        check(lib.skb_retained_operator_stage(self.h, stage, r, m, _stream(stream)))
+
+    def bind_stage(self, stage: int, retained, modal, stream=None) -> Bound:
+        r, m = self._args(retained, modal)
+        return Bound(lib.skb_retained_operator_stage, (self.h, stage, r, m, _stream(stream)), (retained, modal, self))
+
+    def set_path(self, path: int):
+        check(lib.skb_retained_operator_set_path(self.h, path))
+
+    def path(self) -> int:
+        return int(lib.skb_retained_operator_path(self.h))
+
+    def set_fused(self, fused: bool):
+        check(lib.skb_retained_operator_set_fused(self.h, 1 if fused else 0))
+
+    def fused(self) -> bool:
+        return bool(lib.skb_retained_operator_fused(self.h))
+
+    def real(self):
+        """The operator's real planes (x + Nx(y + Ny*plane)) as a torch tensor view."""
+        ptr = lib.skb_retained_operator_real(self.h)
+        return _device_view(ptr, self.w.real_elements, real_dtype(self.precision), self)
+
+    def device_bytes(self) -> int:
+        return int(lib.skb_retained_operator_device_bytes(self.h))
+
+    def close(self):
+        if getattr(self, "h", None) and lib is not None:
+            lib.skb_retained_operator_destroy(self.h)
             self.h = None
 
     __del__ = close

@@ -76,7 +76,10 @@ def test_graph_replays_equal_eager(device, shape, precision):
 def test_operator_calls_equal_stage_launches(device, precision):
     """The evaluation as three operator calls (pruned inverse, pruned forward,
     vertical forward -- the provider calls of the reference's harness), bound
-    once and called again, equals the five single-stage launches bit for bit."""
+    once and called again, equals the five single-stage launches bit for bit.
+    The stage entry points are the two-pass kernels, so the calls are held to
+    that path here (the plane path is checked against them to rounding in
+    test_gpu_operator.py)."""
     import torch
 
     shape = (256, 256, 65, 3)
@@ -87,6 +90,7 @@ def test_operator_calls_equal_stage_launches(device, precision):
     fwd, inv = S.qr_gaussian_family(w.nz, len(groups), 129)
     cd, rd = S.complex_dtype(precision), S.real_dtype(precision)
     pr = S.PrunedPlan(w, modes, precision)
+    pr.set_path(S.SKB_PATH_TWO_PASS)
     vp = S.VerticalPlan(w, w.nz, groups, fwd, inv, precision)
     x = torch.from_numpy(S.spectral_fixture(w, modes, 129, -1.0)).to(device=device, dtype=cd)
     real = torch.empty(w.real_elements, dtype=rd, device=device)
