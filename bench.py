@@ -2,11 +2,12 @@
 """Retained-operator throughput of the hip provider on one MI355X.
 
 One step = one retained-operator evaluation of the 256^2 x Nz65 x 3 workload
-with inputs resident in HBM:
+with inputs resident in HBM, enqueued by ONE C-ABI call
+(skb_retained_operator_execute):
   1. pruned inverse c2r   compact retained (z, field, mode) -> real planes
   2. pruned forward r2c   real planes -> compact retained (2/3-rule truncated)
   3. K^2-grouped vertical GEMM on MFMA -> modal
-The headline runs in FP32 storage (FP32 transforms, FP32 MFMA vertical
+The headline runs in FP32 storage (FP32 transforms, split-BF16 MFMA vertical
 products); the plain FP64 GPU form is timed beside it and supplies the
 reference for the FP32 error.  The CPU column times the FP64 oracle
 (pocketfft + BLAS) on this machine's host cores on a bounded sample.
@@ -53,12 +54,10 @@ def parse():
 
 
 COPY_BYTES = 256 << 20
-KERNEL_NAMES = {  # substring of the device kernel name -> stage
-    "col2_kernel<float, 256, true, 1, 0, false>": "inverse_columns",
-    "row2_c2r_kernel<float, 256, false>": "inverse_rows",
-    "row2_r2c_kernel<float, 256, false>": "forward_rows",
-    "col2_kernel<float, 256, false, 0, 1, false>": "forward_columns",
-    "grouped_gemm_split_kernel<2, 65>": "vertical_gemm",
+KERNEL_NAMES = {  # substring of the device kernel name -> stage of the evaluation
+    "pm_transpose_kernel<true>": "layout",
+    "plane_operator_kernel<86>": "transforms",
+    "grouped_gemm_split_kernel<2, 65, true>": "vertical_gemm",
 }
 
 
@@ -113,73 +112,28 @@ def log(args, msg):
 
 
 class Operator:
-    """Buffers and plans of one retained-operator evaluation at one precision."""
+    """One retained-operator plan (skb_retained_operator: every kernel of an
+    evaluation enqueued by one C-ABI call) with its input and output."""
 
     def __init__(self, S, torch, w, modes, groups, nj, fwd, inv, x64, precision, device):
         self.S, self.torch, self.w, self.nj = S, torch, w, nj
-        self.pruned = S.PrunedPlan(w, modes, precision)
-        self.vert = S.VerticalPlan(w, nj, groups, fwd, inv, precision)
-        cd, rd = S.complex_dtype(precision), S.real_dtype(precision)
+        self.op = S.OperatorPlan(w, modes, nj, groups, fwd, inv, precision)
+        cd = S.complex_dtype(precision)
         self.x = torch.from_numpy(x64).to(device=device, dtype=cd)
-        self.real = torch.empty(w.real_elements, dtype=rd, device=device)
-        self.back = torch.empty(len(modes) * w.planes, dtype=cd, device=device)
-        self.modal = torch.empty(len(modes) * w.fields * nj, dtype=cd, device=device)
-        # per-call form (stream taken at each call, so it can be captured)
-        self.calls = [
-            lambda: self.pruned.inverse_columns(self.x),
-            lambda: self.pruned.inverse_rows(self.real),
-            lambda: self.pruned.forward_rows(self.real),
-            lambda: self.pruned.forward_columns(self.back),
-            lambda: self.vert.forward(self.back, self.modal),
-        ]
-        # one evaluation as the reference's three provider calls (pruned
-        # inverse, pruned forward, vertical forward), each prepared once:
-        # arguments checked at binding, one C-ABI call per operator
-        self.evaluation = [
-            self.pruned.bind_inverse(self.x, self.real),
-            self.pruned.bind_forward(self.real, self.back),
-            self.vert.bind_forward(self.back, self.modal),
-        ]
-        # the same five kernels as prepared single-stage launches, timed one
-        # stage at a time
-        self.stages = [
-            ("inverse_columns", self.pruned.bind_stage("inverse_columns", self.x)),
-            ("inverse_rows", self.pruned.bind_stage("inverse_rows", self.real)),
-            ("forward_rows", self.pruned.bind_stage("forward_rows", self.real)),
-            ("forward_columns", self.pruned.bind_stage("forward_columns", self.back)),
-            ("vertical_gemm", self.vert.bind_forward(self.back, self.modal)),
-        ]
+        self.modal = torch.empty(self.op.modal_elements, dtype=cd, device=device)
+        self.call = self.op.bind_execute(self.x, self.modal)
+        self.plane = self.op.path() == S.SKB_PATH_PLANE
+        # the kernels of one evaluation as separately launchable stages
+        names = ((("layout", S.SKB_OP_STAGE_LAYOUT), ("transforms", S.SKB_OP_STAGE_TRANSFORMS),
+                  ("vertical_gemm", S.SKB_OP_STAGE_VERTICAL)) if self.plane else
+                 (("transforms", S.SKB_OP_STAGE_TRANSFORMS), ("vertical_gemm", S.SKB_OP_STAGE_VERTICAL)))
+        self.stages = [(n, self.op.bind_stage(st, self.x, self.modal)) for n, st in names]
+        # the two transforms as separate kernels (the unfused form), for reference
+        self.unfused = [(n, self.op.bind_stage(st, self.x, self.modal))
+                        for n, st in (("inverse", S.SKB_OP_STAGE_INVERSE), ("forward", S.SKB_OP_STAGE_FORWARD))]
 
     def step(self):
-        for fn in self.evaluation:
-            fn()
-
-    def capture(self):
-        """Records one evaluation (the five kernel launches) into a hipGraph
-        (one launch call per evaluation).  The graph is checked against the
-        eager launches before it is timed: three replays, each after eager
-        work, must reproduce the eager output bit for bit."""
-        torch = self.torch
-        torch.cuda.synchronize()
-        self.step()
-        torch.cuda.synchronize()
-        ref = self.modal.clone()
-        self.graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(self.graph):
-            for fn in self.calls:
-                fn()
-        for _ in range(3):
-            self.modal.zero_()
-            self.step()
-            self.modal.zero_()
-            torch.cuda.synchronize()
-            self.graph.replay()
-            torch.cuda.synchronize()
-            if not torch.equal(self.modal, ref):
-                raise RuntimeError("graph replay differs from the eager evaluation")
-
-    def replay(self):
-        self.graph.replay()
+        self.call()
 
 
 def timed(torch, fn, steps):
@@ -191,37 +145,43 @@ def timed(torch, fn, steps):
     return time.perf_counter() - t0
 
 
-def instrumented(torch, op, steps, reps=50):
-    """Device times from HIP events recorded on the stream the kernels run on
-    (torch's current stream, which the prepared launches are bound to).
+LATENCY_EVALS = 1000
 
-    per_eval: one event pair around each evaluation (latency percentiles).
-    per_stage: for each kernel, `reps` back-to-back launches between one event
-    pair, averaged -- the kernel's launch duration without the event gaps that
-    per-kernel pairs inside an evaluation would add.  Every stage is
-    idempotent (it rewrites its output from its input), so repeating it alone
-    does the same work as inside the evaluation."""
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+
+def per_eval_latency(torch, op, evals):
+    """Device time of each evaluation: one HIP event pair per evaluation on
+    the stream the plan launches on, over `evals` back-to-back evaluations."""
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(evals)]
     torch.cuda.synchronize()
-    for s in range(steps):
-        ev[s][0].record()
+    for a, b in ev:
+        a.record()
         op.step()
-        ev[s][1].record()
+        b.record()
     torch.cuda.synchronize()
-    per_eval = [a.elapsed_time(b) for a, b in ev]
-    per_stage = {}
-    for name, fn in op.stages:
+    return [a.elapsed_time(b) for a, b in ev]
+
+
+def per_stage_times(torch, ops, launches, block=50):
+    """Average launch duration of every stage, HIP events around blocks of
+    `block` back-to-back launches on the plans' stream.  The launches rotate
+    over the plans in `ops` (each with its own buffers), so a stage's working
+    set exceeds the 256 MB Infinity Cache and the rates are HBM rates.  Every
+    stage rewrites its outputs from its inputs, so repeating one alone does
+    the work it does inside an evaluation."""
+    out = {}
+    for idx, (name, _) in enumerate(ops[0].stages + ops[0].unfused):
+        fns = [(op.stages + op.unfused)[idx][1] for op in ops]
         blocks = []
-        for _ in range(max(1, steps // reps)):
-            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        for b in range(max(1, launches // block)):
+            a, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             a.record()
-            for _ in range(reps):
-                fn()
-            b.record()
-            blocks.append((a, b))
+            for i in range(block):
+                fns[(b * block + i) % len(fns)]()
+            e.record()
+            blocks.append((a, e))
         torch.cuda.synchronize()
-        per_stage[name] = [a.elapsed_time(b) / reps for a, b in blocks]
-    return per_stage, per_eval
+        out[name] = [a.elapsed_time(e) / block for a, e in blocks]
+    return out
 
 
 def percentile(values, q):
@@ -233,13 +193,36 @@ def percentile(values, q):
     return v[lo] + (v[hi] - v[lo]) * (k - lo)
 
 
+# A GPU process on the MI355X box owns a 16-core share of the host.
+CPU_SHARE = 16
+
+
+def host_cpu():
+    """lscpu model and topology, and the CPUs this process may run on."""
+    import subprocess
+
+    info = {}
+    try:
+        text = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in text.splitlines():
+            key, _, val = line.partition(":")
+            info[key.strip()] = val.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"model": info.get("Model name"), "sockets": info.get("Socket(s)"),
+            "cores_per_socket": info.get("Core(s) per socket"), "threads_per_core": info.get("Thread(s) per core"),
+            "logical_cpus": info.get("CPU(s)"), "affinity_cpus": affinity}
+
+
 def cpu_baseline(args, w, x64, nj, groups, fwd):
     """FP64 oracle timed on the host cores; also returns its output, used only
     as the checker of the GPU results."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # checker / baseline only
 
-    workers = max(1, min(16, os.cpu_count() or 1))
+    cpu = host_cpu()
+    workers = max(1, min(CPU_SHARE, cpu["affinity_cpus"]))
     ow = O.Workload(w.nx, w.ny, w.nz, w.fields)
     om = O.retained_modes(ow)
     first = groups["first_mode"].copy()
@@ -261,6 +244,9 @@ def cpu_baseline(args, w, x64, nj, groups, fwd):
                   "rfft2/irfft2 (workers=%d, not FFTW), C gather/embed, numpy BLAS vertical"
                   % (n, el, workers),
         "ms_per_eval": 1e3 * el / n,
+        "host": cpu,
+        "threads_note": "workers = the %d-core host share of one GPU on this machine (of %s affinity CPUs)"
+                        % (CPU_SHARE, cpu["affinity_cpus"]),
     }
     return rec, out, O
 
@@ -530,6 +516,7 @@ def run(args, np, torch, S, device):
 
     ops = {p: Operator(S, torch, w, modes, groups, nj, fwd, inv, x64, p, device) for p in (S.SKB_F32, S.SKB_F64)}
     f32, f64 = ops[S.SKB_F32], ops[S.SKB_F64]
+    log(args, "fp32 path: %s" % ("plane-resident transforms" if f32.plane else "two-pass transforms"))
 
     # FP32 versus the FP64 GPU form on identical inputs
     f32.step()
@@ -561,20 +548,42 @@ def run(args, np, torch, S, device):
     wall = timed(torch, f32.step, args.steps)
     value = args.steps / wall
     log(args, "fp32 %.1f evals/s (%.2f us/eval)" % (value, 1e6 * wall / args.steps))
-    per_stage, per_eval = instrumented(torch, f32, args.steps)
-    log(args, "graph replay (one hipGraph per evaluation, checked against eager launches)")
-    f32.capture()
-    graph_steps = max(10, args.steps // 4)
-    wall_graph = timed(torch, f32.replay, graph_steps)
-    log(args, "fp32 graph replay %.1f evals/s" % (graph_steps / wall_graph))
+    # host cost of enqueueing one evaluation (the C-ABI call), measured while
+    # the device works through the queued evaluations
+    torch.cuda.synchronize()
+    n_host = 200
+    t0 = time.perf_counter()
+    for _ in range(n_host):
+        f32.step()
+    host_us = 1e6 * (time.perf_counter() - t0) / n_host
+    torch.cuda.synchronize()
+    lat_evals = max(LATENCY_EVALS, args.steps)
+    log(args, "latency over %d evaluations" % lat_evals)
+    per_eval = per_eval_latency(torch, f32, lat_evals)
+    log(args, "per-stage times over rotated plans")
+    rot = [f32] + [Operator(S, torch, w, modes, groups, nj, fwd, inv, x64, S.SKB_F32, device) for _ in range(7)]
+    per_stage = per_stage_times(torch, rot, max(200, args.steps))
+    # the same stages on one plan (working set resident in the Infinity Cache)
+    per_stage_resident = per_stage_times(torch, [f32], max(200, args.steps))
+    log(args, "rotated evaluations (8 plans, %d MB of buffers)" % (8 * f32.op.device_bytes() >> 20))
+    rot_calls = [o.call for o in rot]
+    k = [0]
+
+    def rot_step():
+        rot_calls[k[0] % len(rot_calls)]()
+        k[0] += 1
+
+    for _ in range(16):
+        rot_step()
+    wall_rot = timed(torch, rot_step, args.steps)
+    del rot_calls
+    rot = None
+    torch.cuda.empty_cache()
     two = None
     if not args.no_extras:
-        # Two evaluations in flight: a second operator (own plans and buffers)
-        # on a second stream, launches alternating between the two.  Kernels of
-        # one evaluation then fill the gaps the other leaves (the GEMM's
-        # multiply phases against the transforms' memory phases).  An extra:
-        # kernels that overlap stretch each other's durations, so the kernel
-        # trace is taken without it.
+        # Two evaluations in flight: a second plan (own buffers) on a second
+        # stream, calls alternating between the two.  Reported beside the
+        # headline, which keeps one evaluation at a time.
         log(args, "extra: two streams")
         side = torch.cuda.Stream(device=device)
         with torch.cuda.stream(side):
@@ -602,26 +611,29 @@ def run(args, np, torch, S, device):
         f64.step()
     f64_steps = max(10, args.steps // 4)
     wall64 = timed(torch, f64.step, f64_steps)
-    _, per_eval64 = instrumented(torch, f64, f64_steps)
+    per_eval64 = per_eval_latency(torch, f64, f64_steps)
 
-    # algorithmic (stage-boundary) bytes attributed to each kernel, FP32 storage
+    # algorithmic (stage-boundary) bytes per launch of each stage, FP32 storage:
+    # each stage reads its input once and writes its output once
     nkl, P = len(modes), w.planes
     ret_b, real_b = nkl * P * 8, w.real_elements * 4
     modal_b = nkl * w.fields * nj * 8
     mat_b = len(groups) * nj * w.nz * 4
     algo = {
-        "inverse_columns": ret_b,
-        "inverse_rows": real_b,
-        "forward_rows": real_b,
-        "forward_columns": ret_b,
+        "layout": 2 * ret_b,                      # compact in, plane-major copy out (not in the yardstick)
+        "transforms": 2 * ret_b + 2 * real_b,     # retained in, real planes out and back in, retained out
+        "inverse": ret_b + real_b,
+        "forward": real_b + ret_b,
         "vertical_gemm": ret_b + modal_b + mat_b,
     }
-    eval_bytes = sum(algo.values())
+    eval_bytes = ret_b + real_b + real_b + ret_b + ret_b + modal_b + mat_b  # the yardstick
     kernels = {}
     for n, ms in per_stage.items():
         avg = sum(ms) / len(ms)
-        kernels[n] = {"avg_us": 1e3 * avg, "algorithmic_bytes": algo[n], "GBps": algo[n] / (avg * 1e-3) / 1e9}
-    dominant = max(kernels, key=lambda n: kernels[n]["avg_us"])
+        res = sum(per_stage_resident[n]) / len(per_stage_resident[n])
+        kernels[n] = {"avg_us": 1e3 * avg, "algorithmic_bytes": algo[n], "GBps": algo[n] / (avg * 1e-3) / 1e9,
+                      "resident_avg_us": 1e3 * res, "in_evaluation": n not in ("inverse", "forward")}
+    dominant = max((n for n in kernels if kernels[n]["in_evaluation"]), key=lambda n: kernels[n]["avg_us"])
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         try:
@@ -640,15 +652,19 @@ def run(args, np, torch, S, device):
         "frac": dk["GBps"] / HBM_PEAK_GBPS,
         "traffic": traffic,
         "algorithmic_bytes_per_launch": dk["algorithmic_bytes"],
+        "timing": "HIP events around blocks of 50 launches rotating over 8 plans (HBM-resident working set)",
     }
     eval_avg_ms = sum(per_eval) / len(per_eval)
     eval_roofline = {
         "bytes_per_eval": eval_bytes,
         "achieved": eval_bytes / (eval_avg_ms * 1e-3) / 1e9,
         "frac": eval_bytes / (eval_avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS,
-        "from": "sum of instrumented kernel times",
+        "from": "mean of per-evaluation HIP event pairs",
         "wall_achieved": eval_bytes * value / 1e9,
         "wall_frac": eval_bytes * value / 1e9 / HBM_PEAK_GBPS,
+        "rotated_evals_per_s": args.steps / wall_rot,
+        "rotated_wall_frac": eval_bytes * args.steps / wall_rot / 1e9 / HBM_PEAK_GBPS,
+        "fused_minimum_bytes": ret_b + modal_b + mat_b,
     }
 
     cpu = None
@@ -680,23 +696,24 @@ def run(args, np, torch, S, device):
                         "K^2-grouped vertical GEMM",
             "nx": 256, "ny": 256, "nz": 65, "fields": 3, "retained_modes": nkl, "vertical_family": args.family,
             "vertical_groups": len(groups), "nj": nj,
-            "launch": "three prepared C-ABI calls per evaluation (pruned inverse, pruned forward, vertical forward; five kernels) on one stream",
+            "launch": "one prepared C-ABI call per evaluation (skb_retained_operator_execute) on one stream",
+            "path": "plane-resident transforms" if f32.plane else "two-pass transforms",
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
         "eval_roofline": eval_roofline,
         "latency_ms": {"p50": percentile(per_eval, 0.5), "p99": percentile(per_eval, 0.99),
-                       "mean": eval_avg_ms},
+                       "mean": eval_avg_ms, "evaluations": len(per_eval)},
+        "host_enqueue_us_per_eval": host_us,
         "kernels": kernels,
         "error_vs_fp64_gpu": {"max_scale_normalized": err_max, "relative_l2": err_l2, "tolerance": 1e-6,
                               "passed": err_max <= 1e-6 and err_l2 <= 1e-6},
         "error_vs_fp64_oracle": err_oracle,
         "two_streams": two,
-        "graph_replay": {"evals_per_s": graph_steps / wall_graph, "ms_per_step": 1e3 * wall_graph / graph_steps,
-                         "note": "the same five kernels recorded into one hipGraph, one replay per evaluation"},
         "fp64": {"value": f64_steps / wall64, "unit": "evals/s", "ms_per_step": 1e3 * wall64 / f64_steps,
                  "p50_ms": percentile(per_eval64, 0.5), "p99_ms": percentile(per_eval64, 0.99),
-                 "bytes_per_eval": 2 * eval_bytes},
+                 "bytes_per_eval": 2 * eval_bytes,
+                 "path": "plane-resident transforms" if f64.plane else "two-pass transforms"},
         "device": torch.cuda.get_device_name(0),
         "library": S.version(),
     }

@@ -242,12 +242,15 @@ int skb_retained_operator_execute(skb_retained_operator* op, const void* retaine
  * path), SKB_OP_STAGE_INVERSE (into the real planes), SKB_OP_STAGE_FORWARD
  * (real planes to the plan's retained intermediate) and SKB_OP_STAGE_VERTICAL
  * (into `modal`).  Each reads what the stage before it left in the plan's
- * buffers, so the four in order on one stream equal one execute. */
+ * buffers, so LAYOUT, INVERSE, FORWARD, VERTICAL in order on one stream
+ * equal one execute.  SKB_OP_STAGE_TRANSFORMS runs INVERSE then FORWARD as
+ * execute does (one kernel on the fused plane path). */
 enum skb_operator_stage {
     SKB_OP_STAGE_LAYOUT = 0,
     SKB_OP_STAGE_INVERSE = 1,
     SKB_OP_STAGE_FORWARD = 2,
-    SKB_OP_STAGE_VERTICAL = 3
+    SKB_OP_STAGE_VERTICAL = 3,
+    SKB_OP_STAGE_TRANSFORMS = 4
 };
 int skb_retained_operator_stage(skb_retained_operator* op, int stage, const void* retained, void* modal,
                                 void* stream);

@@ -69,6 +69,17 @@ void run_stage(skb_retained_operator* op, int stage, const void* retained, void*
                 check_status(skb_pruned_forward(op->pruned, op->real.ptr, op->compact.ptr, s));
             }
             return;
+        case SKB_OP_STAGE_TRANSFORMS:
+            // both transforms as execute runs them
+            if (plane && op->fused) {
+                plane_operator(*t, pm, pm, op->real.ptr, op->planes, s);
+                if (!vertical_pm_supported(op->vertical))
+                    pm_transpose(false, pm, op->compact.ptr, t->modes, op->planes, t->np, s);
+            } else {
+                run_stage(op, SKB_OP_STAGE_INVERSE, retained, modal, s);
+                run_stage(op, SKB_OP_STAGE_FORWARD, retained, modal, s);
+            }
+            return;
         case SKB_OP_STAGE_VERTICAL:
             if (!modal) fail(SKB_INVALID_ARGUMENT, "null modal output");
             if (plane && vertical_pm_supported(op->vertical)) vertical_forward_pm(op->vertical, pm, t->np, modal, s);
@@ -115,18 +126,9 @@ int skb_retained_operator_execute(skb_retained_operator* op, const void* retaine
     return guarded([&] {
         if (!op || !retained || !modal) fail(SKB_INVALID_ARGUMENT, "null argument");
         const hipStream_t s = as_stream(stream);
-        if (op->fused && plane_path(op)) {
-            const PlaneTables* t = pruned_plane_tables(op->pruned);
-            void* pm = pruned_plane_scratch(op->pruned);
-            run_stage(op, SKB_OP_STAGE_LAYOUT, retained, modal, s);
-            plane_operator(*t, pm, pm, op->real.ptr, op->planes, s);
-            if (!vertical_pm_supported(op->vertical))
-                pm_transpose(false, pm, op->compact.ptr, t->modes, op->planes, t->np, s);
-            run_stage(op, SKB_OP_STAGE_VERTICAL, retained, modal, s);
-            return;
-        }
-        for (int stage = SKB_OP_STAGE_LAYOUT; stage <= SKB_OP_STAGE_VERTICAL; ++stage)
-            run_stage(op, stage, retained, modal, s);
+        run_stage(op, SKB_OP_STAGE_LAYOUT, retained, modal, s);
+        run_stage(op, SKB_OP_STAGE_TRANSFORMS, retained, modal, s);
+        run_stage(op, SKB_OP_STAGE_VERTICAL, retained, modal, s);
     });
 }
 

@@ -306,6 +306,7 @@ SKB_PRODUCTS_FP32 = 0
 SKB_PRODUCTS_SPLIT_BF16 = 1
 SKB_PATH_AUTO, SKB_PATH_TWO_PASS, SKB_PATH_PLANE = 0, 1, 2
 SKB_OP_STAGE_LAYOUT, SKB_OP_STAGE_INVERSE, SKB_OP_STAGE_FORWARD, SKB_OP_STAGE_VERTICAL = 0, 1, 2, 3
+SKB_OP_STAGE_TRANSFORMS = 4
 
 
 def synthetic_modal_weights(w: "Workload", modes: np.ndarray) -> np.ndarray:
