@@ -286,6 +286,21 @@ int skb_vertical_plan_create(const skb_workload* workload, uint64_t nj, const sk
 int skb_vertical_plan_destroy(skb_vertical_plan* plan);
 int skb_vertical_forward(skb_vertical_plan* plan, const void* physical, void* modal, void* stream);
 int skb_vertical_inverse(skb_vertical_plan* plan, const void* modal, void* physical, void* stream);
+/* Split layout (VerticalGemmLayout::split, include/skbench/skbench.hpp:125-128;
+ * the split dgemm of src/vertical_gemm.cpp:558-596 and its data accessors,
+ * include/skbench/skbench.hpp:203-210): real and imaginary parts in separate
+ * arrays, each in the interleaved layout's element order.  reserve_split
+ * allocates the plan's interleaved staging once (the provider constructor's
+ * choice of layout); the split calls without it fail with SKB_LOGIC_ERROR.
+ * The matrices are real, so the real and imaginary parts are independent
+ * products (executeForwardReal / executeForwardImaginary, src/vertical_gemm.cpp:
+ * 771-819): a part whose input and output pointers are both null is not
+ * computed and its output array is not touched. */
+int skb_vertical_plan_reserve_split(skb_vertical_plan* plan);
+int skb_vertical_forward_split(skb_vertical_plan* plan, const void* physical_real, const void* physical_imag,
+                               void* modal_real, void* modal_imag, void* stream);
+int skb_vertical_inverse_split(skb_vertical_plan* plan, const void* modal_real, const void* modal_imag,
+                               void* physical_real, void* physical_imag, void* stream);
 /* VerticalGemmProvider::matrixBytesPerDirection. */
 uint64_t skb_vertical_matrix_bytes(const skb_vertical_plan* plan);
 /* Launch shape: workgroups per launch, 0 = one per 16-column tile (the
@@ -506,6 +521,20 @@ int skb_batch_launch(skb_batch* batch, void* stream);
  * captured graph.  Results agree with the default launch shapes to rounding
  * (see skb_pruned_plan_set_grid). */
 int skb_batch_autotune(skb_batch* batch, int reps, void* stream, char* report, size_t capacity);
+
+/* ---- split <-> interleaved complex arrays ---------------------------------
+ * The conversions under the reference's split-layout adapters
+ * (FFTWProvider::forwardSplit / inverseSplit / gatherRetainedToSplitOuter /
+ * embedRetainedFromSplitOuter / scaleRetainedSplitOuter /
+ * gatherRetainedSplitOuter / embedRetainedSplitOuter,
+ * include/skbench/skbench.hpp:374-398; src/fftw_provider.cpp:331-395,
+ * 701-830): interleave writes out[i] = (real[i], imaginary[i]); deinterleave
+ * writes real[i] = scale * in[i].re and imaginary[i] = scale * in[i].im;
+ * scale_split multiplies both arrays in place.  count = complex elements. */
+int skb_interleave(int precision, uint64_t count, const void* real, const void* imaginary, void* out, void* stream);
+int skb_deinterleave(int precision, uint64_t count, const void* in, void* real, void* imaginary, double scale,
+                     void* stream);
+int skb_scale_split(int precision, uint64_t count, void* real, void* imaginary, double scale, void* stream);
 
 /* ---- pointwise advection -----------------------------------------------
  * Replaces PointwiseAdvectionExecutor::execute (src/pointwise_advection.hpp

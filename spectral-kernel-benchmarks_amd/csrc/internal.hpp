@@ -4,9 +4,17 @@
 // products the composed graphs call (vertical.hip).
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include "skb_hip.h"
 
 namespace skb {
+
+// Split <-> interleaved complex arrays of n elements (layout.hip):
+// out[i] = (re[i], im[i]); re[i] = scale * in[i].re, im[i] = scale * in[i].im.
+// deinterleave skips a null re or im array (not both).
+void interleave(int precision, uint64_t n, const void* re, const void* im, void* out, hipStream_t s);
+void deinterleave(int precision, uint64_t n, const void* in, void* re, void* im, double scale, hipStream_t s);
 
 struct PlaneTables;
 // The pruned plan's plane-resident tables and its plane-major scratch

@@ -1196,6 +1196,9 @@ struct skb_vertical_plan {
     // and one field (GemmTile::pad = field), group by group.
     int pm_ntiles = 0;
     skb::DeviceBuffer pm_tiles;
+    // interleaved staging of the split-layout entry points (reserved by
+    // skb_vertical_plan_reserve_split): physical-size and modal-size arrays
+    skb::DeviceBuffer split_physical, split_modal;
 };
 
 using namespace skb;
@@ -1709,6 +1712,58 @@ int skb_vertical_wvm_modal_work(skb_vertical_plan* plan, const void* weights, co
 }
 
 uint64_t skb_vertical_matrix_bytes(const skb_vertical_plan* plan) { return plan ? plan->matrix_bytes : 0; }
+
+}  // extern "C"
+
+namespace {
+
+// Split layout over the interleaved kernels: interleave the input pair into
+// the plan's staging, one grouped launch, deinterleave into the output pair.
+// The matrices are real, so the real (imaginary) output part depends on the
+// real (imaginary) input part alone: a component whose input and output are
+// both null is left out (the other input fills both staged parts and only
+// the requested output part is written).
+void run_split(skb_vertical_plan* plan, bool forward, const void* in_re, const void* in_im, void* out_re,
+               void* out_im, void* stream) {
+    if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
+    if (!plan->split_physical.ptr)
+        fail(SKB_LOGIC_ERROR, "Split GEMM component requested from the complex GEMM provider.");
+    if ((in_re == nullptr) != (out_re == nullptr) || (in_im == nullptr) != (out_im == nullptr) ||
+        (!in_re && !in_im))
+        fail(SKB_INVALID_ARGUMENT, "a split component needs both its input and its output array");
+    const hipStream_t s = as_stream(stream);
+    const uint64_t np = plan->modes * plan->w.nz * plan->w.fields, nm = plan->modes * plan->nj * plan->w.fields;
+    void* stage_in = forward ? plan->split_physical.ptr : plan->split_modal.ptr;
+    void* stage_out = forward ? plan->split_modal.ptr : plan->split_physical.ptr;
+    interleave(plan->precision, forward ? np : nm, in_re ? in_re : in_im, in_im ? in_im : in_re, stage_in, s);
+    run(plan, forward, stage_in, stage_out, stream, false);
+    deinterleave(plan->precision, forward ? nm : np, stage_out, out_re, out_im, 1.0, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int skb_vertical_plan_reserve_split(skb_vertical_plan* plan) {
+    return guarded([&] {
+        if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
+        if (plan->wvm) fail(SKB_LOGIC_ERROR, "the WVM-direct GEMM has no split layout");
+        if (plan->split_physical.ptr) return;
+        const uint64_t cb = complex_bytes(plan->precision);
+        plan->split_physical.allocate(plan->modes * plan->w.nz * plan->w.fields * cb);
+        plan->split_modal.allocate(plan->modes * plan->nj * plan->w.fields * cb);
+    });
+}
+
+int skb_vertical_forward_split(skb_vertical_plan* plan, const void* physical_real, const void* physical_imag,
+                               void* modal_real, void* modal_imag, void* stream) {
+    return guarded([&] { run_split(plan, true, physical_real, physical_imag, modal_real, modal_imag, stream); });
+}
+
+int skb_vertical_inverse_split(skb_vertical_plan* plan, const void* modal_real, const void* modal_imag,
+                               void* physical_real, void* physical_imag, void* stream) {
+    return guarded([&] { run_split(plan, false, modal_real, modal_imag, physical_real, physical_imag, stream); });
+}
 
 int skb_vertical_plan_set_grid(skb_vertical_plan* plan, int workgroups) {
     return guarded([&] {

@@ -110,6 +110,7 @@ struct Staged {
             hip_check(hipStreamSynchronize(s), "upload sync");
         } else {
             hip_check(hipMemcpyAsync(dev, src, elements * 8, hipMemcpyHostToDevice, s), "upload");
+            hip_check(hipStreamSynchronize(s), "upload sync");  // the caller may reuse src on return
         }
     }
     void download(double* dst, hipStream_t s) {
@@ -217,16 +218,16 @@ std::vector<double> makeFixture(const Workload& workload, FixtureKind fixture, s
 struct HipProvider::Impl {
     Workload w;
     Precision precision;
+    DataLayout layout;
     skb_fft_plan* plan = nullptr;
     bool supported = true;
     std::string capability = "hip provider: gfx950 Stockham r2c/c2r in WVM order";
     double allocation_seconds = 0, planning_seconds = 0;
     Stream stream;
-    Staged real, spectrum, retained;
-    std::vector<RetainedMode> cached_modes;
+    Staged real, spectrum, retained, spectrum_re, spectrum_im, retained_re, retained_im;
     std::size_t retained_modes = 0;
 
-    Impl(const Workload& wl, Precision p) : w(wl), precision(p) {
+    Impl(const Workload& wl, DataLayout l, Precision p) : w(wl), precision(p), layout(l) {
         const auto c = to_c(wl);
         const auto t0 = Clock::now();
         const int st = skb_fft_plan_create(&c, c_precision(p), &plan);
@@ -241,6 +242,10 @@ struct HipProvider::Impl {
         const bool single = p == Precision::float32;
         real.allocate(w.realElements(), single);
         spectrum.allocate(2 * w.spectrumElements(), single);
+        if (layout == DataLayout::split) {
+            spectrum_re.allocate(w.spectrumElements(), single);
+            spectrum_im.allocate(w.spectrumElements(), single);
+        }
         allocation_seconds = seconds_since(t1);
     }
     ~Impl() {
@@ -249,24 +254,40 @@ struct HipProvider::Impl {
     void require() const {
         if (!supported) throw std::logic_error(capability);
     }
+    // src/fftw_provider.cpp:468-472
+    void require_layout(DataLayout expected) const {
+        require();
+        if (layout != expected)
+            throw std::logic_error("FFTW execution entry point does not match the planned data layout.");
+    }
+    // Retained staging follows the mode list of the call (a provider serves
+    // any retained set); it is reallocated only when the mode count changes.
     void ensure_retained(std::size_t modes) {
         if (retained_modes == modes) return;
-        retained.~Staged();
-        new (&retained) Staged();
-        retained.allocate(2 * modes * w.planes(), precision == Precision::float32);
+        for (Staged* b : {&retained, &retained_re, &retained_im}) {
+            b->~Staged();
+            new (b) Staged();
+        }
+        const bool single = precision == Precision::float32;
+        retained.allocate(2 * modes * w.planes(), single);
+        retained_re.allocate(modes * w.planes(), single);
+        retained_im.allocate(modes * w.planes(), single);
         retained_modes = modes;
     }
+    int cp() const { return c_precision(precision); }
 };
 
 HipProvider::HipProvider(const Workload& workload, Precision precision)
-    : impl_(std::make_unique<Impl>(workload, precision)) {}
+    : impl_(std::make_unique<Impl>(workload, DataLayout::interleaved, precision)) {}
+HipProvider::HipProvider(const Workload& workload, DataLayout layout, Precision precision)
+    : impl_(std::make_unique<Impl>(workload, layout, precision)) {}
 HipProvider::~HipProvider() = default;
 HipProvider::HipProvider(HipProvider&&) noexcept = default;
 HipProvider& HipProvider::operator=(HipProvider&&) noexcept = default;
 
 void HipProvider::forward(const double* input, Complex* wvmSpectrum) {
     auto& m = *impl_;
-    m.require();
+    m.require_layout(DataLayout::interleaved);
     m.real.upload(input, m.stream.s);
     check(skb_fft_forward(m.plan, m.real.dev, m.spectrum.dev, m.stream.s));
     m.spectrum.download(reinterpret_cast<double*>(wvmSpectrum), m.stream.s);
@@ -274,7 +295,7 @@ void HipProvider::forward(const double* input, Complex* wvmSpectrum) {
 
 void HipProvider::inverse(Complex* wvmSpectrum, double* output) {
     auto& m = *impl_;
-    m.require();
+    m.require_layout(DataLayout::interleaved);
     m.spectrum.upload(reinterpret_cast<const double*>(wvmSpectrum), m.stream.s);
     check(skb_fft_inverse(m.plan, m.spectrum.dev, m.real.dev, m.stream.s));
     m.real.download(output, m.stream.s);
@@ -300,6 +321,112 @@ void HipProvider::embedRetainedOuter(const std::vector<RetainedMode>& modes, con
     m.retained.upload(reinterpret_cast<const double*>(retainedSpectrum), m.stream.s);
     check(skb_fft_embed_retained(m.plan, c.data(), c.size(), m.retained.dev, m.spectrum.dev, m.stream.s));
     m.spectrum.download(reinterpret_cast<double*>(wvmSpectrum), m.stream.s);
+}
+
+// Split forms: the device kernels read and write interleaved arrays, so the
+// split arrays are (de)interleaved on the device around them (skb_interleave /
+// skb_deinterleave, which also applies the gather's scale).
+void HipProvider::gatherRetainedToSplitOuter(const std::vector<RetainedMode>& modes, const Complex* spectrum,
+                                             double* retainedReal, double* retainedImag, double scale) {
+    auto& m = *impl_;
+    m.require();
+    const auto c = to_c(modes);
+    m.ensure_retained(modes.size());
+    const std::uint64_t n = modes.size() * m.w.planes();
+    m.spectrum.upload(reinterpret_cast<const double*>(spectrum), m.stream.s);
+    check(skb_fft_gather_retained(m.plan, c.data(), c.size(), m.spectrum.dev, m.retained.dev, m.stream.s));
+    check(skb_deinterleave(m.cp(), n, m.retained.dev, m.retained_re.dev, m.retained_im.dev, scale, m.stream.s));
+    m.retained_re.download(retainedReal, m.stream.s);
+    m.retained_im.download(retainedImag, m.stream.s);
+}
+
+void HipProvider::embedRetainedFromSplitOuter(const std::vector<RetainedMode>& modes, const double* retainedReal,
+                                              const double* retainedImag, Complex* spectrum) {
+    auto& m = *impl_;
+    m.require();
+    const auto c = to_c(modes);
+    m.ensure_retained(modes.size());
+    const std::uint64_t n = modes.size() * m.w.planes();
+    m.retained_re.upload(retainedReal, m.stream.s);
+    m.retained_im.upload(retainedImag, m.stream.s);
+    check(skb_interleave(m.cp(), n, m.retained_re.dev, m.retained_im.dev, m.retained.dev, m.stream.s));
+    check(skb_fft_embed_retained(m.plan, c.data(), c.size(), m.retained.dev, m.spectrum.dev, m.stream.s));
+    m.spectrum.download(reinterpret_cast<double*>(spectrum), m.stream.s);
+}
+
+void HipProvider::scaleRetainedSplitOuter(const std::vector<RetainedMode>& modes, double* retainedReal,
+                                          double* retainedImag, double scale) {
+    auto& m = *impl_;
+    m.require();
+    m.ensure_retained(modes.size());
+    const std::uint64_t n = modes.size() * m.w.planes();
+    m.retained_re.upload(retainedReal, m.stream.s);
+    m.retained_im.upload(retainedImag, m.stream.s);
+    check(skb_scale_split(m.cp(), n, m.retained_re.dev, m.retained_im.dev, scale, m.stream.s));
+    m.retained_re.download(retainedReal, m.stream.s);
+    m.retained_im.download(retainedImag, m.stream.s);
+}
+
+void HipProvider::forwardSplit(const double* input, double* wvmSpectrumReal, double* wvmSpectrumImag) {
+    auto& m = *impl_;
+    m.require_layout(DataLayout::split);
+    m.real.upload(input, m.stream.s);
+    check(skb_fft_forward(m.plan, m.real.dev, m.spectrum.dev, m.stream.s));
+    check(skb_deinterleave(m.cp(), m.w.spectrumElements(), m.spectrum.dev, m.spectrum_re.dev, m.spectrum_im.dev, 1.0,
+                           m.stream.s));
+    m.spectrum_re.download(wvmSpectrumReal, m.stream.s);
+    m.spectrum_im.download(wvmSpectrumImag, m.stream.s);
+}
+
+void HipProvider::inverseSplit(double* wvmSpectrumReal, double* wvmSpectrumImag, double* output) {
+    auto& m = *impl_;
+    m.require_layout(DataLayout::split);
+    m.spectrum_re.upload(wvmSpectrumReal, m.stream.s);
+    m.spectrum_im.upload(wvmSpectrumImag, m.stream.s);
+    check(skb_interleave(m.cp(), m.w.spectrumElements(), m.spectrum_re.dev, m.spectrum_im.dev, m.spectrum.dev,
+                         m.stream.s));
+    check(skb_fft_inverse(m.plan, m.spectrum.dev, m.real.dev, m.stream.s));
+    m.real.download(output, m.stream.s);
+}
+
+void HipProvider::gatherRetainedSplitOuter(const std::vector<RetainedMode>& modes, const double* spectrumReal,
+                                           const double* spectrumImag, double* retainedReal, double* retainedImag) {
+    auto& m = *impl_;
+    m.require_layout(DataLayout::split);
+    const auto c = to_c(modes);
+    m.ensure_retained(modes.size());
+    const std::uint64_t n = modes.size() * m.w.planes();
+    m.spectrum_re.upload(spectrumReal, m.stream.s);
+    m.spectrum_im.upload(spectrumImag, m.stream.s);
+    check(skb_interleave(m.cp(), m.w.spectrumElements(), m.spectrum_re.dev, m.spectrum_im.dev, m.spectrum.dev,
+                         m.stream.s));
+    check(skb_fft_gather_retained(m.plan, c.data(), c.size(), m.spectrum.dev, m.retained.dev, m.stream.s));
+    check(skb_deinterleave(m.cp(), n, m.retained.dev, m.retained_re.dev, m.retained_im.dev, 1.0, m.stream.s));
+    m.retained_re.download(retainedReal, m.stream.s);
+    m.retained_im.download(retainedImag, m.stream.s);
+}
+
+void HipProvider::embedRetainedSplitOuter(const std::vector<RetainedMode>& modes, const double* retainedReal,
+                                          const double* retainedImag, double* spectrumReal, double* spectrumImag) {
+    auto& m = *impl_;
+    m.require_layout(DataLayout::split);
+    const auto c = to_c(modes);
+    m.ensure_retained(modes.size());
+    const std::uint64_t n = modes.size() * m.w.planes();
+    m.retained_re.upload(retainedReal, m.stream.s);
+    m.retained_im.upload(retainedImag, m.stream.s);
+    check(skb_interleave(m.cp(), n, m.retained_re.dev, m.retained_im.dev, m.retained.dev, m.stream.s));
+    check(skb_fft_embed_retained(m.plan, c.data(), c.size(), m.retained.dev, m.spectrum.dev, m.stream.s));
+    check(skb_deinterleave(m.cp(), m.w.spectrumElements(), m.spectrum.dev, m.spectrum_re.dev, m.spectrum_im.dev, 1.0,
+                           m.stream.s));
+    m.spectrum_re.download(spectrumReal, m.stream.s);
+    m.spectrum_im.download(spectrumImag, m.stream.s);
+}
+
+DataLayout HipProvider::layout() const noexcept { return impl_->layout; }
+std::string HipProvider::splitInPlaceWvmOrderCapability() const {
+    return "unsupported: the device transforms read and write interleaved spectra; the split entry points stage "
+           "through the provider's interleaved spectrum out of place";
 }
 
 void HipProvider::forwardDevice(const void* real, void* wvm, void* stream) {
@@ -370,11 +497,13 @@ void HipPrunedProvider::executeForwardRows(const double* input) {
     auto& m = *impl_;
     m.real.upload(input, m.stream.s);
     check(skb_pruned_forward_rows(m.plan, m.real.dev, m.stream.s));
+    hip_check(hipStreamSynchronize(m.stream.s), "sync");
 }
 
 void HipPrunedProvider::executeForwardColumns() {
     auto& m = *impl_;
     check(skb_pruned_forward_columns(m.plan, m.retained.dev, m.stream.s));
+    hip_check(hipStreamSynchronize(m.stream.s), "sync");
 }
 
 void HipPrunedProvider::gatherForward(Complex* retainedSpectrum) {
@@ -390,6 +519,7 @@ void HipPrunedProvider::embedInverse(const Complex* retainedSpectrum) {
 void HipPrunedProvider::executeInverseColumns() {
     auto& m = *impl_;
     check(skb_pruned_inverse_columns(m.plan, m.retained.dev, m.stream.s));
+    hip_check(hipStreamSynchronize(m.stream.s), "sync");
 }
 
 void HipPrunedProvider::executeInverseRows(double* output) {
@@ -724,72 +854,421 @@ std::string HipStreamingPrunedSplitProvider::libraryIdentity() const { return "l
 std::string HipStreamingPrunedSplitProvider::version() const { return skb_version(); }
 
 // ---- HipVerticalGemmProvider ----------------------------------------------------------
+std::string verticalGemmLayoutName(VerticalGemmLayout layout) {
+    return layout == VerticalGemmLayout::split ? "split" : "complex-interleaved";
+}
+
+// Host arrays of the layout (the reference's Accelerate buffers,
+// src/vertical_gemm.cpp:360-390) plus device staging of the plan precision.
+// Forward direction: physical input -> modal output; inverse: modal input ->
+// physical output.  A direction the buffer policy leaves out has no arrays.
 struct HipVerticalGemmProvider::Impl {
     Workload w;
-    std::size_t nj, modes, groups;
+    std::size_t nj = 0, modes = 0, groups = 0;
+    VerticalGemmLayout layout;
+    VerticalGemmStrategy strategy;
+    VerticalGemmBufferPolicy policy;
+    Precision precision;
     skb_vertical_plan* plan = nullptr;
+    bool available = true;
+    std::string capability_text;
+    double allocation_seconds = 0, preparation_seconds = 0;
     Stream stream;
-    Staged physical, modal;
+    std::size_t physical_count = 0, modal_count = 0;
+    // interleaved layout
+    std::vector<Complex> physical_in, modal_out, modal_in, physical_out;
+    Staged d_physical, d_modal;
+    // split layout
+    std::vector<double> physical_in_re, physical_in_im, modal_out_re, modal_out_im;
+    std::vector<double> modal_in_re, modal_in_im, physical_out_re, physical_out_im;
+    Staged d_physical_re, d_physical_im, d_modal_re, d_modal_im;
 
-    Impl(const Workload& wl, const GroupedVerticalOperators& ops, Precision p)
-        : w(wl), nj(ops.nj), groups(ops.groups.size()) {
+    Impl(const Workload& wl, const GroupedVerticalOperators& ops, VerticalGemmLayout l, VerticalGemmStrategy st,
+         VerticalGemmBufferPolicy bp, Precision p)
+        : w(wl), nj(ops.nj), groups(ops.groups.size()), layout(l), strategy(st), policy(bp), precision(p) {
         if (ops.nz != wl.nz) throw std::invalid_argument("Vertical operator dimensions do not match the workload.");
         if (ops.groups.empty()) throw std::invalid_argument("Vertical GEMM requires at least one group.");
+        if (strategy.outerWorkers == 0) throw std::invalid_argument("Vertical GEMM outer workers must be positive.");
         modes = ops.groups.back().firstMode + ops.groups.back().modeCount;
+        physical_count = modes * wl.planes();
+        modal_count = modes * wl.fields * nj;
         const auto cw = to_c(wl);
-        check(skb_vertical_plan_create(&cw, nj, reinterpret_cast<const skb_mode_group*>(ops.groups.data()), groups,
-                                       ops.forward.data(), ops.inverse.data(), c_precision(p), &plan));
+        const auto t0 = Clock::now();
+        const int st_create =
+            skb_vertical_plan_create(&cw, nj, reinterpret_cast<const skb_mode_group*>(ops.groups.data()), groups,
+                                     ops.forward.data(), ops.inverse.data(), c_precision(p), &plan);
+        preparation_seconds = seconds_since(t0);
+        if (st_create == SKB_UNSUPPORTED) {
+            available = false;
+            capability_text = "unsupported: " + last_error();
+            return;
+        }
+        check(st_create);
+        const auto t1 = Clock::now();
         const bool single = p == Precision::float32;
-        physical.allocate(2 * modes * wl.planes(), single);
-        modal.allocate(2 * modes * wl.fields * nj, single);
+        const bool fwd = policy != VerticalGemmBufferPolicy::inverseOnly;
+        const bool inv = policy != VerticalGemmBufferPolicy::forwardOnly;
+        if (layout == VerticalGemmLayout::complexInterleaved) {
+            if (fwd) {
+                physical_in.resize(physical_count);
+                modal_out.resize(modal_count);
+            }
+            if (inv) {
+                modal_in.resize(modal_count);
+                physical_out.resize(physical_count);
+            }
+            d_physical.allocate(2 * physical_count, single);
+            d_modal.allocate(2 * modal_count, single);
+        } else {
+            check(skb_vertical_plan_reserve_split(plan));
+            if (fwd) {
+                physical_in_re.resize(physical_count);
+                physical_in_im.resize(physical_count);
+                modal_out_re.resize(modal_count);
+                modal_out_im.resize(modal_count);
+            }
+            if (inv) {
+                modal_in_re.resize(modal_count);
+                modal_in_im.resize(modal_count);
+                physical_out_re.resize(physical_count);
+                physical_out_im.resize(physical_count);
+            }
+            d_physical_re.allocate(physical_count, single);
+            d_physical_im.allocate(physical_count, single);
+            d_modal_re.allocate(modal_count, single);
+            d_modal_im.allocate(modal_count, single);
+        }
+        allocation_seconds = seconds_since(t1);
+        capability_text = "supported: hip grouped MFMA GEMM (gfx950), " + verticalGemmLayoutName(layout) +
+                          " layout, one grouped launch per direction";
     }
     ~Impl() {
         if (plan) skb_vertical_plan_destroy(plan);
+    }
+    // src/vertical_gemm.cpp:433-467
+    void require_available() const {
+        if (!available) throw std::runtime_error(capability_text);
+    }
+    void require_forward() const {
+        require_available();
+        if (policy == VerticalGemmBufferPolicy::inverseOnly)
+            throw std::logic_error("Forward GEMM component requested from an inverse-only provider.");
+    }
+    void require_inverse() const {
+        require_available();
+        if (policy == VerticalGemmBufferPolicy::forwardOnly)
+            throw std::logic_error("Inverse GEMM component requested from a forward-only provider.");
+    }
+    void require_split() const {
+        require_available();
+        if (layout != VerticalGemmLayout::split)
+            throw std::logic_error("Split GEMM component requested from the complex GEMM provider.");
+    }
+    void require_interleaved() const {
+        require_available();
+        if (layout != VerticalGemmLayout::complexInterleaved)
+            throw std::logic_error("Interleaved GEMM component requested from the split GEMM provider.");
+    }
+    void validate_modes(const std::vector<RetainedMode>& m) const {
+        if (m.size() != modes)
+            throw std::invalid_argument("Retained-mode count does not match the vertical GEMM columns.");
+    }
+    // One direction of the split product over the host arrays; a null host
+    // pair leaves that part out.
+    void split_product(bool forward, const double* in_re, const double* in_im, double* out_re, double* out_im) {
+        Staged& i_re = forward ? d_physical_re : d_modal_re;
+        Staged& i_im = forward ? d_physical_im : d_modal_im;
+        Staged& o_re = forward ? d_modal_re : d_physical_re;
+        Staged& o_im = forward ? d_modal_im : d_physical_im;
+        if (in_re) i_re.upload(in_re, stream.s);
+        if (in_im) i_im.upload(in_im, stream.s);
+        const void* dir = in_re ? i_re.dev : nullptr;
+        const void* dii = in_im ? i_im.dev : nullptr;
+        void* dor = in_re ? o_re.dev : nullptr;
+        void* doi = in_im ? o_im.dev : nullptr;
+        if (forward) check(skb_vertical_forward_split(plan, dir, dii, dor, doi, stream.s));
+        else check(skb_vertical_inverse_split(plan, dir, dii, dor, doi, stream.s));
+        if (out_re) o_re.download(out_re, stream.s);
+        if (out_im) o_im.download(out_im, stream.s);
     }
 };
 
 HipVerticalGemmProvider::HipVerticalGemmProvider(const Workload& workload, const GroupedVerticalOperators& operators,
                                                  Precision precision)
-    : impl_(std::make_unique<Impl>(workload, operators, precision)) {}
+    : impl_(std::make_unique<Impl>(workload, operators, VerticalGemmLayout::complexInterleaved,
+                                   VerticalGemmStrategy{}, VerticalGemmBufferPolicy::bidirectional, precision)) {}
+HipVerticalGemmProvider::HipVerticalGemmProvider(const Workload& workload, const GroupedVerticalOperators& operators,
+                                                 VerticalGemmLayout layout, Precision precision)
+    : impl_(std::make_unique<Impl>(workload, operators, layout, VerticalGemmStrategy{},
+                                   VerticalGemmBufferPolicy::bidirectional, precision)) {}
+HipVerticalGemmProvider::HipVerticalGemmProvider(const Workload& workload, const GroupedVerticalOperators& operators,
+                                                 VerticalGemmLayout layout, VerticalGemmStrategy strategy,
+                                                 VerticalGemmBufferPolicy bufferPolicy, Precision precision)
+    : impl_(std::make_unique<Impl>(workload, operators, layout, strategy, bufferPolicy, precision)) {}
 HipVerticalGemmProvider::~HipVerticalGemmProvider() = default;
 HipVerticalGemmProvider::HipVerticalGemmProvider(HipVerticalGemmProvider&&) noexcept = default;
 HipVerticalGemmProvider& HipVerticalGemmProvider::operator=(HipVerticalGemmProvider&&) noexcept = default;
 
-bool HipVerticalGemmProvider::supported() const noexcept { return true; }
-std::string HipVerticalGemmProvider::capability() const { return "hip grouped MFMA GEMM (gfx950)"; }
+bool HipVerticalGemmProvider::supported() const noexcept { return impl_->available; }
+std::string HipVerticalGemmProvider::capability() const { return impl_->capability_text; }
+VerticalGemmLayout HipVerticalGemmProvider::layout() const noexcept { return impl_->layout; }
+VerticalGemmStrategy HipVerticalGemmProvider::strategy() const noexcept { return impl_->strategy; }
+VerticalGemmBufferPolicy HipVerticalGemmProvider::bufferPolicy() const noexcept { return impl_->policy; }
+std::size_t HipVerticalGemmProvider::outerWorkers() const noexcept { return impl_->strategy.outerWorkers; }
 std::size_t HipVerticalGemmProvider::columns() const noexcept { return impl_->modes * impl_->w.fields; }
-std::size_t HipVerticalGemmProvider::physicalElements() const noexcept { return impl_->modes * impl_->w.planes(); }
-std::size_t HipVerticalGemmProvider::modalElements() const noexcept {
-    return impl_->modes * impl_->w.fields * impl_->nj;
-}
+std::size_t HipVerticalGemmProvider::physicalElements() const noexcept { return impl_->physical_count; }
+std::size_t HipVerticalGemmProvider::modalElements() const noexcept { return impl_->modal_count; }
 std::size_t HipVerticalGemmProvider::groupCount() const noexcept { return impl_->groups; }
 std::size_t HipVerticalGemmProvider::gemmCallsPerExecution() const noexcept { return 1; }
 std::size_t HipVerticalGemmProvider::matrixBytesPerDirection() const noexcept {
-    return skb_vertical_matrix_bytes(impl_->plan);
+    return impl_->plan ? skb_vertical_matrix_bytes(impl_->plan) : 0;
 }
+std::size_t HipVerticalGemmProvider::persistentBytes() const noexcept {
+    const auto& m = *impl_;
+    const std::size_t e = m.precision == Precision::float32 ? 4 : 8;
+    const std::size_t host = 16 * (m.physical_in.size() + m.modal_out.size() + m.modal_in.size() +
+                                   m.physical_out.size()) +
+                             8 * (m.physical_in_re.size() + m.physical_in_im.size() + m.modal_out_re.size() +
+                                  m.modal_out_im.size() + m.modal_in_re.size() + m.modal_in_im.size() +
+                                  m.physical_out_re.size() + m.physical_out_im.size());
+    const std::size_t device = m.available ? 2 * e * (m.physical_count + m.modal_count) * (m.layout ==
+                                   VerticalGemmLayout::split ? 2 : 1) : 0;
+    return host + device + 2 * matrixBytesPerDirection();
+}
+double HipVerticalGemmProvider::allocationSeconds() const noexcept { return impl_->allocation_seconds; }
+double HipVerticalGemmProvider::matrixPreparationSeconds() const noexcept { return impl_->preparation_seconds; }
+
+// src/vertical_gemm.cpp:698-729
 void HipVerticalGemmProvider::loadPhysicalInput(const Complex* input) {
-    impl_->physical.upload(reinterpret_cast<const double*>(input), impl_->stream.s);
+    auto& m = *impl_;
+    m.require_forward();
+    if (m.layout == VerticalGemmLayout::complexInterleaved) {
+        std::copy_n(input, m.physical_count, m.physical_in.data());
+        return;
+    }
+    for (std::size_t i = 0; i < m.physical_count; ++i) {
+        m.physical_in_re[i] = input[i].real;
+        m.physical_in_im[i] = input[i].imag;
+    }
 }
 void HipVerticalGemmProvider::loadModalInput(const Complex* input) {
-    impl_->modal.upload(reinterpret_cast<const double*>(input), impl_->stream.s);
+    auto& m = *impl_;
+    m.require_inverse();
+    if (m.layout == VerticalGemmLayout::complexInterleaved) {
+        std::copy_n(input, m.modal_count, m.modal_in.data());
+        return;
+    }
+    for (std::size_t i = 0; i < m.modal_count; ++i) {
+        m.modal_in_re[i] = input[i].real;
+        m.modal_in_im[i] = input[i].imag;
+    }
 }
+
+// WVM spectrum (z + Nz*field + P*(kx + (Nx/2+1)*ky)) -> retained physical
+// input (z + Nz*field + P*mode), conjugating modes stored through their
+// Hermitian partner (src/vertical_gemm.cpp:470-490).
+void HipVerticalGemmProvider::packPhysicalInputFromWvm(const std::vector<RetainedMode>& modes,
+                                                       const Complex* wvmSpectrum) {
+    auto& m = *impl_;
+    m.require_forward();
+    m.validate_modes(modes);
+    const std::size_t P = m.w.planes(), half = m.w.nxHalf();
+    for (std::size_t mode = 0; mode < modes.size(); ++mode) {
+        const auto& r = modes[mode];
+        const Complex* src = wvmSpectrum + P * (r.storedKx + half * r.storedKy);
+        const double sign = r.conjugatesStoredValue ? -1.0 : 1.0;
+        for (std::size_t plane = 0; plane < P; ++plane) {
+            const std::size_t dst = plane + P * mode;
+            if (m.layout == VerticalGemmLayout::complexInterleaved) {
+                m.physical_in[dst] = {src[plane].real, sign * src[plane].imag};
+            } else {
+                m.physical_in_re[dst] = src[plane].real;
+                m.physical_in_im[dst] = sign * src[plane].imag;
+            }
+        }
+    }
+}
+
+// Retained physical output -> zero-filled WVM spectrum; a k_x = 0 mode also
+// writes its conjugate at (0, Ny - k_y) unless k_y is 0 or Ny/2
+// (src/vertical_gemm.cpp:492-516).
+void HipVerticalGemmProvider::embedPhysicalOutputToWvm(const std::vector<RetainedMode>& modes,
+                                                       Complex* wvmSpectrum) const {
+    const auto& m = *impl_;
+    m.require_inverse();
+    m.validate_modes(modes);
+    std::fill_n(wvmSpectrum, m.w.spectrumElements(), Complex{});
+    const std::size_t P = m.w.planes(), half = m.w.nxHalf(), ny = m.w.ny;
+    for (std::size_t mode = 0; mode < modes.size(); ++mode) {
+        const auto& r = modes[mode];
+        Complex* dst = wvmSpectrum + P * (r.storedKx + half * r.storedKy);
+        const bool partner = r.storedKx == 0 && r.storedKy != 0 && 2 * r.storedKy != ny;
+        Complex* mirror = partner ? wvmSpectrum + P * (half * ((ny - r.storedKy) % ny)) : nullptr;
+        for (std::size_t plane = 0; plane < P; ++plane) {
+            const std::size_t src = plane + P * mode;
+            Complex v = m.layout == VerticalGemmLayout::complexInterleaved
+                            ? m.physical_out[src]
+                            : Complex{m.physical_out_re[src], m.physical_out_im[src]};
+            if (r.conjugatesStoredValue) v.imag = -v.imag;
+            dst[plane] = v;
+            if (mirror) mirror[plane] = {v.real, -v.imag};
+        }
+    }
+}
+
 void HipVerticalGemmProvider::executeForward() {
-    check(skb_vertical_forward(impl_->plan, impl_->physical.dev, impl_->modal.dev, impl_->stream.s));
+    auto& m = *impl_;
+    m.require_forward();
+    if (m.layout == VerticalGemmLayout::complexInterleaved) {
+        m.d_physical.upload(reinterpret_cast<const double*>(m.physical_in.data()), m.stream.s);
+        check(skb_vertical_forward(m.plan, m.d_physical.dev, m.d_modal.dev, m.stream.s));
+        m.d_modal.download(reinterpret_cast<double*>(m.modal_out.data()), m.stream.s);
+        return;
+    }
+    m.split_product(true, m.physical_in_re.data(), m.physical_in_im.data(), m.modal_out_re.data(),
+                    m.modal_out_im.data());
 }
 void HipVerticalGemmProvider::executeInverse() {
-    check(skb_vertical_inverse(impl_->plan, impl_->modal.dev, impl_->physical.dev, impl_->stream.s));
+    auto& m = *impl_;
+    m.require_inverse();
+    if (m.layout == VerticalGemmLayout::complexInterleaved) {
+        m.d_modal.upload(reinterpret_cast<const double*>(m.modal_in.data()), m.stream.s);
+        check(skb_vertical_inverse(m.plan, m.d_modal.dev, m.d_physical.dev, m.stream.s));
+        m.d_physical.download(reinterpret_cast<double*>(m.physical_out.data()), m.stream.s);
+        return;
+    }
+    m.split_product(false, m.modal_in_re.data(), m.modal_in_im.data(), m.physical_out_re.data(),
+                    m.physical_out_im.data());
 }
+void HipVerticalGemmProvider::executeForwardReal() {
+    auto& m = *impl_;
+    m.require_forward();
+    m.require_split();
+    m.split_product(true, m.physical_in_re.data(), nullptr, m.modal_out_re.data(), nullptr);
+}
+void HipVerticalGemmProvider::executeForwardImaginary() {
+    auto& m = *impl_;
+    m.require_forward();
+    m.require_split();
+    m.split_product(true, nullptr, m.physical_in_im.data(), nullptr, m.modal_out_im.data());
+}
+void HipVerticalGemmProvider::executeInverseReal() {
+    auto& m = *impl_;
+    m.require_inverse();
+    m.require_split();
+    m.split_product(false, m.modal_in_re.data(), nullptr, m.physical_out_re.data(), nullptr);
+}
+void HipVerticalGemmProvider::executeInverseImaginary() {
+    auto& m = *impl_;
+    m.require_inverse();
+    m.require_split();
+    m.split_product(false, nullptr, m.modal_in_im.data(), nullptr, m.physical_out_im.data());
+}
+void HipVerticalGemmProvider::executeSchedulerNoop() {
+    impl_->require_available();
+    hip_check(hipStreamSynchronize(impl_->stream.s), "sync");
+}
+
+// src/vertical_gemm.cpp:826-900
+Complex* HipVerticalGemmProvider::interleavedPhysicalInputData() {
+    impl_->require_forward();
+    impl_->require_interleaved();
+    return impl_->physical_in.data();
+}
+Complex* HipVerticalGemmProvider::interleavedModalInputData() {
+    impl_->require_inverse();
+    impl_->require_interleaved();
+    return impl_->modal_in.data();
+}
+const Complex* HipVerticalGemmProvider::interleavedModalOutputData() const {
+    impl_->require_forward();
+    impl_->require_interleaved();
+    return impl_->modal_out.data();
+}
+const Complex* HipVerticalGemmProvider::interleavedPhysicalOutputData() const {
+    impl_->require_inverse();
+    impl_->require_interleaved();
+    return impl_->physical_out.data();
+}
+double* HipVerticalGemmProvider::splitPhysicalInputRealData() {
+    impl_->require_forward();
+    impl_->require_split();
+    return impl_->physical_in_re.data();
+}
+double* HipVerticalGemmProvider::splitPhysicalInputImaginaryData() {
+    impl_->require_forward();
+    impl_->require_split();
+    return impl_->physical_in_im.data();
+}
+double* HipVerticalGemmProvider::splitModalInputRealData() {
+    impl_->require_inverse();
+    impl_->require_split();
+    return impl_->modal_in_re.data();
+}
+double* HipVerticalGemmProvider::splitModalInputImaginaryData() {
+    impl_->require_inverse();
+    impl_->require_split();
+    return impl_->modal_in_im.data();
+}
+const double* HipVerticalGemmProvider::splitModalOutputRealData() const {
+    impl_->require_forward();
+    impl_->require_split();
+    return impl_->modal_out_re.data();
+}
+const double* HipVerticalGemmProvider::splitModalOutputImaginaryData() const {
+    impl_->require_forward();
+    impl_->require_split();
+    return impl_->modal_out_im.data();
+}
+const double* HipVerticalGemmProvider::splitPhysicalOutputRealData() const {
+    impl_->require_inverse();
+    impl_->require_split();
+    return impl_->physical_out_re.data();
+}
+const double* HipVerticalGemmProvider::splitPhysicalOutputImaginaryData() const {
+    impl_->require_inverse();
+    impl_->require_split();
+    return impl_->physical_out_im.data();
+}
+
 void HipVerticalGemmProvider::copyForwardOutput(Complex* output) const {
-    impl_->modal.download(reinterpret_cast<double*>(output), impl_->stream.s);
+    const auto& m = *impl_;
+    m.require_forward();
+    if (m.layout == VerticalGemmLayout::complexInterleaved) {
+        std::copy_n(m.modal_out.data(), m.modal_count, output);
+        return;
+    }
+    for (std::size_t i = 0; i < m.modal_count; ++i) output[i] = {m.modal_out_re[i], m.modal_out_im[i]};
 }
 void HipVerticalGemmProvider::copyInverseOutput(Complex* output) const {
-    impl_->physical.download(reinterpret_cast<double*>(output), impl_->stream.s);
+    const auto& m = *impl_;
+    m.require_inverse();
+    if (m.layout == VerticalGemmLayout::complexInterleaved) {
+        std::copy_n(m.physical_out.data(), m.physical_count, output);
+        return;
+    }
+    for (std::size_t i = 0; i < m.physical_count; ++i) output[i] = {m.physical_out_re[i], m.physical_out_im[i]};
 }
+
 void HipVerticalGemmProvider::forwardDevice(const void* physical, void* modal, void* stream) {
+    impl_->require_forward();
     check(skb_vertical_forward(impl_->plan, physical, modal, stream));
 }
 void HipVerticalGemmProvider::inverseDevice(const void* modal, void* physical, void* stream) {
+    impl_->require_inverse();
     check(skb_vertical_inverse(impl_->plan, modal, physical, stream));
+}
+void HipVerticalGemmProvider::forwardSplitDevice(const void* physicalReal, const void* physicalImag,
+                                                 void* modalReal, void* modalImag, void* stream) {
+    impl_->require_forward();
+    impl_->require_split();
+    check(skb_vertical_forward_split(impl_->plan, physicalReal, physicalImag, modalReal, modalImag, stream));
+}
+void HipVerticalGemmProvider::inverseSplitDevice(const void* modalReal, const void* modalImag, void* physicalReal,
+                                                 void* physicalImag, void* stream) {
+    impl_->require_inverse();
+    impl_->require_split();
+    check(skb_vertical_inverse_split(impl_->plan, modalReal, modalImag, physicalReal, physicalImag, stream));
 }
 std::string HipVerticalGemmProvider::libraryIdentity() const { return "libskb_hip.so"; }
 

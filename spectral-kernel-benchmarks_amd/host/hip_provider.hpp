@@ -13,8 +13,12 @@
 //                                      (include/skbench/skbench.hpp:534-621)
 // Host-pointer methods are the reference's "adapter" scope: they stage the
 // caller's FP64 arrays through device buffers owned by the provider (allocated
-// at construction; nothing is allocated per call).  The *Device methods are
-// the "primitive" scope on caller-owned device memory.
+// at construction; nothing is allocated per call) and return only after the
+// provider's stream has finished the work, so host results are complete and
+// a host timer around a call measures the call (the reference's CPU calls are
+// synchronous).  The *Device methods are the "primitive" scope on
+// caller-owned device memory: they enqueue on the caller's stream and return
+// at once.
 //
 // The value types below are layout-identical to the reference's
 // (skbench::Complex, Workload, RetainedMode, VerticalModeGroup); the static
@@ -75,6 +79,24 @@ enum class Precision { float64, float32 };
 
 enum class FixtureKind { impulse, sinusoid, random, dc, nyquist, sinusoid5x17 };
 
+// Data layout of the FFT provider's spectra (FFTWDataLayout,
+// include/skbench/skbench.hpp:337-340): interleaved Complex arrays, or
+// separate real and imaginary arrays in the same WVM element order.
+enum class DataLayout { interleaved, split };
+
+// Vertical GEMM options (include/skbench/skbench.hpp:125-151).  The device
+// product is one grouped launch, so the schedule and worker count are kept
+// for API parity and reported back; the buffer policy decides which
+// direction's staging exists, as in the reference.
+enum class VerticalGemmLayout { complexInterleaved, split };
+std::string verticalGemmLayoutName(VerticalGemmLayout layout);
+enum class VerticalGemmBufferPolicy { bidirectional, forwardOnly, inverseOnly };
+enum class VerticalGemmSchedule { serial, outerStatic, outerDynamic };
+struct VerticalGemmStrategy {
+    VerticalGemmSchedule schedule = VerticalGemmSchedule::serial;
+    std::size_t outerWorkers = 1;
+};
+
 // ---- model (src/model.cpp) ------------------------------------------------
 std::vector<RetainedMode> retainedHorizontalModes(const Workload& workload);
 std::vector<VerticalModeGroup> squaredWavenumberGroups(const std::vector<RetainedMode>& modes);
@@ -92,6 +114,7 @@ std::vector<double> makeFixture(const Workload& workload, FixtureKind fixture, s
 class HipProvider {
 public:
     HipProvider(const Workload& workload, Precision precision = Precision::float64);
+    HipProvider(const Workload& workload, DataLayout layout, Precision precision = Precision::float64);
     ~HipProvider();
     HipProvider(HipProvider&&) noexcept;
     HipProvider& operator=(HipProvider&&) noexcept;
@@ -105,11 +128,31 @@ public:
                              Complex* retainedSpectrum);
     void embedRetainedOuter(const std::vector<RetainedMode>& modes, const Complex* retainedSpectrum,
                             Complex* wvmSpectrum);
+    // split forms (include/skbench/skbench.hpp:380-398, src/fftw_provider.cpp:331-395, 701-830):
+    // retained split arrays hold (plane, mode) at plane + P * mode; split
+    // spectra hold the WVM order of the interleaved spectrum.  forwardSplit,
+    // inverseSplit, gatherRetainedSplitOuter and embedRetainedSplitOuter need
+    // the split layout; forward and inverse need the interleaved one.
+    void gatherRetainedToSplitOuter(const std::vector<RetainedMode>& modes, const Complex* spectrum,
+                                    double* retainedReal, double* retainedImag, double scale = 1.0);
+    void embedRetainedFromSplitOuter(const std::vector<RetainedMode>& modes, const double* retainedReal,
+                                     const double* retainedImag, Complex* spectrum);
+    void scaleRetainedSplitOuter(const std::vector<RetainedMode>& modes, double* retainedReal, double* retainedImag,
+                                 double scale);
+    void forwardSplit(const double* input, double* wvmSpectrumReal, double* wvmSpectrumImag);
+    void inverseSplit(double* wvmSpectrumReal, double* wvmSpectrumImag, double* output);
+    void gatherRetainedSplitOuter(const std::vector<RetainedMode>& modes, const double* spectrumReal,
+                                  const double* spectrumImag, double* retainedReal, double* retainedImag);
+    void embedRetainedSplitOuter(const std::vector<RetainedMode>& modes, const double* retainedReal,
+                                 const double* retainedImag, double* spectrumReal, double* spectrumImag);
     // primitive scope (device arrays in the plan's precision)
     void forwardDevice(const void* real, void* wvm, void* stream);
     void inverseDevice(void* wvm, void* real, void* stream);
 
     void executeSchedulerNoop();
+    DataLayout layout() const noexcept;
+    bool splitInPlaceWvmOrderSupported() const noexcept { return false; }
+    std::string splitInPlaceWvmOrderCapability() const;
     bool supported() const noexcept;
     std::string capability() const;
     double allocationSeconds() const noexcept;
@@ -235,29 +278,78 @@ class HipVerticalGemmProvider {
 public:
     HipVerticalGemmProvider(const Workload& workload, const GroupedVerticalOperators& operators,
                             Precision precision = Precision::float64);
+    HipVerticalGemmProvider(const Workload& workload, const GroupedVerticalOperators& operators,
+                            VerticalGemmLayout layout, Precision precision = Precision::float64);
+    HipVerticalGemmProvider(const Workload& workload, const GroupedVerticalOperators& operators,
+                            VerticalGemmLayout layout, VerticalGemmStrategy strategy,
+                            VerticalGemmBufferPolicy bufferPolicy = VerticalGemmBufferPolicy::bidirectional,
+                            Precision precision = Precision::float64);
     ~HipVerticalGemmProvider();
     HipVerticalGemmProvider(HipVerticalGemmProvider&&) noexcept;
     HipVerticalGemmProvider& operator=(HipVerticalGemmProvider&&) noexcept;
     HipVerticalGemmProvider(const HipVerticalGemmProvider&) = delete;
     HipVerticalGemmProvider& operator=(const HipVerticalGemmProvider&) = delete;
 
+    // false (with the library's reason in capability()) when the GEMM is not
+    // instantiated for these dimensions; every execution call then throws
+    // std::runtime_error(capability()), as the reference's unavailable BLAS does.
     bool supported() const noexcept;
     std::string capability() const;
+    VerticalGemmLayout layout() const noexcept;
+    VerticalGemmStrategy strategy() const noexcept;
+    VerticalGemmBufferPolicy bufferPolicy() const noexcept;
+    std::size_t outerWorkers() const noexcept;
     std::size_t columns() const noexcept;
     std::size_t physicalElements() const noexcept;
     std::size_t modalElements() const noexcept;
     std::size_t groupCount() const noexcept;
     std::size_t gemmCallsPerExecution() const noexcept;  // one grouped launch
     std::size_t matrixBytesPerDirection() const noexcept;
+    std::size_t persistentBytes() const noexcept;  // host staging + device staging + device matrices
+    std::size_t schedulerPersistentBytes() const noexcept { return 0; }
+    std::size_t minimumAlignmentBytes() const noexcept { return 256; }
+    double allocationSeconds() const noexcept;
+    double matrixPreparationSeconds() const noexcept;  // plan creation: matrix split and upload
+    double schedulerSetupSeconds() const noexcept { return 0.0; }
+    bool hasOpaqueSchedulerMemory() const noexcept { return false; }
 
+    // adapter scope: the provider's host arrays (FP64, the layout's element
+    // order: retained index z + Nz * field + P * mode, modal index
+    // j + Nj * field + Nj * fields * mode); execute* uploads the input array,
+    // runs the grouped product and downloads the output array before returning.
     void loadPhysicalInput(const Complex* input);
     void loadModalInput(const Complex* input);
+    void packPhysicalInputFromWvm(const std::vector<RetainedMode>& modes, const Complex* wvmSpectrum);
     void executeForward();
     void executeInverse();
+    void executeForwardReal();  // split layout: the real (imaginary) part of the product alone
+    void executeForwardImaginary();
+    void executeInverseReal();
+    void executeInverseImaginary();
+    void executeSchedulerNoop();
+    Complex* interleavedPhysicalInputData();
+    Complex* interleavedModalInputData();
+    const Complex* interleavedModalOutputData() const;
+    const Complex* interleavedPhysicalOutputData() const;
+    double* splitPhysicalInputRealData();
+    double* splitPhysicalInputImaginaryData();
+    double* splitModalInputRealData();
+    double* splitModalInputImaginaryData();
+    const double* splitModalOutputRealData() const;
+    const double* splitModalOutputImaginaryData() const;
+    const double* splitPhysicalOutputRealData() const;
+    const double* splitPhysicalOutputImaginaryData() const;
     void copyForwardOutput(Complex* output) const;
     void copyInverseOutput(Complex* output) const;
+    void embedPhysicalOutputToWvm(const std::vector<RetainedMode>& modes, Complex* wvmSpectrum) const;
+
+    // primitive scope (device arrays in the plan's precision, caller's stream)
     void forwardDevice(const void* physical, void* modal, void* stream);
     void inverseDevice(const void* modal, void* physical, void* stream);
+    void forwardSplitDevice(const void* physicalReal, const void* physicalImag, void* modalReal, void* modalImag,
+                            void* stream);
+    void inverseSplitDevice(const void* modalReal, const void* modalImag, void* physicalReal, void* physicalImag,
+                            void* stream);
     std::string libraryIdentity() const;
 
 private:

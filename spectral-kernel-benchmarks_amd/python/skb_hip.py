@@ -194,6 +194,12 @@ def _load():
         "skb_retained_operator_device_bytes": (u64, [vp]),
         "skb_retained_operator_set_fused": (i32, [vp, i32]),
         "skb_retained_operator_fused": (i32, [vp]),
+        "skb_interleave": (i32, [i32, u64, vp, vp, vp, vp]),
+        "skb_deinterleave": (i32, [i32, u64, vp, vp, vp, dbl, vp]),
+        "skb_scale_split": (i32, [i32, u64, vp, vp, dbl, vp]),
+        "skb_vertical_plan_reserve_split": (i32, [vp]),
+        "skb_vertical_forward_split": (i32, [vp, vp, vp, vp, vp, vp]),
+        "skb_vertical_inverse_split": (i32, [vp, vp, vp, vp, vp, vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -231,7 +237,8 @@ EXPORTED = [
     "skb_retained_operator_create", "skb_retained_operator_destroy", "skb_retained_operator_execute",
     "skb_retained_operator_stage", "skb_retained_operator_set_path", "skb_retained_operator_path",
     "skb_retained_operator_real", "skb_retained_operator_device_bytes", "skb_retained_operator_set_fused",
-    "skb_retained_operator_fused",
+    "skb_retained_operator_fused", "skb_interleave", "skb_deinterleave", "skb_scale_split",
+    "skb_vertical_plan_reserve_split", "skb_vertical_forward_split", "skb_vertical_inverse_split",
 ]
 
 
@@ -437,6 +444,35 @@ def _device_view(ptr, numel: int, dtype, owner):
         raise LogicError(3, "null device pointer")
     typestr = {torch.float32: "<f4", torch.float64: "<f8"}[dtype]
     return torch.as_tensor(_DevicePointer(int(ptr), numel, typestr, owner), device="cuda")
+
+
+def _dev_opt(t, numel: int, dtype, name: str):
+    """_dev, or a null pointer for None (a skipped split component)."""
+    return ctypes.c_void_p(0) if t is None else _dev(t, numel, dtype, name)
+
+
+def interleave(precision: int, real, imaginary, out, stream=None):
+    """out[i] = real[i] + 1j * imaginary[i] on the device (skb_interleave)."""
+    n = out.numel()
+    rd = real_dtype(precision)
+    check(lib.skb_interleave(precision, n, _dev(real, n, rd, "real"), _dev(imaginary, n, rd, "imaginary"),
+                             _dev(out, n, complex_dtype(precision), "out"), _stream(stream)))
+
+
+def deinterleave(precision: int, inp, real, imaginary, scale: float = 1.0, stream=None):
+    """real[i], imaginary[i] = scale * inp[i] parts (skb_deinterleave; None skips a part)."""
+    n = inp.numel()
+    rd = real_dtype(precision)
+    check(lib.skb_deinterleave(precision, n, _dev(inp, n, complex_dtype(precision), "in"),
+                               _dev_opt(real, n, rd, "real"), _dev_opt(imaginary, n, rd, "imaginary"), scale,
+                               _stream(stream)))
+
+
+def scale_split(precision: int, real, imaginary, scale: float, stream=None):
+    n = real.numel()
+    rd = real_dtype(precision)
+    check(lib.skb_scale_split(precision, n, _dev(real, n, rd, "real"), _dev(imaginary, n, rd, "imaginary"), scale,
+                              _stream(stream)))
 
 
 def _stream(stream):
@@ -693,6 +729,25 @@ class VerticalPlan:
         cd = complex_dtype(self.precision)
         check(lib.skb_vertical_inverse(self.h, _dev(modal, self.modes * self.w.fields * self.nj, cd, "modal"),
                                        _dev(physical, self.modes * self.w.planes, cd, "physical"), _stream(stream)))
+
+    def reserve_split(self):
+        """Allocate the interleaved staging of the split-layout calls (once)."""
+        check(lib.skb_vertical_plan_reserve_split(self.h))
+
+    def _split(self, fn, n_in, n_out, in_re, in_im, out_re, out_im, stream):
+        rd = real_dtype(self.precision)
+        check(fn(self.h, _dev_opt(in_re, n_in, rd, "input real"), _dev_opt(in_im, n_in, rd, "input imaginary"),
+                 _dev_opt(out_re, n_out, rd, "output real"), _dev_opt(out_im, n_out, rd, "output imaginary"),
+                 _stream(stream)))
+
+    def forward_split(self, physical_real, physical_imag, modal_real, modal_imag, stream=None):
+        """Split-layout forward product; a part whose input and output are None is left out."""
+        self._split(lib.skb_vertical_forward_split, self.modes * self.w.planes, self.modes * self.w.fields * self.nj,
+                    physical_real, physical_imag, modal_real, modal_imag, stream)
+
+    def inverse_split(self, modal_real, modal_imag, physical_real, physical_imag, stream=None):
+        self._split(lib.skb_vertical_inverse_split, self.modes * self.w.fields * self.nj, self.modes * self.w.planes,
+                    modal_real, modal_imag, physical_real, physical_imag, stream)
 
     def matrix_bytes(self) -> int:
         return int(lib.skb_vertical_matrix_bytes(self.h))

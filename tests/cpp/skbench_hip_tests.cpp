@@ -46,6 +46,13 @@ struct OrMode {
     double radial;
 };
 
+bool same_values(const std::vector<skbhip::Complex>& a, const std::vector<skbhip::Complex>& b) {
+    if (a.size() != b.size()) return false;
+    for (size_t i = 0; i < a.size(); ++i)
+        if (a[i].real != b[i].real || a[i].imag != b[i].imag) return false;
+    return true;
+}
+
 std::vector<OrMode> to_or(const std::vector<skbhip::RetainedMode>& modes) {
     std::vector<OrMode> out(modes.size());
     for (size_t i = 0; i < modes.size(); ++i)
@@ -261,6 +268,201 @@ int main() {
                 threw = true;
             }
             require(threw, "WVM-direct requires Nj = 2(Nz-1)/3");
+        }
+
+        // Split-layout vertical GEMM (VerticalGemmLayout::split,
+        // src/vertical_gemm.cpp:360-390, 698-935).  Every output below is read
+        // right after the call returns, with no device synchronisation of the
+        // test's own: the adapter calls finish their stream before returning.
+        {
+            HipVerticalGemmProvider split(w, ops, VerticalGemmLayout::split);
+            require(split.supported() && split.layout() == VerticalGemmLayout::split, "split vertical provider");
+            double* inRe = split.splitPhysicalInputRealData();
+            double* inIm = split.splitPhysicalInputImaginaryData();
+            for (size_t i = 0; i < phys.size(); ++i) {
+                inRe[i] = phys[i].real;
+                inIm[i] = phys[i].imag;
+            }
+            split.executeForward();
+            const double* outRe = split.splitModalOutputRealData();
+            const double* outIm = split.splitModalOutputImaginaryData();
+            bool same = true;
+            for (size_t i = 0; i < modal.size(); ++i) same = same && outRe[i] == modal[i].real && outIm[i] == modal[i].imag;
+            require(same, "split forward equals the interleaved forward");
+            std::vector<double> keepRe(outRe, outRe + modal.size()), keepIm(outIm, outIm + modal.size());
+            // the parts one at a time: the real part depends on the real input alone
+            for (size_t i = 0; i < phys.size(); ++i) inIm[i] = 1.0e6;
+            split.executeForwardReal();
+            bool realPart = true;
+            for (size_t i = 0; i < modal.size(); ++i) realPart = realPart && outRe[i] == keepRe[i] && outIm[i] == keepIm[i];
+            require(realPart, "executeForwardReal writes the real part alone");
+            for (size_t i = 0; i < phys.size(); ++i) inIm[i] = phys[i].imag;
+            for (size_t i = 0; i < phys.size(); ++i) inRe[i] = -3.0;
+            split.executeForwardImaginary();
+            bool imagPart = true;
+            for (size_t i = 0; i < modal.size(); ++i) imagPart = imagPart && outRe[i] == keepRe[i] && outIm[i] == keepIm[i];
+            require(imagPart, "executeForwardImaginary writes the imaginary part alone");
+            std::vector<Complex> copied(modal.size());
+            split.copyForwardOutput(copied.data());
+            require(or_max_relative_error_complex(copied.data(), modalOracle.data(), copied.size()) <= 1e-12,
+                    "split forward vs scalar products");
+
+            // inverse, interleaved and split, with the modal output as input
+            vertical.loadModalInput(modal.data());
+            vertical.executeInverse();
+            std::vector<Complex> back(phys.size());
+            vertical.copyInverseOutput(back.data());
+            double* mRe = split.splitModalInputRealData();
+            double* mIm = split.splitModalInputImaginaryData();
+            for (size_t i = 0; i < modal.size(); ++i) {
+                mRe[i] = modal[i].real;
+                mIm[i] = modal[i].imag;
+            }
+            split.executeInverseReal();
+            split.executeInverseImaginary();
+            const double* pRe = split.splitPhysicalOutputRealData();
+            const double* pIm = split.splitPhysicalOutputImaginaryData();
+            bool inv = true;
+            for (size_t i = 0; i < back.size(); ++i) inv = inv && pRe[i] == back[i].real && pIm[i] == back[i].imag;
+            require(inv, "split inverse parts equal the interleaved inverse");
+            split.executeInverse();
+            bool inv2 = true;
+            for (size_t i = 0; i < back.size(); ++i) inv2 = inv2 && pRe[i] == back[i].real && pIm[i] == back[i].imag;
+            require(inv2, "split inverse equals the interleaved inverse");
+            // K^2 family: orthonormal rows, so inverse(forward(x)) keeps the
+            // projection of x; compare against the scalar inverse product.
+            const Complex* ip = vertical.interleavedPhysicalOutputData();
+            require(ip[0].real == back[0].real && ip[0].imag == back[0].imag, "interleaved physical output accessor");
+
+            // WVM pack / embed (src/vertical_gemm.cpp:470-516) against the
+            // oracle's gather / embed
+            std::vector<Complex> full(w.spectrumElements());
+            for (size_t i = 0; i < full.size(); ++i)
+                full[i] = {std::cos(0.23 * static_cast<double>(i)), std::sin(0.41 * static_cast<double>(i) + 0.2)};
+            std::vector<Complex> gathered(phys.size());
+            or_gather_retained(8, 8, 7, 2, om.data(), om.size(), reinterpret_cast<const or_complex*>(full.data()),
+                               reinterpret_cast<or_complex*>(gathered.data()));
+            for (auto* provider : {&vertical, &split}) {
+                provider->packPhysicalInputFromWvm(modes, full.data());
+                provider->executeForward();
+                std::vector<Complex> out(modal.size()), expect(modal.size());
+                provider->copyForwardOutput(out.data());
+                vertical.loadPhysicalInput(gathered.data());
+                vertical.executeForward();
+                vertical.copyForwardOutput(expect.data());
+                require(same_values(out, expect), "packPhysicalInputFromWvm equals the gathered load");
+                provider->executeInverse();
+                std::vector<Complex> compactOut(phys.size()), wvmOut(full.size(), Complex{9.0, 9.0}),
+                    wvmOracle(full.size());
+                provider->copyInverseOutput(compactOut.data());
+                provider->embedPhysicalOutputToWvm(modes, wvmOut.data());
+                or_embed_retained(8, 8, 7, 2, om.data(), om.size(), reinterpret_cast<or_complex*>(compactOut.data()),
+                                  reinterpret_cast<or_complex*>(wvmOracle.data()));
+                bool embedOk = true;
+                for (size_t i = 0; i < full.size(); ++i)
+                    embedOk = embedOk && wvmOut[i].real == wvmOracle[i].real && wvmOut[i].imag == wvmOracle[i].imag;
+                require(embedOk, std::string("embedPhysicalOutputToWvm vs oracle embed, ") +
+                                     verticalGemmLayoutName(provider->layout()));
+            }
+
+            // component / policy errors (src/vertical_gemm.cpp:433-467)
+            int logic = 0;
+            auto expectLogic = [&](auto&& f) {
+                try {
+                    f();
+                } catch (const std::logic_error&) {
+                    ++logic;
+                }
+            };
+            expectLogic([&] { (void)split.interleavedPhysicalInputData(); });
+            expectLogic([&] { (void)vertical.splitModalOutputRealData(); });
+            expectLogic([&] { vertical.executeForwardReal(); });
+            HipVerticalGemmProvider forwardOnly(w, ops, VerticalGemmLayout::split, VerticalGemmStrategy{},
+                                                VerticalGemmBufferPolicy::forwardOnly);
+            expectLogic([&] { forwardOnly.executeInverse(); });
+            expectLogic([&] { (void)forwardOnly.splitModalInputRealData(); });
+            require(logic == 5, "layout and buffer-policy errors are std::logic_error");
+            bool wrongModes = false;
+            try {
+                split.packPhysicalInputFromWvm(std::vector<RetainedMode>(modes.begin(), modes.end() - 1), full.data());
+            } catch (const std::invalid_argument&) {
+                wrongModes = true;
+            }
+            require(wrongModes, "packPhysicalInputFromWvm checks the retained-mode count");
+
+            // dimensions the FP64 GEMM does not instantiate: supported() false,
+            // execution throws std::runtime_error with the capability text
+            const Workload deep{8, 8, 257, 1, 1.0, 1.0, true};
+            const auto deepModes = retainedHorizontalModes(deep);
+            HipVerticalGemmProvider unsupported(deep, squaredWavenumberVerticalFixture(deep, deepModes));
+            require(!unsupported.supported() && unsupported.capability().rfind("unsupported", 0) == 0,
+                    "FP64 Nz = 257 reports unsupported");
+            bool runtime = false;
+            try {
+                unsupported.executeForward();
+            } catch (const std::runtime_error&) {
+                runtime = true;
+            }
+            require(runtime, "unsupported vertical provider throws std::runtime_error on execution");
+        }
+
+        // FFT provider split forms (src/fftw_provider.cpp:331-395, 701-830)
+        // against the interleaved forms already checked against the oracle.
+        {
+            HipProvider splitFft(w, DataLayout::split);
+            require(splitFft.layout() == DataLayout::split && !splitFft.splitInPlaceWvmOrderSupported(),
+                    "split FFT provider layout");
+            const auto input = makeFixture(w, FixtureKind::random, 4242);
+            std::vector<Complex> ref(w.spectrumElements());
+            full.forward(input.data(), ref.data());
+            const std::size_t ns = ref.size(), nr = modes.size() * w.planes();
+            std::vector<double> sRe(ns), sIm(ns);
+            splitFft.forwardSplit(input.data(), sRe.data(), sIm.data());
+            bool fwd = true;
+            for (size_t i = 0; i < ns; ++i) fwd = fwd && sRe[i] == ref[i].real && sIm[i] == ref[i].imag;
+            require(fwd, "forwardSplit equals forward");
+            std::vector<Complex> refRet(nr), embedRef(ns);
+            full.gatherRetainedOuter(modes, ref.data(), refRet.data());
+            std::vector<double> rRe(nr), rIm(nr);
+            splitFft.gatherRetainedSplitOuter(modes, sRe.data(), sIm.data(), rRe.data(), rIm.data());
+            bool gat = true;
+            for (size_t i = 0; i < nr; ++i) gat = gat && rRe[i] == refRet[i].real && rIm[i] == refRet[i].imag;
+            require(gat, "gatherRetainedSplitOuter equals gatherRetainedOuter");
+            splitFft.gatherRetainedToSplitOuter(modes, ref.data(), rRe.data(), rIm.data(), 0.5);
+            bool gat2 = true;
+            for (size_t i = 0; i < nr; ++i)
+                gat2 = gat2 && rRe[i] == 0.5 * refRet[i].real && rIm[i] == 0.5 * refRet[i].imag;
+            require(gat2, "gatherRetainedToSplitOuter applies the scale");
+            splitFft.scaleRetainedSplitOuter(modes, rRe.data(), rIm.data(), 2.0);
+            bool sc = true;
+            for (size_t i = 0; i < nr; ++i) sc = sc && rRe[i] == refRet[i].real && rIm[i] == refRet[i].imag;
+            require(sc, "scaleRetainedSplitOuter");
+            full.embedRetainedOuter(modes, refRet.data(), embedRef.data());
+            std::vector<Complex> emb(ns, Complex{5.0, 5.0});
+            splitFft.embedRetainedFromSplitOuter(modes, rRe.data(), rIm.data(), emb.data());
+            require(same_values(emb, embedRef), "embedRetainedFromSplitOuter equals embedRetainedOuter");
+            std::vector<double> eRe(ns, 5.0), eIm(ns, 5.0);
+            splitFft.embedRetainedSplitOuter(modes, rRe.data(), rIm.data(), eRe.data(), eIm.data());
+            bool emb2 = true;
+            for (size_t i = 0; i < ns; ++i) emb2 = emb2 && eRe[i] == embedRef[i].real && eIm[i] == embedRef[i].imag;
+            require(emb2, "embedRetainedSplitOuter equals embedRetainedOuter");
+            std::vector<double> outA(w.realElements()), outB(w.realElements());
+            auto destroyed = embedRef;
+            full.inverse(destroyed.data(), outA.data());
+            splitFft.inverseSplit(eRe.data(), eIm.data(), outB.data());
+            require(outA == outB, "inverseSplit equals inverse");
+            int layoutErrors = 0;
+            try {
+                splitFft.forward(input.data(), ref.data());
+            } catch (const std::logic_error&) {
+                ++layoutErrors;
+            }
+            try {
+                full.forwardSplit(input.data(), sRe.data(), sIm.data());
+            } catch (const std::logic_error&) {
+                ++layoutErrors;
+            }
+            require(layoutErrors == 2, "FFT entry points check the planned data layout");
         }
 
         HipPointwiseAdvectionExecutor advection(37, 0.03125);
