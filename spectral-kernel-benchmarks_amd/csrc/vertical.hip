@@ -1718,26 +1718,33 @@ uint64_t skb_vertical_matrix_bytes(const skb_vertical_plan* plan) { return plan 
 namespace {
 
 // Split layout over the interleaved kernels: interleave the input pair into
-// the plan's staging, one grouped launch, deinterleave into the output pair.
+// interleaved staging, one grouped launch, deinterleave into the output pair.
 // The matrices are real, so the real (imaginary) output part depends on the
 // real (imaginary) input part alone: a component whose input and output are
 // both null is left out (the other input fills both staged parts and only
-// the requested output part is written).
-void run_split(skb_vertical_plan* plan, bool forward, const void* in_re, const void* in_im, void* out_re,
-               void* out_im, void* stream) {
-    if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
-    if (!plan->split_physical.ptr)
-        fail(SKB_LOGIC_ERROR, "Split GEMM component requested from the complex GEMM provider.");
+// the requested output part is written).  Every staged entry is written
+// before it is read, so the staging never needs clearing.
+void run_split_staged(skb_vertical_plan* plan, bool forward, const void* in_re, const void* in_im, void* out_re,
+                      void* out_im, void* stage_physical, void* stage_modal, void* stream) {
     if ((in_re == nullptr) != (out_re == nullptr) || (in_im == nullptr) != (out_im == nullptr) ||
         (!in_re && !in_im))
         fail(SKB_INVALID_ARGUMENT, "a split component needs both its input and its output array");
     const hipStream_t s = as_stream(stream);
     const uint64_t np = plan->modes * plan->w.nz * plan->w.fields, nm = plan->modes * plan->nj * plan->w.fields;
-    void* stage_in = forward ? plan->split_physical.ptr : plan->split_modal.ptr;
-    void* stage_out = forward ? plan->split_modal.ptr : plan->split_physical.ptr;
+    void* stage_in = forward ? stage_physical : stage_modal;
+    void* stage_out = forward ? stage_modal : stage_physical;
     interleave(plan->precision, forward ? np : nm, in_re ? in_re : in_im, in_im ? in_im : in_re, stage_in, s);
     run(plan, forward, stage_in, stage_out, stream, false);
     deinterleave(plan->precision, forward ? nm : np, stage_out, out_re, out_im, 1.0, s);
+}
+
+void run_split(skb_vertical_plan* plan, bool forward, const void* in_re, const void* in_im, void* out_re,
+               void* out_im, void* stream) {
+    if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
+    if (!plan->split_physical.ptr)
+        fail(SKB_LOGIC_ERROR, "Split GEMM component requested from the complex GEMM provider.");
+    run_split_staged(plan, forward, in_re, in_im, out_re, out_im, plan->split_physical.ptr, plan->split_modal.ptr,
+                     stream);
 }
 
 }  // namespace
