@@ -301,6 +301,18 @@ int skb_vertical_forward_split(skb_vertical_plan* plan, const void* physical_rea
                                void* modal_real, void* modal_imag, void* stream);
 int skb_vertical_inverse_split(skb_vertical_plan* plan, const void* modal_real, const void* modal_imag,
                                void* physical_real, void* physical_imag, void* stream);
+/* skb_vertical_forward_split without any allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops): it needs no
+ * skb_vertical_plan_reserve_split and calls no hipMalloc, hipFree or other
+ * allocating routine.  Its interleaved staging lives in the library's static
+ * device scratch of 64 MiB per GPU (held until the process ends); a plan
+ * whose staging, modes*Nz*fields + modes*Nj*fields complex values (the first
+ * part rounded up to 256 bytes), exceeds it gets SKB_INVALID_ARGUMENT.  All
+ * calls on one GPU share that scratch: calls that overlap on one GPU (on
+ * different streams, or from different threads) race. */
+int skb_vertical_forward_split_noalloc(skb_vertical_plan* plan, const void* physical_real, const void* physical_imag,
+                                       void* modal_real, void* modal_imag, void* stream);
 /* VerticalGemmProvider::matrixBytesPerDirection. */
 uint64_t skb_vertical_matrix_bytes(const skb_vertical_plan* plan);
 /* Launch shape: workgroups per launch, 0 = one per 16-column tile (the

@@ -36,6 +36,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -1738,6 +1739,32 @@ void run_split_staged(skb_vertical_plan* plan, bool forward, const void* in_re, 
     deinterleave(plan->precision, forward ? nm : np, stage_out, out_re, out_im, 1.0, s);
 }
 
+// Staging of skb_vertical_forward_split_noalloc: a file-scope device array,
+// one copy per GPU at its own address, held until the process ends.  Host
+// code never passes the array's name to a kernel or copy (on the host it is
+// only a stand-in symbol): it uses the per-device address that
+// hipGetSymbolAddress returns, looked up once per device.
+constexpr uint64_t kNoAllocScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+std::mutex scratch_mutex;
+constexpr int kMaxDevices = 64;
+void* scratch_ptr[kMaxDevices] = {};
+
+void* scratch_device_address() {
+    int device = 0;
+    check_hip(hipGetDevice(&device), "hipGetDevice");
+    if (device < 0 || device >= kMaxDevices) fail(SKB_UNSUPPORTED, "device ordinal above 63");
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[device]) {
+        void* ptr = nullptr;
+        check_hip(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!ptr) fail(SKB_RUNTIME_ERROR, "hipGetSymbolAddress(scratch) returned a null address");
+        scratch_ptr[device] = ptr;
+    }
+    return scratch_ptr[device];
+}
+
 void run_split(skb_vertical_plan* plan, bool forward, const void* in_re, const void* in_im, void* out_re,
                void* out_im, void* stream) {
     if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
@@ -1770,6 +1797,24 @@ int skb_vertical_forward_split(skb_vertical_plan* plan, const void* physical_rea
 int skb_vertical_inverse_split(skb_vertical_plan* plan, const void* modal_real, const void* modal_imag,
                                void* physical_real, void* physical_imag, void* stream) {
     return guarded([&] { run_split(plan, false, modal_real, modal_imag, physical_real, physical_imag, stream); });
+}
+
+int skb_vertical_forward_split_noalloc(skb_vertical_plan* plan, const void* physical_real, const void* physical_imag,
+                                       void* modal_real, void* modal_imag, void* stream) {
+    return guarded([&] {
+        if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
+        if (plan->wvm) fail(SKB_LOGIC_ERROR, "the WVM-direct GEMM has no split layout");
+        const uint64_t cb = complex_bytes(plan->precision);
+        const uint64_t physical_bytes = (plan->modes * plan->w.nz * plan->w.fields * cb + 255) & ~uint64_t(255);
+        const uint64_t modal_bytes = plan->modes * plan->nj * plan->w.fields * cb;
+        if (physical_bytes + modal_bytes > kNoAllocScratchBytes)
+            fail(SKB_INVALID_ARGUMENT, "split staging of " + std::to_string(physical_bytes + modal_bytes) +
+                                           " bytes exceeds the " + std::to_string(kNoAllocScratchBytes) +
+                                           "-byte no-allocation scratch");
+        char* base = static_cast<char*>(scratch_device_address());
+        run_split_staged(plan, true, physical_real, physical_imag, modal_real, modal_imag, base,
+                         base + physical_bytes, stream);
+    });
 }
 
 int skb_vertical_plan_set_grid(skb_vertical_plan* plan, int workgroups) {

@@ -200,6 +200,7 @@ def _load():
         "skb_vertical_plan_reserve_split": (i32, [vp]),
         "skb_vertical_forward_split": (i32, [vp, vp, vp, vp, vp, vp]),
         "skb_vertical_inverse_split": (i32, [vp, vp, vp, vp, vp, vp]),
+        "skb_vertical_forward_split_noalloc": (i32, [vp, vp, vp, vp, vp, vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -239,6 +240,7 @@ EXPORTED = [
     "skb_retained_operator_real", "skb_retained_operator_device_bytes", "skb_retained_operator_set_fused",
     "skb_retained_operator_fused", "skb_interleave", "skb_deinterleave", "skb_scale_split",
     "skb_vertical_plan_reserve_split", "skb_vertical_forward_split", "skb_vertical_inverse_split",
+    "skb_vertical_forward_split_noalloc",
 ]
 
 
@@ -744,6 +746,12 @@ class VerticalPlan:
         """Split-layout forward product; a part whose input and output are None is left out."""
         self._split(lib.skb_vertical_forward_split, self.modes * self.w.planes, self.modes * self.w.fields * self.nj,
                     physical_real, physical_imag, modal_real, modal_imag, stream)
+
+    def forward_split_noalloc(self, physical_real, physical_imag, modal_real, modal_imag, stream=None):
+        """forward_split with its staging in the library's static 64 MiB device scratch: no
+        reserve_split, no allocation; overlapping calls on one GPU race."""
+        self._split(lib.skb_vertical_forward_split_noalloc, self.modes * self.w.planes,
+                    self.modes * self.w.fields * self.nj, physical_real, physical_imag, modal_real, modal_imag, stream)
 
     def inverse_split(self, modal_real, modal_imag, physical_real, physical_imag, stream=None):
         self._split(lib.skb_vertical_inverse_split, self.modes * self.w.fields * self.nj, self.modes * self.w.planes,
