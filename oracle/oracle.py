@@ -361,7 +361,9 @@ def relative_l2_error(actual, expected) -> float:
 
 # ---- FP64 FFT paths (pocketfft) ---------------------------------------------
 def _workers(workers):
-    return workers if workers else (os.cpu_count() or 1)
+    """Threads for the pocketfft calls: the caller's count, else the CPUs this
+    process may run on, at most 16 (one GPU's share of a shared host)."""
+    return workers if workers else max(1, min(16, len(os.sched_getaffinity(0))))
 
 
 def fft_r2c_wvm(w: Workload, real: np.ndarray, workers: int | None = None) -> np.ndarray:
@@ -487,11 +489,15 @@ def vertical_derivative_matrix(nz: int, lz: float = 1.0) -> np.ndarray:
 
 
 def advection_flux(w: Workload, modes: np.ndarray, dz: np.ndarray, spectral: np.ndarray,
-                   use_direct: bool = False) -> np.ndarray:
+                   use_direct: bool = False, z_chunk: int | None = None) -> np.ndarray:
     """Advection flux of one tracer from compact retained (u, v, w, q):
     q_x = i 2pi k/lx q, q_y = i 2pi l/ly q (logical (k, l) of each mode),
     q_z = D q along z; unnormalised inverses of the six fields; product
-    -(u q_x + v q_y + w q_z)/(Nx Ny)^2; raw forward truncated to the modes."""
+    -(u q_x + v q_y + w q_z)/(Nx Ny)^2; raw forward truncated to the modes.
+
+    After q_z every step acts on one level at a time, so `z_chunk` levels at
+    a time give the same values with the full-grid arrays of only that many
+    levels in memory (the 1024^2 x Nz257 grids)."""
     n = len(modes)
     sp = np.asarray(spectral, np.complex128).reshape(n, 4, w.nz)
     u, v, ww, q = sp[:, 0], sp[:, 1], sp[:, 2], sp[:, 3]
@@ -499,6 +505,19 @@ def advection_flux(w: Workload, modes: np.ndarray, dz: np.ndarray, spectral: np.
     l = modes["l"].astype(np.float64)[:, None]
     D = np.asarray(dz, np.float64).reshape(w.nz, w.nz)
     six = np.stack([u, v, ww, 1j * (2 * np.pi / w.lx) * k * q, 1j * (2 * np.pi / w.ly) * l * q, q @ D.T], axis=1)
+    if z_chunk and z_chunk < w.nz:
+        out = np.empty((n, w.nz), np.complex128)
+        for z0 in range(0, w.nz, z_chunk):
+            z1 = min(w.nz, z0 + z_chunk)
+            out[:, z0:z1] = _advection_levels(Workload(w.nx, w.ny, z1 - z0, 4, w.lx, w.ly), modes,
+                                              np.ascontiguousarray(six[:, :, z0:z1]), use_direct).reshape(n, z1 - z0)
+        return out.reshape(-1)
+    return _advection_levels(w, modes, six, use_direct)
+
+
+def _advection_levels(w: Workload, modes: np.ndarray, six: np.ndarray, use_direct: bool) -> np.ndarray:
+    """The per-level part of advection_flux on the six spectral fields
+    (mode, field, z) of w.nz levels."""
     w6 = Workload(w.nx, w.ny, w.nz, 6, w.lx, w.ly)
     w1 = Workload(w.nx, w.ny, w.nz, 1, w.lx, w.ly)
     if use_direct:

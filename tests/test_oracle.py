@@ -440,3 +440,6 @@ def test_advection_flux_oracle_closed_forms_and_direct_dft():
     a = O.advection_flux(w, modes, D, sp, use_direct=True)
     b = O.advection_flux(w, modes, D, sp)
     assert O.max_relative_error(b, a) < 1e-12
+    # level chunks give the same values (q_z is formed before the chunking)
+    for chunk in (1, 2, w.nz - 1):
+        assert np.array_equal(O.advection_flux(w, modes, D, sp, z_chunk=chunk), b)
