@@ -1415,7 +1415,8 @@ struct skb_pruned_plan {
     skb::DeviceBuffer fwd_start, inv_start, fwd, inv, fmap, imap;
     skb::DeviceBuffer scratch;  // T: kc * Ny * ps complex, element (kx, y, p) at ps * (kx + kc * y) + p
     // Plane-resident path (FP32 256 x 256, k_x band 86): tables and the
-    // plane-major copy of the compact operand (mode + np * plane).
+    // chunked plane-major copy of the compact operand (16*P*(mode/16) +
+    // 16*plane + mode%16).
     int path = SKB_PATH_AUTO;
     bool plane_built = false;
     skb::PlaneTables plane;

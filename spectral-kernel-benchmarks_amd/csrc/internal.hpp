@@ -35,9 +35,10 @@ void flux_rows(skb_pruned_plan* pin, skb_pruned_plan* pout, int targets, double 
 void vertical_forward_columns(const skb_vertical_plan* p, const void* in, int in_cs, int in_co, void* out,
                               void* stream);
 
-// Forward product read from the plane-major retained layout (element (mode,
-// z, field) at pm[mode + np*(z + Nz*field)]) into the plan's modal layout;
-// square FP32 split-BF16 plans only (vertical.hip).
+// Forward product read from the chunked plane-major retained layout (element
+// (mode, z, field) at pm[16*P*(mode/16) + 16*(z + Nz*field) + mode%16], np
+// modes, a multiple of 16) into the plan's modal layout; square FP32
+// split-BF16 plans only (vertical.hip).
 bool vertical_pm_supported(const skb_vertical_plan* p);
 void vertical_forward_pm(const skb_vertical_plan* p, const void* pm, int np, void* out, void* stream);
 

@@ -5,10 +5,11 @@
 // pays one C-ABI call per evaluation.
 //
 // Plane path (FP32, 256 x 256 planes, 2/3-rule k_x band): the input is
-// copied into the plane-major layout (mode + np*plane), each plane's inverse
-// and forward 2-D transform runs in one workgroup with the band on chip, and
-// the vertical product reads the forward's plane-major output directly
-// (square split-BF16 families) or through a copy back to the compact layout.
+// copied into the chunked plane-major layout (16*P*(mode/16) + 16*plane +
+// mode%16), each plane's inverse and forward 2-D transform runs in one
+// workgroup with the band on chip, and the vertical product reads the
+// forward's plane-major output directly (square split-BF16 families) or
+// through a copy back to the compact layout.
 // Two-pass path: the pruned plan's row and column kernels and the compact
 // vertical product.
 #include <memory>

@@ -11,16 +11,20 @@
 // direction; the kc x Ny x P intermediate of the two-kernel transforms is
 // gone.
 //
-// Retained values are read and written in the PLANE-MAJOR layout: mode m of
-// plane p at m + np*p, np = modes rounded up to 16, so one plane's modes are
-// one contiguous run.  The caller's compact layout (plane + P*mode) is
-// converted by pm_transpose_kernel, whose workgroups move 16 modes of every
-// plane (one contiguous 16*P run on the compact side, P runs of 128 bytes on
-// the plane-major side).
+// Retained values are read and written in the CHUNKED PLANE-MAJOR layout:
+// modes in chunks of 16, mode m of plane p at 16*P*(m / 16) + 16*p + m % 16
+// (P planes; the last chunk padded to 16 modes, np = modes rounded up to 16).
+// A plane's 16 modes of a chunk are one 128-byte run, and the 16*P values of
+// a chunk are the same bytes as the compact layout's (plane + P*mode) 16
+// modes, transposed: pm_transpose_kernel moves each chunk through LDS with
+// contiguous reads and writes, and the vertical product reads a tile's
+// 16 x Nz block of one field as one contiguous run.
 //
 // Every barrier in these kernels orders LDS only (lds_barrier): the global
-// loads and stores of a plane are never read by another wavefront, so the
-// stores of one row half keep draining while the next half is transformed.
+// loads and stores of a plane are never read by another wavefront (nor read
+// back by the fused operator, whose forward starts from the inverse's row
+// registers), so the stores of one row half keep draining while the next
+// half is transformed.
 //
 // Thread roles in a plane workgroup (tid = 64*wave + 16*g + n):
 //   columns  column c = 64*r + 4*wave + g (round r), lane n of its 16;
@@ -142,9 +146,15 @@ __device__ __forceinline__ void load_codes(uint32_t* c, const uint32_t* __restri
 // k_x = 0 Hermitian partners included (src/model.cpp:332-335).  The imaginary
 // part of the kx = 0 bin is ignored by the row c2r, as FFTW's c2r does.
 // ---------------------------------------------------------------------------
-template <int KC>
-__device__ __forceinline__ void inverse_plane(unsigned char* lds, const cx<float>* pm, float* real, const cx<float>* __restrict__ twg,
-                                              const uint32_t* __restrict__ icodes, int modes, int np, int p) {
+// FWD (the fused operator): right after a half's real rows are stored, the
+// forward row transforms of the same row pairs run on the registers that hold
+// them, so the forward never reads the real plane back; X_a, X_b of both
+// halves end in xa, xb.
+template <int KC, bool FWD>
+__device__ __forceinline__ void inverse_plane(unsigned char* lds, const cx<float>* pm, float* real,
+                                              const cx<float>* __restrict__ twg, const uint32_t* __restrict__ icodes,
+                                              int modes, int np, int planes, int p, cx<float> (&xa)[2][PlaneGeo<KC>::KH],
+                                              cx<float> (&xb)[2][PlaneGeo<KC>::KH]) {
     using G = PlaneGeo<KC>;
     constexpr int N = G::N, RP = G::RP;
     cx<float>* r0 = reinterpret_cast<cx<float>*>(lds);  // staged modes, then the row buffer [128][RP]
@@ -152,14 +162,15 @@ __device__ __forceinline__ void inverse_plane(unsigned char* lds, const cx<float
     float* wt = reinterpret_cast<float*>(lds + G::R0) + kExchangeFloats * wave;
     cx<float>* tw = reinterpret_cast<cx<float>*>(lds + G::R0 + G::WT);
 
-    // 1. the plane's retained values (16-byte pairs; np is even and past an
-    //    odd count the pair reads the row's padding) and the twiddles
+    // 1. the plane's retained values (16-byte pairs of one chunk; past an odd
+    //    count the pair reads the chunk's padding) and the twiddles
     {
-        const __amdgpu_buffer_rsrc_t rs =
-            buffer_rsrc(pm + static_cast<size_t>(np) * p, static_cast<uint32_t>(np) * 8u);
+        const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(pm, static_cast<uint32_t>(np) * planes * 8u);
         const int pairs = (modes + 1) >> 1;
-        for (int i = tid; i < pairs; i += kPlaneThreads)
-            *reinterpret_cast<bf4*>(&r0[2 * i]) = raw_buffer_load_f4(rs, 16 * i, 0, 0);
+        for (int i = tid; i < pairs; i += kPlaneThreads) {
+            const uint32_t at = (static_cast<uint32_t>(i >> 3) * 16u * planes + 16u * p + 2u * (i & 7)) * 8u;
+            *reinterpret_cast<bf4*>(&r0[2 * i]) = raw_buffer_load_f4(rs, static_cast<int>(at), 0, 0);
+        }
         if (tid < N) tw[tid] = twg[tid];
     }
     uint32_t code[G::ROUNDS][8];
@@ -249,6 +260,7 @@ __device__ __forceinline__ void inverse_plane(unsigned char* lds, const cx<float
             buffer_store_real<float>(z[k].x, out, row + 64u * k);
             buffer_store_real<float>(z[k].y, out, row + N * 4u + 64u * k);
         }
+        if constexpr (FWD) forward_row_pair<KC>(z, wt, tw, g, n, xa[h], xb[h]);
     }
 }
 
@@ -257,61 +269,51 @@ __global__ __launch_bounds__(kPlaneThreads) void plane_inverse_kernel(const cx<f
                                                                      float* __restrict__ real,
                                                                      const cx<float>* __restrict__ twg,
                                                                      const uint32_t* __restrict__ icodes, int modes,
-                                                                     int np) {
+                                                                     int np, int planes) {
     __shared__ __align__(16) unsigned char lds[PlaneGeo<KC>::BYTES];
-    inverse_plane<KC>(lds, pm, real, twg, icodes, modes, np, static_cast<int>(blockIdx.x));
+    cx<float> xa[2][PlaneGeo<KC>::KH], xb[2][PlaneGeo<KC>::KH];
+    inverse_plane<KC, false>(lds, pm, real, twg, icodes, modes, np, planes, static_cast<int>(blockIdx.x), xa, xb);
 }
 
 // ---------------------------------------------------------------------------
 // Forward: real plane blockIdx.x -> its plane-major retained values.
 // fcodes: per (column c, lane n) 16 codes of (kx = c, ky = n + 16 k).
 // ---------------------------------------------------------------------------
-// READBACK: the plane was written by this workgroup's inverse pass with the
-// same row decomposition, so each lane reads back the addresses it wrote
-// (program order suffices, no barrier); the loads are volatile so that the
-// compiler keeps them.  Otherwise the twiddles are staged first.
-template <int KC, bool READBACK>
-__device__ __forceinline__ void forward_plane(unsigned char* lds, const float* real, cx<float>* pm, const cx<float>* __restrict__ twg,
-                                              const uint32_t* __restrict__ fcodes, int modes, int np, int p) {
+// Row FFT of the packed row pair z (Re = row 2f, Im = row 2f + 1 of a half)
+// and the Hermitian split into X_a, X_b at kx = n + 16 k < KC.
+template <int KC>
+__device__ __forceinline__ void forward_row_pair(cx<float>* z, float* wt, const cx<float>* tw, int g, int n,
+                                                 cx<float>* xa, cx<float>* xb) {
+    fft256<false>(z, wt, tw, g, n);
+    // Z[n + 16 k]; its partner Z[N - n - 16 k] is register 15 - k of lane
+    // (16 - n) mod 16, or register (16 - k) mod 16 of lane 0 itself
+#pragma unroll
+    for (int k = 0; k < PlaneGeo<KC>::KH; ++k) {
+        const cx<float> pz = z[15 - k];
+        cx<float> zn{from_partner_lane(pz.x), from_partner_lane(pz.y)};
+        if (n == 0) zn = z[(16 - k) & 15];
+        xa[k] = cx<float>{0.5f * (z[k].x + zn.x), 0.5f * (z[k].y - zn.y)};
+        xb[k] = cx<float>{0.5f * (z[k].y + zn.y), 0.5f * (zn.x - z[k].x)};
+    }
+}
+
+// The forward after its row transforms: the band (rows 2f, 2f + 1 of both
+// halves at kx = n + 16 k, held by every lane) moves to columns through LDS,
+// the column transforms run, and the retained outputs go to plane p of pm.
+// The caller's LDS use of the band region [0, R0) must be over (barrier).
+template <int KC>
+__device__ __forceinline__ void forward_columns(unsigned char* lds, const cx<float> (&xa)[2][PlaneGeo<KC>::KH],
+                                                const cx<float> (&xb)[2][PlaneGeo<KC>::KH], cx<float>* pm,
+                                                const uint32_t* __restrict__ fcodes, int modes, int np, int planes,
+                                                int p) {
     using G = PlaneGeo<KC>;
-    constexpr int N = G::N, YP = G::YP, KH = G::KH;
+    constexpr int YP = G::YP, KH = G::KH;
     float* yb = reinterpret_cast<float*>(lds);           // band [KC][YP], real or imaginary parts
     cx<float>* st = reinterpret_cast<cx<float>*>(lds);  // then the output staging [modes]
     const int tid = static_cast<int>(threadIdx.x), wave = tid >> 6, lane = tid & 63, g = lane >> 4, n = lane & 15;
     float* wt = reinterpret_cast<float*>(lds + G::R0) + kExchangeFloats * wave;
-    cx<float>* tw = reinterpret_cast<cx<float>*>(lds + G::R0 + G::WT);
-    const __amdgpu_buffer_rsrc_t in =
-        buffer_rsrc(real + static_cast<size_t>(p) * N * N, static_cast<uint32_t>(N * N * 4));
-    constexpr int kAux = READBACK ? static_cast<int>(0x80000000u) : 0;  // volatile
+    const cx<float>* tw = reinterpret_cast<const cx<float>*>(lds + G::R0 + G::WT);
     const int f = 4 * wave + g;
-    if (!READBACK && tid < N) tw[tid] = twg[tid];
-
-    // 1. rows: X_a, X_b (rows 2f, 2f + 1 of each half) for kx = n + 16 k < KC
-    cx<float> xa[2][KH], xb[2][KH];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const uint32_t row = static_cast<uint32_t>((128 * h + 2 * f) * N + n) * 4u;
-        cx<float> z[16];
-#pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            z[j].x = raw_buffer_load_f1(in, static_cast<int>(row + 64u * j), 0, kAux);
-            z[j].y = raw_buffer_load_f1(in, static_cast<int>(row + N * 4u + 64u * j), 0, kAux);
-        }
-        // (READBACK: the barrier also ends the inverse pass's row-buffer reads
-        // before the band overwrites the region)
-        if (h == 0) lds_barrier();
-        fft256<false>(z, wt, tw, g, n);
-        // Z[n + 16 k]; its partner Z[N - n - 16 k] is register 15 - k of lane
-        // (16 - n) mod 16, or register (16 - k) mod 16 of lane 0 itself
-#pragma unroll
-        for (int k = 0; k < KH; ++k) {
-            const cx<float> pz = z[15 - k];
-            cx<float> zn{from_partner_lane(pz.x), from_partner_lane(pz.y)};
-            if (n == 0) zn = z[(16 - k) & 15];
-            xa[h][k] = cx<float>{0.5f * (z[k].x + zn.x), 0.5f * (z[k].y - zn.y)};
-            xb[h][k] = cx<float>{0.5f * (z[k].y + zn.y), 0.5f * (zn.x - z[k].x)};
-        }
-    }
 
     // 2. the band through LDS, real parts then imaginary parts; the column
     //    lanes gather their inputs after each
@@ -366,14 +368,48 @@ __device__ __forceinline__ void forward_plane(unsigned char* lds, const float* r
     }
     lds_barrier();
 
-    // 4. copy out (16-byte pairs; the pair past an odd count writes the row's padding)
+    // 4. copy out (16-byte pairs of one chunk; the pair past an odd count
+    //    writes the chunk's padding)
     {
-        const __amdgpu_buffer_rsrc_t rs =
-            buffer_rsrc(pm + static_cast<size_t>(np) * p, static_cast<uint32_t>(np) * 8u);
+        const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(pm, static_cast<uint32_t>(np) * planes * 8u);
         const int pairs = (modes + 1) >> 1;
-        for (int i = tid; i < pairs; i += kPlaneThreads)
-            raw_buffer_store_f4(*reinterpret_cast<const bf4*>(&st[2 * i]), rs, 16 * i, 0, 0);
+        for (int i = tid; i < pairs; i += kPlaneThreads) {
+            const uint32_t at = (static_cast<uint32_t>(i >> 3) * 16u * planes + 16u * p + 2u * (i & 7)) * 8u;
+            raw_buffer_store_f4(*reinterpret_cast<const bf4*>(&st[2 * i]), rs, static_cast<int>(at), 0, 0);
+        }
     }
+}
+
+template <int KC>
+__device__ __forceinline__ void forward_plane(unsigned char* lds, const float* real, cx<float>* pm,
+                                              const cx<float>* __restrict__ twg, const uint32_t* __restrict__ fcodes,
+                                              int modes, int np, int planes, int p) {
+    using G = PlaneGeo<KC>;
+    constexpr int N = G::N, KH = G::KH;
+    const int tid = static_cast<int>(threadIdx.x), wave = tid >> 6, lane = tid & 63, g = lane >> 4, n = lane & 15;
+    float* wt = reinterpret_cast<float*>(lds + G::R0) + kExchangeFloats * wave;
+    cx<float>* tw = reinterpret_cast<cx<float>*>(lds + G::R0 + G::WT);
+    const __amdgpu_buffer_rsrc_t in =
+        buffer_rsrc(real + static_cast<size_t>(p) * N * N, static_cast<uint32_t>(N * N * 4));
+    const int f = 4 * wave + g;
+    if (tid < N) tw[tid] = twg[tid];
+
+    // 1. rows: X_a, X_b (rows 2f, 2f + 1 of each half) for kx = n + 16 k < KC
+    cx<float> xa[2][KH], xb[2][KH];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t row = static_cast<uint32_t>((128 * h + 2 * f) * N + n) * 4u;
+        cx<float> z[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            z[j].x = raw_buffer_load_f1(in, static_cast<int>(row + 64u * j), 0, 0);
+            z[j].y = raw_buffer_load_f1(in, static_cast<int>(row + N * 4u + 64u * j), 0, 0);
+        }
+        if (h == 0) lds_barrier();  // the twiddles are staged
+        forward_row_pair<KC>(z, wt, tw, g, n, xa[h], xb[h]);
+    }
+    lds_barrier();  // (the band region held nothing else; kept for the caller contract)
+    forward_columns<KC>(lds, xa, xb, pm, fcodes, modes, np, planes, p);
 }
 
 template <int KC>
@@ -381,16 +417,18 @@ __global__ __launch_bounds__(kPlaneThreads) void plane_forward_kernel(const floa
                                                                      cx<float>* __restrict__ pm,
                                                                      const cx<float>* __restrict__ twg,
                                                                      const uint32_t* __restrict__ fcodes, int modes,
-                                                                     int np) {
+                                                                     int np, int planes) {
     __shared__ __align__(16) unsigned char lds[PlaneGeo<KC>::BYTES];
-    forward_plane<KC, false>(lds, real, pm, twg, fcodes, modes, np, static_cast<int>(blockIdx.x));
+    forward_plane<KC>(lds, real, pm, twg, fcodes, modes, np, planes, static_cast<int>(blockIdx.x));
 }
 
 // ---------------------------------------------------------------------------
 // The retained operator's two transforms of plane blockIdx.x in one
-// workgroup: plane-major values -> inverse -> real plane (written to memory)
-// -> read back -> forward -> plane-major values (may alias the input: each
-// workgroup reads its plane's row of pm in full before it writes it).
+// workgroup: plane-major values -> inverse -> real plane (written to memory,
+// the physical-space result) and, from the same registers, the forward row
+// transforms -> forward columns -> plane-major values (may alias the input:
+// each workgroup reads its plane's values in full before it writes them, and
+// no other workgroup touches them).
 // ---------------------------------------------------------------------------
 template <int KC>
 __global__ __launch_bounds__(kPlaneThreads) void plane_operator_kernel(const cx<float>* pm_in,
@@ -399,68 +437,66 @@ __global__ __launch_bounds__(kPlaneThreads) void plane_operator_kernel(const cx<
                                                                       const cx<float>* __restrict__ twg,
                                                                       const uint32_t* __restrict__ icodes,
                                                                       const uint32_t* __restrict__ fcodes, int modes,
-                                                                      int np) {
+                                                                      int np, int planes) {
     __shared__ __align__(16) unsigned char lds[PlaneGeo<KC>::BYTES];
     const int p = static_cast<int>(blockIdx.x);
-    inverse_plane<KC>(lds, pm_in, real, twg, icodes, modes, np, p);
-    forward_plane<KC, true>(lds, real, pm_out, twg, fcodes, modes, np, p);
+    cx<float> xa[2][PlaneGeo<KC>::KH], xb[2][PlaneGeo<KC>::KH];
+    inverse_plane<KC, true>(lds, pm_in, real, twg, icodes, modes, np, planes, p, xa, xb);
+    lds_barrier();  // every row-buffer read of the inverse is done before the band overwrites it
+    forward_columns<KC>(lds, xa, xb, pm_out, fcodes, modes, np, planes, p);
 }
 
 // ---------------------------------------------------------------------------
-// Compact (plane + P*mode) <-> plane-major (mode + np*plane).  Workgroup b
-// moves modes [16b, 16b + 16) of every plane through an LDS tile [P][17].
+// Compact (plane + P*mode) <-> chunked plane-major.  Workgroup b moves chunk
+// b: the compact run [16*P*b, 16*P*b + nm*P) (nm = the chunk's modes, 16 but
+// in the last chunk) and the chunked run [16*P*b, 16*P*b + 16*P), through an
+// LDS tile [P][17].  Both sides are contiguous.  The chunk's padding modes
+// are written as zeros.
 // ---------------------------------------------------------------------------
 template <bool TO_PM>
 __global__ __launch_bounds__(256) void pm_transpose_kernel(const cx<float>* __restrict__ in,
-                                                           cx<float>* __restrict__ out, int modes, int planes,
-                                                           int np) {
-    // 16-byte accesses (two complex values) on both sides: the compact run of
-    // a tile starts at 16 * P * 8 bytes (a multiple of 128) and its length
-    // nm * P is odd only for a last partial tile (one single tail element);
-    // a plane-major row segment of 16 values is 8 pairs.
+                                                           cx<float>* __restrict__ out, int modes, int planes) {
+    // 16-byte accesses (two complex values) on both sides: a chunk starts at
+    // 16 * P * 8 bytes (a multiple of 128); its compact run nm * P is odd only
+    // for a last partial chunk (one single tail element).
     extern __shared__ __align__(16) unsigned char pm_tile_lds[];
     cx<float>* tile = reinterpret_cast<cx<float>*>(pm_tile_lds);
-    const int m0 = 16 * static_cast<int>(blockIdx.x);
-    const int nm = modes - m0 < 16 ? modes - m0 : 16;
+    const size_t base = static_cast<size_t>(16) * planes * blockIdx.x;
+    const int nm = modes - 16 * static_cast<int>(blockIdx.x) < 16 ? modes - 16 * static_cast<int>(blockIdx.x) : 16;
     const int tid = static_cast<int>(threadIdx.x);
-    const int run = nm * planes;  // compact elements of the tile
+    const int run = nm * planes;  // compact elements of the chunk
     if constexpr (TO_PM) {
-        const bf4* src = reinterpret_cast<const bf4*>(in + static_cast<size_t>(m0) * planes);
+        const bf4* src = reinterpret_cast<const bf4*>(in + base);
         for (int e2 = tid; 2 * e2 < run; e2 += 256) {
             const int e = 2 * e2;
+            const int m = e / planes, pl = e - m * planes;
             if (e + 1 < run) {
                 const bf4 q = src[e2];
-                const int m = e / planes, pl = e - m * planes;
                 tile[pl * 17 + m] = cx<float>{q.x, q.y};
                 const int m1 = pl + 1 == planes ? m + 1 : m, p1 = pl + 1 == planes ? 0 : pl + 1;
                 tile[p1 * 17 + m1] = cx<float>{q.z, q.w};
             } else {
-                const int m = e / planes, pl = e - m * planes;
-                tile[pl * 17 + m] = in[static_cast<size_t>(m0) * planes + e];
+                tile[pl * 17 + m] = in[base + e];
             }
         }
         lds_barrier();
-        for (int e = tid; e < 8 * planes; e += 256) {
-            const int mp = e & 7, pl = e >> 3, m = 2 * mp;
-            const cx<float> a = tile[pl * 17 + m], b = tile[pl * 17 + m + 1];
-            cx<float>* dst = out + static_cast<size_t>(np) * pl + m0 + m;
-            if (m + 1 < nm) *reinterpret_cast<bf4*>(dst) = bf4{a.x, a.y, b.x, b.y};
-            else if (m < nm) *dst = a;
+        bf4* dst = reinterpret_cast<bf4*>(out + base);
+        for (int e2 = tid; e2 < 8 * planes; e2 += 256) {
+            const int pl = e2 >> 3, m = 2 * (e2 & 7);
+            const cx<float> a = m < nm ? tile[pl * 17 + m] : cx<float>{0.f, 0.f};
+            const cx<float> b = m + 1 < nm ? tile[pl * 17 + m + 1] : cx<float>{0.f, 0.f};
+            dst[e2] = bf4{a.x, a.y, b.x, b.y};
         }
     } else {
-        for (int e = tid; e < 8 * planes; e += 256) {
-            const int mp = e & 7, pl = e >> 3, m = 2 * mp;
-            const cx<float>* srcp = in + static_cast<size_t>(np) * pl + m0 + m;
-            if (m + 1 < nm) {
-                const bf4 q = *reinterpret_cast<const bf4*>(srcp);
-                tile[pl * 17 + m] = cx<float>{q.x, q.y};
-                tile[pl * 17 + m + 1] = cx<float>{q.z, q.w};
-            } else if (m < nm) {
-                tile[pl * 17 + m] = *srcp;
-            }
+        const bf4* src = reinterpret_cast<const bf4*>(in + base);
+        for (int e2 = tid; e2 < 8 * planes; e2 += 256) {
+            const int pl = e2 >> 3, m = 2 * (e2 & 7);
+            const bf4 q = src[e2];
+            tile[pl * 17 + m] = cx<float>{q.x, q.y};
+            tile[pl * 17 + m + 1] = cx<float>{q.z, q.w};
         }
         lds_barrier();
-        bf4* dst = reinterpret_cast<bf4*>(out + static_cast<size_t>(m0) * planes);
+        bf4* dst = reinterpret_cast<bf4*>(out + base);
         for (int e2 = tid; 2 * e2 < run; e2 += 256) {
             const int e = 2 * e2;
             const int m = e / planes, pl = e - m * planes;
@@ -470,44 +506,42 @@ __global__ __launch_bounds__(256) void pm_transpose_kernel(const cx<float>* __re
                 const cx<float> b = tile[p1 * 17 + m1];
                 dst[e2] = bf4{a.x, a.y, b.x, b.y};
             } else {
-                out[static_cast<size_t>(m0) * planes + e] = a;
+                out[base + e] = a;
             }
         }
     }
 }
 
-// The same move with 8-byte accesses, for compact arrays that are not 16-byte
-// aligned.
+// The same move with 8-byte accesses on the compact side, for compact arrays
+// that are not 16-byte aligned.
 template <bool TO_PM>
 __global__ __launch_bounds__(256) void pm_transpose_scalar_kernel(const cx<float>* __restrict__ in,
                                                                   cx<float>* __restrict__ out, int modes,
-                                                                  int planes, int np) {
+                                                                  int planes) {
     extern __shared__ __align__(16) unsigned char pm_tile_lds[];
     cx<float>* tile = reinterpret_cast<cx<float>*>(pm_tile_lds);
-    const int m0 = 16 * static_cast<int>(blockIdx.x);
-    const int nm = modes - m0 < 16 ? modes - m0 : 16;
+    const size_t base = static_cast<size_t>(16) * planes * blockIdx.x;
+    const int nm = modes - 16 * static_cast<int>(blockIdx.x) < 16 ? modes - 16 * static_cast<int>(blockIdx.x) : 16;
     const int tid = static_cast<int>(threadIdx.x);
     if constexpr (TO_PM) {
-        const cx<float>* src = in + static_cast<size_t>(m0) * planes;
         for (int e = tid; e < nm * planes; e += 256) {
             const int m = e / planes, pl = e - m * planes;
-            tile[pl * 17 + m] = src[e];
+            tile[pl * 17 + m] = in[base + e];
         }
         lds_barrier();
         for (int e = tid; e < 16 * planes; e += 256) {
             const int m = e & 15, pl = e >> 4;
-            if (m < nm) out[static_cast<size_t>(np) * pl + m0 + m] = tile[pl * 17 + m];
+            out[base + e] = m < nm ? tile[pl * 17 + m] : cx<float>{0.f, 0.f};
         }
     } else {
         for (int e = tid; e < 16 * planes; e += 256) {
             const int m = e & 15, pl = e >> 4;
-            if (m < nm) tile[pl * 17 + m] = in[static_cast<size_t>(np) * pl + m0 + m];
+            tile[pl * 17 + m] = in[base + e];
         }
         lds_barrier();
-        cx<float>* dst = out + static_cast<size_t>(m0) * planes;
         for (int e = tid; e < nm * planes; e += 256) {
             const int m = e / planes, pl = e - m * planes;
-            dst[e] = tile[pl * 17 + m];
+            out[base + e] = tile[pl * 17 + m];
         }
     }
 }
@@ -553,14 +587,14 @@ void PlaneTables::build(const skb_workload& w, uint64_t kc_, uint64_t modes_, co
 void plane_inverse(const PlaneTables& t, const void* pm, void* real, int planes, hipStream_t s) {
     hipLaunchKernelGGL((plane_inverse_kernel<kPlaneKc>), dim3(planes), dim3(kPlaneThreads), 0, s,
                        static_cast<const cx<float>*>(pm), static_cast<float*>(real), t.tw.as<cx<float>>(),
-                       t.icodes.as<uint32_t>(), t.modes, t.np);
+                       t.icodes.as<uint32_t>(), t.modes, t.np, planes);
     check_hip(hipGetLastError(), "plane inverse launch");
 }
 
 void plane_forward(const PlaneTables& t, const void* real, void* pm, int planes, hipStream_t s) {
     hipLaunchKernelGGL((plane_forward_kernel<kPlaneKc>), dim3(planes), dim3(kPlaneThreads), 0, s,
                        static_cast<const float*>(real), static_cast<cx<float>*>(pm), t.tw.as<cx<float>>(),
-                       t.fcodes.as<uint32_t>(), t.modes, t.np);
+                       t.fcodes.as<uint32_t>(), t.modes, t.np, planes);
     check_hip(hipGetLastError(), "plane forward launch");
 }
 
@@ -568,20 +602,21 @@ void plane_operator(const PlaneTables& t, const void* pm_in, void* pm_out, void*
     hipLaunchKernelGGL((plane_operator_kernel<kPlaneKc>), dim3(planes), dim3(kPlaneThreads), 0, s,
                        static_cast<const cx<float>*>(pm_in), static_cast<cx<float>*>(pm_out),
                        static_cast<float*>(real), t.tw.as<cx<float>>(), t.icodes.as<uint32_t>(),
-                       t.fcodes.as<uint32_t>(), t.modes, t.np);
+                       t.fcodes.as<uint32_t>(), t.modes, t.np, planes);
     check_hip(hipGetLastError(), "plane operator launch");
 }
 
 void pm_transpose(bool to_pm, const void* in, void* out, int modes, int planes, int np, hipStream_t s) {
+    (void)np;  // the chunk count follows from modes
     const dim3 grid((modes + 15) / 16);
     const size_t lds = static_cast<size_t>(planes) * 17 * sizeof(cx<float>);
     const auto* i = static_cast<const cx<float>*>(in);
     auto* o = static_cast<cx<float>*>(out);
     const bool wide = (reinterpret_cast<uintptr_t>(in) % 16 == 0) && (reinterpret_cast<uintptr_t>(out) % 16 == 0);
-    if (wide && to_pm) hipLaunchKernelGGL(pm_transpose_kernel<true>, grid, dim3(256), lds, s, i, o, modes, planes, np);
-    else if (wide) hipLaunchKernelGGL(pm_transpose_kernel<false>, grid, dim3(256), lds, s, i, o, modes, planes, np);
-    else if (to_pm) hipLaunchKernelGGL(pm_transpose_scalar_kernel<true>, grid, dim3(256), lds, s, i, o, modes, planes, np);
-    else hipLaunchKernelGGL(pm_transpose_scalar_kernel<false>, grid, dim3(256), lds, s, i, o, modes, planes, np);
+    if (wide && to_pm) hipLaunchKernelGGL(pm_transpose_kernel<true>, grid, dim3(256), lds, s, i, o, modes, planes);
+    else if (wide) hipLaunchKernelGGL(pm_transpose_kernel<false>, grid, dim3(256), lds, s, i, o, modes, planes);
+    else if (to_pm) hipLaunchKernelGGL(pm_transpose_scalar_kernel<true>, grid, dim3(256), lds, s, i, o, modes, planes);
+    else hipLaunchKernelGGL(pm_transpose_scalar_kernel<false>, grid, dim3(256), lds, s, i, o, modes, planes);
     check_hip(hipGetLastError(), "plane-major transpose launch");
 }
 

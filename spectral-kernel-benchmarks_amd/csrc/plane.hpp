@@ -15,7 +15,8 @@ constexpr int kPlaneKc = 86;
 constexpr int kPlaneMaxModes = 11440;  // retained values staged in one workgroup's LDS
 constexpr int kPlaneMaxPlanes = 480;   // LDS tile of the layout transpose
 
-// Plane stride of the plane-major retained layout: modes rounded up to 16.
+// Modes of the chunked plane-major retained layout (modes rounded up to 16):
+// mode m of plane p at 16*P*(m / 16) + 16*p + m % 16, for P planes.
 uint64_t plane_np(uint64_t modes);
 bool plane_supported(const skb_workload& w, uint64_t kc, uint64_t modes, int precision);
 
@@ -30,13 +31,13 @@ struct PlaneTables {
                const std::vector<int>& imap);
 };
 
-// pm: plane-major retained values (mode + np * plane), real: x + Nx(y + Ny*plane).
+// pm: chunked plane-major retained values (np * planes), real: x + Nx(y + Ny*plane).
 void plane_inverse(const PlaneTables& t, const void* pm, void* real, int planes, hipStream_t s);
 void plane_forward(const PlaneTables& t, const void* real, void* pm, int planes, hipStream_t s);
 // Both transforms of each plane in one workgroup: pm_in -> real planes ->
 // pm_out (pm_out may equal pm_in).
 void plane_operator(const PlaneTables& t, const void* pm_in, void* pm_out, void* real, int planes, hipStream_t s);
-// compact (plane + P*mode) -> plane-major (to_pm) or back.
+// compact (plane + P*mode) -> chunked plane-major (to_pm) or back.
 void pm_transpose(bool to_pm, const void* in, void* out, int modes, int planes, int np, hipStream_t s);
 
 }  // namespace skb

@@ -123,11 +123,12 @@ struct GemmArgs {
     // [group][16*MB].
     const uint16_t* a3;
     const float* atail;
-    // plane-major input (split kernel, PM = true): element (mode, z, field) at
-    // in[mode + pm_np * (z + pm_nz * field)]; a tile is up to 16 consecutive
-    // modes of one group and one field (GemmTile::pad = field), written to the
-    // modal layout out + out_ld * (field + F * mode).
-    int pm_np, pm_nz;
+    // chunked plane-major input (split kernel, PM = true): element (mode, z,
+    // field) at in[16*P*(mode / 16) + 16*(z + pm_nz * field) + mode % 16],
+    // P = pm_planes, pm_np modes (a multiple of 16); a tile is up to 16
+    // consecutive modes of one group and one field (GemmTile::pad = field),
+    // written to the modal layout out + out_ld * (field + F * mode).
+    int pm_np, pm_nz, pm_planes;
 };
 
 template <typename R, int KS, bool WVM>
@@ -512,21 +513,24 @@ __global__ __launch_bounds__(256) void grouped_gemm_split_kernel(const GemmArgs<
     auto load_b = [&](BRegs& bv, const GemmTile& t) {
         if constexpr (PM) {
             // column j = e % 16 of the tile, rows z, z + 1 with z = 2 (e / 16):
-            // 16 lanes read one 128-byte run of consecutive modes per row
-            const cx<float>* base =
-                g.in + static_cast<size_t>(g.pm_np) * static_cast<size_t>(g.pm_nz * t.pad) + t.col_begin;
-            const uint32_t np = static_cast<uint32_t>(g.pm_np);
-            const uint32_t bytes = (np * static_cast<uint32_t>(g.K - 1) + static_cast<uint32_t>(t.ncols)) * CB;
-            const __amdgpu_buffer_rsrc_t r = buffer_rsrc(base, bytes);
+            // 16 lanes read one 128-byte run of consecutive modes per row (two
+            // pieces when the tile's modes straddle two chunks)
+            const uint32_t P = static_cast<uint32_t>(g.pm_planes);
+            const __amdgpu_buffer_rsrc_t r = buffer_rsrc(g.in, static_cast<uint32_t>(g.pm_np) * P * CB);
+            const uint32_t plane0 = static_cast<uint32_t>(g.pm_nz * t.pad);
+            auto at = [&](int c, int z) {
+                const uint32_t m = static_cast<uint32_t>(t.col_begin + c);
+                return (16u * P * (m >> 4) + 16u * (plane0 + static_cast<uint32_t>(z)) + (m & 15u)) * CB;
+            };
 #pragma unroll
             for (int n = 0; n < NK; ++n) {
                 const int e = tid + 256 * n, c = e & 15, z = 2 * (e >> 4);
                 const bool ok = c < t.ncols;
-                const cx<float> lo = buffer_load_cx<float>(r, ok ? (np * z + c) * CB : kBufferPast);
-                const cx<float> hi = buffer_load_cx<float>(r, ok ? (np * (z + 1) + c) * CB : kBufferPast);
+                const cx<float> lo = buffer_load_cx<float>(r, ok ? at(c, z) : kBufferPast);
+                const cx<float> hi = buffer_load_cx<float>(r, ok ? at(c, z + 1) : kBufferPast);
                 bv.pair[n] = bf4{lo.x, lo.y, hi.x, hi.y};
             }
-            bv.tail = buffer_load_cx<float>(r, tid < t.ncols ? (np * KB + tid) * CB : kBufferPast);
+            bv.tail = buffer_load_cx<float>(r, tid < t.ncols ? at(tid, KB) : kBufferPast);
         } else {
             const cx<float>* base = g.in + static_cast<size_t>(g.in_ld) * t.col_begin;
             const uint32_t bytes = static_cast<uint32_t>(in_step * (t.ncols - 1) + g.K) * CB;
@@ -603,121 +607,136 @@ __global__ __launch_bounds__(256) void grouped_gemm_split_kernel(const GemmArgs<
             }
         }
     };
-    GemmTile cur = tile_record(g.tiles, t_begin);
-    int group = cur.group;
+    // Barriers guard LDS only (lds_barrier): the output stores and the
+    // prefetched B loads stay in flight across them.
+    // The B blocks of the first DEPTH tiles of the range are requested with
+    // the group's operand; each later block is requested as the block
+    // DEPTH - 1 tiles ahead of it is staged (a deeper prefetch measured
+    // slower: the workgroups' first blocks then queue behind their later ones).
+    constexpr int DEPTH = 2;
+    GemmTile rec[DEPTH];
+#pragma unroll
+    for (int i = 0; i < DEPTH; ++i) rec[i] = tile_record(g.tiles, t_begin + i < t_end ? t_begin + i : t_begin);
+    int group = rec[0].group;
     issue_group(group);
+    BRegs pre[DEPTH];
+#pragma unroll
+    for (int i = 0; i < DEPTH; ++i)
+        if (t_begin + i < t_end) load_b(pre[i], rec[i]);
     // rows K .. KP-1 of the FP32 copies stay zero (the vector-ALU rows read them)
     for (int e = tid; e < NB * 16 * (KP - 32 * NK - 1); e += 256) {
         const int b = e / (16 * (KP - 32 * NK - 1)), f = e - b * 16 * (KP - 32 * NK - 1);
         const int c = f / (KP - 32 * NK - 1);
         sbb[b][c * CS + 32 * NK + 1 + (f - c * (KP - 32 * NK - 1))] = cx<float>{0.f, 0.f};
     }
-    BRegs bv;
-    load_b(bv, cur);
-    GemmTile nxt = tile_record(g.tiles, t_begin + 1 < t_end ? t_begin + 1 : t_begin);
-    BRegs bn;
-    load_b(bn, nxt);
-    put_b(bv, 0);
+    put_b(pre[0], 0);
     put_row();
-    bv = bn;
-    __syncthreads();
+    lds_barrier();
     int cb = 0;  // buffer holding the current tile's block
-    for (int tile = t_begin; tile < t_end; ++tile) {
-        const cx<float>* sb = sbb[cb];
-        const uint16_t(*sh)[16 * BS] = shb[cb];
-        A4 re[SL], im[SL];
+    for (int base = t_begin; base < t_end; base += DEPTH) {
 #pragma unroll
-        for (int sl = 0; sl < SL; ++sl) {
-            const int mb = wave + 4 * sl;
-            if (mb < MB) {
-                A4 r = {0, 0, 0, 0}, i = {0, 0, 0, 0};
-#pragma unroll
-                for (int t = 0; t < NK; ++t) {
-                    const int off = j * BS + 32 * t + 8 * q;
-                    const bf16x8 r0 = *reinterpret_cast<const bf16x8*>(&sh[0][off]);
-                    const bf16x8 r1 = *reinterpret_cast<const bf16x8*>(&sh[1][off]);
-                    const bf16x8 r2 = *reinterpret_cast<const bf16x8*>(&sh[2][off]);
-                    const bf16x8 i0 = *reinterpret_cast<const bf16x8*>(&sh[3][off]);
-                    const bf16x8 i1 = *reinterpret_cast<const bf16x8*>(&sh[4][off]);
-                    const bf16x8 i2 = *reinterpret_cast<const bf16x8*>(&sh[5][off]);
-                    const bf16x8* A = a[sl][t];
-                    r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[2], r0, r, 0, 0, 0);
-                    i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[2], i0, i, 0, 0, 0);
-                    r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], r1, r, 0, 0, 0);
-                    i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], i1, i, 0, 0, 0);
-                    r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], r2, r, 0, 0, 0);
-                    i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], i2, i, 0, 0, 0);
-                    r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], r0, r, 0, 0, 0);
-                    i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], i0, i, 0, 0, 0);
-                    r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], r1, r, 0, 0, 0);
-                    i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], i1, i, 0, 0, 0);
-                    r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], r0, r, 0, 0, 0);
-                    i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], i0, i, 0, 0, 0);
-                }
-                // k = K - 1 in FP32: rows 4q + e of the block, column j
-                const cx<float> bt = sb[j * CS + KB];
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    r[e] += at4[sl][e] * bt.x;
-                    i[e] += at4[sl][e] * bt.y;
-                }
-                re[sl] = r;
-                im[sl] = i;
-            }
-        }
-        cx<float> vrow{0.f, 0.f};
-        if (VR > 0 && wave < VR) {
-#pragma unroll
-            for (int s = 0; s < KS; ++s) {
-                const float av = sar[wave * KP + q * KS + s];
-                const cx<float> b = sb[j * CS + q * KS + s];
-                vrow.x += av * b.x;
-                vrow.y += av * b.y;
-            }
-            vrow.x += __shfl_xor(vrow.x, 16);
-            vrow.y += __shfl_xor(vrow.y, 16);
-            vrow.x += __shfl_xor(vrow.x, 32);
-            vrow.y += __shfl_xor(vrow.y, 32);
-        }
-        {
-            // results straight from the accumulators: lane (j, q) holds rows
-            // 4q .. 4q+3 of column j in each block, two 16-byte row pairs
-            const int ld = PM ? g.out_ld * g.F : g.out_ld;
-            cx<float>* obase = PM ? g.out + static_cast<size_t>(g.out_ld) * (cur.pad + static_cast<size_t>(g.F) * cur.col_begin)
-                                  : g.out + static_cast<size_t>(ld) * cur.col_begin;
-            const __amdgpu_buffer_rsrc_t r =
-                buffer_rsrc(obase, static_cast<uint32_t>(ld * (cur.ncols - 1) + M) * CB);
-            const uint32_t col = static_cast<uint32_t>(ld * j) * CB;
+        for (int slot = 0; slot < DEPTH; ++slot) {
+            const int tile = base + slot;
+            if (tile >= t_end) break;  // uniform across the workgroup
+            const GemmTile cur = rec[slot];
+            const cx<float>* sb = sbb[cb];
+            const uint16_t(*sh)[16 * BS] = shb[cb];
+            A4 re[SL], im[SL];
 #pragma unroll
             for (int sl = 0; sl < SL; ++sl) {
                 const int mb = wave + 4 * sl;
                 if (mb < MB) {
-                    const uint32_t at = col + static_cast<uint32_t>(16 * mb + 4 * q) * CB;
-                    raw_buffer_store_f4(bf4{re[sl][0], im[sl][0], re[sl][1], im[sl][1]}, r, static_cast<int>(at), 0, 0);
-                    raw_buffer_store_f4(bf4{re[sl][2], im[sl][2], re[sl][3], im[sl][3]}, r,
-                                        static_cast<int>(at + 2 * CB), 0, 0);
+                    A4 r = {0, 0, 0, 0}, i = {0, 0, 0, 0};
+#pragma unroll
+                    for (int t = 0; t < NK; ++t) {
+                        const int off = j * BS + 32 * t + 8 * q;
+                        const bf16x8 r0 = *reinterpret_cast<const bf16x8*>(&sh[0][off]);
+                        const bf16x8 r1 = *reinterpret_cast<const bf16x8*>(&sh[1][off]);
+                        const bf16x8 r2 = *reinterpret_cast<const bf16x8*>(&sh[2][off]);
+                        const bf16x8 i0 = *reinterpret_cast<const bf16x8*>(&sh[3][off]);
+                        const bf16x8 i1 = *reinterpret_cast<const bf16x8*>(&sh[4][off]);
+                        const bf16x8 i2 = *reinterpret_cast<const bf16x8*>(&sh[5][off]);
+                        const bf16x8* A = a[sl][t];
+                        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[2], r0, r, 0, 0, 0);
+                        i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[2], i0, i, 0, 0, 0);
+                        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], r1, r, 0, 0, 0);
+                        i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], i1, i, 0, 0, 0);
+                        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], r2, r, 0, 0, 0);
+                        i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], i2, i, 0, 0, 0);
+                        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], r0, r, 0, 0, 0);
+                        i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], i0, i, 0, 0, 0);
+                        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], r1, r, 0, 0, 0);
+                        i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], i1, i, 0, 0, 0);
+                        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], r0, r, 0, 0, 0);
+                        i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], i0, i, 0, 0, 0);
+                    }
+                    // k = K - 1 in FP32: rows 4q + e of the block, column j
+                    const cx<float> bt = sb[j * CS + KB];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        r[e] += at4[sl][e] * bt.x;
+                        i[e] += at4[sl][e] * bt.y;
+                    }
+                    re[sl] = r;
+                    im[sl] = i;
                 }
             }
-            if (VR > 0 && wave < VR)
-                buffer_store_cx<float>(vrow, r, q == 0 ? col + static_cast<uint32_t>(16 * MB + wave) * CB : kBufferPast);
-        }
-        if (tile + 1 < t_end) {
-            if constexpr (NB == 1) __syncthreads();  // every wavefront is done with the block
-            put_b(bv, NB == 1 ? 0 : cb ^ 1);  // the B block prefetched for tile + 1
-            cur = nxt;
-            if (cur.group != group) {
-                group = cur.group;
-                issue_group(group);
-                __syncthreads();  // every vector-ALU row of this tile has read the old coefficients
-                put_row();
-                __builtin_amdgcn_s_waitcnt(0);
+            cx<float> vrow{0.f, 0.f};
+            if (VR > 0 && wave < VR) {
+#pragma unroll
+                for (int s = 0; s < KS; ++s) {
+                    const float av = sar[wave * KP + q * KS + s];
+                    const cx<float> b = sb[j * CS + q * KS + s];
+                    vrow.x += av * b.x;
+                    vrow.y += av * b.y;
+                }
+                vrow.x += __shfl_xor(vrow.x, 16);
+                vrow.y += __shfl_xor(vrow.y, 16);
+                vrow.x += __shfl_xor(vrow.x, 32);
+                vrow.y += __shfl_xor(vrow.y, 32);
             }
-            if (tile + 2 < t_end) {
-                nxt = tile_record(g.tiles, tile + 2);
-                load_b(bv, nxt);
+            {
+                // results straight from the accumulators: lane (j, q) holds rows
+                // 4q .. 4q+3 of column j in each block, two 16-byte row pairs
+                const int ld = PM ? g.out_ld * g.F : g.out_ld;
+                cx<float>* obase = PM ? g.out + static_cast<size_t>(g.out_ld) *
+                                                    (cur.pad + static_cast<size_t>(g.F) * cur.col_begin)
+                                      : g.out + static_cast<size_t>(ld) * cur.col_begin;
+                const __amdgpu_buffer_rsrc_t r =
+                    buffer_rsrc(obase, static_cast<uint32_t>(ld * (cur.ncols - 1) + M) * CB);
+                const uint32_t col = static_cast<uint32_t>(ld * j) * CB;
+#pragma unroll
+                for (int sl = 0; sl < SL; ++sl) {
+                    const int mb = wave + 4 * sl;
+                    if (mb < MB) {
+                        const uint32_t at = col + static_cast<uint32_t>(16 * mb + 4 * q) * CB;
+                        raw_buffer_store_f4(bf4{re[sl][0], im[sl][0], re[sl][1], im[sl][1]}, r, static_cast<int>(at),
+                                            0, 0);
+                        raw_buffer_store_f4(bf4{re[sl][2], im[sl][2], re[sl][3], im[sl][3]}, r,
+                                            static_cast<int>(at + 2 * CB), 0, 0);
+                    }
+                }
+                if (VR > 0 && wave < VR)
+                    buffer_store_cx<float>(vrow, r,
+                                           q == 0 ? col + static_cast<uint32_t>(16 * MB + wave) * CB : kBufferPast);
             }
-            __syncthreads();  // block of tile + 1 complete; this tile's block is no longer read
-            if constexpr (NB == 2) cb ^= 1;
+            if (tile + 1 < t_end) {
+                const int next = slot + 1 < DEPTH ? slot + 1 : 0;  // constant once the slot loop is unrolled
+                if constexpr (NB == 1) lds_barrier();  // every wavefront is done with the block
+                put_b(pre[next], NB == 1 ? 0 : cb ^ 1);  // the B block of tile + 1
+                if (rec[next].group != group) {
+                    group = rec[next].group;
+                    issue_group(group);
+                    lds_barrier();  // every vector-ALU row of this tile has read the old coefficients
+                    put_row();
+                }
+                if (tile + DEPTH < t_end) {  // refill the slot this tile's block came from
+                    rec[slot] = tile_record(g.tiles, tile + DEPTH);
+                    load_b(pre[slot], rec[slot]);
+                }
+                lds_barrier();  // block of tile + 1 complete; this tile's block is no longer read
+                if constexpr (NB == 2) cb ^= 1;
+            }
         }
     }
 }
@@ -1363,6 +1382,8 @@ void vertical_forward_pm(const skb_vertical_plan* p, const void* pm, int np, voi
     a.ntiles = p->pm_ntiles;
     a.pm_np = np;
     a.pm_nz = static_cast<int>(p->w.nz);
+    a.pm_planes = static_cast<int>(p->w.nz * p->w.fields);
+    if (np % 16 != 0) fail(SKB_INVALID_ARGUMENT, "the chunked plane-major layout holds whole chunks of 16 modes");
     const hipStream_t s = as_stream(stream);
     switch (a.K) {
         case 65: launch_gemm_split<2, 65, true>(a, p->grid, s); break;
