@@ -1715,7 +1715,8 @@ int skb_pruned_plan_create(const skb_workload* workload, const skb_retained_mode
         const int line = precision == SKB_F64 ? 8 : 16;
         p->ps = workload->nx >= 64 && workload->ny >= 64 ? (p->planes + line - 1) / line * line : p->planes;
         p->scratch.allocate(p->lists.kc * workload->ny * static_cast<uint64_t>(p->ps) * complex_bytes(precision));
-        if (plane_supported(*workload, p->lists.kc, mode_count, precision)) {
+        if (plane_supported(*workload, p->lists.kc, mode_count, precision) &&
+            plane_band_empty(p->lists.fmap, p->lists.imap, p->lists.kc)) {
             p->plane.build(*workload, p->lists.kc, mode_count, p->lists.fmap, p->lists.imap);
             p->pm.allocate(static_cast<uint64_t>(p->plane.np) * p->planes * sizeof(cx<float>));
             p->plane_built = true;

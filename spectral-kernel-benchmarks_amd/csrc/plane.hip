@@ -90,6 +90,13 @@ struct PlaneGeo {
     static constexpr int RP = KC | 1;              // row-buffer pitch (complex, odd: column writes spread over banks)
     static constexpr int YP = N + 1;               // band-buffer pitch (floats, odd)
     static constexpr int KH = (KC + 15) / 16;      // registers k of a row FFT holding kx = n + 16k < KC
+    // Registers j of a column FFT whose k_y = 16 j + n (n < 16) lie in the
+    // band |k_y| >= KC that a 2/3-rule set never retains: [16 J0, 16 J1 + 15]
+    // inside (KC - 1, N - KC + 1).  Plans use the plane path only when their
+    // mode set leaves these slots empty (plane_band_empty), so the inverse
+    // columns take them as zero and the forward columns skip them.
+    static constexpr int J0 = (KC + 15) / 16, J1 = (N - KC + 1 - 16) / 16;
+    static constexpr bool empty(int j) { return j >= J0 && j <= J1; }
     static constexpr int SMAX = kPlaneMaxModes;    // retained values staged per plane
     static constexpr int R0 = std::max(std::max(H * RP * 8, KC * YP * 4), SMAX * 8);
     static constexpr int WT = 16 * kExchangeFloats * 4;  // one exchange scratch per wavefront
@@ -189,6 +196,10 @@ __device__ __forceinline__ void inverse_plane(unsigned char* lds, const cx<float
         if (64 * r + 4 * wave < KC) {
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
+                if (G::empty(j)) {
+                    y[r][j] = cx<float>{0.f, 0.f};
+                    continue;
+                }
                 const int e = unpack_code(code[r], j);
                 const bool ok = e != 0xFFFF && c < KC;
                 const cx<float> x = r0[ok ? (e >> 1) : 0];
@@ -358,6 +369,7 @@ __device__ __forceinline__ void forward_columns(unsigned char* lds, const cx<flo
             load_codes(code, fcodes, (c < KC ? c : KC - 1) * 16 + n);
 #pragma unroll
             for (int k = 0; k < 16; ++k) {
+                if (G::empty(k)) continue;  // no retained k_y there: these outputs are never computed
                 const int e = unpack_code(code, k);
                 if (e != 0xFFFF && c < KC) {
                     const cx<float> x = cv[r][k];
@@ -549,6 +561,16 @@ __global__ __launch_bounds__(256) void pm_transpose_scalar_kernel(const cx<float
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
+bool plane_band_empty(const std::vector<int>& fmap, const std::vector<int>& imap, uint64_t kc) {
+    using G = PlaneGeo<kPlaneKc>;
+    if (kc != kPlaneKc) return false;
+    for (int c = 0; c < kPlaneKc; ++c)
+        for (int ky = 16 * G::J0; ky < 16 * G::J1 + 16; ++ky)
+            if (fmap[static_cast<size_t>(c) * kPlaneN + ky] >= 0 || imap[static_cast<size_t>(c) * kPlaneN + ky] >= 0)
+                return false;
+    return true;
+}
+
 bool plane_supported(const skb_workload& w, uint64_t kc, uint64_t modes, int precision) {
     const uint64_t planes = w.nz * w.fields;
     return precision == SKB_F32 && w.nx == kPlaneN && w.ny == kPlaneN && kc == kPlaneKc &&

@@ -19,6 +19,10 @@ constexpr int kPlaneMaxPlanes = 480;   // LDS tile of the layout transpose
 // mode m of plane p at 16*P*(m / 16) + 16*p + m % 16, for P planes.
 uint64_t plane_np(uint64_t modes);
 bool plane_supported(const skb_workload& w, uint64_t kc, uint64_t modes, int precision);
+// The mode set leaves the k_y slots the plane kernels skip empty (k_y in
+// [96, 159] for every active k_x): true for every 2/3-rule set at 256^2.
+// fmap / imap: dense [kx * Ny + ky] -> code or -1 (see PlaneTables).
+bool plane_band_empty(const std::vector<int>& fmap, const std::vector<int>& imap, uint64_t kc);
 
 // Device tables of one mode set: twiddles and the packed 16-bit column codes
 // (2*mode + conj per (kx, ky) slot, 0xFFFF where empty) in the order each

@@ -254,3 +254,23 @@ def test_plane_path_with_8_byte_aligned_compact_arrays(device):
     torch.cuda.synchronize()
     assert torch.equal(out_a, out_big[1:])
     assert out_big[0].item() == 0
+
+
+def test_mode_set_outside_the_band_takes_two_pass(device):
+    """A mode with k_y in the band the plane kernels skip (here l = 100) sends
+    the plan to the two-pass kernels, which still match the oracle."""
+    import torch
+
+    w = S.Workload(256, 256, 7, 1)
+    modes = S.retained_modes(w)
+    extra = np.zeros(1, dtype=modes.dtype)
+    extra["k"], extra["l"], extra["stored_kx"], extra["stored_ky"], extra["radial"] = 1, 100, 1, 100, 100.0
+    modes = np.concatenate([modes, extra])
+    plan = S.PrunedPlan(w, modes, S.SKB_F32)
+    assert plan.path() == S.SKB_PATH_TWO_PASS
+    om = O.retained_modes(ow(w))
+    om = np.concatenate([om, extra.astype(om.dtype)])
+    x = O.spectral_gaussian(ow(w), om, 17)
+    real = torch.empty(w.real_elements, dtype=torch.float32, device=device)
+    plan.inverse(dev(x, torch.complex64, device), real)
+    assert O.max_relative_error(host(real), O.retained_inverse(ow(w), om, x)) <= TOL32
