@@ -124,6 +124,19 @@ int skb_fft_forward(skb_fft_plan* plan, const void* real, void* wvm, void* strea
 /* FFTWProvider::inverse: unnormalised c2r; DESTROYS `wvm` (as FFTW c2r may,
  * src/fftw_provider.cpp:1290-1292). */
 int skb_fft_inverse(skb_fft_plan* plan, void* wvm, void* real, void* stream);
+/* FFTWInterleavedWvmFieldViewProvider::forwardField / inverseFields
+ * (include/skbench/skbench.hpp:423-465; src/fftw_provider.cpp:1175-1230):
+ * the full transforms of one field held inside an interleaved WVM array.
+ * The plan's workload is the field's (planes = Nz * fields, normally
+ * fields = 1); the view holds element (kx, ky, plane) at
+ * spectrum[plane + frequency_stride * (kx + (Nx/2+1) * ky)] with
+ * frequency_stride >= planes (a field f of an F-field array: spectrum =
+ * base + f * Nz, stride F * Nz).  forward_view writes the view's planes only;
+ * inverse_view may overwrite the view, as FFTW's c2r does.  A stride below
+ * the plane count is SKB_INVALID_ARGUMENT. */
+int skb_fft_forward_view(skb_fft_plan* plan, const void* real, void* spectrum, uint64_t frequency_stride,
+                         void* stream);
+int skb_fft_inverse_view(skb_fft_plan* plan, void* spectrum, uint64_t frequency_stride, void* real, void* stream);
 /* FFTWProvider::gatherRetainedOuter / embedRetainedOuter
  * (src/fftw_provider.cpp:318-329; src/model.cpp:310-339). */
 int skb_fft_gather_retained(skb_fft_plan* plan, const skb_retained_mode* modes, uint64_t mode_count,

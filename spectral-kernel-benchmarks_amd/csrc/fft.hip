@@ -511,10 +511,11 @@ __global__ __launch_bounds__((RowGeo<R, NX>::THREADS)) void row2_r2c_kernel(cons
         // their own partners.  A wavefront holds S consecutive plane pairs of
         // 64/S values of k1, so each store instruction writes runs of S
         // plane pairs (256 bytes in FP32) per kx.  Each store is one 16-byte
-        // plane pair (FP32, even ps: the pair's second plane is padding when
-        // it does not exist) or two elements, through a buffer resource over
-        // the row: an element past kc or past the last plane goes past the
-        // row and is dropped.
+        // plane pair (FP32, even ps, both planes present) or two elements,
+        // through a buffer resource over the row: an element past kc or past
+        // the last plane goes past the row and is dropped.  Planes past the
+        // last one are never written (a strided WVM field view interleaves
+        // other fields there).
         {
             constexpr uint32_t CB = sizeof(cx<R>);
             const int yu = __builtin_amdgcn_readfirstlane(y), p0u = __builtin_amdgcn_readfirstlane(p0);
@@ -538,7 +539,7 @@ __global__ __launch_bounds__((RowGeo<R, NX>::THREADS)) void row2_r2c_kernel(cons
                 const bool ok = kx < kc && has_a;
                 const uint32_t off = ok ? static_cast<uint32_t>(ps * kx + pa) * CB : kBufferPast;
                 if constexpr (sizeof(R) == 4) {
-                    if (wide) {
+                    if (wide && has_b) {
                         raw_buffer_store_f4(bf4{xa.x, xa.y, xb.x, xb.y}, rs, static_cast<int>(off), 0, 0);
                         continue;
                     }
@@ -1393,6 +1394,9 @@ struct skb_fft_plan {
     skb_workload w{};
     int precision = SKB_F64;
     skb::Twiddles tw;
+    // Contiguous spectrum that strided field views go through when the grid
+    // is below the two-pass kernels' 64 x 64 (allocated at creation then).
+    skb::DeviceBuffer view_stage;
     // Device copy of the mode table last handed to gather/embed.
     std::vector<skb_retained_mode> cached_modes;
     skb::DeviceBuffer gather_table, embed_table;
@@ -1455,26 +1459,54 @@ void refresh_mode_tables(skb_fft_plan* plan, const skb_retained_mode* modes, uin
     plan->cached_modes.assign(modes, modes + n);
 }
 
+// ps: frequency stride of the spectrum (the plane count, or a WVM field
+// view's stride; ps > planes needs Nx, Ny >= 64).
 template <typename R>
-void fft_forward(skb_fft_plan* p, const void* real, void* wvm, hipStream_t s) {
+void fft_forward(skb_fft_plan* p, const void* real, void* wvm, hipStream_t s, int ps = 0) {
     const auto& w = p->w;
     const int planes = static_cast<int>(w.nz * w.fields);
     const int nxh = static_cast<int>(w.nx / 2 + 1);
-    row_pass<R>(false, static_cast<int>(w.nx), real, wvm, p->tw.x.ptr, static_cast<int>(w.ny), planes, planes, nxh, 0,
-                s);
-    col_pass<R>(ColVariant::FwdStrided, static_cast<int>(w.ny), wvm, wvm, p->tw.y.ptr, planes, planes, nxh, nxh, nxh,
+    if (ps == 0) ps = planes;
+    row_pass<R>(false, static_cast<int>(w.nx), real, wvm, p->tw.x.ptr, static_cast<int>(w.ny), planes, ps, nxh, 0, s);
+    col_pass<R>(ColVariant::FwdStrided, static_cast<int>(w.ny), wvm, wvm, p->tw.y.ptr, planes, ps, nxh, nxh, nxh,
                 nullptr, nullptr, nullptr, 0, s);
 }
 
 template <typename R>
-void fft_inverse(skb_fft_plan* p, void* wvm, void* real, hipStream_t s) {
+void fft_inverse(skb_fft_plan* p, void* wvm, void* real, hipStream_t s, int ps = 0) {
     const auto& w = p->w;
     const int planes = static_cast<int>(w.nz * w.fields);
     const int nxh = static_cast<int>(w.nx / 2 + 1);
-    col_pass<R>(ColVariant::InvStrided, static_cast<int>(w.ny), wvm, wvm, p->tw.y.ptr, planes, planes, nxh, nxh, nxh,
+    if (ps == 0) ps = planes;
+    col_pass<R>(ColVariant::InvStrided, static_cast<int>(w.ny), wvm, wvm, p->tw.y.ptr, planes, ps, nxh, nxh, nxh,
                 nullptr, nullptr, nullptr, 0, s);
-    row_pass<R>(true, static_cast<int>(w.nx), wvm, real, p->tw.x.ptr, static_cast<int>(w.ny), planes, planes, nxh, 0,
-                s);
+    row_pass<R>(true, static_cast<int>(w.nx), wvm, real, p->tw.x.ptr, static_cast<int>(w.ny), planes, ps, nxh, 0, s);
+}
+
+// A WVM field view of stride `stride`: native for Nx, Ny >= 64, otherwise
+// through the plan's contiguous stage.
+void check_view(const skb_fft_plan* p, uint64_t stride) {
+    const uint64_t planes = p->w.nz * p->w.fields, freqs = (p->w.nx / 2 + 1) * p->w.ny;
+    if (stride < planes) fail(SKB_INVALID_ARGUMENT, "WVM field-view frequency strides must contain every vertical plane.");
+    if (stride * freqs > (1ull << 31)) fail(SKB_UNSUPPORTED, "WVM field view exceeds 2^31 elements");
+}
+
+template <typename R>
+void fft_forward_view(skb_fft_plan* p, const void* real, void* view, uint64_t stride, hipStream_t s) {
+    check_view(p, stride);
+    const int planes = static_cast<int>(p->w.nz * p->w.fields);
+    if (!p->view_stage.ptr) return fft_forward<R>(p, real, view, s, static_cast<int>(stride));
+    fft_forward<R>(p, real, p->view_stage.ptr, s);
+    copy_planes(p->precision, (p->w.nx / 2 + 1) * p->w.ny, planes, p->view_stage.ptr, planes, view, stride, s);
+}
+
+template <typename R>
+void fft_inverse_view(skb_fft_plan* p, void* view, uint64_t stride, void* real, hipStream_t s) {
+    check_view(p, stride);
+    const int planes = static_cast<int>(p->w.nz * p->w.fields);
+    if (!p->view_stage.ptr) return fft_inverse<R>(p, view, real, s, static_cast<int>(stride));
+    copy_planes(p->precision, (p->w.nx / 2 + 1) * p->w.ny, planes, view, stride, p->view_stage.ptr, planes, s);
+    fft_inverse<R>(p, p->view_stage.ptr, real, s);
 }
 
 // The four stages of the pruned transforms; T is the plan's kc x Ny x ps
@@ -1594,6 +1626,9 @@ int skb_fft_plan_create(const skb_workload* workload, int precision, skb_fft_pla
         p->w = *workload;
         p->precision = precision;
         p->tw.build(precision, workload->nx, workload->ny);
+        if (workload->nx < 64 || workload->ny < 64)
+            p->view_stage.allocate((workload->nx / 2 + 1) * workload->ny * workload->nz * workload->fields *
+                                   complex_bytes(precision));
         *plan = p.release();
     });
 }
@@ -1617,6 +1652,27 @@ int skb_fft_inverse(skb_fft_plan* plan, void* wvm, void* real, void* stream) {
         require_ptrs(real, wvm);
         if (plan->precision == SKB_F64) fft_inverse<double>(plan, wvm, real, as_stream(stream));
         else fft_inverse<float>(plan, wvm, real, as_stream(stream));
+    });
+}
+
+int skb_fft_forward_view(skb_fft_plan* plan, const void* real, void* spectrum, uint64_t frequency_stride,
+                         void* stream) {
+    return guarded([&] {
+        if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
+        require_ptrs(real, spectrum);
+        if (plan->precision == SKB_F64)
+            fft_forward_view<double>(plan, real, spectrum, frequency_stride, as_stream(stream));
+        else fft_forward_view<float>(plan, real, spectrum, frequency_stride, as_stream(stream));
+    });
+}
+
+int skb_fft_inverse_view(skb_fft_plan* plan, void* spectrum, uint64_t frequency_stride, void* real, void* stream) {
+    return guarded([&] {
+        if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
+        require_ptrs(real, spectrum);
+        if (plan->precision == SKB_F64)
+            fft_inverse_view<double>(plan, spectrum, frequency_stride, real, as_stream(stream));
+        else fft_inverse_view<float>(plan, spectrum, frequency_stride, real, as_stream(stream));
     });
 }
 

@@ -15,6 +15,9 @@ namespace skb {
 // deinterleave skips a null re or im array (not both).
 void interleave(int precision, uint64_t n, const void* re, const void* im, void* out, hipStream_t s);
 void deinterleave(int precision, uint64_t n, const void* in, void* re, void* im, double scale, hipStream_t s);
+// dst[p + dst_stride*f] = src[p + src_stride*f] for f < frequencies, p < planes.
+void copy_planes(int precision, uint64_t frequencies, int planes, const void* src, uint64_t src_stride, void* dst,
+                 uint64_t dst_stride, hipStream_t s);
 
 struct PlaneTables;
 // The pruned plan's plane-resident tables and its plane-major scratch

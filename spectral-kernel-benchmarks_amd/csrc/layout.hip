@@ -37,7 +37,32 @@ __global__ void scale_split_kernel(R* __restrict__ re, R* __restrict__ im, R sca
     }
 }
 
+template <typename R>
+__global__ void copy_planes_kernel(const cx<R>* __restrict__ src, uint64_t src_stride, cx<R>* __restrict__ dst,
+                                   uint64_t dst_stride, uint32_t planes, size_t n) {
+    for (size_t e = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; e < n;
+         e += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const size_t f = e / planes, p = e - f * planes;
+        dst[p + dst_stride * f] = src[p + src_stride * f];
+    }
+}
+
 }  // namespace
+
+void copy_planes(int precision, uint64_t frequencies, int planes, const void* src, uint64_t src_stride, void* dst,
+                 uint64_t dst_stride, hipStream_t s) {
+    const size_t n = static_cast<size_t>(frequencies) * static_cast<size_t>(planes);
+    if (n == 0) return;
+    if (precision == SKB_F64)
+        hipLaunchKernelGGL(copy_planes_kernel<double>, stream_grid(n), dim3(256), 0, s,
+                           static_cast<const cx<double>*>(src), src_stride, static_cast<cx<double>*>(dst), dst_stride,
+                           static_cast<uint32_t>(planes), n);
+    else
+        hipLaunchKernelGGL(copy_planes_kernel<float>, stream_grid(n), dim3(256), 0, s,
+                           static_cast<const cx<float>*>(src), src_stride, static_cast<cx<float>*>(dst), dst_stride,
+                           static_cast<uint32_t>(planes), n);
+    check_hip(hipGetLastError(), "plane copy launch");
+}
 
 void interleave(int precision, uint64_t n, const void* re, const void* im, void* out, hipStream_t s) {
     if (n == 0) return;

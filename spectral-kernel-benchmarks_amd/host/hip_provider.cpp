@@ -853,6 +853,146 @@ StreamingInversePreparationPolicy HipStreamingPrunedSplitProvider::inversePrepar
 std::string HipStreamingPrunedSplitProvider::libraryIdentity() const { return "libskb_hip.so"; }
 std::string HipStreamingPrunedSplitProvider::version() const { return skb_version(); }
 
+// ---- HipInterleavedWvmFieldViewProvider ------------------------------------------------
+// Host views are packed into (and unpacked from) a contiguous one-field
+// spectrum on the host side of the adapter; device views go to the library's
+// strided entry points unchanged.
+struct HipInterleavedWvmFieldViewProvider::Impl {
+    Workload w;
+    VerticalGemmBufferPolicy policy;
+    std::vector<std::size_t> strides;
+    skb_fft_plan* plan = nullptr;
+    Stream stream;
+    Staged real, spectrum;
+    std::vector<double> host;  // packed one-field spectrum (interleaved doubles)
+    double allocation_seconds = 0, planning_seconds = 0;
+
+    // src/fftw_provider.cpp:1066-1110
+    Impl(const Workload& wl, std::vector<std::size_t> st, VerticalGemmBufferPolicy bp, Precision p)
+        : w(wl), policy(bp), strides(std::move(st)) {
+        if (w.fields != 1) throw std::invalid_argument("WVM field-view FFTW requires a one-field logical workload.");
+        if (strides.empty())
+            throw std::invalid_argument("WVM field-view FFTW requires at least one native frequency stride.");
+        for (const auto s : strides)
+            if (s < w.nz)
+                throw std::invalid_argument("WVM field-view frequency strides must contain every vertical plane.");
+        std::sort(strides.begin(), strides.end());
+        strides.erase(std::unique(strides.begin(), strides.end()), strides.end());
+        const auto cw = to_c(w);
+        const auto t0 = Clock::now();
+        check(skb_fft_plan_create(&cw, c_precision(p), &plan));
+        planning_seconds = seconds_since(t0);
+        const auto t1 = Clock::now();
+        const bool single = p == Precision::float32;
+        real.allocate(w.realElements(), single);
+        spectrum.allocate(2 * w.spectrumElements(), single);
+        host.resize(2 * w.spectrumElements());
+        allocation_seconds = seconds_since(t1);
+    }
+    ~Impl() {
+        if (plan) skb_fft_plan_destroy(plan);
+    }
+    void require_forward() const {
+        if (policy == VerticalGemmBufferPolicy::inverseOnly)
+            throw std::logic_error("Forward transform requested from an inverse-only WVM field-view provider.");
+    }
+    void require_inverse() const {
+        if (policy == VerticalGemmBufferPolicy::forwardOnly)
+            throw std::logic_error("Inverse transform requested from a forward-only WVM field-view provider.");
+    }
+    void require_stride(std::size_t s) const {
+        if (!std::binary_search(strides.begin(), strides.end(), s))
+            throw std::invalid_argument("WVM field view uses a frequency stride that was not planned.");
+    }
+    template <typename V>
+    void validate_view(const V& v) const {
+        if (v.data == nullptr) throw std::invalid_argument("WVM FFTW field view is null.");
+        require_stride(v.frequencyStride);
+    }
+    std::size_t frequencies() const { return w.nxHalf() * w.ny; }
+};
+
+HipInterleavedWvmFieldViewProvider::HipInterleavedWvmFieldViewProvider(const Workload& singleFieldWorkload,
+                                                                       std::vector<std::size_t> frequencyStrides,
+                                                                       VerticalGemmBufferPolicy bufferPolicy,
+                                                                       Precision precision)
+    : impl_(std::make_unique<Impl>(singleFieldWorkload, std::move(frequencyStrides), bufferPolicy, precision)) {}
+HipInterleavedWvmFieldViewProvider::~HipInterleavedWvmFieldViewProvider() = default;
+HipInterleavedWvmFieldViewProvider::HipInterleavedWvmFieldViewProvider(HipInterleavedWvmFieldViewProvider&&) noexcept =
+    default;
+HipInterleavedWvmFieldViewProvider& HipInterleavedWvmFieldViewProvider::operator=(
+    HipInterleavedWvmFieldViewProvider&&) noexcept = default;
+
+void HipInterleavedWvmFieldViewProvider::inverseFields(MutableFieldView* fields, std::size_t fieldCount,
+                                                       double* output) {
+    auto& m = *impl_;
+    m.require_inverse();
+    if (fields == nullptr || fieldCount == 0)
+        throw std::invalid_argument("WVM field-view inverse requires at least one field.");
+    for (std::size_t f = 0; f < fieldCount; ++f) m.validate_view(fields[f]);
+    if (output == nullptr) throw std::invalid_argument("WVM field-view inverse output is null.");
+    const std::size_t nz = m.w.nz, freqs = m.frequencies(), vol = m.w.realElements();
+    for (std::size_t f = 0; f < fieldCount; ++f) {
+        const Complex* v = fields[f].data;
+        const std::size_t s = fields[f].frequencyStride;
+        for (std::size_t q = 0; q < freqs; ++q)
+            for (std::size_t z = 0; z < nz; ++z) {
+                m.host[2 * (z + nz * q)] = v[z + s * q].real;
+                m.host[2 * (z + nz * q) + 1] = v[z + s * q].imag;
+            }
+        m.spectrum.upload(m.host.data(), m.stream.s);
+        check(skb_fft_inverse(m.plan, m.spectrum.dev, m.real.dev, m.stream.s));
+        m.real.download(output + f * vol, m.stream.s);
+    }
+}
+
+void HipInterleavedWvmFieldViewProvider::forwardField(const double* input, MutableFieldView output) {
+    auto& m = *impl_;
+    m.require_forward();
+    m.validate_view(output);
+    if (input == nullptr) throw std::invalid_argument("WVM field-view forward input is null.");
+    m.real.upload(input, m.stream.s);
+    check(skb_fft_forward(m.plan, m.real.dev, m.spectrum.dev, m.stream.s));
+    m.spectrum.download(m.host.data(), m.stream.s);
+    const std::size_t nz = m.w.nz, freqs = m.frequencies(), s = output.frequencyStride;
+    for (std::size_t q = 0; q < freqs; ++q)
+        for (std::size_t z = 0; z < nz; ++z)
+            output.data[z + s * q] = {m.host[2 * (z + nz * q)], m.host[2 * (z + nz * q) + 1]};
+}
+
+void HipInterleavedWvmFieldViewProvider::inverseFieldsDevice(const DeviceFieldView* fields, std::size_t fieldCount,
+                                                             void* output, void* stream) {
+    auto& m = *impl_;
+    m.require_inverse();
+    if (fields == nullptr || fieldCount == 0)
+        throw std::invalid_argument("WVM field-view inverse requires at least one field.");
+    for (std::size_t f = 0; f < fieldCount; ++f) m.validate_view(fields[f]);
+    if (output == nullptr) throw std::invalid_argument("WVM field-view inverse output is null.");
+    const std::size_t bytes = m.w.realElements() * (m.real.fp32 ? 4 : 8);
+    for (std::size_t f = 0; f < fieldCount; ++f)
+        check(skb_fft_inverse_view(m.plan, fields[f].data, fields[f].frequencyStride,
+                                   static_cast<char*>(output) + f * bytes, stream));
+}
+
+void HipInterleavedWvmFieldViewProvider::forwardFieldDevice(const void* input, DeviceFieldView output, void* stream) {
+    auto& m = *impl_;
+    m.require_forward();
+    m.validate_view(output);
+    if (input == nullptr) throw std::invalid_argument("WVM field-view forward input is null.");
+    check(skb_fft_forward_view(m.plan, input, output.data, output.frequencyStride, stream));
+}
+
+void HipInterleavedWvmFieldViewProvider::executeSchedulerNoop() {
+    hip_check(hipStreamSynchronize(impl_->stream.s), "sync");
+}
+double HipInterleavedWvmFieldViewProvider::allocationSeconds() const noexcept { return impl_->allocation_seconds; }
+double HipInterleavedWvmFieldViewProvider::planningSeconds() const noexcept { return impl_->planning_seconds; }
+std::size_t HipInterleavedWvmFieldViewProvider::planningBytes() const noexcept {
+    return impl_->w.realElements() * 8 + impl_->strides.back() * impl_->frequencies() * 16;
+}
+std::string HipInterleavedWvmFieldViewProvider::libraryIdentity() const { return "libskb_hip.so"; }
+std::string HipInterleavedWvmFieldViewProvider::version() const { return skb_version(); }
+
 // ---- HipVerticalGemmProvider ----------------------------------------------------------
 std::string verticalGemmLayoutName(VerticalGemmLayout layout) {
     return layout == VerticalGemmLayout::split ? "split" : "complex-interleaved";

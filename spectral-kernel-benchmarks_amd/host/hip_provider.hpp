@@ -11,6 +11,8 @@
 //                                      (include/skbench/skbench.hpp:220-257)
 //   HipStreamingPrunedSplitProvider <- skbench::FFTWStreamingPrunedSplitProvider
 //                                      (include/skbench/skbench.hpp:534-621)
+//   HipInterleavedWvmFieldViewProvider <- skbench::FFTWInterleavedWvmFieldViewProvider
+//                                      (include/skbench/skbench.hpp:423-465)
 // Host-pointer methods are the reference's "adapter" scope: they stage the
 // caller's FP64 arrays through device buffers owned by the provider (allocated
 // at construction; nothing is allocated per call) and return only after the
@@ -266,6 +268,54 @@ public:
     StreamingInversePreparationPolicy inversePreparationPolicy() const noexcept;
     bool rowInversePreservesInput() const noexcept { return true; }
     bool completeHalfSpectrumMaterialized() const noexcept { return false; }
+    std::string libraryIdentity() const;
+    std::string version() const;
+
+private:
+    struct Impl;
+    std::unique_ptr<Impl> impl_;
+};
+
+// Full transforms of single fields that live inside interleaved WVM arrays
+// with their native frequency stride (include/skbench/skbench.hpp:423-465,
+// src/fftw_provider.cpp:919-1260).  A view holds element (kx, ky, z) at
+// data[z + frequencyStride * (kx + (Nx/2+1) * ky)]; the provider is built for
+// a one-field workload and the list of strides its views may use.
+class HipInterleavedWvmFieldViewProvider {
+public:
+    struct MutableFieldView {
+        Complex* data = nullptr;
+        std::size_t frequencyStride = 0;
+    };
+    struct DeviceFieldView {
+        void* data = nullptr;  // device array of the plan's precision
+        std::size_t frequencyStride = 0;
+    };
+
+    HipInterleavedWvmFieldViewProvider(const Workload& singleFieldWorkload, std::vector<std::size_t> frequencyStrides,
+                                       VerticalGemmBufferPolicy bufferPolicy = VerticalGemmBufferPolicy::bidirectional,
+                                       Precision precision = Precision::float64);
+    ~HipInterleavedWvmFieldViewProvider();
+    HipInterleavedWvmFieldViewProvider(HipInterleavedWvmFieldViewProvider&&) noexcept;
+    HipInterleavedWvmFieldViewProvider& operator=(HipInterleavedWvmFieldViewProvider&&) noexcept;
+    HipInterleavedWvmFieldViewProvider(const HipInterleavedWvmFieldViewProvider&) = delete;
+    HipInterleavedWvmFieldViewProvider& operator=(const HipInterleavedWvmFieldViewProvider&) = delete;
+
+    // adapter scope (host arrays): the output holds field-major real volumes
+    // in the order supplied; the inverse may overwrite each view
+    void inverseFields(MutableFieldView* fields, std::size_t fieldCount, double* output);
+    void forwardField(const double* input, MutableFieldView output);
+    // primitive scope (device views and volumes, caller's stream)
+    void inverseFieldsDevice(const DeviceFieldView* fields, std::size_t fieldCount, void* output, void* stream);
+    void forwardFieldDevice(const void* input, DeviceFieldView output, void* stream);
+    void executeSchedulerNoop();
+    bool inverseInputIsDestructive() const noexcept { return true; }
+    double otherSetupSeconds() const noexcept { return 0.0; }
+    double allocationSeconds() const noexcept;
+    double planningSeconds() const noexcept;
+    std::size_t planningBytes() const noexcept;
+    std::size_t internalWorkers() const noexcept { return 1; }
+    std::size_t outerWorkers() const noexcept { return 1; }
     std::string libraryIdentity() const;
     std::string version() const;
 

@@ -201,6 +201,8 @@ def _load():
         "skb_vertical_forward_split": (i32, [vp, vp, vp, vp, vp, vp]),
         "skb_vertical_inverse_split": (i32, [vp, vp, vp, vp, vp, vp]),
         "skb_vertical_forward_split_noalloc": (i32, [vp, vp, vp, vp, vp, vp]),
+        "skb_fft_forward_view": (i32, [vp, vp, vp, u64, vp]),
+        "skb_fft_inverse_view": (i32, [vp, vp, u64, vp, vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -240,7 +242,7 @@ EXPORTED = [
     "skb_retained_operator_real", "skb_retained_operator_device_bytes", "skb_retained_operator_set_fused",
     "skb_retained_operator_fused", "skb_interleave", "skb_deinterleave", "skb_scale_split",
     "skb_vertical_plan_reserve_split", "skb_vertical_forward_split", "skb_vertical_inverse_split",
-    "skb_vertical_forward_split_noalloc",
+    "skb_vertical_forward_split_noalloc", "skb_fft_forward_view", "skb_fft_inverse_view",
 ]
 
 
@@ -525,6 +527,30 @@ class FftPlan:
         w = self.w
         check(lib.skb_fft_inverse(self.h, _dev(wvm, w.spectrum_elements, complex_dtype(self.precision), "wvm"),
                                   _dev(real, w.real_elements, real_dtype(self.precision), "real"), _stream(stream)))
+
+    def _view(self, base, offset: int, stride: int):
+        """Pointer to a WVM field view inside `base` (a device complex array):
+        element (kx, ky, plane) at base[offset + plane + stride * (kx + nx_half * ky)]."""
+        w = self.w
+        cd = complex_dtype(self.precision)
+        _dev(base, base.numel(), cd, "spectrum")
+        last = offset + w.planes - 1 + stride * (w.nx_half * w.ny - 1)
+        if offset < 0 or stride < w.planes or last >= base.numel():
+            raise InvalidArgument(1, "field view outside its array")
+        return ctypes.c_void_p(base.data_ptr() + offset * base.element_size())
+
+    def forward_view(self, real, base, offset: int, stride: int, stream=None):
+        """FFTWInterleavedWvmFieldViewProvider::forwardField into a view of `base`."""
+        w = self.w
+        check(lib.skb_fft_forward_view(self.h, _dev(real, w.real_elements, real_dtype(self.precision), "real"),
+                                       self._view(base, offset, stride), stride, _stream(stream)))
+
+    def inverse_view(self, base, offset: int, stride: int, real, stream=None):
+        """FFTWInterleavedWvmFieldViewProvider::inverseFields for one view (may overwrite it)."""
+        w = self.w
+        check(lib.skb_fft_inverse_view(self.h, self._view(base, offset, stride), stride,
+                                       _dev(real, w.real_elements, real_dtype(self.precision), "real"),
+                                       _stream(stream)))
 
     def gather_retained(self, modes, wvm, retained, stream=None):
         w = self.w

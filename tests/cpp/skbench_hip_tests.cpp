@@ -465,6 +465,50 @@ int main() {
             require(layoutErrors == 2, "FFT entry points check the planned data layout");
         }
 
+        // Native-stride WVM field views (include/skbench/skbench.hpp:423-465):
+        // the two fields of the 8x8x7x2 WVM array as views of stride 14
+        {
+            const Workload one{8, 8, 7, 1, 1.0, 1.0, true};
+            HipInterleavedWvmFieldViewProvider views(one, {14, 7});
+            require(views.inverseInputIsDestructive(), "field-view inverse may overwrite its views");
+            const auto input = makeFixture(w, FixtureKind::random, 77);
+            std::vector<Complex> expect(w.spectrumElements()), got(w.spectrumElements(), Complex{3.0, 3.0});
+            or_direct_r2c(8, 8, 7, 2, input.data(), reinterpret_cast<or_complex*>(expect.data()));
+            const std::size_t vol = one.realElements();
+            views.forwardField(input.data() + vol, {got.data() + 7, 14});
+            bool untouched = true;
+            for (std::size_t q = 0; q < w.nxHalf() * w.ny; ++q)
+                for (std::size_t z = 0; z < 7; ++z)
+                    untouched = untouched && got[z + 14 * q].real == 3.0 && got[z + 14 * q].imag == 3.0;
+            require(untouched, "forwardField writes its own view only");
+            views.forwardField(input.data(), {got.data(), 14});
+            require(or_max_relative_error_complex(got.data(), expect.data(), got.size()) <= 1e-12,
+                    "forwardField views vs direct DFT");
+            std::vector<double> back(w.realElements());
+            HipInterleavedWvmFieldViewProvider::MutableFieldView fv[2] = {{got.data(), 14}, {got.data() + 7, 14}};
+            views.inverseFields(fv, 2, back.data());
+            require(or_max_relative_error_real(back.data(), input.data(), back.size(), 1.0 / 64.0) <= 1e-12,
+                    "inverseFields round trip");
+            int errors = 0;
+            try {
+                views.forwardField(input.data(), {got.data(), 9});
+            } catch (const std::invalid_argument&) {
+                ++errors;
+            }
+            try {
+                HipInterleavedWvmFieldViewProvider bad(w, {14});
+            } catch (const std::invalid_argument&) {
+                ++errors;
+            }
+            try {
+                HipInterleavedWvmFieldViewProvider inv(one, {14}, VerticalGemmBufferPolicy::inverseOnly);
+                inv.forwardField(input.data(), {got.data(), 14});
+            } catch (const std::logic_error&) {
+                ++errors;
+            }
+            require(errors == 3, "field-view stride, workload and policy checks");
+        }
+
         HipPointwiseAdvectionExecutor advection(37, 0.03125);
         std::vector<double> shared(111), derivative(111), target(37), expected(37);
         for (size_t i = 0; i < 111; ++i) {
