@@ -7,8 +7,11 @@ CSV and its `compare` summary (src/json.cpp:282-332; src/main.cpp:229-238).
         --json out.json --csv out.csv
     python report.py compare --input out.csv
     python report.py check out.json
+    python report.py validate --profile smoke
 
-Kernels: fft, pruned-horizontal, vertical-gemm, spectral-pipeline.  Each
+Kernels: fft, pruned-horizontal, vertical-gemm, spectral-pipeline,
+production-lifetime-flux.  `validate` is the reference's validateBenchmark
+(src/benchmark.cpp:4898-5003, src/main.cpp:163-173) for the hip provider.  Each
 record carries primitive-scope timings (device time of one call per sample,
 after `warmups` untimed calls, median reported as in src/benchmark.cpp:91-111)
 and correctness metrics computed by the harness itself -- two independent
@@ -356,6 +359,123 @@ def write_json(rec: dict, path: str):
 # ---- measuring the hip provider ----------------------------------------------------------
 
 
+# ---- validate (src/benchmark.cpp:4898-5003) ------------------------------------------
+# The command's own host references on the 8 x 8 validation grid, as the
+# reference's validateBenchmark uses its host directR2C / gatherRetained /
+# verticalForward (src/model.cpp:296-355, 663-716): explicit DFT sums, the
+# mode permutation and the vertical products, in numpy.
+VALIDATE_FIXTURES = ("impulse", "sinusoid", "random", "dc", "nyquist")
+
+
+def _host_r2c(w, x) -> np.ndarray:
+    """Direct DFT sums of every plane into WVM order (plane + P*(kx + nxh*ky))."""
+    nxh, P = w.nx // 2 + 1, w.planes
+    ex = np.exp(-2j * np.pi * np.outer(np.arange(nxh), np.arange(w.nx)) / w.nx)
+    ey = np.exp(-2j * np.pi * np.outer(np.arange(w.ny), np.arange(w.ny)) / w.ny)
+    planes = np.asarray(x, np.float64).reshape(P, w.ny, w.nx)
+    spec = np.einsum("ky,pyx,qx->kqp", ey, planes, ex)  # [ky][kx][plane]
+    return np.ascontiguousarray(spec.reshape(-1))
+
+
+def _host_gather(w, modes, spec) -> np.ndarray:
+    nxh, P = w.nx // 2 + 1, w.planes
+    s = spec.reshape(w.ny, nxh, P)
+    out = s[modes["stored_ky"].astype(np.int64), modes["stored_kx"].astype(np.int64)].copy()
+    conj = modes["conj"] != 0
+    out[conj] = np.conj(out[conj])
+    return out.reshape(-1)
+
+
+def validate(profile: str = "smoke", stream=sys.stdout) -> int:
+    """validateBenchmark for the hip provider: full and pruned transforms of
+    the five fixtures against direct DFT sums, the retained gather, the
+    vertical projection of the retained modes and its modal round trip
+    (FP64, 1e-12).  Prints the reference's message lines; returns 0 when
+    every check passes, 1 otherwise."""
+    import torch
+
+    import skb_hip as S
+
+    if profile not in PROFILES:
+        raise ValueError("unknown profile %r" % profile)
+    shape = PROFILES[profile][:4] if profile == "smoke" else (8, 8, 7, 2)
+    w = S.Workload(*shape)
+    device = torch.device("cuda:0")
+    tol = 1e-12
+    modes = S.retained_modes(w)
+    nj = w.nj
+    f_mat, i_mat = S.orthonormal_vertical(w.nz, nj)
+    groups = np.zeros(1, S.GROUP_DTYPE)
+    groups["first_mode"], groups["mode_count"] = 0, len(modes)
+    full = S.FftPlan(w, S.SKB_F64)
+    pruned = S.PrunedPlan(w, modes, S.SKB_F64)
+    vplan = S.VerticalPlan(w, nj, groups, f_mat, i_mat, S.SKB_F64)
+    F = f_mat.reshape(nj, w.nz)
+    n = w.nx * w.ny
+    cd = torch.complex128
+    passed = True
+    lines = []
+
+    def dev(a, dtype):
+        return torch.from_numpy(np.ascontiguousarray(a)).to(device=device, dtype=dtype)
+
+    def host(t):
+        torch.cuda.synchronize()
+        return t.detach().cpu().numpy()
+
+    for i, kind in enumerate(VALIDATE_FIXTURES):
+        x = S.make_fixture(w, kind, 129 + i)
+        oracle = _host_r2c(w, x)
+        d_x = dev(x, torch.float64)
+        spec = torch.empty(w.spectrum_elements, dtype=cd, device=device)
+        full.forward(d_x, spec)
+        err = max_relative_error(host(spec), oracle)
+        passed &= err <= tol
+        lines.append("hip/%s/forward=%f" % (kind, err))
+        work = dev(oracle, cd)
+        real = torch.empty(w.real_elements, dtype=torch.float64, device=device)
+        full.inverse(work, real)
+        err = max_relative_error(host(real) / n, x)
+        passed &= err <= tol
+        lines.append("hip/%s/inverse=%f" % (kind, err))
+        retained_ref = _host_gather(w, modes, oracle)
+        ret = torch.empty(len(modes) * w.planes, dtype=cd, device=device)
+        pruned.forward(d_x, ret)
+        err = max_relative_error(host(ret), retained_ref)
+        passed &= err <= tol
+        lines.append("operator/%s/retained=%f" % (kind, err))
+        # pruned inverse against embed + full inverse on the device
+        d_ref = dev(retained_ref, cd)
+        back = torch.empty_like(real)
+        pruned.inverse(d_ref, back)
+        emb = torch.empty(w.spectrum_elements, dtype=cd, device=device)
+        full.embed_retained(modes, d_ref, emb)
+        back_full = torch.empty_like(real)
+        full.inverse(emb, back_full)
+        err = max_relative_error(host(back), host(back_full))
+        passed &= err <= tol
+        lines.append("hip-pruned/%s/inverse=%f" % (kind, err))
+        modal = torch.empty(len(modes) * w.fields * nj, dtype=cd, device=device)
+        vplan.forward(ret, modal)
+        expect = (retained_ref.reshape(len(modes), w.fields, w.nz) @ F.T).reshape(-1)
+        err = max_relative_error(host(modal), expect)
+        passed &= err <= tol
+        lines.append("operator/%s/horizontal-vertical=%f" % (kind, err))
+        phys = torch.empty_like(ret)
+        vplan.inverse(modal, phys)
+        again = torch.empty_like(modal)
+        vplan.forward(phys, again)
+        err = max_relative_error(host(again), host(modal))
+        passed &= err <= tol
+        lines.append("operator/%s/vertical-modal-round-trip=%f" % (kind, err))
+    lines.append("retained-mode-count=%d" % len(modes))
+    lines.append("retained-mode-order-hash=%s" % S.mode_order_hash(modes))
+    lines.append("validation=passed" if passed else "validation=failed")
+    for line in lines:
+        print(line, file=stream)
+    return 0 if passed else 1
+
+
 def _device_samples(torch, fn, warmups: int, samples: int):
     for _ in range(warmups):
         fn()
@@ -520,8 +640,66 @@ def run(kernel: str, profile: str = "smoke", precision_id: str = "float64", seed
                                       "real grid -> real grid", "x=1,y=Nx,z=Nx*Ny", 0)
         prov = provider_record(S, "hip", "spectral-pipeline-v1", "real-grid", timings, corr, ledger, 0, 0, fwd_dir,
                                fwd_dir, gemm_calls=2)
+    elif kernel == "production-lifetime-flux":
+        # src/benchmark.cpp:3957-4234, 4440-4580: 15 modal inputs [U, V, W,
+        # q_t x, q_t y, q_t z for four targets] -> four modal targets
+        targets = 4
+        wt = S.Workload(nx, ny, nz, targets)
+        plan = S.FluxPlan(wt, modes, w.nj, groups, fwd, inv, precision)
+        nin = plan.input_elements
+        gin = torch.Generator(device=device)
+        gin.manual_seed(seed)
+        xin = torch.complex(torch.randn(nin, generator=gin, device=device, dtype=torch.float64),
+                            torch.randn(nin, generator=gin, device=device, dtype=torch.float64))
+        dc = int(np.nonzero((modes["k"] == 0) & (modes["l"] == 0))[0][0])
+        xin.view(len(modes), 3 + 3 * targets, w.nj)[dc] = xin.view(len(modes), 3 + 3 * targets, w.nj)[dc].real.to(
+            torch.complex128)
+        d_in = xin.to(cd)
+        keep = d_in.clone()
+        out_f = torch.empty(plan.output_elements, dtype=cd, device=device)
+        out_s = torch.empty_like(out_f)
+        plan.set_fused(True)
+        t_f = _device_samples(torch, lambda: plan.execute(d_in, out_f), warmups, samples)
+        plan.set_fused(False)
+        t_s = _device_samples(torch, lambda: plan.execute(d_in, out_s), warmups, samples)
+        plan.set_fused(True)
+        o_f, o_s = host(out_f), host(out_s)
+        corr = [metric("fused physical-space pass versus separate-kernel composition", o_f, o_s, tol),
+                metric("caller modal input preserved", host(d_in), host(keep), tol)]
+        per = o_f.reshape(len(modes), targets, w.nj)
+        corr.append({"name": "DC boundary modal outputs are real", "maximumRelativeError":
+                     float(np.abs(per[dc].imag).max() / max(np.abs(per).max(), 1e-300)),
+                     "relativeL2Error": 0.0, "tolerance": tol,
+                     "passed": bool(np.abs(per[dc].imag).max() <= tol * max(np.abs(per).max(), 1e-300))})
+        if precision == S.SKB_F32:
+            p64 = S.FluxPlan(wt, modes, w.nj, groups, fwd, inv, S.SKB_F64)
+            o64 = torch.empty(plan.output_elements, dtype=torch.complex128, device=device)
+            p64.execute(xin, o64)
+            corr.append(metric("FP32 flux versus FP64 flux of the same input", o_f, host(o64), tol))
+        p64_ = per
+        for t in range(targets):
+            ref = o_s.reshape(len(modes), targets, w.nj)[:, t]
+            corr.append(metric("target %d fused versus separate kernels" % t, p64_[:, t], ref, tol))
+        in_bytes = nin * cb
+        out_bytes = plan.output_elements * cb
+        vol = wt.nx * wt.ny * wt.nz
+        timings = [timing("uninstrumented-total", "production-lifetime four-target spectral-flux composition",
+                          "forward", in_bytes + out_bytes, t_f),
+                   timing("uninstrumented-total",
+                          "production-lifetime flux, separate kernels (real volumes in memory)", "forward",
+                          in_bytes + out_bytes + (15 + 4) * vol * cb, t_s)]
+        ledger = [("raw inverse vertical MM (15 modal inputs)", "executed", "one MFMA launch"),
+                  ("horizontal reconstruction", "fused", "inverse columns, then rows fused with the products"),
+                  ("pointwise advection", "fused", "products formed in registers between the row transforms"),
+                  ("horizontal forward and retention", "executed", "forward columns to compact retained"),
+                  ("raw forward vertical MM (4 modal targets)", "executed", "one MFMA launch")]
+        fwd_dir = execution_direction("out-of-place", False, "modal-compact", "modal-compact",
+                                      "15 modal fields -> 4 modal targets", "j=1,field=Nj,mode=Nj*F", 0)
+        prov = provider_record(S, "hip", "production-lifetime-flux-v1", "modal-compact", timings, corr, ledger, 0, 0,
+                               fwd_dir, fwd_dir, gemm_calls=2)
     else:
-        raise ValueError("kernel must be fft, pruned-horizontal, vertical-gemm or spectral-pipeline")
+        raise ValueError("kernel must be fft, pruned-horizontal, vertical-gemm, spectral-pipeline or "
+                         "production-lifetime-flux")
     return assemble(S, kernel, profile, precision_id, seed, warmups, samples, w, modes, groups, family_id, [prov],
                     torch.cuda.get_device_name(0))
 
@@ -542,7 +720,11 @@ def main(argv=None) -> int:
     c.add_argument("--input", required=True)
     k = sub.add_parser("check")
     k.add_argument("path")
+    v = sub.add_parser("validate")
+    v.add_argument("--profile", default="smoke")
     a = ap.parse_args(argv)
+    if a.command == "validate":
+        return validate(a.profile)
     if a.command == "compare":
         return compare_csv(a.input)
     if a.command == "check":

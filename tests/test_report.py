@@ -67,6 +67,20 @@ def test_csv_and_compare(tmp_path):
     assert rows[1] == "hip\tprimitive\traw FFT\tforward\t0.020000\t3"
 
 
+def test_validate_host_references_match_the_oracle():
+    """The validate command's host DFT sums and gather equal the oracle's."""
+    import oracle as O
+
+    w = S.Workload(8, 8, 7, 2)
+    ow = O.Workload(8, 8, 7, 2)
+    for kind in R.VALIDATE_FIXTURES:
+        x = O.make_fixture(ow, kind, 5)
+        spec = R._host_r2c(w, x)
+        assert O.max_relative_error(spec, O.direct_r2c(ow, x)) <= 1e-13
+        assert np.array_equal(R._host_gather(w, S.retained_modes(w), spec),
+                              O.gather_retained(ow, O.retained_modes(ow), spec))
+
+
 def test_probe_columns_follow_reference_rule():
     assert R.probe_columns(1) == [0]
     assert R.probe_columns(5) == [0, 1, 2, 3, 4]
@@ -75,7 +89,8 @@ def test_probe_columns_follow_reference_rule():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kernel", ["fft", "pruned-horizontal", "vertical-gemm", "spectral-pipeline"])
+@pytest.mark.parametrize("kernel", ["fft", "pruned-horizontal", "vertical-gemm", "spectral-pipeline",
+                                    "production-lifetime-flux"])
 @pytest.mark.parametrize("precision", ["float64", "float32"])
 def test_report_runs_on_device(device, kernel, precision, tmp_path):
     print("[case] report %s %s" % (kernel, precision), flush=True)
@@ -94,3 +109,18 @@ def test_report_quick_profile_fp32_vertical(device):
     rec = R.run("vertical-gemm", "quick", "float32", warmups=1, samples=5)
     assert rec["status"] == "passed" and R.check_record(rec) == []
     assert rec["workload"]["permutationHashes"]["retainedModeOrder"] == "fnv1a64:22e98f1dcda40ad5"
+
+
+@pytest.mark.gpu
+def test_validate_passes_and_prints_the_reference_lines(device):
+    print("[case] report validate", flush=True)
+    out = io.StringIO()
+    assert R.validate("smoke", out) == 0
+    lines = out.getvalue().splitlines()
+    assert lines[-1] == "validation=passed"
+    assert "retained-mode-count=11" in lines  # Nkl of the 8 x 8 smoke grid
+    assert "retained-mode-order-hash=fnv1a64:" in " ".join(lines)
+    for kind in R.VALIDATE_FIXTURES:
+        for check in ("hip/%s/forward=" % kind, "hip/%s/inverse=" % kind, "operator/%s/retained=" % kind,
+                      "operator/%s/horizontal-vertical=" % kind, "operator/%s/vertical-modal-round-trip=" % kind):
+            assert any(line.startswith(check) for line in lines), check
