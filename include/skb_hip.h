@@ -547,6 +547,17 @@ int skb_batch_launch(skb_batch* batch, void* stream);
  * (see skb_pruned_plan_set_grid). */
 int skb_batch_autotune(skb_batch* batch, int reps, void* stream, char* report, size_t capacity);
 
+/* ---- allocation accounting ---------------------------------------------------
+ * Device allocations (count, bytes) the library has made in this process,
+ * for a steady-state check that repeated executions allocate nothing
+ * (docs/benchmark-contract.md:98, 196; tests/allocation_tracker.cpp:42-89).
+ * Allocation happens at plan creation, on the first call with a new
+ * retained-mode list (gather / embed tables), and when an entry point that
+ * documents it is first used (skb_flux_set_fused(plan, 0), the unfused
+ * pipeline's modal array, skb_vertical_plan_reserve_split). */
+uint64_t skb_device_allocations(void);
+uint64_t skb_device_allocated_bytes(void);
+
 /* ---- split <-> interleaved complex arrays ---------------------------------
  * The conversions under the reference's split-layout adapters
  * (FFTWProvider::forwardSplit / inverseSplit / gatherRetainedToSplitOuter /

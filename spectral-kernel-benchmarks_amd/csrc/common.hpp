@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <algorithm>
 #include <cstdint>
 #include <stdexcept>
@@ -50,6 +52,12 @@ int guarded(F&& body) noexcept {
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Every device allocation the library makes goes through DeviceBuffer and
+// is counted here (skb_device_allocations), so a caller can check that
+// repeated executions allocate nothing.
+inline std::atomic<uint64_t> device_allocations{0};
+inline std::atomic<uint64_t> device_allocated_bytes{0};
+
 // Device allocation owned by a plan (allocated at creation only).
 struct DeviceBuffer {
     void* ptr = nullptr;
@@ -67,6 +75,8 @@ struct DeviceBuffer {
             fail(SKB_BAD_ALLOC, "hipMalloc of " + std::to_string(n) + " bytes failed");
         }
         bytes = n;
+        device_allocations.fetch_add(1, std::memory_order_relaxed);
+        device_allocated_bytes.fetch_add(n, std::memory_order_relaxed);
     }
     void upload(const void* host, size_t n) {
         allocate(n);

@@ -1431,11 +1431,19 @@ using namespace skb;
 
 namespace {
 
-void refresh_mode_tables(skb_fft_plan* plan, const skb_retained_mode* modes, uint64_t n) {
+// The gather / embed tables follow the mode list of the call; a new list is
+// uploaded (allocating) on its first use, which must not happen while the
+// stream is being captured into a graph.
+void refresh_mode_tables(skb_fft_plan* plan, const skb_retained_mode* modes, uint64_t n, hipStream_t stream) {
     if (!modes || n == 0) fail(SKB_INVALID_ARGUMENT, "retained modes required");
     if (plan->cached_modes.size() == n &&
         std::memcmp(plan->cached_modes.data(), modes, n * sizeof(skb_retained_mode)) == 0)
         return;
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    check_hip(hipStreamIsCapturing(stream, &capture), "hipStreamIsCapturing");
+    if (capture != hipStreamCaptureStatusNone)
+        fail(SKB_LOGIC_ERROR, "a new retained-mode list needs its tables built before graph capture: "
+                              "call once with it outside the capture");
     const auto& w = plan->w;
     const uint64_t nxh = w.nx / 2 + 1;
     std::vector<ColEntry> g(n), e;
@@ -1681,7 +1689,7 @@ int skb_fft_gather_retained(skb_fft_plan* plan, const skb_retained_mode* modes, 
     return guarded([&] {
         if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
         require_ptrs(wvm, retained);
-        refresh_mode_tables(plan, modes, mode_count);
+        refresh_mode_tables(plan, modes, mode_count, as_stream(stream));
         const int planes = static_cast<int>(plan->w.nz * plan->w.fields);
         const int nxh = static_cast<int>(plan->w.nx / 2 + 1);
         const size_t total = mode_count * static_cast<size_t>(planes);
@@ -1702,7 +1710,7 @@ int skb_fft_gather_retained_modal(skb_fft_plan* plan, const skb_retained_mode* m
     return guarded([&] {
         if (!plan || nj == 0) fail(SKB_INVALID_ARGUMENT, "null plan or zero vertical modes");
         require_ptrs(full_modal, modal);
-        refresh_mode_tables(plan, modes, mode_count);
+        refresh_mode_tables(plan, modes, mode_count, as_stream(stream));
         const int block = static_cast<int>(nj * plan->w.fields);
         const int nxh = static_cast<int>(plan->w.nx / 2 + 1);
         const size_t total = mode_count * static_cast<size_t>(block);
@@ -1723,7 +1731,7 @@ int skb_fft_embed_retained(skb_fft_plan* plan, const skb_retained_mode* modes, u
     return guarded([&] {
         if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
         require_ptrs(wvm, retained);
-        refresh_mode_tables(plan, modes, mode_count);
+        refresh_mode_tables(plan, modes, mode_count, as_stream(stream));
         const auto& w = plan->w;
         const int planes = static_cast<int>(w.nz * w.fields);
         const int nxh = static_cast<int>(w.nx / 2 + 1);

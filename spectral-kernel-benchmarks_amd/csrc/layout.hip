@@ -111,6 +111,10 @@ int skb_deinterleave(int precision, uint64_t count, const void* in, void* real, 
     });
 }
 
+uint64_t skb_device_allocations(void) { return device_allocations.load(std::memory_order_relaxed); }
+
+uint64_t skb_device_allocated_bytes(void) { return device_allocated_bytes.load(std::memory_order_relaxed); }
+
 int skb_scale_split(int precision, uint64_t count, void* real, void* imaginary, double scale, void* stream) {
     return guarded([&] {
         check_precision(precision);

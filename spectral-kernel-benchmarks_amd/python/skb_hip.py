@@ -203,6 +203,8 @@ def _load():
         "skb_vertical_forward_split_noalloc": (i32, [vp, vp, vp, vp, vp, vp]),
         "skb_fft_forward_view": (i32, [vp, vp, vp, u64, vp]),
         "skb_fft_inverse_view": (i32, [vp, vp, u64, vp, vp]),
+        "skb_device_allocations": (u64, []),
+        "skb_device_allocated_bytes": (u64, []),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -242,8 +244,14 @@ EXPORTED = [
     "skb_retained_operator_real", "skb_retained_operator_device_bytes", "skb_retained_operator_set_fused",
     "skb_retained_operator_fused", "skb_interleave", "skb_deinterleave", "skb_scale_split",
     "skb_vertical_plan_reserve_split", "skb_vertical_forward_split", "skb_vertical_inverse_split",
-    "skb_vertical_forward_split_noalloc", "skb_fft_forward_view", "skb_fft_inverse_view",
+    "skb_vertical_forward_split_noalloc", "skb_fft_forward_view", "skb_fft_inverse_view", "skb_device_allocations",
+    "skb_device_allocated_bytes",
 ]
+
+
+def device_allocations() -> tuple:
+    """(count, bytes) of the device allocations the library has made."""
+    return int(lib.skb_device_allocations()), int(lib.skb_device_allocated_bytes())
 
 
 def last_error() -> str:
