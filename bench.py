@@ -44,6 +44,11 @@ def parse():
                     help="skip the secondary measurements (advection flux 512^2 x Nz129, batched uneven workload, "
                          "serving loop)")
     ap.add_argument("--serving-requests", type=int, default=1500)
+    ap.add_argument("--serving-lanes", choices=("size", "class", "executor"), default="size",
+                    help="serving streams: one lane for the large classes and one per small class, one per "
+                         "class, or one per executor")
+    ap.add_argument("--serving-reserve-cus", type=int, default=32,
+                    help="CUs kept free of large-class kernels in the serving loop (0: no cap)")
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for rocprofv3: warmup, --steps evals, then one 256 MiB device copy "
                          "(the calibration kernel for FETCH_SIZE / WRITE_SIZE)")
@@ -442,8 +447,13 @@ def bench_serving(args, S, torch, device):
     labels = V.assign_clients(V.DEFAULT_MIX, 64, 5)
     clients = [V.Client(i, classes[labels[i]], V.initial_state(classes[labels[i]], 500 + i, device))
                for i in range(64)]
-    server = V.Server(classes, budget_bytes=64 << 30, dt=1e-3, device=device, max_slots=4)
-    # capacity: mix-weighted single-request service time
+    budget = 16 << 30  # binds: the 1024^2 class gets fewer executors than max_slots
+    server = V.Server(classes, budget_bytes=budget, dt=1e-3, device=device, max_slots=4,
+                      lanes=getattr(args, "serving_lanes", "size"),
+                      reserve_cus=getattr(args, "serving_reserve_cus", 32))
+    # capacity: mix-weighted single-request service time, measured without
+    # the CU reserve (the arrival rate does not depend on it)
+    server.set_reserve(False)
     svc = []
     for c, (_, _, share) in zip(classes, V.DEFAULT_MIX):
         cl = next(x for x in clients if x.cls is c)
@@ -453,6 +463,7 @@ def bench_serving(args, S, torch, device):
             server.step_alone(cl)
         svc.append(share * (time.perf_counter() - t0) / 3)
         cl.steps = 0
+    server.set_reserve(True)
     rate = 0.7 / sum(svc)
     arrivals = V.poisson_arrivals(rate, args.serving_requests, 64, 9)
     samples = []
@@ -469,8 +480,10 @@ def bench_serving(args, S, torch, device):
         worst = max(worst, (f.to(torch.complex128) - ref).abs().max().item() / ref.abs().max().item())
     summ.update({"arrival_rate_per_s": rate, "clients": 64, "slots": server.slots(),
                  "max_in_flight": server.max_inflight, "sampled_checks": len(samples),
-                 "sampled_fp32_vs_fp64_max_rel": worst, "hbm_budget_bytes": 64 << 30,
-                 "executor_bytes": server.used})
+                 "sampled_fp32_vs_fp64_max_rel": worst, "hbm_budget_bytes": budget,
+                 "executor_bytes": server.used, "admission": server.admission()})
+    for k, v in summ.get("per_class", {}).items():
+        v["p99_over_p50"] = v["p99_ms"] / v["p50_ms"] if v["p50_ms"] > 0 else None
     return summ
 
 
@@ -621,7 +634,7 @@ def run(args, np, torch, S, device):
     mat_b = len(groups) * nj * w.nz * 4
     algo = {
         "layout": 2 * ret_b,                      # compact in, plane-major copy out (not in the yardstick)
-        "transforms": 2 * ret_b + 2 * real_b,     # retained in, real planes out and back in, retained out
+        "transforms": 2 * ret_b + real_b,         # retained in, real planes out, retained out (no read-back)
         "inverse": ret_b + real_b,
         "forward": real_b + ret_b,
         "vertical_gemm": ret_b + modal_b + mat_b,

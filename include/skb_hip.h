@@ -507,6 +507,12 @@ int skb_advection_flux_plan_create(const skb_workload* workload, const skb_retai
                                    const double* dz, int precision, skb_advection_flux_plan** plan);
 int skb_advection_flux_plan_destroy(skb_advection_flux_plan* plan);
 int skb_advection_flux_set_fused(skb_advection_flux_plan* plan, int fused);
+/* Caps every kernel of the plan at `workgroups` workgroups, which then loop
+ * over the work items (0 = one workgroup per work item, the default).  A
+ * serving loop caps its large requests so that kernels of small requests on
+ * other streams find free CUs.  Results are unchanged.  No reference
+ * counterpart (the reference runs one workload per process). */
+int skb_advection_flux_set_workgroups(skb_advection_flux_plan* plan, int workgroups);
 int skb_advection_flux_execute(skb_advection_flux_plan* plan, const void* spectral, void* flux, void* stream);
 /* Device memory the plan holds (intermediates and scratch), for admission
  * control by an HBM budget. */

@@ -829,23 +829,15 @@ struct FluxGeo {
     static constexpr int BUF = S * (M1 > ZS ? M1 : ZS);
 };
 
-// The c2r rows of u, v, w stay in registers for all targets.  FP32 only: in
-// FP64 the register set does not fit, and the separate-kernel path is faster.
+// One work item of the fused flux rows: row y of a tile of TP planes.
 template <typename R, int NX>
-__global__ __launch_bounds__((RowGeo<R, NX>::THREADS)) void flux_rows_kernel(const cx<R>* __restrict__ tin,
-                                                                          cx<R>* __restrict__ tout,
-                                                                          const cx<R>* __restrict__ tw, int nz,
-                                                                          int ps_in, int ps_out, int kc, int targets,
-                                                                          R scale, int items) {
+__device__ __forceinline__ void flux_rows_item(const cx<R>* __restrict__ tin, cx<R>* __restrict__ tout, cx<R>* buf,
+                                               const cx<R>* twl, int nz, int ps_in, int ps_out, int kc, int targets,
+                                               R scale, int item) {
     using G = RowGeo<R, NX>;
-    constexpr int N2 = G::N2, TP = 2 * G::PAIRS, NT = G::THREADS;
-    __shared__ cx<R> buf[FluxGeo<R, NX>::BUF];
-    __shared__ cx<R> twl[NX];
-    const int item = xcd_item(static_cast<int>(blockIdx.x), items);
-    if (item >= items) return;  // uniform across the workgroup
+    constexpr int N2 = G::N2, TP = 2 * G::PAIRS;
     const int ztiles = (nz + TP - 1) / TP;
     const int y = item / ztiles, z0 = (item % ztiles) * TP;
-    for (int i = threadIdx.x; i < NX; i += NT) twl[i] = tw[i];
     const size_t rin = static_cast<size_t>(ps_in) * static_cast<size_t>(kc) * y;
     const size_t rout = static_cast<size_t>(ps_out) * static_cast<size_t>(kc) * y;
     cx<R> sh[3][N2];
@@ -869,6 +861,33 @@ __global__ __launch_bounds__((RowGeo<R, NX>::THREADS)) void flux_rows_kernel(con
 #pragma unroll
         for (int k = 0; k < N2; ++k) acc[k] = cx<R>{-scale * acc[k].x, -scale * acc[k].y};
         flux_r2c_store<R, NX>(acc, tout, buf, twl, rout, ps_out, kc, tg * nz + z0, z0, nz);
+    }
+}
+
+// The c2r rows of u, v, w stay in registers for all targets.  FP32 only: in
+// FP64 the register set does not fit, and the separate-kernel path is faster.
+// LOOP: fewer workgroups than items, each taking items blockIdx.x,
+// blockIdx.x + gridDim.x, ... (a capped launch that leaves CUs to other
+// streams); otherwise one workgroup per item in the XCD-aware order.
+template <typename R, int NX, bool LOOP>
+__global__ __launch_bounds__((RowGeo<R, NX>::THREADS)) void flux_rows_kernel(const cx<R>* __restrict__ tin,
+                                                                          cx<R>* __restrict__ tout,
+                                                                          const cx<R>* __restrict__ tw, int nz,
+                                                                          int ps_in, int ps_out, int kc, int targets,
+                                                                          R scale, int items) {
+    using G = RowGeo<R, NX>;
+    constexpr int NT = G::THREADS;
+    __shared__ cx<R> buf[FluxGeo<R, NX>::BUF];
+    __shared__ cx<R> twl[NX];
+    int item = first_item<LOOP>(items);
+    if (item >= items) return;  // uniform across the workgroup
+    for (int i = threadIdx.x; i < NX; i += NT) twl[i] = tw[i];
+    for (;;) {
+        flux_rows_item<R, NX>(tin, tout, buf, twl, nz, ps_in, ps_out, kc, targets, scale, item);
+        if (!LOOP) break;
+        item += static_cast<int>(gridDim.x);
+        if (item >= items) break;
+        __syncthreads();  // buf free for the next item
     }
 }
 
@@ -1953,16 +1972,23 @@ bool flux_rows_supported(const skb_pruned_plan* pin, const skb_pruned_plan* pout
 
 namespace {
 template <typename R, int NX>
-void launch_flux_rows(skb_pruned_plan* pin, skb_pruned_plan* pout, int targets, double scale, hipStream_t s) {
+void launch_flux_rows(skb_pruned_plan* pin, skb_pruned_plan* pout, int targets, double scale, int workgroups,
+                      hipStream_t s) {
     if constexpr (NX >= 64) {
         using G = RowGeo<R, NX>;
         constexpr int TP = 2 * G::PAIRS;
         const int nz = static_cast<int>(pin->w.nz);
         const int items = static_cast<int>(pin->w.ny) * ((nz + TP - 1) / TP);
-        hipLaunchKernelGGL((flux_rows_kernel<R, NX>), dim3(xcd_grid(items)), dim3(G::THREADS), 0, s,
-                           static_cast<const cx<R>*>(pin->scratch.ptr), static_cast<cx<R>*>(pout->scratch.ptr),
-                           pin->tw.x.as<cx<R>>(), nz, pin->ps, pout->ps, pin->kc, targets, static_cast<R>(scale),
-                           items);
+        const int grid = launch_grid(items, workgroups);
+        const auto* tin = static_cast<const cx<R>*>(pin->scratch.ptr);
+        auto* tout = static_cast<cx<R>*>(pout->scratch.ptr);
+        const auto* tw = pin->tw.x.as<cx<R>>();
+        if (grid < items)
+            hipLaunchKernelGGL((flux_rows_kernel<R, NX, true>), dim3(grid), dim3(G::THREADS), 0, s, tin, tout, tw, nz,
+                               pin->ps, pout->ps, pin->kc, targets, static_cast<R>(scale), items);
+        else
+            hipLaunchKernelGGL((flux_rows_kernel<R, NX, false>), dim3(grid), dim3(G::THREADS), 0, s, tin, tout, tw,
+                               nz, pin->ps, pout->ps, pin->kc, targets, static_cast<R>(scale), items);
         check_launch("flux rows launch");
     } else {
         fail(SKB_UNSUPPORTED, "fused flux rows need Nx >= 64");
@@ -1970,11 +1996,12 @@ void launch_flux_rows(skb_pruned_plan* pin, skb_pruned_plan* pout, int targets, 
 }
 
 template <typename R>
-void flux_rows_t(skb_pruned_plan* pin, skb_pruned_plan* pout, int targets, double scale, hipStream_t s) {
+void flux_rows_t(skb_pruned_plan* pin, skb_pruned_plan* pout, int targets, double scale, int workgroups,
+                 hipStream_t s) {
     switch (pin->w.nx) {
-#define SKB_FLUX_CASE(N)                                             \
-    case N:                                                          \
-        launch_flux_rows<R, N>(pin, pout, targets, scale, s);        \
+#define SKB_FLUX_CASE(N)                                                     \
+    case N:                                                                  \
+        launch_flux_rows<R, N>(pin, pout, targets, scale, workgroups, s);    \
         return;
         SKB_POW2_CASES(SKB_FLUX_CASE)
 #undef SKB_FLUX_CASE
@@ -2014,9 +2041,10 @@ void flux_inverse_columns(skb_pruned_plan* pin, const FluxSource* sources, int c
     else flux_inverse_columns_t<float>(pin, sources, count, as_stream(stream));
 }
 
-void flux_rows(skb_pruned_plan* pin, skb_pruned_plan* pout, int targets, double scale, void* stream) {
+void flux_rows(skb_pruned_plan* pin, skb_pruned_plan* pout, int targets, double scale, void* stream,
+               int workgroups) {
     if (!flux_rows_supported(pin, pout)) fail(SKB_RUNTIME_ERROR, "fused flux rows: mismatched plans");
-    flux_rows_t<float>(pin, pout, targets, scale, as_stream(stream));
+    flux_rows_t<float>(pin, pout, targets, scale, workgroups, as_stream(stream));
 }
 
 }  // namespace skb

@@ -63,6 +63,7 @@ struct skb_advection_flux_plan {
     skb_pruned_plan* pout = nullptr;  // 1 field
     skb::DeviceBuffer qz, real_in, real_out;
     bool fused = true;
+    int workgroups = 0;  // launch cap of every kernel (0 = one workgroup per work item)
     void allocate_real() {
         if (real_in.ptr) return;
         const uint64_t volume = w.nx * w.ny * w.nz, rb = skb::real_bytes(precision);
@@ -213,6 +214,22 @@ int skb_advection_flux_set_fused(skb_advection_flux_plan* plan, int fused) {
     });
 }
 
+// Caps every kernel of the plan at `workgroups` workgroups (persistent
+// workgroups loop over the work items; 0 restores one per item).  Results
+// are unchanged: each work item is computed the same way.
+int skb_advection_flux_set_workgroups(skb_advection_flux_plan* plan, int workgroups) {
+    return guarded([&] {
+        if (!plan) fail(SKB_INVALID_ARGUMENT, "null plan");
+        if (workgroups < 0) fail(SKB_INVALID_ARGUMENT, "workgroups must be >= 0");
+        for (int stage = 0; stage < 4; ++stage) {
+            check_status(skb_pruned_plan_set_grid(plan->pin, stage, workgroups));
+            check_status(skb_pruned_plan_set_grid(plan->pout, stage, workgroups));
+        }
+        check_status(skb_vertical_plan_set_grid(plan->dz, workgroups));
+        plan->workgroups = workgroups;
+    });
+}
+
 // Device bytes owned by the plan (its sub-plans' scratch included).
 uint64_t skb_advection_flux_device_bytes(const skb_advection_flux_plan* plan) {
     if (!plan) return 0;
@@ -232,7 +249,7 @@ int skb_advection_flux_execute(skb_advection_flux_plan* plan, const void* spectr
                                        {spectral, 4, 3, kFactorDy},   {p.qz.ptr, 1, 0, kFactorNone}};
         flux_inverse_columns(p.pin, sources, 6, stream);
         if (p.fused && flux_rows_supported(p.pin, p.pout)) {
-            flux_rows(p.pin, p.pout, 1, scale, stream);
+            flux_rows(p.pin, p.pout, 1, scale, stream, p.workgroups);
             check_status(skb_pruned_forward_columns(p.pout, flux, stream));
         } else {
             const uint64_t volume = w.nx * w.ny * w.nz;

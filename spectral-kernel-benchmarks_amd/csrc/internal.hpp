@@ -31,7 +31,9 @@ int pruned_planes(const skb_pruned_plan* p);
 bool flux_rows_supported(const skb_pruned_plan* pin, const skb_pruned_plan* pout);
 // pout's intermediate <- forward rows of -scale * (u*qx + v*qy + w*qz) per
 // target, from the inverse rows of pin's intermediate.
-void flux_rows(skb_pruned_plan* pin, skb_pruned_plan* pout, int targets, double scale, void* stream);
+// workgroups > 0 caps the launch (persistent workgroups); 0 = one per item.
+void flux_rows(skb_pruned_plan* pin, skb_pruned_plan* pout, int targets, double scale, void* stream,
+               int workgroups = 0);
 
 // Vertical forward product of the plan's columns read from a wider compact
 // array: column c at in + K*(in_cs*c + in_co) (vertical.hip).

@@ -148,6 +148,7 @@ def _load():
         "skb_advection_flux_plan_create": (i32, [W, vp, u64, vp, i32, pp]),
         "skb_advection_flux_plan_destroy": (i32, [vp]),
         "skb_advection_flux_set_fused": (i32, [vp, i32]),
+        "skb_advection_flux_set_workgroups": (i32, [vp, i32]),
         "skb_advection_flux_execute": (i32, [vp, vp, vp, vp]),
         "skb_vertical_derivative_matrix": (i32, [u64, dbl, vp]),
         "skb_advection_flux_device_bytes": (u64, [vp]),
@@ -230,7 +231,8 @@ EXPORTED = [
     "skb_vertical_wvm_forward", "skb_vertical_wvm_inverse", "skb_vertical_wvm_modal_elements",
     "skb_vertical_wvm_initialize_modal", "skb_vertical_wvm_initialize_spectrum", "skb_flux_plan_create",
     "skb_flux_plan_destroy", "skb_flux_set_fused", "skb_flux_execute", "skb_advection_flux_plan_create",
-    "skb_advection_flux_plan_destroy", "skb_advection_flux_set_fused", "skb_advection_flux_execute",
+    "skb_advection_flux_plan_destroy", "skb_advection_flux_set_fused", "skb_advection_flux_set_workgroups",
+    "skb_advection_flux_execute",
     "skb_vertical_derivative_matrix", "skb_batch_create", "skb_batch_destroy", "skb_batch_add_class",
     "skb_batch_bind", "skb_batch_run", "skb_batch_capture", "skb_batch_launch", "skb_batch_autotune",
     "skb_advection_flux_device_bytes", "skb_fft_gather_retained_modal", "skb_synthetic_modal_weights",
@@ -1014,6 +1016,11 @@ class AdvectionFluxPlan:
 
     def set_fused(self, fused: bool):
         check(lib.skb_advection_flux_set_fused(self.h, 1 if fused else 0))
+
+    def set_workgroups(self, workgroups: int):
+        """Caps every kernel of the plan at `workgroups` workgroups (0: one per
+        work item); results are unchanged."""
+        check(lib.skb_advection_flux_set_workgroups(self.h, int(workgroups)))
 
     def device_bytes(self) -> int:
         return int(lib.skb_advection_flux_device_bytes(self.h))

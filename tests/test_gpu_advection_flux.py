@@ -65,3 +65,37 @@ def test_advection_flux_512_nz129_fp32(device):
     note("advection flux 512^2 x 129 fp32 fused")
     outs, ref = run_case((512, 512, 129, 4), S.SKB_F32, device, fused_modes=(True,))
     assert O.max_relative_error(outs[True], ref) <= 1e-6
+
+
+@pytest.mark.parametrize("precision,shape", [(S.SKB_F32, (256, 256, 65, 4)), (S.SKB_F64, (64, 64, 17, 4))])
+def test_advection_flux_workgroup_cap_is_bit_identical(device, precision, shape):
+    """A capped plan (persistent workgroups looping over the work items, the
+    serving loop's CU reserve) computes every item as the uncapped plan does:
+    outputs must be bit-identical, fused and separate stages, for caps below
+    and above the CU count; the uncapped result is also held to the oracle."""
+    import torch
+
+    note("advection flux workgroup cap %s p=%d" % (shape, precision))
+    w = S.Workload(*shape)
+    modes = S.retained_modes(w)
+    dz = S.vertical_derivative_matrix(w.nz, 1.0)
+    plan = S.AdvectionFluxPlan(w, modes, dz, precision)
+    spec = S.spectral_fixture(w, modes, 7, -5.0 / 6.0)
+    cd = S.complex_dtype(precision)
+    d_spec = torch.from_numpy(spec).to(device=device, dtype=cd)
+    for fused in (True, False):
+        plan.set_fused(fused)
+        plan.set_workgroups(0)
+        ref = torch.empty(len(modes) * w.nz, dtype=cd, device=device)
+        plan.execute(d_spec, ref)
+        for cap in (8, 64, 224, 4096):
+            plan.set_workgroups(cap)
+            out = torch.full_like(ref, float("nan"))
+            plan.execute(d_spec, out)
+            torch.cuda.synchronize()
+            assert torch.equal(out, ref), (fused, cap)
+    if shape[0] <= 64:
+        oref = O.advection_flux(O.Workload(*shape), O.retained_modes(O.Workload(*shape)), dz, host(d_spec))
+        assert O.max_relative_error(host(ref), oref) <= 1e-12
+    with pytest.raises(S.SkbError):
+        plan.set_workgroups(-1)

@@ -44,7 +44,7 @@ def test_serving_mix_equals_alone_and_overlaps(device):
     note("serving mix vs alone")
     classes, clients = build(device, 8, 7)
     init = [c.state.clone() for c in clients]
-    server = V.Server(classes, budget_bytes=2 << 30, dt=0.05, device=device, max_slots=3)
+    server = V.Server(classes, budget_bytes=2 << 30, dt=0.05, device=device, max_slots=3, work_cap_ms=1e9)
     assert all(n >= 1 for n in server.slots().values())
     arrivals = V.poisson_arrivals(rate=50000.0, count=40, clients=len(clients), seed=11)
     done = server.run(clients, arrivals)
@@ -81,3 +81,40 @@ def test_serving_samples_against_fp64(device):
         out = torch.empty(len(cls.modes) * cls.w.nz, dtype=torch.complex128, device=device)
         plans64[cls.key].execute(x.to(torch.complex128), out)
         assert rel(f.cpu().numpy(), out.cpu().numpy()) <= TOL
+
+
+def test_serving_admission_caps_large_work_and_budget_binds(device):
+    """Size-aware admission: large requests in flight never exceed the work
+    cap (one of the largest class at a time), small ones are not held back,
+    results still equal the steps run alone (the large classes run capped to
+    leave CUs free, which must not change a bit), and a tight HBM budget
+    leaves a class below max_slots."""
+    import torch
+
+    note("serving admission")
+    classes, clients = build(device, 8, 7)
+    init = [c.state.clone() for c in clients]
+    probe = V.Server(classes, budget_bytes=8 << 30, dt=0.05, device=device, max_slots=3, work_cap_ms=1e9)
+    per_exec = {k: v[0].plan.device_bytes() for k, v in probe.pools.items()}
+    del probe
+    tight = sum(per_exec.values()) + min(per_exec.values()) // 2  # one executor each, then nothing fits
+    server = V.Server(classes, budget_bytes=tight, dt=0.05, device=device, max_slots=3)
+    adm = server.admission()
+    assert adm["budget_limited_classes"], adm
+    # size lanes: every small-class lane sits on a hardware queue of its own
+    assert adm["lanes"] == "size" and adm["lane_check"] == "distinct", adm
+    assert all(n == 1 for n in server.slots().values())
+    done = server.run(clients, V.poisson_arrivals(rate=50000.0, count=30, clients=len(clients), seed=13))
+    assert len(done) == 30
+    largest = max(server.cost.values())
+    assert server.max_large_work <= max(server.work_cap, largest) + 1e-9
+    for i, c in enumerate(clients):
+        if not c.steps:
+            continue
+        alone = V.Client(i, c.cls, init[i].clone())
+        for _ in range(c.steps):
+            server.step_alone(alone)
+        torch.cuda.synchronize()
+        assert rel(c.state.cpu().numpy(), alone.state.cpu().numpy()) <= TOL, i
+    summary = V.latency_summary(done, clients)
+    assert set(summary["per_class"]) == {str(c.key) for c in classes if any(x.cls is c and x.steps for x in clients)}
