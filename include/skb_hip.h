@@ -280,8 +280,28 @@ int skb_retained_operator_path(const skb_retained_operator* op);
  * identical. */
 int skb_retained_operator_set_fused(skb_retained_operator* op, int fused);
 int skb_retained_operator_fused(const skb_retained_operator* op);
+/* Batched evaluation.  reserve_lanes(n) gives the plan n lanes (n in
+ * [1, 8]): streams of its own, each checked at reservation to sit on a
+ * hardware queue no other lane uses (lanes_apart), and, past the first lane,
+ * a second set of intermediates (real planes and plane-major blocks); it
+ * allocates, and n = 1 releases them.  Plane path only (elsewhere it keeps
+ * one lane).  execute_batch runs `count` independent evaluations
+ * (retained[i] -> modal[i], host arrays of device pointers) alternating over
+ * the lanes, forked from `stream` and joined back to it: every output is
+ * complete when `stream` passes the call, and each equals the same
+ * evaluation run alone bit for bit.  Outputs must not alias each other or an
+ * input.  One evaluation's layout copy and vertical product then run on the
+ * CUs another evaluation's plane workgroups leave idle.  With one lane the
+ * batch runs in order on `stream`.  No reference counterpart (the reference
+ * evaluates one operator at a time on the host). */
+int skb_retained_operator_reserve_lanes(skb_retained_operator* op, int lanes);
+/* Lanes reserved (>= 1); *apart = 1 when every lane has a queue of its own. */
+int skb_retained_operator_lanes(const skb_retained_operator* op, int* apart);
+int skb_retained_operator_execute_batch(skb_retained_operator* op, int count, const void* const* retained,
+                                        void* const* modal, void* stream);
 /* Device pointer to the real planes (x + Nx(y + Ny*plane)) of the last
- * evaluation, and the plan's device allocation in bytes. */
+ * evaluation (after a batch: the last one run on the first lane), and the
+ * plan's device allocation in bytes. */
 void* skb_retained_operator_real(const skb_retained_operator* op);
 uint64_t skb_retained_operator_device_bytes(const skb_retained_operator* op);
 

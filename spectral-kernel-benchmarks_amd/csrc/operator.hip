@@ -12,11 +12,40 @@
 // through a copy back to the compact layout.
 // Two-pass path: the pruned plan's row and column kernels and the compact
 // vertical product.
+// Batched execute: evaluations of a batch alternate over the plan's lanes
+// (own streams and intermediates), forked from and joined back to the
+// caller's stream, so one evaluation's layout and vertical product run on
+// the CUs another evaluation's 195 plane workgroups leave idle.
+#include <algorithm>
 #include <memory>
+#include <vector>
 
 #include "common.hpp"
 #include "internal.hpp"
 #include "plane.hpp"
+
+namespace skb {
+// One lane of a batched execute: a stream of the plan's own and the
+// intermediates one evaluation in flight needs (lane 0 uses the plan's).
+struct OperatorLane {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    DeviceBuffer pm, real, compact;
+    ~OperatorLane() {
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+        if (done) (void)hipEventDestroy(done);
+    }
+};
+// The intermediates one evaluation writes and reads.
+struct OperatorBuffers {
+    void* pm;
+    void* real;
+    void* compact;
+};
+}  // namespace skb
 
 struct skb_retained_operator {
     skb_workload w{};
@@ -30,7 +59,19 @@ struct skb_retained_operator {
     skb::DeviceBuffer real;     // Nx * Ny * P real values
     skb::DeviceBuffer compact;  // the forward's compact output on the two-pass path (and as the vertical input
                                 // when the vertical plan does not read the plane-major layout)
+    // batched execute (skb_retained_operator_reserve_lanes)
+    std::vector<std::unique_ptr<skb::OperatorLane>> lanes;
+    hipEvent_t fork = nullptr;
+    bool lanes_apart = true;  // every lane on a hardware queue of its own
+    skb::DeviceBuffer probe;  // target of the queue check's tiny kernel
+    void drop_lanes() {
+        lanes.clear();
+        if (fork) (void)hipEventDestroy(fork);
+        fork = nullptr;
+        lanes_apart = true;
+    }
     ~skb_retained_operator() {
+        drop_lanes();
         if (vertical) skb_vertical_plan_destroy(vertical);
         if (pruned) skb_pruned_plan_destroy(pruned);
     }
@@ -44,10 +85,15 @@ bool plane_path(const skb_retained_operator* op) {
     return op->path != SKB_PATH_TWO_PASS && pruned_plane_tables(op->pruned) != nullptr;
 }
 
-void run_stage(skb_retained_operator* op, int stage, const void* retained, void* modal, hipStream_t s) {
+OperatorBuffers own_buffers(skb_retained_operator* op) {
+    return {pruned_plane_scratch(op->pruned), op->real.ptr, op->compact.ptr};
+}
+
+void run_stage(skb_retained_operator* op, int stage, const void* retained, void* modal, hipStream_t s,
+               const OperatorBuffers& b) {
     const bool plane = plane_path(op);
     const PlaneTables* t = pruned_plane_tables(op->pruned);
-    void* pm = pruned_plane_scratch(op->pruned);
+    void* pm = b.pm;
     switch (stage) {
         case SKB_OP_STAGE_LAYOUT:
             if (!retained) fail(SKB_INVALID_ARGUMENT, "null retained input");
@@ -55,39 +101,74 @@ void run_stage(skb_retained_operator* op, int stage, const void* retained, void*
             return;
         case SKB_OP_STAGE_INVERSE:
             if (plane) {
-                plane_inverse(*t, pm, op->real.ptr, op->planes, s);
+                plane_inverse(*t, pm, b.real, op->planes, s);
             } else {
                 if (!retained) fail(SKB_INVALID_ARGUMENT, "null retained input");
-                check_status(skb_pruned_inverse(op->pruned, retained, op->real.ptr, s));
+                check_status(skb_pruned_inverse(op->pruned, retained, b.real, s));
             }
             return;
         case SKB_OP_STAGE_FORWARD:
             if (plane) {
-                plane_forward(*t, op->real.ptr, pm, op->planes, s);
+                plane_forward(*t, b.real, pm, op->planes, s);
                 if (!vertical_pm_supported(op->vertical))
-                    pm_transpose(false, pm, op->compact.ptr, t->modes, op->planes, t->np, s);
+                    pm_transpose(false, pm, b.compact, t->modes, op->planes, t->np, s);
             } else {
-                check_status(skb_pruned_forward(op->pruned, op->real.ptr, op->compact.ptr, s));
+                check_status(skb_pruned_forward(op->pruned, b.real, b.compact, s));
             }
             return;
         case SKB_OP_STAGE_TRANSFORMS:
             // both transforms as execute runs them
             if (plane && op->fused) {
-                plane_operator(*t, pm, pm, op->real.ptr, op->planes, s);
+                plane_operator(*t, pm, pm, b.real, op->planes, s);
                 if (!vertical_pm_supported(op->vertical))
-                    pm_transpose(false, pm, op->compact.ptr, t->modes, op->planes, t->np, s);
+                    pm_transpose(false, pm, b.compact, t->modes, op->planes, t->np, s);
             } else {
-                run_stage(op, SKB_OP_STAGE_INVERSE, retained, modal, s);
-                run_stage(op, SKB_OP_STAGE_FORWARD, retained, modal, s);
+                run_stage(op, SKB_OP_STAGE_INVERSE, retained, modal, s, b);
+                run_stage(op, SKB_OP_STAGE_FORWARD, retained, modal, s, b);
             }
             return;
         case SKB_OP_STAGE_VERTICAL:
             if (!modal) fail(SKB_INVALID_ARGUMENT, "null modal output");
             if (plane && vertical_pm_supported(op->vertical)) vertical_forward_pm(op->vertical, pm, t->np, modal, s);
-            else check_status(skb_vertical_forward(op->vertical, op->compact.ptr, modal, s));
+            else check_status(skb_vertical_forward(op->vertical, b.compact, modal, s));
             return;
     }
     fail(SKB_INVALID_ARGUMENT, "bad operator stage");
+}
+
+void run_eval(skb_retained_operator* op, const void* retained, void* modal, hipStream_t s, const OperatorBuffers& b) {
+    run_stage(op, SKB_OP_STAGE_LAYOUT, retained, modal, s, b);
+    run_stage(op, SKB_OP_STAGE_TRANSFORMS, retained, modal, s, b);
+    run_stage(op, SKB_OP_STAGE_VERTICAL, retained, modal, s, b);
+}
+
+// Queue check of the lanes: a bounded spin (`ticks` of the device's
+// constant-rate wall clock) and a 64-word store.
+__global__ void lane_spin_kernel(uint64_t ticks) {
+    const uint64_t t0 = wall_clock64();
+    while (wall_clock64() - t0 < ticks) __builtin_amdgcn_s_sleep(8);
+}
+__global__ void lane_touch_kernel(int* p) { p[threadIdx.x] = 1; }  // 64 ints
+
+// True when work on stream b waits for work on stream a: the two streams
+// share one hardware queue (in-order), so b's store completes only after
+// a's spin has ended.
+bool shares_queue(hipStream_t a, hipStream_t b, int* probe, uint64_t ticks) {
+    hipEvent_t ea = nullptr, eb = nullptr;
+    check_hip(hipEventCreateWithFlags(&ea, hipEventDisableTiming), "hipEventCreate");
+    check_hip(hipEventCreateWithFlags(&eb, hipEventDisableTiming), "hipEventCreate");
+    hipLaunchKernelGGL(lane_spin_kernel, dim3(1), dim3(64), 0, a, ticks);
+    check_hip(hipGetLastError(), "lane spin launch");
+    check_hip(hipEventRecord(ea, a), "hipEventRecord");
+    hipLaunchKernelGGL(lane_touch_kernel, dim3(1), dim3(64), 0, b, probe);
+    check_hip(hipGetLastError(), "lane touch launch");
+    check_hip(hipEventRecord(eb, b), "hipEventRecord");
+    check_hip(hipEventSynchronize(eb), "hipEventSynchronize");
+    const bool shared = hipEventQuery(ea) == hipSuccess;
+    check_hip(hipStreamSynchronize(a), "hipStreamSynchronize");
+    (void)hipEventDestroy(ea);
+    (void)hipEventDestroy(eb);
+    return shared;
 }
 
 }  // namespace
@@ -126,10 +207,93 @@ int skb_retained_operator_destroy(skb_retained_operator* op) {
 int skb_retained_operator_execute(skb_retained_operator* op, const void* retained, void* modal, void* stream) {
     return guarded([&] {
         if (!op || !retained || !modal) fail(SKB_INVALID_ARGUMENT, "null argument");
+        run_eval(op, retained, modal, as_stream(stream), own_buffers(op));
+    });
+}
+
+int skb_retained_operator_reserve_lanes(skb_retained_operator* op, int lanes) {
+    return guarded([&] {
+        if (!op) fail(SKB_INVALID_ARGUMENT, "null plan");
+        if (lanes < 1 || lanes > 8) fail(SKB_INVALID_ARGUMENT, "lanes must lie in [1, 8]");
+        op->drop_lanes();
+        if (lanes == 1 || !plane_path(op)) return;
+        const PlaneTables* t = pruned_plane_tables(op->pruned);
+        const uint64_t pm_bytes = static_cast<uint64_t>(t->np) * op->planes * complex_bytes(op->precision);
+        if (!op->probe.ptr) op->probe.allocate(256);
+        int device = 0, rate_khz = 0;
+        check_hip(hipGetDevice(&device), "hipGetDevice");
+        check_hip(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, device), "wall clock rate");
+        const uint64_t ticks = static_cast<uint64_t>(rate_khz > 0 ? rate_khz : 100000) * 2;  // 2 ms
+        check_hip(hipEventCreateWithFlags(&op->fork, hipEventDisableTiming), "hipEventCreate");
+        // Streams are drawn until one sits on a hardware queue no other lane
+        // uses (at most 8 draws per lane; the last one is kept otherwise).
+        // Rejected streams stay alive until the lanes are chosen: the runtime
+        // gives a new stream its least-used queue, which a destroyed stream
+        // would hand straight back.
+        std::vector<hipStream_t> rejected;
+        struct Release {
+            std::vector<hipStream_t>& v;
+            ~Release() {
+                for (hipStream_t st : v) (void)hipStreamDestroy(st);
+            }
+        } release{rejected};
+        for (int k = 0; k < lanes; ++k) {
+            auto lane = std::make_unique<OperatorLane>();
+            for (int attempt = 0; attempt < 8; ++attempt) {
+                hipStream_t st = nullptr;
+                check_hip(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+                bool shared = false;
+                for (const auto& o : op->lanes)
+                    if (shares_queue(o->stream, st, static_cast<int*>(op->probe.ptr), ticks)) shared = true;
+                if (!shared || attempt == 7) {
+                    if (shared) op->lanes_apart = false;
+                    lane->stream = st;
+                    break;
+                }
+                rejected.push_back(st);
+            }
+            check_hip(hipEventCreateWithFlags(&lane->done, hipEventDisableTiming), "hipEventCreate");
+            if (k > 0) {
+                lane->pm.allocate(pm_bytes);
+                lane->real.allocate(op->real.bytes);
+                lane->compact.allocate(op->compact.bytes);
+            }
+            op->lanes.push_back(std::move(lane));
+        }
+    });
+}
+
+int skb_retained_operator_lanes(const skb_retained_operator* op, int* apart) {
+    if (apart) *apart = op && op->lanes_apart ? 1 : 0;
+    return op ? std::max<int>(1, static_cast<int>(op->lanes.size())) : 0;
+}
+
+int skb_retained_operator_execute_batch(skb_retained_operator* op, int count, const void* const* retained,
+                                        void* const* modal, void* stream) {
+    return guarded([&] {
+        if (!op || count < 0 || (count > 0 && (!retained || !modal))) fail(SKB_INVALID_ARGUMENT, "null argument");
+        for (int i = 0; i < count; ++i)
+            if (!retained[i] || !modal[i]) fail(SKB_INVALID_ARGUMENT, "null retained input or modal output");
         const hipStream_t s = as_stream(stream);
-        run_stage(op, SKB_OP_STAGE_LAYOUT, retained, modal, s);
-        run_stage(op, SKB_OP_STAGE_TRANSFORMS, retained, modal, s);
-        run_stage(op, SKB_OP_STAGE_VERTICAL, retained, modal, s);
+        const int nl = static_cast<int>(op->lanes.size());
+        if (nl <= 1 || count <= 1 || !plane_path(op)) {
+            for (int i = 0; i < count; ++i) run_eval(op, retained[i], modal[i], s, own_buffers(op));
+            return;
+        }
+        const int used = std::min(nl, count);
+        check_hip(hipEventRecord(op->fork, s), "hipEventRecord");
+        for (int k = 0; k < used; ++k) check_hip(hipStreamWaitEvent(op->lanes[k]->stream, op->fork, 0), "wait");
+        for (int i = 0; i < count; ++i) {
+            const int k = i % used;
+            const auto& lane = *op->lanes[k];
+            const OperatorBuffers b = k == 0 ? own_buffers(op)
+                                             : OperatorBuffers{lane.pm.ptr, lane.real.ptr, lane.compact.ptr};
+            run_eval(op, retained[i], modal[i], lane.stream, b);
+        }
+        for (int k = 0; k < used; ++k) {
+            check_hip(hipEventRecord(op->lanes[k]->done, op->lanes[k]->stream), "hipEventRecord");
+            check_hip(hipStreamWaitEvent(s, op->lanes[k]->done, 0), "wait");
+        }
     });
 }
 
@@ -146,7 +310,7 @@ int skb_retained_operator_stage(skb_retained_operator* op, int stage, const void
                                 void* stream) {
     return guarded([&] {
         if (!op) fail(SKB_INVALID_ARGUMENT, "null plan");
-        run_stage(op, stage, retained, modal, as_stream(stream));
+        run_stage(op, stage, retained, modal, as_stream(stream), own_buffers(op));
     });
 }
 
@@ -155,6 +319,7 @@ int skb_retained_operator_set_path(skb_retained_operator* op, int path) {
         if (!op) fail(SKB_INVALID_ARGUMENT, "null plan");
         check_status(skb_pruned_plan_set_path(op->pruned, path));
         op->path = path;
+        if (!plane_path(op)) op->drop_lanes();  // lanes hold plane-path intermediates
     });
 }
 
@@ -166,9 +331,11 @@ void* skb_retained_operator_real(const skb_retained_operator* op) { return op ? 
 
 uint64_t skb_retained_operator_device_bytes(const skb_retained_operator* op) {
     if (!op) return 0;
+    uint64_t lanes = 0;
+    for (const auto& l : op->lanes) lanes += l->pm.bytes + l->real.bytes + l->compact.bytes;
     return op->real.bytes + op->compact.bytes + skb_pruned_scratch_bytes(op->pruned) +
            (pruned_plane_scratch(op->pruned) ? plane_np(op->modes) * op->planes * 8 : 0) +
-           2 * skb_vertical_matrix_bytes(op->vertical);
+           2 * skb_vertical_matrix_bytes(op->vertical) + lanes;
 }
 
 }  // extern "C"

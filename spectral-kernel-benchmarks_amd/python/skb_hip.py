@@ -189,6 +189,9 @@ def _load():
         "skb_retained_operator_destroy": (i32, [vp]),
         "skb_retained_operator_execute": (i32, [vp, vp, vp, vp]),
         "skb_retained_operator_stage": (i32, [vp, i32, vp, vp, vp]),
+        "skb_retained_operator_reserve_lanes": (i32, [vp, i32]),
+        "skb_retained_operator_lanes": (i32, [vp, ctypes.POINTER(i32)]),
+        "skb_retained_operator_execute_batch": (i32, [vp, i32, vp, vp, vp]),
         "skb_retained_operator_set_path": (i32, [vp, i32]),
         "skb_retained_operator_path": (i32, [vp]),
         "skb_retained_operator_real": (vp, [vp]),
@@ -243,6 +246,7 @@ EXPORTED = [
     "skb_type1_vertical_schedule", "skb_type1_fill", "skb_pruned_plan_set_path", "skb_pruned_plan_path",
     "skb_retained_operator_create", "skb_retained_operator_destroy", "skb_retained_operator_execute",
     "skb_retained_operator_stage", "skb_retained_operator_set_path", "skb_retained_operator_path",
+    "skb_retained_operator_reserve_lanes", "skb_retained_operator_lanes", "skb_retained_operator_execute_batch",
     "skb_retained_operator_real", "skb_retained_operator_device_bytes", "skb_retained_operator_set_fused",
     "skb_retained_operator_fused", "skb_interleave", "skb_deinterleave", "skb_scale_split",
     "skb_vertical_plan_reserve_split", "skb_vertical_forward_split", "skb_vertical_inverse_split",
@@ -881,6 +885,38 @@ class OperatorPlan:
 
     def fused(self) -> bool:
         return bool(lib.skb_retained_operator_fused(self.h))
+
+    def reserve_lanes(self, lanes: int):
+        """Lanes of the batched execute (own streams and intermediates; 1 releases them)."""
+        check(lib.skb_retained_operator_reserve_lanes(self.h, int(lanes)))
+
+    def lanes(self):
+        """(lanes reserved, every lane on a hardware queue of its own)"""
+        apart = ctypes.c_int32(0)
+        n = int(lib.skb_retained_operator_lanes(self.h, ctypes.byref(apart)))
+        return n, bool(apart.value)
+
+    def _batch_args(self, retained, modal):
+        if len(retained) != len(modal):
+            raise ValueError("one modal output per retained input")
+        pairs = [self._args(r, m) for r, m in zip(retained, modal)]
+        n = len(pairs)
+        ra = (ctypes.c_void_p * max(1, n))(*[p[0] for p in pairs])
+        ma = (ctypes.c_void_p * max(1, n))(*[p[1] for p in pairs])
+        outs = {m.data_ptr() for m in modal}
+        if len(outs) != n or outs & {r.data_ptr() for r in retained}:
+            raise ValueError("batch outputs must not alias each other or an input")
+        return n, ra, ma
+
+    def execute_batch(self, retained, modal, stream=None):
+        """Independent evaluations retained[i] -> modal[i], spread over the lanes."""
+        n, ra, ma = self._batch_args(retained, modal)
+        check(lib.skb_retained_operator_execute_batch(self.h, n, ra, ma, _stream(stream)))
+
+    def bind_execute_batch(self, retained, modal, stream=None) -> Bound:
+        n, ra, ma = self._batch_args(retained, modal)
+        return Bound(lib.skb_retained_operator_execute_batch, (self.h, n, ra, ma, _stream(stream)),
+                     (list(retained), list(modal), ra, ma, self))
 
     def real(self):
         """The operator's real planes (x + Nx(y + Ny*plane)) as a torch tensor view."""
