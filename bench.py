@@ -1,12 +1,19 @@
 #!/usr/bin/env python3
 """Retained-operator throughput of the hip provider on one MI355X.
 
-One step = one retained-operator evaluation of the 256^2 x Nz65 x 3 workload
-with inputs resident in HBM, enqueued by ONE C-ABI call
-(skb_retained_operator_execute):
+One step = one batch of --batch (32) independent retained-operator
+evaluations of the 256^2 x Nz65 x 3 workload, distinct inputs and outputs
+resident in HBM, enqueued by ONE C-ABI call
+(skb_retained_operator_execute_batch), each evaluation:
   1. pruned inverse c2r   compact retained (z, field, mode) -> real planes
   2. pruned forward r2c   real planes -> compact retained (2/3-rule truncated)
   3. K^2-grouped vertical GEMM on MFMA -> modal
+The evaluations alternate over the plan's lanes (streams on separate
+hardware queues), so one evaluation's layout copy and vertical product use
+the CUs another's 195 plane workgroups leave idle.  `value` = evaluations
+per second over the timed steps; the single-call rate (one
+skb_retained_operator_execute per evaluation, in order) and per-evaluation
+latencies are reported beside it.
 The headline runs in FP32 storage (FP32 transforms, split-BF16 MFMA vertical
 products); the plain FP64 GPU form is timed beside it and supplies the
 reference for the FP32 error.  The CPU column times the FP64 oracle
@@ -33,8 +40,12 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_retained_operator_256_nz6
 def parse():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400)
-    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--steps", type=int, default=40,
+                    help="timed steps; one step is one batch of --batch evaluations")
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--batch", type=int, default=32,
+                    help="independent evaluations per step (one execute_batch call)")
+    ap.add_argument("--lanes", type=int, default=3, help="lanes of the batched execute (1: in order)")
     ap.add_argument("--family", choices=["qr32", "k2"], default="qr32",
                     help="qr32: 32 K^2 bins of 65x65 QR-Gaussian matrices; k2: reference 2045x(42x65) family")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -50,8 +61,9 @@ def parse():
     ap.add_argument("--serving-reserve-cus", type=int, default=32,
                     help="CUs kept free of large-class kernels in the serving loop (0: no cap)")
     ap.add_argument("--profile-run", action="store_true",
-                    help="short run for rocprofv3: warmup, --steps evals, then one 256 MiB device copy "
-                         "(the calibration kernel for FETCH_SIZE / WRITE_SIZE)")
+                    help="short run for rocprofv3: warmup, then every stage in blocks of 50 launches rotating "
+                         "over 8 plans (max(200, --steps) launches each, as the roofline timing), then one "
+                         "256 MiB device copy (the calibration kernel for FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="turn two rocprofv3 --pmc counter_collection.csv files (FETCH_SIZE run, WRITE_SIZE run) "
                          "into " + os.path.relpath(TRAFFIC_FILE, ROOT) + " and exit")
@@ -151,6 +163,7 @@ def timed(torch, fn, steps):
 
 
 LATENCY_EVALS = 1000
+SINGLE_EVALS = 400
 
 
 def per_eval_latency(torch, op, evals):
@@ -542,10 +555,13 @@ def run(args, np, torch, S, device):
     log(args, "fp32 vs fp64 max-scale err %.3e, rel L2 %.3e" % (err_max, err_l2))
 
     if args.profile_run:
+        # the launches the roofline is timed on: every stage in blocks of 50
+        # launches rotating over 8 plans (HBM-resident working set)
         for _ in range(args.warmup):
             f32.step()
-        for _ in range(args.steps):
-            f32.step()
+        rot = [f32] + [Operator(S, torch, w, modes, groups, nj, fwd, inv, x64, S.SKB_F32, device) for _ in range(7)]
+        per_stage_times(torch, rot, max(200, args.steps))
+        del rot
         src = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device=device).fill_(1.0)
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
@@ -554,13 +570,14 @@ def run(args, np, torch, S, device):
         log(args, "profile run done")
         return 0
 
-    log(args, "warmup %d" % args.warmup)
-    for _ in range(args.warmup):
+    # single-call rate: one evaluation per C-ABI call, in order on one stream
+    n_single = max(SINGLE_EVALS, args.steps)
+    log(args, "single-call evaluations: warmup %d, timed %d" % (args.warmup * 8, n_single))
+    for _ in range(args.warmup * 8):
         f32.step()
-    log(args, "timed %d steps (fp32)" % args.steps)
-    wall = timed(torch, f32.step, args.steps)
-    value = args.steps / wall
-    log(args, "fp32 %.1f evals/s (%.2f us/eval)" % (value, 1e6 * wall / args.steps))
+    wall_single = timed(torch, f32.step, n_single)
+    single = n_single / wall_single
+    log(args, "fp32 single call %.1f evals/s (%.2f us/eval)" % (single, 1e6 * wall_single / n_single))
     # host cost of enqueueing one evaluation (the C-ABI call), measured while
     # the device works through the queued evaluations
     torch.cuda.synchronize()
@@ -570,6 +587,37 @@ def run(args, np, torch, S, device):
         f32.step()
     host_us = 1e6 * (time.perf_counter() - t0) / n_host
     torch.cuda.synchronize()
+
+    # headline: one step = one batch of independent evaluations (distinct
+    # HBM-resident inputs and outputs) spread over the plan's lanes
+    B = args.batch
+    f32.op.reserve_lanes(args.lanes)
+    lanes, lanes_apart = f32.op.lanes()
+    xs = [f32.x * (1.0 + 1e-3 * k) for k in range(B)]
+    outs = [torch.empty_like(f32.modal) for _ in range(B)]
+    batch_call = f32.op.bind_execute_batch(xs, outs)
+    batch_call()
+    for k in sorted({0, B // 2, B - 1}):  # every evaluation equals the same one run alone
+        f32.op.execute(xs[k], f32.modal)
+        torch.cuda.synchronize()
+        if not torch.equal(outs[k], f32.modal):
+            raise RuntimeError("batched evaluation %d differs from the same evaluation run alone" % k)
+    log(args, "batched: %d evaluations per step over %d lanes (apart %s); warmup %d, timed %d steps"
+        % (B, lanes, lanes_apart, args.warmup, args.steps))
+    for _ in range(args.warmup):
+        batch_call()
+    wall = timed(torch, batch_call, args.steps)
+    value = B * args.steps / wall
+    log(args, "fp32 batched %.1f evals/s (%.2f us/eval)" % (value, 1e6 * wall / (B * args.steps)))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        batch_call()
+    host_batch_us = 1e6 * (time.perf_counter() - t0) / (10 * B)
+    torch.cuda.synchronize()
+    del xs, outs, batch_call
+    f32.op.reserve_lanes(1)
+    torch.cuda.empty_cache()
     lat_evals = max(LATENCY_EVALS, args.steps)
     log(args, "latency over %d evaluations" % lat_evals)
     per_eval = per_eval_latency(torch, f32, lat_evals)
@@ -588,37 +636,10 @@ def run(args, np, torch, S, device):
 
     for _ in range(16):
         rot_step()
-    wall_rot = timed(torch, rot_step, args.steps)
+    wall_rot = timed(torch, rot_step, n_single)
     del rot_calls
     rot = None
     torch.cuda.empty_cache()
-    two = None
-    if not args.no_extras:
-        # Two evaluations in flight: a second plan (own buffers) on a second
-        # stream, calls alternating between the two.  Reported beside the
-        # headline, which keeps one evaluation at a time.
-        log(args, "extra: two streams")
-        side = torch.cuda.Stream(device=device)
-        with torch.cuda.stream(side):
-            f32b = Operator(S, torch, w, modes, groups, nj, fwd, inv, x64, S.SKB_F32, device)
-        pair_steps = 2 * (args.steps // 2)
-        for _ in range(4):
-            f32.step()
-            f32b.step()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(pair_steps // 2):
-            f32.step()
-            f32b.step()
-        torch.cuda.synchronize()
-        wall_pair = time.perf_counter() - t0
-        log(args, "fp32 two streams %.1f evals/s" % (pair_steps / wall_pair))
-        if not torch.equal(f32b.modal, f32.modal):
-            raise RuntimeError("the second stream's evaluation differs from the first")
-        del f32b
-        two = {"evals_per_s": pair_steps / wall_pair, "ms_per_eval": 1e3 * wall_pair / pair_steps,
-               "note": "two evaluations in flight on two streams (own plans and buffers), same inputs; "
-                       "outputs checked equal"}
     log(args, "fp64 form")
     for _ in range(max(2, args.warmup // 4)):
         f64.step()
@@ -675,8 +696,10 @@ def run(args, np, torch, S, device):
         "from": "mean of per-evaluation HIP event pairs",
         "wall_achieved": eval_bytes * value / 1e9,
         "wall_frac": eval_bytes * value / 1e9 / HBM_PEAK_GBPS,
-        "rotated_evals_per_s": args.steps / wall_rot,
-        "rotated_wall_frac": eval_bytes * args.steps / wall_rot / 1e9 / HBM_PEAK_GBPS,
+        "wall_from": "the headline rate (batched evaluations)",
+        "single_call_wall_frac": eval_bytes * single / 1e9 / HBM_PEAK_GBPS,
+        "rotated_evals_per_s": n_single / wall_rot,
+        "rotated_wall_frac": eval_bytes * n_single / wall_rot / 1e9 / HBM_PEAK_GBPS,
         "fused_minimum_bytes": ret_b + modal_b + mat_b,
     }
 
@@ -699,6 +722,7 @@ def run(args, np, torch, S, device):
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": 1e3 * wall / args.steps,
+        "evals_per_step": B,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -709,7 +733,10 @@ def run(args, np, torch, S, device):
                         "K^2-grouped vertical GEMM",
             "nx": 256, "ny": 256, "nz": 65, "fields": 3, "retained_modes": nkl, "vertical_family": args.family,
             "vertical_groups": len(groups), "nj": nj,
-            "launch": "one prepared C-ABI call per evaluation (skb_retained_operator_execute) on one stream",
+            "launch": "one prepared C-ABI call per step: skb_retained_operator_execute_batch of %d independent "
+                      "evaluations (distinct inputs and outputs) over %d lanes on separate hardware queues, "
+                      "forked from and joined to one stream" % (B, lanes),
+            "evals_per_step": B, "lanes": lanes, "lanes_apart": lanes_apart,
             "path": "plane-resident transforms" if f32.plane else "two-pass transforms",
         },
         "roofline": roofline,
@@ -718,11 +745,13 @@ def run(args, np, torch, S, device):
         "latency_ms": {"p50": percentile(per_eval, 0.5), "p99": percentile(per_eval, 0.99),
                        "mean": eval_avg_ms, "evaluations": len(per_eval)},
         "host_enqueue_us_per_eval": host_us,
+        "host_enqueue_us_per_eval_batched": host_batch_us,
+        "single_call": {"evals_per_s": single, "ms_per_eval": 1e3 * wall_single / n_single, "evaluations": n_single,
+                        "note": "one skb_retained_operator_execute per evaluation, in order on one stream"},
         "kernels": kernels,
         "error_vs_fp64_gpu": {"max_scale_normalized": err_max, "relative_l2": err_l2, "tolerance": 1e-6,
                               "passed": err_max <= 1e-6 and err_l2 <= 1e-6},
         "error_vs_fp64_oracle": err_oracle,
-        "two_streams": two,
         "fp64": {"value": f64_steps / wall64, "unit": "evals/s", "ms_per_step": 1e3 * wall64 / f64_steps,
                  "p50_ms": percentile(per_eval64, 0.5), "p99_ms": percentile(per_eval64, 0.99),
                  "bytes_per_eval": 2 * eval_bytes,
