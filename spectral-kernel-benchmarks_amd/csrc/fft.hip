@@ -1819,7 +1819,7 @@ int skb_pruned_forward(skb_pruned_plan* plan, const void* real, void* retained, 
         const hipStream_t s = as_stream(stream);
         if (uses_plane(plan)) {
             plane_forward(plan->plane, real, plan->pm.ptr, plan->planes, s);
-            pm_transpose(false, plan->pm.ptr, retained, plan->plane.modes, plan->planes, plan->plane.np, s);
+            pm_transpose(plan->plane, false, plan->pm.ptr, retained, plan->planes, s);
             return;
         }
         SKB_BY_PRECISION(plan, stage_forward_rows, plan, real, s);
@@ -1833,7 +1833,7 @@ int skb_pruned_inverse(skb_pruned_plan* plan, const void* retained, void* real, 
         require_ptrs(real, retained);
         const hipStream_t s = as_stream(stream);
         if (uses_plane(plan)) {
-            pm_transpose(true, retained, plan->pm.ptr, plan->plane.modes, plan->planes, plan->plane.np, s);
+            pm_transpose(plan->plane, true, retained, plan->pm.ptr, plan->planes, s);
             plane_inverse(plan->plane, plan->pm.ptr, real, plan->planes, s);
             return;
         }

@@ -42,10 +42,12 @@ void vertical_forward_columns(const skb_vertical_plan* p, const void* in, int in
 
 // Forward product read from the chunked plane-major retained layout (element
 // (mode, z, field) at pm[16*P*(mode/16) + 16*(z + Nz*field) + mode%16], np
-// modes, a multiple of 16) into the plan's modal layout; square FP32
-// split-BF16 plans only (vertical.hip).
+// modes, a multiple of 16; slot values, conjugated for the modes flagged in
+// cmask, bit m % 16 of word m / 16) into the plan's modal layout (the
+// caller's values); square FP32 split-BF16 plans only (vertical.hip).
 bool vertical_pm_supported(const skb_vertical_plan* p);
-void vertical_forward_pm(const skb_vertical_plan* p, const void* pm, int np, void* out, void* stream);
+void vertical_forward_pm(const skb_vertical_plan* p, const void* pm, int np, const uint32_t* cmask, void* out,
+                         void* stream);
 
 // Modal round trip in one kernel: out[:, c] = I_g (w[:, c] .* (F_g in[:, c]))
 // over a compact plan's columns (weights in the plan's precision, modal

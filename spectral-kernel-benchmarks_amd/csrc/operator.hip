@@ -97,7 +97,7 @@ void run_stage(skb_retained_operator* op, int stage, const void* retained, void*
     switch (stage) {
         case SKB_OP_STAGE_LAYOUT:
             if (!retained) fail(SKB_INVALID_ARGUMENT, "null retained input");
-            if (plane) pm_transpose(true, retained, pm, t->modes, op->planes, t->np, s);
+            if (plane) pm_transpose(*t, true, retained, pm, op->planes, s);
             return;
         case SKB_OP_STAGE_INVERSE:
             if (plane) {
@@ -111,7 +111,7 @@ void run_stage(skb_retained_operator* op, int stage, const void* retained, void*
             if (plane) {
                 plane_forward(*t, b.real, pm, op->planes, s);
                 if (!vertical_pm_supported(op->vertical))
-                    pm_transpose(false, pm, b.compact, t->modes, op->planes, t->np, s);
+                    pm_transpose(*t, false, pm, b.compact, op->planes, s);
             } else {
                 check_status(skb_pruned_forward(op->pruned, b.real, b.compact, s));
             }
@@ -121,7 +121,7 @@ void run_stage(skb_retained_operator* op, int stage, const void* retained, void*
             if (plane && op->fused) {
                 plane_operator(*t, pm, pm, b.real, op->planes, s);
                 if (!vertical_pm_supported(op->vertical))
-                    pm_transpose(false, pm, b.compact, t->modes, op->planes, t->np, s);
+                    pm_transpose(*t, false, pm, b.compact, op->planes, s);
             } else {
                 run_stage(op, SKB_OP_STAGE_INVERSE, retained, modal, s, b);
                 run_stage(op, SKB_OP_STAGE_FORWARD, retained, modal, s, b);
@@ -129,7 +129,7 @@ void run_stage(skb_retained_operator* op, int stage, const void* retained, void*
             return;
         case SKB_OP_STAGE_VERTICAL:
             if (!modal) fail(SKB_INVALID_ARGUMENT, "null modal output");
-            if (plane && vertical_pm_supported(op->vertical)) vertical_forward_pm(op->vertical, pm, t->np, modal, s);
+            if (plane && vertical_pm_supported(op->vertical)) vertical_forward_pm(op->vertical, pm, t->np, t->cmask.as<uint32_t>(), modal, s);
             else check_status(skb_vertical_forward(op->vertical, b.compact, modal, s));
             return;
     }

@@ -14,6 +14,12 @@
 // Retained values are read and written in the CHUNKED PLANE-MAJOR layout:
 // modes in chunks of 16, mode m of plane p at 16*P*(m / 16) + 16*p + m % 16
 // (P planes; the last chunk padded to 16 modes, np = modes rounded up to 16).
+// The layout holds SLOT values: the value of the half-spectrum slot a mode is
+// stored at, i.e. the conjugate of the caller's value for modes stored
+// conjugated (k < 0).  The moves to and from the caller's compact layout
+// apply that conjugation (PlaneTables::cmask), so the transforms need none:
+// a mode's slot takes the value as it is, a k_x = 0 Hermitian partner its
+// conjugate.
 // A plane's 16 modes of a chunk are one 128-byte run, and the 16*P values of
 // a chunk are the same bytes as the compact layout's (plane + P*mode) 16
 // modes, transposed: pm_transpose_kernel moves each chunk through LDS with
@@ -98,20 +104,22 @@ struct PlaneGeo {
     static constexpr int J0 = (KC + 15) / 16, J1 = (N - KC + 1 - 16) / 16;
     static constexpr bool empty(int j) { return j >= J0 && j <= J1; }
     static constexpr int SMAX = kPlaneMaxModes;    // retained values staged per plane
-    static constexpr int R0 = std::max(std::max(H * RP * 8, KC * YP * 4), SMAX * 8);
+    static constexpr int Z = SMAX;                 // staged slot holding zero (inverse) / trash (forward)
+    static constexpr int R0 = std::max(std::max(H * RP * 8, KC * YP * 4), (SMAX + 1) * 8);
     static constexpr int WT = 16 * kExchangeFloats * 4;  // one exchange scratch per wavefront
     static constexpr int BYTES = R0 + WT + N * 8;
     static_assert(KC <= 128 && BYTES <= 160 * 1024, "plane workgroup exceeds the CU's LDS");
 };
 
 // One 256-point FFT per 16-lane group: lane n holds x[16 j + n] in v[j]; on
-// return it holds X[n + 16 k] in v[k].  tw[i] = exp(-2 pi i i / 256) in LDS;
-// INV uses the conjugate (unnormalised inverse).
+// return it holds X[n + 16 k] in v[k].  tw[16 k + n] = exp(-2 pi i n k / 256)
+// in LDS (the 16 lanes of a group read 16 consecutive entries, no bank
+// conflicts); INV uses the conjugate (unnormalised inverse).
 template <bool INV>
 __device__ __forceinline__ void fft256(cx<float>* v, float* s, const cx<float>* tw, int g, int n) {
     rdft<16, INV>(v);
 #pragma unroll
-    for (int k = 1; k < 16; ++k) v[k] = INV ? cmulc(v[k], tw[n * k]) : cmul(v[k], tw[n * k]);
+    for (int k = 1; k < 16; ++k) v[k] = INV ? cmulc(v[k], tw[16 * k + n]) : cmul(v[k], tw[16 * k + n]);
     wave_exchange16(s, v, g, n);
     rdft<16, INV>(v);
 }
@@ -125,8 +133,12 @@ __device__ __forceinline__ float from_partner_lane(float v) {
     return __int_as_float(x);
 }
 
-// Code k (0..15) of a lane's 16 packed 16-bit column codes: 2*mode + conj, or
-// 0xFFFF where the (kx, ky) slot holds no retained value.
+// Code k (0..15) of a lane's 16 packed 16-bit column codes: the staged index
+// of the slot's value (its mode, or kPlaneMaxModes, a staged zero, where the
+// (kx, ky) slot holds no retained value), plus kPartner on the inverse's
+// k_x = 0 Hermitian partners, which take the conjugate.
+constexpr int kPartner = 0x8000, kCodeIndex = 0x7FFF;
+static_assert(kPlaneMaxModes <= kCodeIndex, "staged indices need 15 bits");
 __device__ __forceinline__ int unpack_code(const uint32_t* c, int k) {
     const uint32_t w = c[k >> 1];
     return static_cast<int>((k & 1) ? (w >> 16) : (w & 0xFFFFu));
@@ -170,16 +182,30 @@ __device__ __forceinline__ void inverse_plane(unsigned char* lds, const cx<float
     cx<float>* tw = reinterpret_cast<cx<float>*>(lds + G::R0 + G::WT);
 
     // 1. the plane's retained values (16-byte pairs of one chunk; past an odd
-    //    count the pair reads the chunk's padding) and the twiddles
+    //    count the pair reads the chunk's padding) and the twiddles.  Every
+    //    load of a thread is issued before the first LDS write, so the plane
+    //    waits for one memory latency, not one per pair.
     {
         const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(pm, static_cast<uint32_t>(np) * planes * 8u);
         const int pairs = (modes + 1) >> 1;
-        for (int i = tid; i < pairs; i += kPlaneThreads) {
+        constexpr int IT = (G::SMAX / 2 + kPlaneThreads - 1) / kPlaneThreads;
+        bf4 v[IT];
+#pragma unroll
+        for (int k = 0; k < IT; ++k) {
+            const int i = tid + kPlaneThreads * k;
             const uint32_t at = (static_cast<uint32_t>(i >> 3) * 16u * planes + 16u * p + 2u * (i & 7)) * 8u;
-            *reinterpret_cast<bf4*>(&r0[2 * i]) = raw_buffer_load_f4(rs, static_cast<int>(at), 0, 0);
+            v[k] = raw_buffer_load_f4(rs, static_cast<int>(i < pairs ? at : kBufferPast), 0, 0);
         }
-        if (tid < N) tw[tid] = twg[tid];
+        const cx<float> t = tid < N ? twg[tid] : cx<float>{0.f, 0.f};
+#pragma unroll
+        for (int k = 0; k < IT; ++k) {
+            const int i = tid + kPlaneThreads * k;
+            if (i < pairs) *reinterpret_cast<bf4*>(&r0[2 * i]) = v[k];
+        }
+        if (tid < N) tw[tid] = t;
+        if (tid == 0) r0[G::Z] = cx<float>{0.f, 0.f};
     }
+    const int uw = __builtin_amdgcn_readfirstlane(wave);  // wavefront index, uniform
     uint32_t code[G::ROUNDS][8];
 #pragma unroll
     for (int r = 0; r < G::ROUNDS; ++r) {
@@ -201,9 +227,9 @@ __device__ __forceinline__ void inverse_plane(unsigned char* lds, const cx<float
                     continue;
                 }
                 const int e = unpack_code(code[r], j);
-                const bool ok = e != 0xFFFF && c < KC;
-                const cx<float> x = r0[ok ? (e >> 1) : 0];
-                y[r][j] = ok ? cx<float>{x.x, (e & 1) ? -x.y : x.y} : cx<float>{0.f, 0.f};
+                const cx<float> x = r0[e & kCodeIndex];
+                // partners live in column 0 alone (wave 0, round 0)
+                y[r][j] = (r == 0 && uw == 0 && (e & kPartner)) ? cx<float>{x.x, -x.y} : x;
             }
         }
     }
@@ -370,11 +396,8 @@ __device__ __forceinline__ void forward_columns(unsigned char* lds, const cx<flo
 #pragma unroll
             for (int k = 0; k < 16; ++k) {
                 if (G::empty(k)) continue;  // no retained k_y there: these outputs are never computed
-                const int e = unpack_code(code, k);
-                if (e != 0xFFFF && c < KC) {
-                    const cx<float> x = cv[r][k];
-                    st[e >> 1] = cx<float>{x.x, (e & 1) ? -x.y : x.y};
-                }
+                // empty slots (and the columns past KC) write the trash slot
+                st[c < KC ? unpack_code(code, k) : G::Z] = cv[r][k];
             }
         }
     }
@@ -463,64 +486,101 @@ __global__ __launch_bounds__(kPlaneThreads) void plane_operator_kernel(const cx<
 // b: the compact run [16*P*b, 16*P*b + nm*P) (nm = the chunk's modes, 16 but
 // in the last chunk) and the chunked run [16*P*b, 16*P*b + 16*P), through an
 // LDS tile [P][17].  Both sides are contiguous.  The chunk's padding modes
-// are written as zeros.
+// are written as zeros.  Modes stored conjugated (bit m of the chunk's cmask
+// word) are conjugated on the way: the compact side holds the caller's
+// values, the chunked side slot values.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ cx<float> conj_if(cx<float> v, uint32_t mask, int m) {
+    return ((mask >> m) & 1u) ? cx<float>{v.x, -v.y} : v;
+}
+
+__device__ __forceinline__ uint32_t chunk_mask(const uint32_t* cmask, int b) {
+    using ConstWord = const __attribute__((address_space(4))) uint32_t;
+    return *((ConstWord*)cmask + b);
+}
+
 template <bool TO_PM>
 __global__ __launch_bounds__(256) void pm_transpose_kernel(const cx<float>* __restrict__ in,
-                                                           cx<float>* __restrict__ out, int modes, int planes) {
+                                                           cx<float>* __restrict__ out,
+                                                           const uint32_t* __restrict__ cmask, int modes,
+                                                           int planes) {
     // 16-byte accesses (two complex values) on both sides: a chunk starts at
     // 16 * P * 8 bytes (a multiple of 128); its compact run nm * P is odd only
-    // for a last partial chunk (one single tail element).
+    // for a last partial chunk (one single tail element).  Loads go out in
+    // batches of U per thread before any of them is used, so a chunk waits
+    // for about one memory latency (P <= 256: one batch).
+    constexpr int U = 8;
     extern __shared__ __align__(16) unsigned char pm_tile_lds[];
     cx<float>* tile = reinterpret_cast<cx<float>*>(pm_tile_lds);
-    const size_t base = static_cast<size_t>(16) * planes * blockIdx.x;
-    const int nm = modes - 16 * static_cast<int>(blockIdx.x) < 16 ? modes - 16 * static_cast<int>(blockIdx.x) : 16;
+    const int blk = static_cast<int>(blockIdx.x);
+    const size_t base = static_cast<size_t>(16) * planes * blk;
+    const int nm = modes - 16 * blk < 16 ? modes - 16 * blk : 16;
+    const uint32_t mask = chunk_mask(cmask, blk);
     const int tid = static_cast<int>(threadIdx.x);
     const int run = nm * planes;  // compact elements of the chunk
+    const int full = run >> 1;    // whole pairs of the compact run
+    // compact element e <-> tile slot (plane, mode), conjugated as its mode says
+    auto put = [&](int e, cx<float> v) {
+        const int m = e / planes;
+        tile[(e - m * planes) * 17 + m] = conj_if(v, mask, m);
+    };
+    auto get = [&](int e) {
+        const int m = e / planes;
+        return conj_if(tile[(e - m * planes) * 17 + m], mask, m);
+    };
     if constexpr (TO_PM) {
-        const bf4* src = reinterpret_cast<const bf4*>(in + base);
-        for (int e2 = tid; 2 * e2 < run; e2 += 256) {
-            const int e = 2 * e2;
-            const int m = e / planes, pl = e - m * planes;
-            if (e + 1 < run) {
-                const bf4 q = src[e2];
-                tile[pl * 17 + m] = cx<float>{q.x, q.y};
-                const int m1 = pl + 1 == planes ? m + 1 : m, p1 = pl + 1 == planes ? 0 : pl + 1;
-                tile[p1 * 17 + m1] = cx<float>{q.z, q.w};
-            } else {
-                tile[pl * 17 + m] = in[base + e];
+        const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(in + base, static_cast<uint32_t>(run) * 8u);
+        for (int b = 0; b < full; b += 256 * U) {
+            bf4 q[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int e2 = b + tid + 256 * u;
+                q[u] = raw_buffer_load_f4(rs, static_cast<int>(e2 < full ? 16u * e2 : kBufferPast), 0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int e2 = b + tid + 256 * u;
+                if (e2 < full) {
+                    put(2 * e2, cx<float>{q[u].x, q[u].y});
+                    put(2 * e2 + 1, cx<float>{q[u].z, q[u].w});
+                }
             }
         }
+        if ((run & 1) && tid == 0) put(run - 1, in[base + run - 1]);
         lds_barrier();
         bf4* dst = reinterpret_cast<bf4*>(out + base);
         for (int e2 = tid; e2 < 8 * planes; e2 += 256) {
             const int pl = e2 >> 3, m = 2 * (e2 & 7);
             const cx<float> a = m < nm ? tile[pl * 17 + m] : cx<float>{0.f, 0.f};
-            const cx<float> b = m + 1 < nm ? tile[pl * 17 + m + 1] : cx<float>{0.f, 0.f};
-            dst[e2] = bf4{a.x, a.y, b.x, b.y};
+            const cx<float> c = m + 1 < nm ? tile[pl * 17 + m + 1] : cx<float>{0.f, 0.f};
+            dst[e2] = bf4{a.x, a.y, c.x, c.y};
         }
     } else {
-        const bf4* src = reinterpret_cast<const bf4*>(in + base);
-        for (int e2 = tid; e2 < 8 * planes; e2 += 256) {
-            const int pl = e2 >> 3, m = 2 * (e2 & 7);
-            const bf4 q = src[e2];
-            tile[pl * 17 + m] = cx<float>{q.x, q.y};
-            tile[pl * 17 + m + 1] = cx<float>{q.z, q.w};
+        const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(in + base, static_cast<uint32_t>(16 * planes) * 8u);
+        for (int b = 0; b < 8 * planes; b += 256 * U) {
+            bf4 q[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int e2 = b + tid + 256 * u;
+                q[u] = raw_buffer_load_f4(rs, static_cast<int>(e2 < 8 * planes ? 16u * e2 : kBufferPast), 0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int e2 = b + tid + 256 * u;
+                if (e2 < 8 * planes) {
+                    const int pl = e2 >> 3, m = 2 * (e2 & 7);
+                    tile[pl * 17 + m] = cx<float>{q[u].x, q[u].y};
+                    tile[pl * 17 + m + 1] = cx<float>{q[u].z, q[u].w};
+                }
+            }
         }
         lds_barrier();
         bf4* dst = reinterpret_cast<bf4*>(out + base);
-        for (int e2 = tid; 2 * e2 < run; e2 += 256) {
-            const int e = 2 * e2;
-            const int m = e / planes, pl = e - m * planes;
-            const cx<float> a = tile[pl * 17 + m];
-            if (e + 1 < run) {
-                const int m1 = pl + 1 == planes ? m + 1 : m, p1 = pl + 1 == planes ? 0 : pl + 1;
-                const cx<float> b = tile[p1 * 17 + m1];
-                dst[e2] = bf4{a.x, a.y, b.x, b.y};
-            } else {
-                out[base + e] = a;
-            }
+        for (int e2 = tid; e2 < full; e2 += 256) {
+            const cx<float> a = get(2 * e2), c = get(2 * e2 + 1);
+            dst[e2] = bf4{a.x, a.y, c.x, c.y};
         }
+        if ((run & 1) && tid == 0) out[base + run - 1] = get(run - 1);
     }
 }
 
@@ -528,17 +588,20 @@ __global__ __launch_bounds__(256) void pm_transpose_kernel(const cx<float>* __re
 // that are not 16-byte aligned.
 template <bool TO_PM>
 __global__ __launch_bounds__(256) void pm_transpose_scalar_kernel(const cx<float>* __restrict__ in,
-                                                                  cx<float>* __restrict__ out, int modes,
+                                                                  cx<float>* __restrict__ out,
+                                                                  const uint32_t* __restrict__ cmask, int modes,
                                                                   int planes) {
     extern __shared__ __align__(16) unsigned char pm_tile_lds[];
     cx<float>* tile = reinterpret_cast<cx<float>*>(pm_tile_lds);
-    const size_t base = static_cast<size_t>(16) * planes * blockIdx.x;
-    const int nm = modes - 16 * static_cast<int>(blockIdx.x) < 16 ? modes - 16 * static_cast<int>(blockIdx.x) : 16;
+    const int blk = static_cast<int>(blockIdx.x);
+    const size_t base = static_cast<size_t>(16) * planes * blk;
+    const int nm = modes - 16 * blk < 16 ? modes - 16 * blk : 16;
+    const uint32_t mask = chunk_mask(cmask, blk);
     const int tid = static_cast<int>(threadIdx.x);
     if constexpr (TO_PM) {
         for (int e = tid; e < nm * planes; e += 256) {
             const int m = e / planes, pl = e - m * planes;
-            tile[pl * 17 + m] = in[base + e];
+            tile[pl * 17 + m] = conj_if(in[base + e], mask, m);
         }
         lds_barrier();
         for (int e = tid; e < 16 * planes; e += 256) {
@@ -553,7 +616,7 @@ __global__ __launch_bounds__(256) void pm_transpose_scalar_kernel(const cx<float
         lds_barrier();
         for (int e = tid; e < nm * planes; e += 256) {
             const int m = e / planes, pl = e - m * planes;
-            out[base + e] = tile[pl * 17 + m];
+            out[base + e] = conj_if(tile[pl * 17 + m], mask, m);
         }
     }
 }
@@ -589,20 +652,34 @@ void PlaneTables::build(const skb_workload& w, uint64_t kc_, uint64_t modes_, co
     const int N = kPlaneN, KC = kPlaneKc;
     // per (column c, lane n) 16 codes, two per 32-bit word
     std::vector<uint16_t> f(static_cast<size_t>(KC) * 16 * 16), i(f.size());
-    auto pack = [](int code) { return static_cast<uint16_t>(code < 0 ? 0xFFFF : code); };
+    // Slot codes (2*mode + conj from the column lists) -> staged indices.  A
+    // slot the forward writes holds its mode's slot value, so the inverse
+    // reads it as it is; the inverse's extra slots are the k_x = 0 partners,
+    // whose value is the conjugate (kPartner).  Every mode's conj flag goes to
+    // cmask (bit m % 16 of word m / 16).
+    std::vector<uint32_t> cm(static_cast<size_t>(np / 16), 0u);
     for (int c = 0; c < KC; ++c)
         for (int n = 0; n < 16; ++n)
             for (int k = 0; k < 16; ++k) {
-                f[(static_cast<size_t>(c) * 16 + n) * 16 + k] = pack(fmap[static_cast<size_t>(c) * N + n + 16 * k]);
-                i[(static_cast<size_t>(c) * 16 + n) * 16 + k] = pack(imap[static_cast<size_t>(c) * N + 16 * k + n]);
+                const int fc = fmap[static_cast<size_t>(c) * N + n + 16 * k];
+                const int fi = fmap[static_cast<size_t>(c) * N + 16 * k + n];
+                const int ic = imap[static_cast<size_t>(c) * N + 16 * k + n];
+                if (fc >= 0 && (fc & 1)) cm[static_cast<size_t>(fc >> 1) / 16] |= 1u << ((fc >> 1) % 16);
+                if (fi >= 0 && fi != ic) fail(SKB_LOGIC_ERROR, "plane codes: forward and inverse slots disagree");
+                f[(static_cast<size_t>(c) * 16 + n) * 16 + k] = static_cast<uint16_t>(fc < 0 ? kPlaneMaxModes : fc >> 1);
+                i[(static_cast<size_t>(c) * 16 + n) * 16 + k] = static_cast<uint16_t>(
+                    ic < 0 ? kPlaneMaxModes : (ic >> 1) | (fi < 0 ? kPartner : 0));
             }
+    cmask.upload(cm.data(), cm.size() * sizeof(uint32_t));
     fcodes.upload(f.data(), f.size() * sizeof(uint16_t));
     icodes.upload(i.data(), i.size() * sizeof(uint16_t));
+    // twiddles of the 16 x 16 split, W^(n k) at 16 k + n
     std::vector<cx<float>> t(N);
-    for (int k = 0; k < N; ++k) {
-        const double a = 2.0 * M_PI * k / N;
-        t[k] = cx<float>{static_cast<float>(std::cos(a)), static_cast<float>(-std::sin(a))};
-    }
+    for (int k = 0; k < 16; ++k)
+        for (int n = 0; n < 16; ++n) {
+            const double a = 2.0 * M_PI * (n * k) / N;
+            t[16 * k + n] = cx<float>{static_cast<float>(std::cos(a)), static_cast<float>(-std::sin(a))};
+        }
     tw.upload(t.data(), t.size() * sizeof(t[0]));
 }
 
@@ -628,17 +705,18 @@ void plane_operator(const PlaneTables& t, const void* pm_in, void* pm_out, void*
     check_hip(hipGetLastError(), "plane operator launch");
 }
 
-void pm_transpose(bool to_pm, const void* in, void* out, int modes, int planes, int np, hipStream_t s) {
-    (void)np;  // the chunk count follows from modes
+void pm_transpose(const PlaneTables& t, bool to_pm, const void* in, void* out, int planes, hipStream_t s) {
+    const int modes = t.modes;
     const dim3 grid((modes + 15) / 16);
     const size_t lds = static_cast<size_t>(planes) * 17 * sizeof(cx<float>);
     const auto* i = static_cast<const cx<float>*>(in);
     auto* o = static_cast<cx<float>*>(out);
+    const uint32_t* m = t.cmask.as<uint32_t>();
     const bool wide = (reinterpret_cast<uintptr_t>(in) % 16 == 0) && (reinterpret_cast<uintptr_t>(out) % 16 == 0);
-    if (wide && to_pm) hipLaunchKernelGGL(pm_transpose_kernel<true>, grid, dim3(256), lds, s, i, o, modes, planes);
-    else if (wide) hipLaunchKernelGGL(pm_transpose_kernel<false>, grid, dim3(256), lds, s, i, o, modes, planes);
-    else if (to_pm) hipLaunchKernelGGL(pm_transpose_scalar_kernel<true>, grid, dim3(256), lds, s, i, o, modes, planes);
-    else hipLaunchKernelGGL(pm_transpose_scalar_kernel<false>, grid, dim3(256), lds, s, i, o, modes, planes);
+    if (wide && to_pm) hipLaunchKernelGGL(pm_transpose_kernel<true>, grid, dim3(256), lds, s, i, o, m, modes, planes);
+    else if (wide) hipLaunchKernelGGL(pm_transpose_kernel<false>, grid, dim3(256), lds, s, i, o, m, modes, planes);
+    else if (to_pm) hipLaunchKernelGGL(pm_transpose_scalar_kernel<true>, grid, dim3(256), lds, s, i, o, m, modes, planes);
+    else hipLaunchKernelGGL(pm_transpose_scalar_kernel<false>, grid, dim3(256), lds, s, i, o, m, modes, planes);
     check_hip(hipGetLastError(), "plane-major transpose launch");
 }
 

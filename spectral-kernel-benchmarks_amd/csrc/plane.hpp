@@ -24,12 +24,14 @@ bool plane_supported(const skb_workload& w, uint64_t kc, uint64_t modes, int pre
 // fmap / imap: dense [kx * Ny + ky] -> code or -1 (see PlaneTables).
 bool plane_band_empty(const std::vector<int>& fmap, const std::vector<int>& imap, uint64_t kc);
 
-// Device tables of one mode set: twiddles and the packed 16-bit column codes
-// (2*mode + conj per (kx, ky) slot, 0xFFFF where empty) in the order each
-// kernel's lanes read them.  fmap / imap: dense [kx * Ny + ky] -> 2*mode +
-// conj or -1, forward (stored modes) and inverse (k_x = 0 partners added).
+// Device tables of one mode set: twiddles, the packed 16-bit column codes
+// (staged index per (kx, ky) slot, the k_x = 0 partners flagged) in the order
+// each kernel's lanes read them, and the modes' conj flags.  fmap / imap:
+// dense [kx * Ny + ky] -> 2*mode + conj or -1, forward (stored modes) and
+// inverse (k_x = 0 partners added).
 struct PlaneTables {
     DeviceBuffer tw, fcodes, icodes;
+    DeviceBuffer cmask;  // per 16-mode chunk: bit m % 16 set when mode m is stored conjugated
     int modes = 0, np = 0;
     void build(const skb_workload& w, uint64_t kc, uint64_t modes, const std::vector<int>& fmap,
                const std::vector<int>& imap);
@@ -41,7 +43,8 @@ void plane_forward(const PlaneTables& t, const void* real, void* pm, int planes,
 // Both transforms of each plane in one workgroup: pm_in -> real planes ->
 // pm_out (pm_out may equal pm_in).
 void plane_operator(const PlaneTables& t, const void* pm_in, void* pm_out, void* real, int planes, hipStream_t s);
-// compact (plane + P*mode) -> chunked plane-major (to_pm) or back.
-void pm_transpose(bool to_pm, const void* in, void* out, int modes, int planes, int np, hipStream_t s);
+// compact (plane + P*mode, the caller's values) -> chunked plane-major (slot
+// values) (to_pm) or back.
+void pm_transpose(const PlaneTables& t, bool to_pm, const void* in, void* out, int planes, hipStream_t s);
 
 }  // namespace skb

@@ -126,9 +126,13 @@ struct GemmArgs {
     // chunked plane-major input (split kernel, PM = true): element (mode, z,
     // field) at in[16*P*(mode / 16) + 16*(z + pm_nz * field) + mode % 16],
     // P = pm_planes, pm_np modes (a multiple of 16); a tile is up to 16
-    // consecutive modes of one group and one field (GemmTile::pad = field),
-    // written to the modal layout out + out_ld * (field + F * mode).
+    // consecutive modes of one chunk, one group and one field
+    // (GemmTile::pad = field), written to the modal layout
+    // out + out_ld * (field + F * mode).  The input holds slot values: modes
+    // with bit m % 16 of pm_cmask[m / 16] set are stored conjugated, and the
+    // product, real-matrix times column, is conjugated back on output.
     int pm_np, pm_nz, pm_planes;
+    const uint32_t* pm_cmask;
 };
 
 template <typename R, int KS, bool WVM>
@@ -694,6 +698,16 @@ __global__ __launch_bounds__(256) void grouped_gemm_split_kernel(const GemmArgs<
                 vrow.y += __shfl_xor(vrow.y, 16);
                 vrow.x += __shfl_xor(vrow.x, 32);
                 vrow.y += __shfl_xor(vrow.y, 32);
+            }
+            if constexpr (PM) {
+                // slot values back to the caller's values (uniform word, lane bit)
+                using ConstWord = const __attribute__((address_space(4))) uint32_t;
+                const uint32_t word = *((ConstWord*)g.pm_cmask + (cur.col_begin >> 4));
+                if ((word >> ((cur.col_begin & 15) + j)) & 1u) {
+#pragma unroll
+                    for (int sl = 0; sl < SL; ++sl) im[sl] = -im[sl];
+                    vrow.y = -vrow.y;
+                }
             }
             {
                 // results straight from the accumulators: lane (j, q) holds rows
@@ -1283,13 +1297,18 @@ std::unique_ptr<skb_vertical_plan> make_plan(const skb_workload* workload, uint6
     p->fwd.build(forward, group_count, inj, inz, precision);
     p->inv.build(inverse, group_count, inz, inj, precision);
     p->tiles.upload(tiles.data(), tiles.size() * sizeof(GemmTile));
+    // Plane-major tiles stay inside one 16-mode chunk (a group's first and
+    // last tiles may be partial): a tile's 16 modes at one z are then one
+    // 128-byte line that no other tile reads.
     std::vector<GemmTile> pm;
     for (uint64_t g = 0; g < group_count; ++g)
-        for (uint64_t f = 0; f < fields; ++f)
-            for (uint64_t m = 0; m < groups[g].mode_count; m += 16)
-                pm.push_back(GemmTile{static_cast<int>(g), static_cast<int>(groups[g].first_mode + m),
-                                      static_cast<int>(std::min<uint64_t>(16, groups[g].mode_count - m)),
+        for (uint64_t f = 0; f < fields; ++f) {
+            const uint64_t first = groups[g].first_mode, end = first + groups[g].mode_count;
+            for (uint64_t m = first; m < end; m = (m / 16 + 1) * 16)
+                pm.push_back(GemmTile{static_cast<int>(g), static_cast<int>(m),
+                                      static_cast<int>(std::min<uint64_t>((m / 16 + 1) * 16, end) - m),
                                       static_cast<int>(f)});
+        }
     p->pm_ntiles = static_cast<int>(pm.size());
     p->pm_tiles.upload(pm.data(), pm.size() * sizeof(GemmTile));
     p->matrix_bytes = nj * nz * group_count * (precision == SKB_F64 ? 8 : 4);
@@ -1374,7 +1393,8 @@ bool vertical_pm_supported(const skb_vertical_plan* p) {
     return M == K && (K == 33 || K == 65 || K == 129);
 }
 
-void vertical_forward_pm(const skb_vertical_plan* p, const void* pm, int np, void* out, void* stream) {
+void vertical_forward_pm(const skb_vertical_plan* p, const void* pm, int np, const uint32_t* cmask, void* out,
+                         void* stream) {
     if (!vertical_pm_supported(p)) fail(SKB_UNSUPPORTED, "plane-major vertical product: unsupported plan");
     if (!pm || !out || np < static_cast<int>(p->modes)) fail(SKB_INVALID_ARGUMENT, "bad plane-major operand");
     auto a = args<float>(p, p->fwd, pm, out);
@@ -1383,6 +1403,8 @@ void vertical_forward_pm(const skb_vertical_plan* p, const void* pm, int np, voi
     a.pm_np = np;
     a.pm_nz = static_cast<int>(p->w.nz);
     a.pm_planes = static_cast<int>(p->w.nz * p->w.fields);
+    a.pm_cmask = cmask;
+    if (!cmask) fail(SKB_INVALID_ARGUMENT, "plane-major vertical product: null conj mask");
     if (np % 16 != 0) fail(SKB_INVALID_ARGUMENT, "the chunked plane-major layout holds whole chunks of 16 modes");
     const hipStream_t s = as_stream(stream);
     switch (a.K) {
