@@ -8,10 +8,12 @@ resident in HBM, enqueued by ONE C-ABI call
   1. pruned inverse c2r   compact retained (z, field, mode) -> real planes
   2. pruned forward r2c   real planes -> compact retained (2/3-rule truncated)
   3. K^2-grouped vertical GEMM on MFMA -> modal
-The evaluations alternate over the plan's lanes (streams on separate
-hardware queues), so one evaluation's layout copy and vertical product use
-the CUs another's 195 plane workgroups leave idle.  `value` = evaluations
-per second over the timed steps; the single-call rate (one
+The batch runs stage-batched (skb_retained_operator_reserve_batch): each
+stage is one launch over all evaluations of the step (layout copies, plane
+transforms, vertical products), so the 195 plane workgroups of successive
+evaluations fill every CU and the vertical product runs in steady state
+(--batch-mode lanes: evaluations alternate over streams on separate hardware
+queues instead).  `value` = evaluations per second over the timed steps; the single-call rate (one
 skb_retained_operator_execute per evaluation, in order) and per-evaluation
 latencies are reported beside it.
 The headline runs in FP32 storage (FP32 transforms, split-BF16 MFMA vertical
@@ -45,7 +47,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--batch", type=int, default=32,
                     help="independent evaluations per step (one execute_batch call)")
-    ap.add_argument("--lanes", type=int, default=3, help="lanes of the batched execute (1: in order)")
+    ap.add_argument("--batch-mode", choices=("stage", "lanes"), default="stage",
+                    help="stage: one launch per stage over the batch; lanes: evaluations over --lanes streams")
+    ap.add_argument("--lanes", type=int, default=3, help="lanes of --batch-mode lanes (1: in order)")
     ap.add_argument("--family", choices=["qr32", "k2"], default="qr32",
                     help="qr32: 32 K^2 bins of 65x65 QR-Gaussian matrices; k2: reference 2045x(42x65) family")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -589,10 +593,14 @@ def run(args, np, torch, S, device):
     torch.cuda.synchronize()
 
     # headline: one step = one batch of independent evaluations (distinct
-    # HBM-resident inputs and outputs) spread over the plan's lanes
+    # HBM-resident inputs and outputs), stage-batched (or over lanes)
     B = args.batch
-    f32.op.reserve_lanes(args.lanes)
-    lanes, lanes_apart = f32.op.lanes()
+    if args.batch_mode == "stage":
+        f32.op.reserve_batch(min(B, 32))
+        lanes, lanes_apart = 1, True
+    else:
+        f32.op.reserve_lanes(args.lanes)
+        lanes, lanes_apart = f32.op.lanes()
     xs = [f32.x * (1.0 + 1e-3 * k) for k in range(B)]
     outs = [torch.empty_like(f32.modal) for _ in range(B)]
     batch_call = f32.op.bind_execute_batch(xs, outs)
@@ -602,8 +610,8 @@ def run(args, np, torch, S, device):
         torch.cuda.synchronize()
         if not torch.equal(outs[k], f32.modal):
             raise RuntimeError("batched evaluation %d differs from the same evaluation run alone" % k)
-    log(args, "batched: %d evaluations per step over %d lanes (apart %s); warmup %d, timed %d steps"
-        % (B, lanes, lanes_apart, args.warmup, args.steps))
+    log(args, "batched (%s): %d evaluations per step, %d lanes (apart %s); warmup %d, timed %d steps"
+        % (args.batch_mode, B, lanes, lanes_apart, args.warmup, args.steps))
     for _ in range(args.warmup):
         batch_call()
     wall = timed(torch, batch_call, args.steps)
@@ -616,6 +624,7 @@ def run(args, np, torch, S, device):
     host_batch_us = 1e6 * (time.perf_counter() - t0) / (10 * B)
     torch.cuda.synchronize()
     del xs, outs, batch_call
+    f32.op.reserve_batch(0)
     f32.op.reserve_lanes(1)
     torch.cuda.empty_cache()
     lat_evals = max(LATENCY_EVALS, args.steps)
@@ -733,10 +742,11 @@ def run(args, np, torch, S, device):
                         "K^2-grouped vertical GEMM",
             "nx": 256, "ny": 256, "nz": 65, "fields": 3, "retained_modes": nkl, "vertical_family": args.family,
             "vertical_groups": len(groups), "nj": nj,
-            "launch": "one prepared C-ABI call per step: skb_retained_operator_execute_batch of %d independent "
-                      "evaluations (distinct inputs and outputs) over %d lanes on separate hardware queues, "
-                      "forked from and joined to one stream" % (B, lanes),
-            "evals_per_step": B, "lanes": lanes, "lanes_apart": lanes_apart,
+            "launch": ("one prepared C-ABI call per step: skb_retained_operator_execute_batch of %d independent "
+                       "evaluations (distinct inputs and outputs), " % B) +
+                      ("stage-batched: one launch per stage over the batch" if args.batch_mode == "stage" else
+                       "over %d lanes on separate hardware queues, forked from and joined to one stream" % lanes),
+            "evals_per_step": B, "batch_mode": args.batch_mode, "lanes": lanes, "lanes_apart": lanes_apart,
             "path": "plane-resident transforms" if f32.plane else "two-pass transforms",
         },
         "roofline": roofline,

@@ -297,6 +297,21 @@ int skb_retained_operator_fused(const skb_retained_operator* op);
 int skb_retained_operator_reserve_lanes(skb_retained_operator* op, int lanes);
 /* Lanes reserved (>= 1); *apart = 1 when every lane has a queue of its own. */
 int skb_retained_operator_lanes(const skb_retained_operator* op, int* apart);
+/* Stage-batched evaluation.  reserve_batch(n), n in [2, 32], gives the plan
+ * the plane-major blocks and real planes of n evaluations (it allocates; 0
+ * or 1 releases them).  execute_batch then takes the evaluations in groups
+ * of up to n and runs each stage as ONE launch over the group: the layout
+ * copies, the plane transforms (the plane workgroups of all evaluations fill
+ * every CU) and the vertical products (persistent workgroups over all
+ * evaluations' tiles).  It takes precedence over the lanes, runs in order on
+ * `stream`, allocates nothing, needs 16-byte aligned inputs, and each output
+ * equals the same evaluation run alone bit for bit.  Plane path with a
+ * plane-major vertical product (square split-BF16 FP32 families) only;
+ * elsewhere reserve_batch fails with SKB_UNSUPPORTED.  No reference
+ * counterpart. */
+int skb_retained_operator_reserve_batch(skb_retained_operator* op, int evaluations);
+/* Evaluations of one stage-batched group (0 when none is reserved). */
+int skb_retained_operator_batch(const skb_retained_operator* op);
 int skb_retained_operator_execute_batch(skb_retained_operator* op, int count, const void* const* retained,
                                         void* const* modal, void* stream);
 /* Device pointer to the real planes (x + Nx(y + Ny*plane)) of the last

@@ -48,6 +48,13 @@ void vertical_forward_columns(const skb_vertical_plan* p, const void* in, int in
 bool vertical_pm_supported(const skb_vertical_plan* p);
 void vertical_forward_pm(const skb_vertical_plan* p, const void* pm, int np, const uint32_t* cmask, void* out,
                          void* stream);
+// The same for `count` evaluations in one launch: evaluation e reads the
+// block at pm + e * pm_stride (complex elements) and writes out[e].
+struct BatchOutputs {
+    void* out[kBatchMax];
+};
+void vertical_forward_pm_batch(const skb_vertical_plan* p, const void* pm, size_t pm_stride, int np,
+                               const uint32_t* cmask, const BatchOutputs& out, int count, void* stream);
 
 // Modal round trip in one kernel: out[:, c] = I_g (w[:, c] .* (F_g in[:, c]))
 // over a compact plan's columns (weights in the plan's precision, modal

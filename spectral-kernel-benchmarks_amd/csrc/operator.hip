@@ -12,12 +12,16 @@
 // through a copy back to the compact layout.
 // Two-pass path: the pruned plan's row and column kernels and the compact
 // vertical product.
-// Batched execute: evaluations of a batch alternate over the plan's lanes
-// (own streams and intermediates), forked from and joined back to the
-// caller's stream, so one evaluation's layout and vertical product run on
-// the CUs another evaluation's 195 plane workgroups leave idle.
+// Batched execute, two forms.  Stage-batched (reserve_batch): up to
+// kBatchMax evaluations per group, each stage one launch over the whole
+// group (layout copies, plane transforms, vertical products), with per-
+// evaluation intermediates; the plane workgroups of successive evaluations
+// fill every CU and the vertical product runs in steady state.  Lanes
+// (reserve_lanes): evaluations alternate over the plan's lanes (own streams
+// and intermediates), forked from and joined back to the caller's stream.
 #include <algorithm>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -64,6 +68,10 @@ struct skb_retained_operator {
     hipEvent_t fork = nullptr;
     bool lanes_apart = true;  // every lane on a hardware queue of its own
     skb::DeviceBuffer probe;  // target of the queue check's tiny kernel
+    // stage-batched execute (skb_retained_operator_reserve_batch): the
+    // plane-major blocks and real planes of batch_cap evaluations
+    int batch_cap = 0;
+    skb::DeviceBuffer batch_pm, batch_real;
     void drop_lanes() {
         lanes.clear();
         if (fork) (void)hipEventDestroy(fork);
@@ -268,6 +276,26 @@ int skb_retained_operator_lanes(const skb_retained_operator* op, int* apart) {
     return op ? std::max<int>(1, static_cast<int>(op->lanes.size())) : 0;
 }
 
+int skb_retained_operator_reserve_batch(skb_retained_operator* op, int evaluations) {
+    return guarded([&] {
+        if (!op) fail(SKB_INVALID_ARGUMENT, "null plan");
+        if (evaluations < 0 || evaluations > kBatchMax)
+            fail(SKB_INVALID_ARGUMENT, "batched evaluations must lie in [0, " + std::to_string(kBatchMax) + "]");
+        op->batch_cap = 0;
+        op->batch_pm.reset();
+        op->batch_real.reset();
+        if (evaluations < 2) return;
+        if (!plane_path(op) || !vertical_pm_supported(op->vertical))
+            fail(SKB_UNSUPPORTED, "stage-batched execute: plane path with a plane-major vertical product only");
+        const PlaneTables* t = pruned_plane_tables(op->pruned);
+        op->batch_pm.allocate(static_cast<uint64_t>(evaluations) * t->np * op->planes * complex_bytes(op->precision));
+        op->batch_real.allocate(static_cast<uint64_t>(evaluations) * op->real.bytes);
+        op->batch_cap = evaluations;
+    });
+}
+
+int skb_retained_operator_batch(const skb_retained_operator* op) { return op ? op->batch_cap : 0; }
+
 int skb_retained_operator_execute_batch(skb_retained_operator* op, int count, const void* const* retained,
                                         void* const* modal, void* stream) {
     return guarded([&] {
@@ -275,6 +303,27 @@ int skb_retained_operator_execute_batch(skb_retained_operator* op, int count, co
         for (int i = 0; i < count; ++i)
             if (!retained[i] || !modal[i]) fail(SKB_INVALID_ARGUMENT, "null retained input or modal output");
         const hipStream_t s = as_stream(stream);
+        if (op->batch_cap >= 2 && op->fused && plane_path(op) && vertical_pm_supported(op->vertical)) {
+            // stage-batched: groups of up to batch_cap evaluations, one launch per stage
+            const PlaneTables* t = pruned_plane_tables(op->pruned);
+            const size_t pm_stride = static_cast<size_t>(t->np) * op->planes;
+            const size_t real_stride = op->real.bytes / real_bytes(op->precision);
+            for (int i0 = 0; i0 < count; i0 += op->batch_cap) {
+                const int n = std::min(op->batch_cap, count - i0);
+                BatchInputs in{};
+                BatchOutputs out{};
+                for (int e = 0; e < n; ++e) {
+                    in.in[e] = retained[i0 + e];
+                    out.out[e] = modal[i0 + e];
+                }
+                pm_transpose_batch(*t, in, n, op->batch_pm.ptr, pm_stride, op->planes, s);
+                plane_operator(*t, op->batch_pm.ptr, op->batch_pm.ptr, op->batch_real.ptr, op->planes, s, n, pm_stride,
+                               real_stride);
+                vertical_forward_pm_batch(op->vertical, op->batch_pm.ptr, pm_stride, t->np, t->cmask.as<uint32_t>(),
+                                          out, n, s);
+            }
+            return;
+        }
         const int nl = static_cast<int>(op->lanes.size());
         if (nl <= 1 || count <= 1 || !plane_path(op)) {
             for (int i = 0; i < count; ++i) run_eval(op, retained[i], modal[i], s, own_buffers(op));
@@ -333,6 +382,7 @@ uint64_t skb_retained_operator_device_bytes(const skb_retained_operator* op) {
     if (!op) return 0;
     uint64_t lanes = 0;
     for (const auto& l : op->lanes) lanes += l->pm.bytes + l->real.bytes + l->compact.bytes;
+    lanes += op->batch_pm.bytes + op->batch_real.bytes;
     return op->real.bytes + op->compact.bytes + skb_pruned_scratch_bytes(op->pruned) +
            (pruned_plane_scratch(op->pruned) ? plane_np(op->modes) * op->planes * 8 : 0) +
            2 * skb_vertical_matrix_bytes(op->vertical) + lanes;

@@ -463,7 +463,8 @@ __global__ __launch_bounds__(kPlaneThreads) void plane_forward_kernel(const floa
 // the physical-space result) and, from the same registers, the forward row
 // transforms -> forward columns -> plane-major values (may alias the input:
 // each workgroup reads its plane's values in full before it writes them, and
-// no other workgroup touches them).
+// no other workgroup touches them).  blockIdx.y is the evaluation of a batch:
+// its arrays sit pm_stride complex and real_stride real elements further on.
 // ---------------------------------------------------------------------------
 template <int KC>
 __global__ __launch_bounds__(kPlaneThreads) void plane_operator_kernel(const cx<float>* pm_in,
@@ -472,9 +473,13 @@ __global__ __launch_bounds__(kPlaneThreads) void plane_operator_kernel(const cx<
                                                                       const cx<float>* __restrict__ twg,
                                                                       const uint32_t* __restrict__ icodes,
                                                                       const uint32_t* __restrict__ fcodes, int modes,
-                                                                      int np, int planes) {
+                                                                      int np, int planes, size_t pm_stride,
+                                                                      size_t real_stride) {
     __shared__ __align__(16) unsigned char lds[PlaneGeo<KC>::BYTES];
     const int p = static_cast<int>(blockIdx.x);
+    pm_in += pm_stride * blockIdx.y;
+    pm_out += pm_stride * blockIdx.y;
+    real += real_stride * blockIdx.y;
     cx<float> xa[2][PlaneGeo<KC>::KH], xb[2][PlaneGeo<KC>::KH];
     inverse_plane<KC, true>(lds, pm_in, real, twg, icodes, modes, np, planes, p, xa, xb);
     lds_barrier();  // every row-buffer read of the inverse is done before the band overwrites it
@@ -500,10 +505,8 @@ __device__ __forceinline__ uint32_t chunk_mask(const uint32_t* cmask, int b) {
 }
 
 template <bool TO_PM>
-__global__ __launch_bounds__(256) void pm_transpose_kernel(const cx<float>* __restrict__ in,
-                                                           cx<float>* __restrict__ out,
-                                                           const uint32_t* __restrict__ cmask, int modes,
-                                                           int planes) {
+__device__ __forceinline__ void pm_transpose_chunk(const cx<float>* __restrict__ in, cx<float>* __restrict__ out,
+                                                   const uint32_t* __restrict__ cmask, int modes, int planes) {
     // 16-byte accesses (two complex values) on both sides: a chunk starts at
     // 16 * P * 8 bytes (a multiple of 128); its compact run nm * P is odd only
     // for a last partial chunk (one single tail element).  Loads go out in
@@ -582,6 +585,24 @@ __global__ __launch_bounds__(256) void pm_transpose_kernel(const cx<float>* __re
         }
         if ((run & 1) && tid == 0) out[base + run - 1] = get(run - 1);
     }
+}
+
+template <bool TO_PM>
+__global__ __launch_bounds__(256) void pm_transpose_kernel(const cx<float>* __restrict__ in,
+                                                           cx<float>* __restrict__ out,
+                                                           const uint32_t* __restrict__ cmask, int modes,
+                                                           int planes) {
+    pm_transpose_chunk<TO_PM>(in, out, cmask, modes, planes);
+}
+
+// Compact -> chunked plane-major for the evaluations of a batch: blockIdx.y
+// is the evaluation, its compact input src.in[y], its block at out +
+// y * out_stride.
+__global__ __launch_bounds__(256) void pm_to_pm_batch_kernel(const BatchInputs src, cx<float>* __restrict__ out,
+                                                             size_t out_stride, const uint32_t* __restrict__ cmask,
+                                                             int modes, int planes) {
+    pm_transpose_chunk<true>(static_cast<const cx<float>*>(src.in[blockIdx.y]), out + out_stride * blockIdx.y, cmask,
+                             modes, planes);
 }
 
 // The same move with 8-byte accesses on the compact side, for compact arrays
@@ -697,11 +718,12 @@ void plane_forward(const PlaneTables& t, const void* real, void* pm, int planes,
     check_hip(hipGetLastError(), "plane forward launch");
 }
 
-void plane_operator(const PlaneTables& t, const void* pm_in, void* pm_out, void* real, int planes, hipStream_t s) {
-    hipLaunchKernelGGL((plane_operator_kernel<kPlaneKc>), dim3(planes), dim3(kPlaneThreads), 0, s,
+void plane_operator(const PlaneTables& t, const void* pm_in, void* pm_out, void* real, int planes, hipStream_t s,
+                    int count, size_t pm_stride, size_t real_stride) {
+    hipLaunchKernelGGL((plane_operator_kernel<kPlaneKc>), dim3(planes, count), dim3(kPlaneThreads), 0, s,
                        static_cast<const cx<float>*>(pm_in), static_cast<cx<float>*>(pm_out),
                        static_cast<float*>(real), t.tw.as<cx<float>>(), t.icodes.as<uint32_t>(),
-                       t.fcodes.as<uint32_t>(), t.modes, t.np, planes);
+                       t.fcodes.as<uint32_t>(), t.modes, t.np, planes, pm_stride, real_stride);
     check_hip(hipGetLastError(), "plane operator launch");
 }
 
@@ -717,6 +739,21 @@ void pm_transpose(const PlaneTables& t, bool to_pm, const void* in, void* out, i
     else if (wide) hipLaunchKernelGGL(pm_transpose_kernel<false>, grid, dim3(256), lds, s, i, o, m, modes, planes);
     else if (to_pm) hipLaunchKernelGGL(pm_transpose_scalar_kernel<true>, grid, dim3(256), lds, s, i, o, m, modes, planes);
     else hipLaunchKernelGGL(pm_transpose_scalar_kernel<false>, grid, dim3(256), lds, s, i, o, m, modes, planes);
+    check_hip(hipGetLastError(), "plane-major transpose launch");
+}
+
+void pm_transpose_batch(const PlaneTables& t, const BatchInputs& in, int count, void* out, size_t out_stride,
+                        int planes, hipStream_t s) {
+    if (count < 1 || count > kBatchMax) fail(SKB_INVALID_ARGUMENT, "batch of evaluations out of range");
+    for (int e = 0; e < count; ++e)
+        if (reinterpret_cast<uintptr_t>(in.in[e]) % 16 != 0)
+            fail(SKB_INVALID_ARGUMENT, "batched evaluations need 16-byte aligned retained inputs");
+    if (reinterpret_cast<uintptr_t>(out) % 16 != 0 || (out_stride * sizeof(cx<float>)) % 16 != 0)
+        fail(SKB_LOGIC_ERROR, "batched plane-major blocks must be 16-byte aligned");
+    const dim3 grid((t.modes + 15) / 16, count);
+    const size_t lds = static_cast<size_t>(planes) * 17 * sizeof(cx<float>);
+    hipLaunchKernelGGL(pm_to_pm_batch_kernel, grid, dim3(256), lds, s, in, static_cast<cx<float>*>(out), out_stride,
+                       t.cmask.as<uint32_t>(), t.modes, planes);
     check_hip(hipGetLastError(), "plane-major transpose launch");
 }
 

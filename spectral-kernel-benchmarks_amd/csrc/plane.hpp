@@ -41,10 +41,20 @@ struct PlaneTables {
 void plane_inverse(const PlaneTables& t, const void* pm, void* real, int planes, hipStream_t s);
 void plane_forward(const PlaneTables& t, const void* real, void* pm, int planes, hipStream_t s);
 // Both transforms of each plane in one workgroup: pm_in -> real planes ->
-// pm_out (pm_out may equal pm_in).
-void plane_operator(const PlaneTables& t, const void* pm_in, void* pm_out, void* real, int planes, hipStream_t s);
+// pm_out (pm_out may equal pm_in), for `count` evaluations whose arrays sit
+// pm_stride complex / real_stride real elements apart.
+void plane_operator(const PlaneTables& t, const void* pm_in, void* pm_out, void* real, int planes, hipStream_t s,
+                    int count = 1, size_t pm_stride = 0, size_t real_stride = 0);
 // compact (plane + P*mode, the caller's values) -> chunked plane-major (slot
 // values) (to_pm) or back.
 void pm_transpose(const PlaneTables& t, bool to_pm, const void* in, void* out, int planes, hipStream_t s);
+// The compact inputs of the evaluations of one batched launch.
+struct BatchInputs {
+    const void* in[kBatchMax];
+};
+// compact -> chunked plane-major for `count` evaluations: in.in[e] -> out +
+// e * out_stride (complex elements); 16-byte aligned arrays.
+void pm_transpose_batch(const PlaneTables& t, const BatchInputs& in, int count, void* out, size_t out_stride,
+                        int planes, hipStream_t s);
 
 }  // namespace skb

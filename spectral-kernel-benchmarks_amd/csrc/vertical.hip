@@ -480,8 +480,17 @@ __device__ __forceinline__ uint32_t bf16_pair(float lo, float hi) {
     return __builtin_bit_cast(uint32_t, v);
 }
 
+// A batch of plane-major evaluations in one launch (PM): virtual tile v is
+// tile v % ntiles of evaluation v / ntiles, whose input block sits
+// in_stride complex elements further on and whose output is out[e].
+struct GemmBatch {
+    cx<float>* out[kBatchMax];
+    int count;
+    size_t in_stride;
+};
+
 template <int NK, int M, bool PM>
-__global__ __launch_bounds__(256) void grouped_gemm_split_kernel(const GemmArgs<float> g) {
+__global__ __launch_bounds__(256) void grouped_gemm_split_kernel(const GemmArgs<float> g, const GemmBatch bt) {
     using A4 = floatx4;
     constexpr int KS = (32 * NK + 1 + 3) / 4;  // FP32 copy: K rounded up to 4
     constexpr int KP = 4 * KS;
@@ -502,10 +511,23 @@ __global__ __launch_bounds__(256) void grouped_gemm_split_kernel(const GemmArgs<
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int j = lane & 15, q = lane >> 4;
     const int nwg = static_cast<int>(gridDim.x), wg = static_cast<int>(blockIdx.x);
-    const int t_begin = static_cast<int>((static_cast<long long>(g.ntiles) * wg) / nwg);
-    const int t_end = static_cast<int>((static_cast<long long>(g.ntiles) * (wg + 1)) / nwg);
+    const int total = PM ? g.ntiles * bt.count : g.ntiles;
+    const int t_begin = static_cast<int>((static_cast<long long>(total) * wg) / nwg);
+    const int t_end = static_cast<int>((static_cast<long long>(total) * (wg + 1)) / nwg);
     if (t_begin >= t_end) return;  // uniform across the workgroup
     const int in_step = g.in_ld;
+    // tile record of virtual tile v; PM: the evaluation rides in pad's upper
+    // bits (pad = field + 256 * evaluation)
+    auto record = [&](int v) {
+        if constexpr (PM) {
+            const int e = v / g.ntiles;
+            GemmTile r = tile_record(g.tiles, v - e * g.ntiles);
+            r.pad += e << 8;
+            return r;
+        } else {
+            return tile_record(g.tiles, v);
+        }
+    };
     // B staging: the first KB rows of each column as pairs of complex
     // elements (one 16-byte load, split into bf16 pairs written as 32-bit
     // words), NK pairs per thread; the last row k = KB as one 8-byte load by
@@ -520,8 +542,9 @@ __global__ __launch_bounds__(256) void grouped_gemm_split_kernel(const GemmArgs<
             // 16 lanes read one 128-byte run of consecutive modes per row (two
             // pieces when the tile's modes straddle two chunks)
             const uint32_t P = static_cast<uint32_t>(g.pm_planes);
-            const __amdgpu_buffer_rsrc_t r = buffer_rsrc(g.in, static_cast<uint32_t>(g.pm_np) * P * CB);
-            const uint32_t plane0 = static_cast<uint32_t>(g.pm_nz * t.pad);
+            const __amdgpu_buffer_rsrc_t r =
+                buffer_rsrc(g.in + bt.in_stride * (t.pad >> 8), static_cast<uint32_t>(g.pm_np) * P * CB);
+            const uint32_t plane0 = static_cast<uint32_t>(g.pm_nz * (t.pad & 255));
             auto at = [&](int c, int z) {
                 const uint32_t m = static_cast<uint32_t>(t.col_begin + c);
                 return (16u * P * (m >> 4) + 16u * (plane0 + static_cast<uint32_t>(z)) + (m & 15u)) * CB;
@@ -620,7 +643,7 @@ __global__ __launch_bounds__(256) void grouped_gemm_split_kernel(const GemmArgs<
     constexpr int DEPTH = 2;
     GemmTile rec[DEPTH];
 #pragma unroll
-    for (int i = 0; i < DEPTH; ++i) rec[i] = tile_record(g.tiles, t_begin + i < t_end ? t_begin + i : t_begin);
+    for (int i = 0; i < DEPTH; ++i) rec[i] = record(t_begin + i < t_end ? t_begin + i : t_begin);
     int group = rec[0].group;
     issue_group(group);
     BRegs pre[DEPTH];
@@ -713,8 +736,8 @@ __global__ __launch_bounds__(256) void grouped_gemm_split_kernel(const GemmArgs<
                 // results straight from the accumulators: lane (j, q) holds rows
                 // 4q .. 4q+3 of column j in each block, two 16-byte row pairs
                 const int ld = PM ? g.out_ld * g.F : g.out_ld;
-                cx<float>* obase = PM ? g.out + static_cast<size_t>(g.out_ld) *
-                                                    (cur.pad + static_cast<size_t>(g.F) * cur.col_begin)
+                cx<float>* obase = PM ? bt.out[cur.pad >> 8] + static_cast<size_t>(g.out_ld) *
+                                                                   ((cur.pad & 255) + static_cast<size_t>(g.F) * cur.col_begin)
                                       : g.out + static_cast<size_t>(ld) * cur.col_begin;
                 const __amdgpu_buffer_rsrc_t r =
                     buffer_rsrc(obase, static_cast<uint32_t>(ld * (cur.ncols - 1) + M) * CB);
@@ -745,13 +768,238 @@ __global__ __launch_bounds__(256) void grouped_gemm_split_kernel(const GemmArgs<
                     put_row();
                 }
                 if (tile + DEPTH < t_end) {  // refill the slot this tile's block came from
-                    rec[slot] = tile_record(g.tiles, tile + DEPTH);
+                    rec[slot] = record(tile + DEPTH);
                     load_b(pre[slot], rec[slot]);
                 }
                 lds_barrier();  // block of tile + 1 complete; this tile's block is no longer read
                 if constexpr (NB == 2) cb ^= 1;
             }
         }
+    }
+}
+
+// Batched plane-major split-BF16 product (FP32 square families, K = M =
+// 32*NK + 1): the tiles of many evaluations in one launch, one wavefront per
+// chunk range.  A chunk range is up to 16 modes of one 16-mode chunk and one
+// group; for one evaluation its F fields are one contiguous run of the modal
+// layout (j + M*(field + F*mode)).  Workgroup (b, y) takes block b (up to
+// kPmWaves ranges of ONE group, PmBlock) for the evaluations
+// [y*epw, (y+1)*epw): it stages the group's split operand in LDS once, then
+// every wavefront works through the (evaluation, field) tiles of its range
+// with no further barrier, the B block of the next tile in flight while the
+// current one is multiplied.  A B block is loaded straight into
+// MFMA operand order (lane (j, q) holds z = 32t + 8q + e of mode j; each load
+// instruction reads four whole 128-byte lines), split into bf16 hi/mid/lo in
+// registers, and gives all 16*MB MFMA rows, the FP32 last column and the
+// vector-ALU row M - 1, stored straight from the registers; the next tile's
+// block is in flight meanwhile.  Rows 0 .. 16*MB - 1 use the products, order
+// and rounding of grouped_gemm_split_kernel (equal bit for bit); row M - 1
+// sums in another order (equal to rounding).  Slot values of modes stored
+// conjugated (pm_cmask) are conjugated back on output.
+struct PmBlock {
+    int group;
+    int range_begin;  // index into the range list
+    int ranges;       // 1 .. kPmWaves
+    int pad;
+};
+
+__device__ __forceinline__ PmBlock pm_block_record(const PmBlock* blocks, int i) {
+    using ConstBlock = const __attribute__((address_space(4))) PmBlock;
+    const ConstBlock* b = (ConstBlock*)blocks + __builtin_amdgcn_readfirstlane(i);
+    return PmBlock{b->group, b->range_begin, b->ranges, b->pad};
+}
+
+// hi = RNE_bf16(x), mid = RNE_bf16(x - hi), lo = RNE_bf16(x - hi - mid) of
+// two values, packed as bf16 pairs (a in the low half).
+__device__ __forceinline__ void split3_pair(float a, float b, uint32_t& h, uint32_t& m, uint32_t& l) {
+    h = bf16_pair(a, b);
+    const float ra = a - __uint_as_float(h << 16), rb = b - __uint_as_float(h & 0xFFFF0000u);
+    m = bf16_pair(ra, rb);
+    l = bf16_pair(ra - __uint_as_float(m << 16), rb - __uint_as_float(m & 0xFFFF0000u));
+}
+
+constexpr int kPmWaves = 4;  // chunk ranges (wavefronts) per workgroup
+
+template <int NK, int M>
+__global__ __launch_bounds__(64 * kPmWaves, 2) void gemm_pm_range_kernel(const GemmArgs<float> g, const PmBlock* blocks,
+                                                                      const GemmBatch bt, int epw) {
+    constexpr int W = kPmWaves;
+    constexpr int MB = rows_mb(M), VR = rows_vr(M);
+    constexpr int KB = 32 * NK;  // k on the matrix cores; k = KB is the FP32 last column
+    constexpr int KS = (KB + 1 + 3) / 4, KP = 4 * KS;
+    constexpr int AF = MB * NK * 3 * 64;  // 16-byte A fragments of one group
+    constexpr uint32_t CB = sizeof(cx<float>);
+    static_assert(M == KB + 1 && 16 * MB + VR == M && VR <= 1, "square split family");
+    __shared__ __align__(16) bf16x8 sa[AF];
+    __shared__ float stail[16 * MB];
+    __shared__ float srow[VR > 0 ? KP : 1];
+    const int tid = static_cast<int>(threadIdx.x), lane = tid & 63, wave = tid >> 6;
+    const int j = lane & 15, q = lane >> 4;
+    const PmBlock blk = pm_block_record(blocks, static_cast<int>(blockIdx.x));
+    const int F = g.F;
+    const int e0 = static_cast<int>(blockIdx.y) * epw;
+    const int ne = bt.count - e0 < epw ? bt.count - e0 : epw;
+    const bool active = wave < blk.ranges;  // uniform per wavefront
+    const int n = active ? ne * F : 0;      // (evaluation, field) tiles of this wavefront
+
+    // 1. the group's operand (requested first: the barrier below waits for
+    //    it, and the B blocks requested after it stay in flight)
+    constexpr int PER = (AF + 64 * W - 1) / (64 * W);
+    bf16x8 av[PER];
+    float tv = 0.f, rv0 = 0.f, rv1 = 0.f;
+    {
+        const bf16x8* src = reinterpret_cast<const bf16x8*>(g.a3) + static_cast<size_t>(blk.group) * AF;
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int at = tid + 64 * W * i;
+            if (at < AF) av[i] = src[at];
+        }
+        if (tid < 16 * MB) tv = g.atail[static_cast<size_t>(blk.group) * 16 * MB + tid];
+        if constexpr (VR > 0) {
+            const float* ar = g.arow + static_cast<size_t>(blk.group) * KP;
+            if (tid < KP) rv0 = ar[tid];
+            if (tid + 64 * W < KP) rv1 = ar[tid + 64 * W];
+        }
+    }
+
+    // 2. the B blocks of this wavefront's first two tiles
+    const uint32_t P = static_cast<uint32_t>(g.pm_planes);
+    const GemmTile rng = tile_record(g.tiles, blk.range_begin + (active ? wave : 0));
+    const uint32_t m = static_cast<uint32_t>(rng.col_begin + j);
+    const bool jok = j < rng.ncols;
+    struct BBlock {
+        cx<float> v[NK][8];
+        cx<float> t;  // k = KB
+    };
+    auto load_tile = [&](int i, BBlock& b) {
+        const bool on = jok && i < n;
+        const int e = e0 + i / F, f = i - (i / F) * F;
+        const __amdgpu_buffer_rsrc_t rin =
+            buffer_rsrc(g.in + bt.in_stride * (on ? e : 0), static_cast<uint32_t>(g.pm_np) * P * CB);
+        const uint32_t col = 16u * P * (m >> 4) + 16u * static_cast<uint32_t>(g.pm_nz * f) + (m & 15u);
+#pragma unroll
+        for (int t = 0; t < NK; ++t)
+#pragma unroll
+            for (int e8 = 0; e8 < 8; ++e8) {
+                const uint32_t z = static_cast<uint32_t>(32 * t + 8 * q + e8);
+                b.v[t][e8] = buffer_load_cx<float>(rin, on ? (col + 16u * z) * CB : kBufferPast);
+            }
+        b.t = buffer_load_cx<float>(rin, on ? (col + 16u * KB) * CB : kBufferPast);
+    };
+    BBlock cur, nxt;
+    load_tile(0, cur);
+
+    // 3. stage the operand
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int at = tid + 64 * W * i;
+        if (at < AF) sa[at] = av[i];
+    }
+    if (tid < 16 * MB) stail[tid] = tv;
+    if constexpr (VR > 0) {
+        if (tid < KP) srow[tid] = rv0;
+        if (tid + 64 * W < KP) srow[tid + 64 * W] = rv1;
+    }
+    lds_barrier();
+    if (!active) return;
+
+    // conjugation back to the caller's values: bit j of this range's modes
+    using ConstWord = const __attribute__((address_space(4))) uint32_t;
+    const uint32_t cword = *((ConstWord*)g.pm_cmask + (rng.col_begin >> 4));
+    const bool cj = (cword >> ((rng.col_begin & 15) + j)) & 1u;
+
+    // tile i of this wavefront from block b
+    auto process = [&](int i, const BBlock& b) {
+        const int e = e0 + i / F, f = i - (i / F) * F;
+        // the range's run of evaluation e: mode j, field f, row r at (F j + f) M + r
+        cx<float>* run = bt.out[e] + static_cast<size_t>(g.out_ld) * F * rng.col_begin;
+        const __amdgpu_buffer_rsrc_t rout = buffer_rsrc(run, static_cast<uint32_t>(rng.ncols * F * M) * CB);
+        const uint32_t ocol = static_cast<uint32_t>((F * j + f) * M) * CB;
+        // row M - 1 on the vector ALU: this lane's k = 32 t + 8 q + e, then
+        // the sum over q
+        if constexpr (VR > 0) {
+            cx<float> v{0.f, 0.f};
+#pragma unroll
+            for (int t = 0; t < NK; ++t)
+#pragma unroll
+                for (int e8 = 0; e8 < 8; ++e8) {
+                    const float a = srow[32 * t + 8 * q + e8];
+                    v.x += a * b.v[t][e8].x;
+                    v.y += a * b.v[t][e8].y;
+                }
+            if (q == 0) {
+                v.x += srow[KB] * b.t.x;
+                v.y += srow[KB] * b.t.y;
+            }
+            v.x += __shfl_xor(v.x, 16);
+            v.y += __shfl_xor(v.y, 16);
+            v.x += __shfl_xor(v.x, 32);
+            v.y += __shfl_xor(v.y, 32);
+            if (cj) v.y = -v.y;
+            buffer_store_cx<float>(v, rout, q == 0 && jok ? ocol + static_cast<uint32_t>(16 * MB) * CB : kBufferPast);
+        }
+        // rows 16 mb + 4 q + e of column j on the matrix cores: for each k
+        // block t, B split into bf16 planes [re/im][hi, mid, lo], then the
+        // products of every row block
+        floatx4 r[MB], im[MB];
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) r[mb] = im[mb] = floatx4{0, 0, 0, 0};
+#pragma unroll
+        for (int t = 0; t < NK; ++t) {
+            bf16x8 bb[2][3];
+            {
+                uint32_t w[2][3][4];
+#pragma unroll
+                for (int e4 = 0; e4 < 4; ++e4) {
+                    split3_pair(b.v[t][2 * e4].x, b.v[t][2 * e4 + 1].x, w[0][0][e4], w[0][1][e4], w[0][2][e4]);
+                    split3_pair(b.v[t][2 * e4].y, b.v[t][2 * e4 + 1].y, w[1][0][e4], w[1][1][e4], w[1][2][e4]);
+                }
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+#pragma unroll
+                    for (int p = 0; p < 3; ++p) {
+                        typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+                        const u32x4 u{w[c][p][0], w[c][p][1], w[c][p][2], w[c][p][3]};
+                        bb[c][p] = __builtin_bit_cast(bf16x8, u);
+                    }
+            }
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) {
+                const bf16x8* A = &sa[((mb * NK + t) * 3) * 64 + lane];
+                const bf16x8 a0 = A[0], a1 = A[64], a2 = A[128];
+                r[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, bb[0][0], r[mb], 0, 0, 0);
+                im[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, bb[1][0], im[mb], 0, 0, 0);
+                r[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bb[0][1], r[mb], 0, 0, 0);
+                im[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bb[1][1], im[mb], 0, 0, 0);
+                r[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bb[0][2], r[mb], 0, 0, 0);
+                im[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bb[1][2], im[mb], 0, 0, 0);
+                r[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bb[0][0], r[mb], 0, 0, 0);
+                im[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bb[1][0], im[mb], 0, 0, 0);
+                r[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bb[0][1], r[mb], 0, 0, 0);
+                im[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bb[1][1], im[mb], 0, 0, 0);
+                r[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bb[0][0], r[mb], 0, 0, 0);
+                im[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bb[1][0], im[mb], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) {
+#pragma unroll
+            for (int e4 = 0; e4 < 4; ++e4) {
+                const float at = stail[16 * mb + 4 * q + e4];
+                r[mb][e4] += at * b.t.x;
+                im[mb][e4] += at * b.t.y;
+            }
+            if (cj) im[mb] = -im[mb];
+            const uint32_t at = jok ? ocol + static_cast<uint32_t>(16 * mb + 4 * q) * CB : kBufferPast;
+            raw_buffer_store_f4(bf4{r[mb][0], im[mb][0], r[mb][1], im[mb][1]}, rout, static_cast<int>(at), 0, 0);
+            raw_buffer_store_f4(bf4{r[mb][2], im[mb][2], r[mb][3], im[mb][3]}, rout,
+                                static_cast<int>(jok ? at + 2 * CB : kBufferPast), 0, 0);
+        }
+    };
+    for (int i = 0; i < n; ++i) {  // uniform per wavefront
+        if (i + 1 < n) load_tile(i + 1, nxt);  // in flight while tile i is multiplied
+        process(i, cur);
+        cur = nxt;
     }
 }
 
@@ -1031,6 +1279,9 @@ namespace {
 // Workgroups of the fixed-height streaming kernel when the plan does not set
 // a count: three per CU of the 256-CU part.
 constexpr int kStreamGrid = 768;
+// Evaluations one wavefront of the batched plane-major product takes (its
+// chunk range's tiles of that many evaluations, two always in flight).
+constexpr int kPmEvalsPerWave = 1;
 
 template <typename R, int KS, int M>
 void launch_gemm_fixed(const GemmArgs<R>& a, int req, hipStream_t s) {
@@ -1043,9 +1294,13 @@ void launch_gemm_fixed(const GemmArgs<R>& a, int req, hipStream_t s) {
 // families at Nz = 33, 65, 129 (M = K = Nz for square families, Nj x Nz
 // forward and Nz x Nj inverse for the truncated ones); FP64 up to Nz = 65.
 template <int NK, int M, bool PM = false>
-void launch_gemm_split(const GemmArgs<float>& a, int req, hipStream_t s) {
-    const int grid = std::min(a.ntiles, req > 0 ? req : kStreamGrid);
-    hipLaunchKernelGGL((grouped_gemm_split_kernel<NK, M, PM>), dim3(grid), dim3(256), 0, s, a);
+void launch_gemm_split(const GemmArgs<float>& a, int req, hipStream_t s, const GemmBatch* batch = nullptr) {
+    GemmBatch b{};
+    b.count = 1;
+    b.out[0] = a.out;
+    if (batch) b = *batch;
+    const int grid = std::min(a.ntiles * b.count, req > 0 ? req : kStreamGrid);
+    hipLaunchKernelGGL((grouped_gemm_split_kernel<NK, M, PM>), dim3(grid), dim3(256), 0, s, a, b);
     check_hip(hipGetLastError(), "vertical GEMM launch");
 }
 
@@ -1230,6 +1485,10 @@ struct skb_vertical_plan {
     // and one field (GemmTile::pad = field), group by group.
     int pm_ntiles = 0;
     skb::DeviceBuffer pm_tiles;
+    // batched plane-major product (Nz 33 and 65): the chunk ranges (group,
+    // first mode, modes) and the workgroup blocks over them
+    int pm_nranges = 0, pm_nblocks = 0;
+    skb::DeviceBuffer pm_ranges, pm_blocks;
     // interleaved staging of the split-layout entry points (reserved by
     // skb_vertical_plan_reserve_split): physical-size and modal-size arrays
     skb::DeviceBuffer split_physical, split_modal;
@@ -1311,6 +1570,26 @@ std::unique_ptr<skb_vertical_plan> make_plan(const skb_workload* workload, uint6
         }
     p->pm_ntiles = static_cast<int>(pm.size());
     p->pm_tiles.upload(pm.data(), pm.size() * sizeof(GemmTile));
+    if (precision == SKB_F32 && nj == nz && (nz == 33 || nz == 65)) {
+        std::vector<GemmTile> ranges;
+        for (uint64_t g = 0; g < group_count; ++g) {
+            const uint64_t first = groups[g].first_mode, end = first + groups[g].mode_count;
+            for (uint64_t m = first; m < end; m = (m / 16 + 1) * 16)
+                ranges.push_back(GemmTile{static_cast<int>(g), static_cast<int>(m),
+                                          static_cast<int>(std::min<uint64_t>((m / 16 + 1) * 16, end) - m), 0});
+        }
+        std::vector<PmBlock> blocks;
+        for (int r = 0; r < static_cast<int>(ranges.size());) {
+            int k = 1;
+            while (k < kPmWaves && r + k < static_cast<int>(ranges.size()) && ranges[r + k].group == ranges[r].group) ++k;
+            blocks.push_back(PmBlock{ranges[r].group, r, k, 0});
+            r += k;
+        }
+        p->pm_nranges = static_cast<int>(ranges.size());
+        p->pm_ranges.upload(ranges.data(), ranges.size() * sizeof(GemmTile));
+        p->pm_nblocks = static_cast<int>(blocks.size());
+        p->pm_blocks.upload(blocks.data(), blocks.size() * sizeof(PmBlock));
+    }
     p->matrix_bytes = nj * nz * group_count * (precision == SKB_F64 ? 8 : 4);
     return p;
 }
@@ -1395,22 +1674,52 @@ bool vertical_pm_supported(const skb_vertical_plan* p) {
 
 void vertical_forward_pm(const skb_vertical_plan* p, const void* pm, int np, const uint32_t* cmask, void* out,
                          void* stream) {
+    // one evaluation of the batched product: the same kernels, so a batched
+    // evaluation equals the evaluation run alone bit for bit
+    BatchOutputs o{};
+    o.out[0] = out;
+    vertical_forward_pm_batch(p, pm, 0, np, cmask, o, 1, stream);
+}
+
+void vertical_forward_pm_batch(const skb_vertical_plan* p, const void* pm, size_t pm_stride, int np,
+                               const uint32_t* cmask, const BatchOutputs& out, int count, void* stream) {
     if (!vertical_pm_supported(p)) fail(SKB_UNSUPPORTED, "plane-major vertical product: unsupported plan");
-    if (!pm || !out || np < static_cast<int>(p->modes)) fail(SKB_INVALID_ARGUMENT, "bad plane-major operand");
-    auto a = args<float>(p, p->fwd, pm, out);
+    if (count < 1 || count > kBatchMax) fail(SKB_INVALID_ARGUMENT, "batch of evaluations out of range");
+    if (!pm || !cmask || np < static_cast<int>(p->modes) || np % 16 != 0)
+        fail(SKB_INVALID_ARGUMENT, "bad plane-major operand");
+    if (static_cast<long long>(p->pm_ntiles) * count >= (1ll << 31) || p->w.fields > 255)
+        fail(SKB_UNSUPPORTED, "plane-major vertical product: batch too large");
+    GemmBatch b{};
+    b.count = count;
+    b.in_stride = pm_stride;
+    for (int e = 0; e < count; ++e) {
+        if (!out.out[e]) fail(SKB_INVALID_ARGUMENT, "null modal output");
+        b.out[e] = static_cast<cx<float>*>(out.out[e]);
+    }
+    auto a = args<float>(p, p->fwd, pm, b.out[0]);
     a.tiles = p->pm_tiles.as<GemmTile>();
     a.ntiles = p->pm_ntiles;
     a.pm_np = np;
     a.pm_nz = static_cast<int>(p->w.nz);
     a.pm_planes = static_cast<int>(p->w.nz * p->w.fields);
     a.pm_cmask = cmask;
-    if (!cmask) fail(SKB_INVALID_ARGUMENT, "plane-major vertical product: null conj mask");
-    if (np % 16 != 0) fail(SKB_INVALID_ARGUMENT, "the chunked plane-major layout holds whole chunks of 16 modes");
     const hipStream_t s = as_stream(stream);
+    if (p->pm_nblocks > 0 && p->grid == 0) {
+        // one wavefront per chunk range, kPmEvalsPerWave evaluations each
+        a.tiles = p->pm_ranges.as<GemmTile>();
+        const int epw = std::min(count, kPmEvalsPerWave);
+        const dim3 grid(p->pm_nblocks, (count + epw - 1) / epw), block(64 * kPmWaves);
+        if (a.K == 65)
+            hipLaunchKernelGGL((gemm_pm_range_kernel<2, 65>), grid, block, 0, s, a, p->pm_blocks.as<PmBlock>(), b, epw);
+        else
+            hipLaunchKernelGGL((gemm_pm_range_kernel<1, 33>), grid, block, 0, s, a, p->pm_blocks.as<PmBlock>(), b, epw);
+        check_hip(hipGetLastError(), "vertical GEMM launch");
+        return;
+    }
     switch (a.K) {
-        case 65: launch_gemm_split<2, 65, true>(a, p->grid, s); break;
-        case 33: launch_gemm_split<1, 33, true>(a, p->grid, s); break;
-        default: launch_gemm_split<4, 129, true>(a, p->grid, s); break;
+        case 65: launch_gemm_split<2, 65, true>(a, p->grid, s, &b); break;
+        case 33: launch_gemm_split<1, 33, true>(a, p->grid, s, &b); break;
+        default: launch_gemm_split<4, 129, true>(a, p->grid, s, &b); break;
     }
 }
 

@@ -192,6 +192,8 @@ def _load():
         "skb_retained_operator_reserve_lanes": (i32, [vp, i32]),
         "skb_retained_operator_lanes": (i32, [vp, ctypes.POINTER(i32)]),
         "skb_retained_operator_execute_batch": (i32, [vp, i32, vp, vp, vp]),
+        "skb_retained_operator_reserve_batch": (i32, [vp, i32]),
+        "skb_retained_operator_batch": (i32, [vp]),
         "skb_retained_operator_set_path": (i32, [vp, i32]),
         "skb_retained_operator_path": (i32, [vp]),
         "skb_retained_operator_real": (vp, [vp]),
@@ -247,6 +249,7 @@ EXPORTED = [
     "skb_retained_operator_create", "skb_retained_operator_destroy", "skb_retained_operator_execute",
     "skb_retained_operator_stage", "skb_retained_operator_set_path", "skb_retained_operator_path",
     "skb_retained_operator_reserve_lanes", "skb_retained_operator_lanes", "skb_retained_operator_execute_batch",
+    "skb_retained_operator_reserve_batch", "skb_retained_operator_batch",
     "skb_retained_operator_real", "skb_retained_operator_device_bytes", "skb_retained_operator_set_fused",
     "skb_retained_operator_fused", "skb_interleave", "skb_deinterleave", "skb_scale_split",
     "skb_vertical_plan_reserve_split", "skb_vertical_forward_split", "skb_vertical_inverse_split",
@@ -886,6 +889,13 @@ class OperatorPlan:
     def fused(self) -> bool:
         return bool(lib.skb_retained_operator_fused(self.h))
 
+    def reserve_batch(self, evaluations: int):
+        """Stage-batched execute_batch over groups of up to `evaluations` (0 or 1 releases)."""
+        check(lib.skb_retained_operator_reserve_batch(self.h, int(evaluations)))
+
+    def batch(self) -> int:
+        return int(lib.skb_retained_operator_batch(self.h))
+
     def reserve_lanes(self, lanes: int):
         """Lanes of the batched execute (own streams and intermediates; 1 releases them)."""
         check(lib.skb_retained_operator_reserve_lanes(self.h, int(lanes)))
@@ -909,7 +919,7 @@ class OperatorPlan:
         return n, ra, ma
 
     def execute_batch(self, retained, modal, stream=None):
-        """Independent evaluations retained[i] -> modal[i], spread over the lanes."""
+        """Independent evaluations retained[i] -> modal[i]: stage-batched groups (reserve_batch) or lanes."""
         n, ra, ma = self._batch_args(retained, modal)
         check(lib.skb_retained_operator_execute_batch(self.h, n, ra, ma, _stream(stream)))
 

@@ -120,3 +120,63 @@ def test_batch_allocates_nothing_and_matches_fp64(device):
         op.execute_batch(xs[:2], [outs[0], outs[0]])
     with pytest.raises(S.SkbError):
         op.reserve_lanes(0)
+
+
+@pytest.mark.parametrize("nz,cap", [(65, 4), (65, 32), (33, 8)])
+def test_stage_batch_equals_alone_bit_for_bit(device, nz, cap):
+    """reserve_batch: every stage one launch over a group of evaluations."""
+    import torch
+
+    note("operator stage batch nz=%d cap=%d" % (nz, cap))
+    w = S.Workload(256, 256, nz, 3)
+    op, *_, xs = make(w, S.SKB_F32, device, 9)
+    ref = alone(op, xs, device)
+    op.reserve_batch(cap)
+    assert op.batch() == cap
+    for count in (1, 3, cap, 9):
+        if count > 9:
+            continue
+        outs = [torch.full((op.modal_elements,), float("nan"), dtype=torch.complex64, device=device)
+                for _ in range(count)]
+        op.execute_batch(xs[:count], outs)
+        torch.cuda.synchronize()
+        for k in range(count):
+            assert torch.equal(outs[k], ref[k]), (count, k)
+    # reversed order, several groups when cap < 9: the intermediates carry nothing over
+    outs = [torch.empty(op.modal_elements, dtype=torch.complex64, device=device) for _ in range(9)]
+    op.execute_batch(xs[::-1], outs)
+    torch.cuda.synchronize()
+    for k in range(9):
+        assert torch.equal(outs[k], ref[8 - k]), k
+    # steady state allocates nothing
+    call = op.bind_execute_batch(xs, outs)
+    call()
+    torch.cuda.synchronize()
+    before = S.device_allocations()
+    for _ in range(3):
+        call()
+    torch.cuda.synchronize()
+    assert S.device_allocations() == before
+    # the unfused transforms take the lane / in-order path, same results
+    op.set_fused(False)
+    outs = [torch.empty(op.modal_elements, dtype=torch.complex64, device=device) for _ in range(3)]
+    op.execute_batch(xs[:3], outs)
+    torch.cuda.synchronize()
+    for k in range(3):
+        assert torch.equal(outs[k], ref[k]), k
+    op.reserve_batch(0)
+    assert op.batch() == 0
+
+
+def test_stage_batch_reservation_limits(device):
+    note("operator stage batch limits")
+    w = S.Workload(256, 256, 65, 3)
+    op, modes, groups, f, i, xs = make(w, S.SKB_F32, device, 1)
+    with pytest.raises(ValueError):
+        op.reserve_batch(33)
+    with pytest.raises(ValueError):
+        op.reserve_batch(-1)
+    op64 = S.OperatorPlan(w, modes, w.nz, groups, f, i, S.SKB_F64)
+    with pytest.raises(S.Unsupported):
+        op64.reserve_batch(4)
+    assert op64.batch() == 0
