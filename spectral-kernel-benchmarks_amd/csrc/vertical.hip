@@ -500,6 +500,7 @@ __global__ __launch_bounds__(256) void grouped_gemm_split_kernel(const GemmArgs<
     constexpr int BS = KB + 8;       // bf16 plane column stride (16-byte reads spread over banks)
     constexpr uint32_t CB = sizeof(cx<float>);
     static_assert(16 * MB <= M, "every MFMA row exists");
+    static_assert(VR <= 1, "one vector-ALU row");
     // B blocks, double-buffered where LDS allows three workgroups per CU (the
     // next tile's block is staged while the current one is multiplied, one
     // barrier per tile); the largest height keeps one buffer and a barrier
@@ -710,10 +711,22 @@ __global__ __launch_bounds__(256) void grouped_gemm_split_kernel(const GemmArgs<
             }
             cx<float> vrow{0.f, 0.f};
             if (VR > 0 && wave < VR) {
+                // row 16 MB (VR = 1 for these shapes): lane q sums k = 32 t +
+                // 8 q + e (and k = KB on q = 0), then the sum over q -- the
+                // order of gemm_pm_range_kernel, which gives the same bits
 #pragma unroll
-                for (int s = 0; s < KS; ++s) {
-                    const float av = sar[wave * KP + q * KS + s];
-                    const cx<float> b = sb[j * CS + q * KS + s];
+                for (int t = 0; t < NK; ++t)
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const int k = 32 * t + 8 * q + e;
+                        const float av = sar[k];
+                        const cx<float> b = sb[j * CS + k];
+                        vrow.x += av * b.x;
+                        vrow.y += av * b.y;
+                    }
+                if (q == 0) {
+                    const float av = sar[KB];
+                    const cx<float> b = sb[j * CS + KB];
                     vrow.x += av * b.x;
                     vrow.y += av * b.y;
                 }
@@ -783,10 +796,10 @@ __global__ __launch_bounds__(256) void grouped_gemm_split_kernel(const GemmArgs<
 // chunk range.  A chunk range is up to 16 modes of one 16-mode chunk and one
 // group; for one evaluation its F fields are one contiguous run of the modal
 // layout (j + M*(field + F*mode)).  Workgroup (b, y) takes block b (up to
-// kPmWaves ranges of ONE group, PmBlock) for the evaluations
-// [y*epw, (y+1)*epw): it stages the group's split operand in LDS once, then
-// every wavefront works through the (evaluation, field) tiles of its range
-// with no further barrier, the B block of the next tile in flight while the
+// kPmWaves ranges of ONE group, PmBlock) for the (evaluation, field) tiles
+// [y*tpw, (y+1)*tpw) of each range, in the order e*F + f: it stages the
+// group's split operand in LDS once, then every wavefront works through
+// those tiles of its range with no further barrier, the B block of the next tile in flight while the
 // current one is multiplied.  A B block is loaded straight into
 // MFMA operand order (lane (j, q) holds z = 32t + 8q + e of mode j; each load
 // instruction reads four whole 128-byte lines), split into bf16 hi/mid/lo in
@@ -822,7 +835,7 @@ constexpr int kPmWaves = 4;  // chunk ranges (wavefronts) per workgroup
 
 template <int NK, int M>
 __global__ __launch_bounds__(64 * kPmWaves, 2) void gemm_pm_range_kernel(const GemmArgs<float> g, const PmBlock* blocks,
-                                                                      const GemmBatch bt, int epw) {
+                                                                      const GemmBatch bt, int tpw) {
     constexpr int W = kPmWaves;
     constexpr int MB = rows_mb(M), VR = rows_vr(M);
     constexpr int KB = 32 * NK;  // k on the matrix cores; k = KB is the FP32 last column
@@ -837,10 +850,10 @@ __global__ __launch_bounds__(64 * kPmWaves, 2) void gemm_pm_range_kernel(const G
     const int j = lane & 15, q = lane >> 4;
     const PmBlock blk = pm_block_record(blocks, static_cast<int>(blockIdx.x));
     const int F = g.F;
-    const int e0 = static_cast<int>(blockIdx.y) * epw;
-    const int ne = bt.count - e0 < epw ? bt.count - e0 : epw;
+    const int i0 = static_cast<int>(blockIdx.y) * tpw;  // first tile e*F + f of this workgroup
+    const int all = bt.count * F;
     const bool active = wave < blk.ranges;  // uniform per wavefront
-    const int n = active ? ne * F : 0;      // (evaluation, field) tiles of this wavefront
+    const int n = active ? (all - i0 < tpw ? all - i0 : tpw) : 0;  // tiles of this wavefront
 
     // 1. the group's operand (requested first: the barrier below waits for
     //    it, and the B blocks requested after it stay in flight)
@@ -873,9 +886,10 @@ __global__ __launch_bounds__(64 * kPmWaves, 2) void gemm_pm_range_kernel(const G
     };
     auto load_tile = [&](int i, BBlock& b) {
         const bool on = jok && i < n;
-        const int e = e0 + i / F, f = i - (i / F) * F;
-        const __amdgpu_buffer_rsrc_t rin =
-            buffer_rsrc(g.in + bt.in_stride * (on ? e : 0), static_cast<uint32_t>(g.pm_np) * P * CB);
+        // the resource must be wave-uniform: evaluation from the uniform i only
+        const int iu = i < n ? i : 0;
+        const int e = (i0 + iu) / F, f = i0 + iu - e * F;
+        const __amdgpu_buffer_rsrc_t rin = buffer_rsrc(g.in + bt.in_stride * e, static_cast<uint32_t>(g.pm_np) * P * CB);
         const uint32_t col = 16u * P * (m >> 4) + 16u * static_cast<uint32_t>(g.pm_nz * f) + (m & 15u);
 #pragma unroll
         for (int t = 0; t < NK; ++t)
@@ -910,7 +924,7 @@ __global__ __launch_bounds__(64 * kPmWaves, 2) void gemm_pm_range_kernel(const G
 
     // tile i of this wavefront from block b
     auto process = [&](int i, const BBlock& b) {
-        const int e = e0 + i / F, f = i - (i / F) * F;
+        const int e = (i0 + i) / F, f = i0 + i - e * F;
         // the range's run of evaluation e: mode j, field f, row r at (F j + f) M + r
         cx<float>* run = bt.out[e] + static_cast<size_t>(g.out_ld) * F * rng.col_begin;
         const __amdgpu_buffer_rsrc_t rout = buffer_rsrc(run, static_cast<uint32_t>(rng.ncols * F * M) * CB);
@@ -1674,11 +1688,24 @@ bool vertical_pm_supported(const skb_vertical_plan* p) {
 
 void vertical_forward_pm(const skb_vertical_plan* p, const void* pm, int np, const uint32_t* cmask, void* out,
                          void* stream) {
-    // one evaluation of the batched product: the same kernels, so a batched
-    // evaluation equals the evaluation run alone bit for bit
-    BatchOutputs o{};
-    o.out[0] = out;
-    vertical_forward_pm_batch(p, pm, 0, np, cmask, o, 1, stream);
+    // one evaluation: the persistent split kernel (its row 16 MB sums in the
+    // batched kernel's order, so a batched evaluation equals this bit for bit)
+    if (!vertical_pm_supported(p)) fail(SKB_UNSUPPORTED, "plane-major vertical product: unsupported plan");
+    if (!pm || !out || !cmask || np < static_cast<int>(p->modes) || np % 16 != 0)
+        fail(SKB_INVALID_ARGUMENT, "bad plane-major operand");
+    auto a = args<float>(p, p->fwd, pm, out);
+    a.tiles = p->pm_tiles.as<GemmTile>();
+    a.ntiles = p->pm_ntiles;
+    a.pm_np = np;
+    a.pm_nz = static_cast<int>(p->w.nz);
+    a.pm_planes = static_cast<int>(p->w.nz * p->w.fields);
+    a.pm_cmask = cmask;
+    const hipStream_t s = as_stream(stream);
+    switch (a.K) {
+        case 65: launch_gemm_split<2, 65, true>(a, p->grid, s); break;
+        case 33: launch_gemm_split<1, 33, true>(a, p->grid, s); break;
+        default: launch_gemm_split<4, 129, true>(a, p->grid, s); break;
+    }
 }
 
 void vertical_forward_pm_batch(const skb_vertical_plan* p, const void* pm, size_t pm_stride, int np,
@@ -1707,12 +1734,13 @@ void vertical_forward_pm_batch(const skb_vertical_plan* p, const void* pm, size_
     if (p->pm_nblocks > 0 && p->grid == 0) {
         // one wavefront per chunk range, kPmEvalsPerWave evaluations each
         a.tiles = p->pm_ranges.as<GemmTile>();
-        const int epw = std::min(count, kPmEvalsPerWave);
-        const dim3 grid(p->pm_nblocks, (count + epw - 1) / epw), block(64 * kPmWaves);
+        // tiles per wavefront: the fields of kPmEvalsPerWave evaluations
+        const int F = static_cast<int>(p->w.fields), tpw = kPmEvalsPerWave * F;
+        const dim3 grid(p->pm_nblocks, (count * F + tpw - 1) / tpw), block(64 * kPmWaves);
         if (a.K == 65)
-            hipLaunchKernelGGL((gemm_pm_range_kernel<2, 65>), grid, block, 0, s, a, p->pm_blocks.as<PmBlock>(), b, epw);
+            hipLaunchKernelGGL((gemm_pm_range_kernel<2, 65>), grid, block, 0, s, a, p->pm_blocks.as<PmBlock>(), b, tpw);
         else
-            hipLaunchKernelGGL((gemm_pm_range_kernel<1, 33>), grid, block, 0, s, a, p->pm_blocks.as<PmBlock>(), b, epw);
+            hipLaunchKernelGGL((gemm_pm_range_kernel<1, 33>), grid, block, 0, s, a, p->pm_blocks.as<PmBlock>(), b, tpw);
         check_hip(hipGetLastError(), "vertical GEMM launch");
         return;
     }
