@@ -75,11 +75,12 @@ def parse():
 
 
 COPY_BYTES = 256 << 20
-KERNEL_NAMES = {  # substring of the device kernel name -> stage of the evaluation
-    "pm_transpose_kernel<true>": "layout",
+KERNEL_NAMES = {  # substring of the device kernel name -> stage of the (stage-batched) evaluation
+    "pm_to_pm_batch_kernel": "layout",
     "plane_operator_kernel<86>": "transforms",
-    "grouped_gemm_split_kernel<2, 65, true>": "vertical_gemm",
+    "gemm_pm_range_kernel<2, 65>": "vertical_gemm",
 }
+BATCH_STAGES = (("layout", 0), ("transforms", 4), ("vertical_gemm", 3))  # SKB_OP_STAGE_* of each
 
 
 def summarize_pmc(fetch_csv, write_csv, family):
@@ -199,6 +200,29 @@ def per_stage_times(torch, ops, launches, block=50):
             a.record()
             for i in range(block):
                 fns[(b * block + i) % len(fns)]()
+            e.record()
+            blocks.append((a, e))
+        torch.cuda.synchronize()
+        out[name] = [a.elapsed_time(e) / block for a, e in blocks]
+    return out
+
+
+def batched_stage_times(torch, op, xs, outs, calls, block=10):
+    """Average duration of one call of each stage of the stage-batched
+    execute over the batch (xs -> outs: one launch per group of evaluations),
+    HIP events around blocks of `block` back-to-back calls on the plan's
+    stream.  The inputs and outputs are the headline's (distinct arrays), so
+    the working set is HBM-resident."""
+    out = {}
+    for name, stage in BATCH_STAGES:
+        fn = op.bind_execute_batch_stage(stage, xs, outs)
+        fn()
+        blocks = []
+        for _ in range(max(1, calls // block)):
+            a, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(block):
+                fn()
             e.record()
             blocks.append((a, e))
         torch.cuda.synchronize()
@@ -559,13 +583,18 @@ def run(args, np, torch, S, device):
     log(args, "fp32 vs fp64 max-scale err %.3e, rel L2 %.3e" % (err_max, err_l2))
 
     if args.profile_run:
-        # the launches the roofline is timed on: every stage in blocks of 50
-        # launches rotating over 8 plans (HBM-resident working set)
+        # the launches the roofline is timed on: every stage of the
+        # stage-batched execute over the headline's batch, in blocks
+        B = args.batch
+        f32.op.reserve_batch(min(B, 32))
+        xs = [f32.x * (1.0 + 1e-3 * k) for k in range(B)]
+        outs = [torch.empty_like(f32.modal) for _ in range(B)]
+        batch_call = f32.op.bind_execute_batch(xs, outs)
         for _ in range(args.warmup):
-            f32.step()
-        rot = [f32] + [Operator(S, torch, w, modes, groups, nj, fwd, inv, x64, S.SKB_F32, device) for _ in range(7)]
-        per_stage_times(torch, rot, max(200, args.steps))
-        del rot
+            batch_call()
+        batched_stage_times(torch, f32.op, xs, outs, max(20, args.steps))
+        del xs, outs, batch_call
+        f32.op.reserve_batch(0)
         src = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device=device).fill_(1.0)
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
@@ -623,6 +652,10 @@ def run(args, np, torch, S, device):
         batch_call()
     host_batch_us = 1e6 * (time.perf_counter() - t0) / (10 * B)
     torch.cuda.synchronize()
+    batched_stages = None
+    if args.batch_mode == "stage":
+        log(args, "per-stage times of the stage-batched execute")
+        batched_stages = batched_stage_times(torch, f32.op, xs, outs, max(20, args.steps))
     del xs, outs, batch_call
     f32.op.reserve_batch(0)
     f32.op.reserve_lanes(1)
@@ -670,13 +703,28 @@ def run(args, np, torch, S, device):
         "vertical_gemm": ret_b + modal_b + mat_b,
     }
     eval_bytes = ret_b + real_b + real_b + ret_b + ret_b + modal_b + mat_b  # the yardstick
-    kernels = {}
+    kernels = {}  # one evaluation per launch (single-call form)
     for n, ms in per_stage.items():
         avg = sum(ms) / len(ms)
         res = sum(per_stage_resident[n]) / len(per_stage_resident[n])
         kernels[n] = {"avg_us": 1e3 * avg, "algorithmic_bytes": algo[n], "GBps": algo[n] / (avg * 1e-3) / 1e9,
                       "resident_avg_us": 1e3 * res, "in_evaluation": n not in ("inverse", "forward")}
-    dominant = max((n for n in kernels if kernels[n]["in_evaluation"]), key=lambda n: kernels[n]["avg_us"])
+    kernels_batched = None
+    if batched_stages:
+        # one launch per group of `cap` evaluations; a call holds B / cap launches
+        cap = min(B, 32)
+        launches = (B + cap - 1) // cap
+        kernels_batched = {}
+        for n, ms in batched_stages.items():
+            call_ms = sum(ms) / len(ms)
+            kernels_batched[n] = {"evaluations_per_launch": cap, "avg_us_per_launch": 1e3 * call_ms / launches,
+                                  "us_per_evaluation": 1e3 * call_ms / B,
+                                  "algorithmic_bytes_per_launch": algo[n] * cap,
+                                  "GBps": algo[n] * B / (call_ms * 1e-3) / 1e9}
+    if kernels_batched:
+        dominant = max(kernels_batched, key=lambda n: kernels_batched[n]["us_per_evaluation"])
+    else:
+        dominant = max((n for n in kernels if kernels[n]["in_evaluation"]), key=lambda n: kernels[n]["avg_us"])
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         try:
@@ -685,18 +733,34 @@ def run(args, np, torch, S, device):
                 traffic = tf.get("kernels", {}).get(dominant, {}).get("hbm_bytes")
         except ValueError:
             traffic = None
-    dk = kernels[dominant]
-    roofline = {
-        "bound": "hbm",
-        "kernel": dominant,
-        "achieved": dk["GBps"],
-        "peak": HBM_PEAK_GBPS,
-        "unit": "GB/s",
-        "frac": dk["GBps"] / HBM_PEAK_GBPS,
-        "traffic": traffic,
-        "algorithmic_bytes_per_launch": dk["algorithmic_bytes"],
-        "timing": "HIP events around blocks of 50 launches rotating over 8 plans (HBM-resident working set)",
-    }
+    if kernels_batched:
+        dk = kernels_batched[dominant]
+        roofline = {
+            "bound": "hbm",
+            "kernel": dominant,
+            "achieved": dk["GBps"],
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": dk["GBps"] / HBM_PEAK_GBPS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": dk["algorithmic_bytes_per_launch"],
+            "evaluations_per_launch": dk["evaluations_per_launch"],
+            "timing": "HIP events around blocks of 10 stage-batched calls over the headline's %d distinct inputs "
+                      "and outputs (the launches of the timed loop)" % B,
+        }
+    else:
+        dk = kernels[dominant]
+        roofline = {
+            "bound": "hbm",
+            "kernel": dominant,
+            "achieved": dk["GBps"],
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": dk["GBps"] / HBM_PEAK_GBPS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": dk["algorithmic_bytes"],
+            "timing": "HIP events around blocks of 50 launches rotating over 8 plans (HBM-resident working set)",
+        }
     eval_avg_ms = sum(per_eval) / len(per_eval)
     eval_roofline = {
         "bytes_per_eval": eval_bytes,
@@ -759,6 +823,7 @@ def run(args, np, torch, S, device):
         "single_call": {"evals_per_s": single, "ms_per_eval": 1e3 * wall_single / n_single, "evaluations": n_single,
                         "note": "one skb_retained_operator_execute per evaluation, in order on one stream"},
         "kernels": kernels,
+        "kernels_batched": kernels_batched,
         "error_vs_fp64_gpu": {"max_scale_normalized": err_max, "relative_l2": err_l2, "tolerance": 1e-6,
                               "passed": err_max <= 1e-6 and err_l2 <= 1e-6},
         "error_vs_fp64_oracle": err_oracle,

@@ -312,6 +312,14 @@ int skb_retained_operator_lanes(const skb_retained_operator* op, int* apart);
 int skb_retained_operator_reserve_batch(skb_retained_operator* op, int evaluations);
 /* Evaluations of one stage-batched group (0 when none is reserved). */
 int skb_retained_operator_batch(const skb_retained_operator* op);
+/* One stage of the stage-batched execute (SKB_OP_STAGE_LAYOUT, _TRANSFORMS
+ * or _VERTICAL), one launch per group, for per-kernel timing.  The groups
+ * share the plan's intermediates, so the three calls in that order equal
+ * execute_batch when count is at most the reserved batch; a larger count
+ * launches the same work (every group overwrites the last).  SKB_LOGIC_ERROR
+ * when no batch is reserved.  No reference counterpart. */
+int skb_retained_operator_execute_batch_stage(skb_retained_operator* op, int stage, int count,
+                                              const void* const* retained, void* const* modal, void* stream);
 int skb_retained_operator_execute_batch(skb_retained_operator* op, int count, const void* const* retained,
                                         void* const* modal, void* stream);
 /* Device pointer to the real planes (x + Nx(y + Ny*plane)) of the last

@@ -296,32 +296,68 @@ int skb_retained_operator_reserve_batch(skb_retained_operator* op, int evaluatio
 
 int skb_retained_operator_batch(const skb_retained_operator* op) { return op ? op->batch_cap : 0; }
 
+namespace {
+
+bool stage_batched(const skb_retained_operator* op) {
+    return op->batch_cap >= 2 && op->fused && plane_path(op) && vertical_pm_supported(op->vertical);
+}
+
+// Stage-batched groups of up to batch_cap evaluations: the stages in `stages`
+// (bit 0 layout, 1 transforms, 2 vertical), one launch each per group.
+void run_batch_groups(skb_retained_operator* op, int count, const void* const* retained, void* const* modal,
+                      hipStream_t s, int stages) {
+    const PlaneTables* t = pruned_plane_tables(op->pruned);
+    const size_t pm_stride = static_cast<size_t>(t->np) * op->planes;
+    const size_t real_stride = op->real.bytes / real_bytes(op->precision);
+    for (int i0 = 0; i0 < count; i0 += op->batch_cap) {
+        const int n = std::min(op->batch_cap, count - i0);
+        BatchInputs in{};
+        BatchOutputs out{};
+        for (int e = 0; e < n; ++e) {
+            in.in[e] = retained[i0 + e];
+            out.out[e] = modal[i0 + e];
+        }
+        if (stages & 1) pm_transpose_batch(*t, in, n, op->batch_pm.ptr, pm_stride, op->planes, s);
+        if (stages & 2)
+            plane_operator(*t, op->batch_pm.ptr, op->batch_pm.ptr, op->batch_real.ptr, op->planes, s, n, pm_stride,
+                           real_stride);
+        if (stages & 4)
+            vertical_forward_pm_batch(op->vertical, op->batch_pm.ptr, pm_stride, t->np, t->cmask.as<uint32_t>(), out,
+                                      n, s);
+    }
+}
+
+void check_batch_args(const skb_retained_operator* op, int count, const void* const* retained, void* const* modal) {
+    if (!op || count < 0 || (count > 0 && (!retained || !modal))) fail(SKB_INVALID_ARGUMENT, "null argument");
+    for (int i = 0; i < count; ++i)
+        if (!retained[i] || !modal[i]) fail(SKB_INVALID_ARGUMENT, "null retained input or modal output");
+}
+
+}  // namespace
+
+int skb_retained_operator_execute_batch_stage(skb_retained_operator* op, int stage, int count,
+                                              const void* const* retained, void* const* modal, void* stream) {
+    return guarded([&] {
+        check_batch_args(op, count, retained, modal);
+        if (!stage_batched(op)) fail(SKB_LOGIC_ERROR, "no stage-batched execute reserved (reserve_batch)");
+        int mask = 0;
+        switch (stage) {
+            case SKB_OP_STAGE_LAYOUT: mask = 1; break;
+            case SKB_OP_STAGE_TRANSFORMS: mask = 2; break;
+            case SKB_OP_STAGE_VERTICAL: mask = 4; break;
+            default: fail(SKB_INVALID_ARGUMENT, "batched stages are LAYOUT, TRANSFORMS and VERTICAL");
+        }
+        run_batch_groups(op, count, retained, modal, as_stream(stream), mask);
+    });
+}
+
 int skb_retained_operator_execute_batch(skb_retained_operator* op, int count, const void* const* retained,
                                         void* const* modal, void* stream) {
     return guarded([&] {
-        if (!op || count < 0 || (count > 0 && (!retained || !modal))) fail(SKB_INVALID_ARGUMENT, "null argument");
-        for (int i = 0; i < count; ++i)
-            if (!retained[i] || !modal[i]) fail(SKB_INVALID_ARGUMENT, "null retained input or modal output");
+        check_batch_args(op, count, retained, modal);
         const hipStream_t s = as_stream(stream);
-        if (op->batch_cap >= 2 && op->fused && plane_path(op) && vertical_pm_supported(op->vertical)) {
-            // stage-batched: groups of up to batch_cap evaluations, one launch per stage
-            const PlaneTables* t = pruned_plane_tables(op->pruned);
-            const size_t pm_stride = static_cast<size_t>(t->np) * op->planes;
-            const size_t real_stride = op->real.bytes / real_bytes(op->precision);
-            for (int i0 = 0; i0 < count; i0 += op->batch_cap) {
-                const int n = std::min(op->batch_cap, count - i0);
-                BatchInputs in{};
-                BatchOutputs out{};
-                for (int e = 0; e < n; ++e) {
-                    in.in[e] = retained[i0 + e];
-                    out.out[e] = modal[i0 + e];
-                }
-                pm_transpose_batch(*t, in, n, op->batch_pm.ptr, pm_stride, op->planes, s);
-                plane_operator(*t, op->batch_pm.ptr, op->batch_pm.ptr, op->batch_real.ptr, op->planes, s, n, pm_stride,
-                               real_stride);
-                vertical_forward_pm_batch(op->vertical, op->batch_pm.ptr, pm_stride, t->np, t->cmask.as<uint32_t>(),
-                                          out, n, s);
-            }
+        if (stage_batched(op)) {
+            run_batch_groups(op, count, retained, modal, s, 7);
             return;
         }
         const int nl = static_cast<int>(op->lanes.size());

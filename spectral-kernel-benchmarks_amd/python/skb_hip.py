@@ -194,6 +194,7 @@ def _load():
         "skb_retained_operator_execute_batch": (i32, [vp, i32, vp, vp, vp]),
         "skb_retained_operator_reserve_batch": (i32, [vp, i32]),
         "skb_retained_operator_batch": (i32, [vp]),
+        "skb_retained_operator_execute_batch_stage": (i32, [vp, i32, i32, vp, vp, vp]),
         "skb_retained_operator_set_path": (i32, [vp, i32]),
         "skb_retained_operator_path": (i32, [vp]),
         "skb_retained_operator_real": (vp, [vp]),
@@ -249,7 +250,7 @@ EXPORTED = [
     "skb_retained_operator_create", "skb_retained_operator_destroy", "skb_retained_operator_execute",
     "skb_retained_operator_stage", "skb_retained_operator_set_path", "skb_retained_operator_path",
     "skb_retained_operator_reserve_lanes", "skb_retained_operator_lanes", "skb_retained_operator_execute_batch",
-    "skb_retained_operator_reserve_batch", "skb_retained_operator_batch",
+    "skb_retained_operator_reserve_batch", "skb_retained_operator_batch", "skb_retained_operator_execute_batch_stage",
     "skb_retained_operator_real", "skb_retained_operator_device_bytes", "skb_retained_operator_set_fused",
     "skb_retained_operator_fused", "skb_interleave", "skb_deinterleave", "skb_scale_split",
     "skb_vertical_plan_reserve_split", "skb_vertical_forward_split", "skb_vertical_inverse_split",
@@ -922,6 +923,12 @@ class OperatorPlan:
         """Independent evaluations retained[i] -> modal[i]: stage-batched groups (reserve_batch) or lanes."""
         n, ra, ma = self._batch_args(retained, modal)
         check(lib.skb_retained_operator_execute_batch(self.h, n, ra, ma, _stream(stream)))
+
+    def bind_execute_batch_stage(self, stage: int, retained, modal, stream=None) -> Bound:
+        """One stage of the stage-batched execute (one launch per group), prepared."""
+        n, ra, ma = self._batch_args(retained, modal)
+        return Bound(lib.skb_retained_operator_execute_batch_stage, (self.h, int(stage), n, ra, ma, _stream(stream)),
+                     (list(retained), list(modal), ra, ma, self))
 
     def bind_execute_batch(self, retained, modal, stream=None) -> Bound:
         n, ra, ma = self._batch_args(retained, modal)

@@ -168,6 +168,29 @@ def test_stage_batch_equals_alone_bit_for_bit(device, nz, cap):
     assert op.batch() == 0
 
 
+def test_stage_batch_stages_compose(device):
+    """execute_batch_stage LAYOUT, TRANSFORMS, VERTICAL in order over one
+    group equal the batch (and each evaluation alone) bit for bit."""
+    import torch
+
+    note("operator stage batch by stage")
+    w = S.Workload(256, 256, 65, 3)
+    op, *_, xs = make(w, S.SKB_F32, device, 5)
+    ref = alone(op, xs, device)
+    with pytest.raises(S.SkbError):
+        op.bind_execute_batch_stage(S.SKB_OP_STAGE_LAYOUT, xs, [torch.empty_like(r) for r in ref])()
+    op.reserve_batch(8)
+    outs = [torch.full((op.modal_elements,), float("nan"), dtype=torch.complex64, device=device) for _ in range(5)]
+    for stage in (S.SKB_OP_STAGE_LAYOUT, S.SKB_OP_STAGE_TRANSFORMS, S.SKB_OP_STAGE_VERTICAL):
+        op.bind_execute_batch_stage(stage, xs, outs)()
+    torch.cuda.synchronize()
+    for k in range(5):
+        assert torch.equal(outs[k], ref[k]), k
+    with pytest.raises(ValueError):
+        op.bind_execute_batch_stage(S.SKB_OP_STAGE_INVERSE, xs, outs)()
+    op.reserve_batch(0)
+
+
 def test_stage_batch_reservation_limits(device):
     note("operator stage batch limits")
     w = S.Workload(256, 256, 65, 3)
