@@ -90,19 +90,20 @@ def summarize_pmc(fetch_csv, write_csv, family):
     import csv
 
     def per_kernel(path, counter):
-        """(mean KiB per dispatch by kernel name, largest single copy dispatch in KiB)"""
-        sums, counts, copy_max = {}, {}, 0.0
+        """(median KiB per dispatch by kernel name, largest single copy dispatch
+        in KiB).  The median keeps the batched launches' figure when a few
+        single-evaluation dispatches of the same kernel share the run."""
+        values, copy_max = {}, 0.0
         with open(path, newline="") as handle:
             for row in csv.DictReader(handle):
                 if row.get("Counter_Name") != counter:
                     continue
                 name = row.get("Kernel_Name", "")
                 value = float(row["Counter_Value"])
-                sums[name] = sums.get(name, 0.0) + value
-                counts[name] = counts.get(name, 0) + 1
+                values.setdefault(name, []).append(value)
                 if "copybuffer" in name.lower():
                     copy_max = max(copy_max, value)
-        return {k: sums[k] / counts[k] for k in sums}, copy_max
+        return {k: percentile(v, 0.5) for k, v in values.items()}, copy_max
 
     # The calibration dispatch is the COPY_BYTES device-to-device copy, by far
     # the largest copy of the run; the other copies are small host transfers.
