@@ -104,6 +104,7 @@ struct PlaneGeo {
     static constexpr int J0 = (KC + 15) / 16, J1 = (N - KC + 1 - 16) / 16;
     static constexpr bool empty(int j) { return j >= J0 && j <= J1; }
     static constexpr int SMAX = kPlaneMaxModes;    // retained values staged per plane
+    static constexpr int IT = (SMAX / 2 + kPlaneThreads - 1) / kPlaneThreads;  // 16-byte input loads per thread
     static constexpr int Z = SMAX;                 // staged slot holding zero (inverse) / trash (forward)
     static constexpr int R0 = std::max(std::max(H * RP * 8, KC * YP * 4), (SMAX + 1) * 8);
     static constexpr int WT = 16 * kExchangeFloats * 4;  // one exchange scratch per wavefront
@@ -169,41 +170,76 @@ __device__ __forceinline__ void load_codes(uint32_t* c, const uint32_t* __restri
 // forward row transforms of the same row pairs run on the registers that hold
 // them, so the forward never reads the real plane back; X_a, X_b of both
 // halves end in xa, xb.
+// Thread indexing of the plane functions: `wbase` = 64 * wavefront index (a
+// scalar), plane_lane() the lane, recomputed from the execution mask where it
+// is used.  The lane is opaque to the compiler, so the addresses derived from
+// it are computed in the phase that uses them instead of being held in
+// registers across the persistent operator's loop over planes.
+__device__ __forceinline__ int plane_lane() {
+    uint32_t z = 0;
+    asm volatile("" : "+s"(z));
+    return static_cast<int>(__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, z)));
+}
+
+__device__ __forceinline__ int plane_wbase() {
+    return __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x)) & ~63;
+}
+
+// The plane's retained values as 16-byte pairs of its chunk lines (past an
+// odd count a pair reads the chunk's padding), into registers: issued ahead
+// of their use so that the plane waits for one memory latency, or none when
+// the loads run under the previous plane's work.  live == false reads
+// nothing (an empty buffer range: every load returns zero) but still defines
+// v, so the previous plane's values need not stay in registers.
+template <int KC>
+__device__ __forceinline__ void load_plane_values(int wbase, bf4 (&v)[PlaneGeo<KC>::IT], const cx<float>* pm, int modes,
+                                                  int np, int planes, int p, bool live = true) {
+    const int tid = wbase + plane_lane();
+    const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(pm, live ? static_cast<uint32_t>(np) * planes * 8u : 0u);
+    const int pairs = (modes + 1) >> 1;
+#pragma unroll
+    for (int k = 0; k < PlaneGeo<KC>::IT; ++k) {
+        const int i = tid + kPlaneThreads * k;
+        const uint32_t at = (static_cast<uint32_t>(i >> 3) * 16u * planes + 16u * p + 2u * (i & 7)) * 8u;
+        v[k] = raw_buffer_load_f4(rs, static_cast<int>(i < pairs ? at : kBufferPast), 0, 0);
+    }
+}
+
+// The twiddle table into its LDS region (read by every FFT of the workgroup;
+// nothing else writes the region).
+template <int KC>
+__device__ __forceinline__ void stage_twiddles(int wbase, unsigned char* lds, const cx<float>* __restrict__ twg) {
+    using G = PlaneGeo<KC>;
+    const int tid = wbase + plane_lane();
+    cx<float>* tw = reinterpret_cast<cx<float>*>(lds + G::R0 + G::WT);
+    if (tid < G::N) tw[tid] = twg[tid];
+}
+
 template <int KC, bool FWD>
-__device__ __forceinline__ void inverse_plane(unsigned char* lds, const cx<float>* pm, float* real,
-                                              const cx<float>* __restrict__ twg, const uint32_t* __restrict__ icodes,
-                                              int modes, int np, int planes, int p, cx<float> (&xa)[2][PlaneGeo<KC>::KH],
+__device__ __forceinline__ void inverse_plane(int wbase, unsigned char* lds, const bf4 (&v)[PlaneGeo<KC>::IT], float* real,
+                                              const uint32_t* __restrict__ icodes, int modes, int p,
+                                              cx<float> (&xa)[2][PlaneGeo<KC>::KH],
                                               cx<float> (&xb)[2][PlaneGeo<KC>::KH]) {
     using G = PlaneGeo<KC>;
     constexpr int N = G::N, RP = G::RP;
     cx<float>* r0 = reinterpret_cast<cx<float>*>(lds);  // staged modes, then the row buffer [128][RP]
-    const int tid = static_cast<int>(threadIdx.x), wave = tid >> 6, lane = tid & 63, g = lane >> 4, n = lane & 15;
+    const int lane = plane_lane(), tid = wbase + lane, wave = wbase >> 6, g = lane >> 4, n = lane & 15;
     float* wt = reinterpret_cast<float*>(lds + G::R0) + kExchangeFloats * wave;
     cx<float>* tw = reinterpret_cast<cx<float>*>(lds + G::R0 + G::WT);
 
-    // 1. the plane's retained values (16-byte pairs of one chunk; past an odd
-    //    count the pair reads the chunk's padding) and the twiddles.  Every
-    //    load of a thread is issued before the first LDS write, so the plane
-    //    waits for one memory latency, not one per pair.
+    // 1. the plane's retained values (loaded by the caller, load_plane_values)
+    //    staged in LDS, with the zero slot after them; the caller has staged
+    //    the twiddles (stage_twiddles) before this call
     {
-        const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(pm, static_cast<uint32_t>(np) * planes * 8u);
         const int pairs = (modes + 1) >> 1;
-        constexpr int IT = (G::SMAX / 2 + kPlaneThreads - 1) / kPlaneThreads;
-        bf4 v[IT];
 #pragma unroll
-        for (int k = 0; k < IT; ++k) {
-            const int i = tid + kPlaneThreads * k;
-            const uint32_t at = (static_cast<uint32_t>(i >> 3) * 16u * planes + 16u * p + 2u * (i & 7)) * 8u;
-            v[k] = raw_buffer_load_f4(rs, static_cast<int>(i < pairs ? at : kBufferPast), 0, 0);
-        }
-        const cx<float> t = tid < N ? twg[tid] : cx<float>{0.f, 0.f};
-#pragma unroll
-        for (int k = 0; k < IT; ++k) {
+        for (int k = 0; k < G::IT; ++k) {
             const int i = tid + kPlaneThreads * k;
             if (i < pairs) *reinterpret_cast<bf4*>(&r0[2 * i]) = v[k];
         }
-        if (tid < N) tw[tid] = t;
-        if (tid == 0) r0[G::Z] = cx<float>{0.f, 0.f};
+        float zero = 0.f;
+        asm volatile("" : "+v"(zero));  // made here, not held across the operator's loop
+        if (tid == 0) r0[G::Z] = cx<float>{zero, zero};
     }
     const int uw = __builtin_amdgcn_readfirstlane(wave);  // wavefront index, uniform
     uint32_t code[G::ROUNDS][8];
@@ -309,7 +345,12 @@ __global__ __launch_bounds__(kPlaneThreads) void plane_inverse_kernel(const cx<f
                                                                      int np, int planes) {
     __shared__ __align__(16) unsigned char lds[PlaneGeo<KC>::BYTES];
     cx<float> xa[2][PlaneGeo<KC>::KH], xb[2][PlaneGeo<KC>::KH];
-    inverse_plane<KC, false>(lds, pm, real, twg, icodes, modes, np, planes, static_cast<int>(blockIdx.x), xa, xb);
+    const int p = static_cast<int>(blockIdx.x);
+    const int wbase = plane_wbase();
+    bf4 v[PlaneGeo<KC>::IT];
+    load_plane_values<KC>(wbase, v, pm, modes, np, planes, p);
+    stage_twiddles<KC>(wbase, lds, twg);
+    inverse_plane<KC, false>(wbase, lds, v, real, icodes, modes, p, xa, xb);
 }
 
 // ---------------------------------------------------------------------------
@@ -338,16 +379,18 @@ __device__ __forceinline__ void forward_row_pair(cx<float>* z, float* wt, const 
 // halves at kx = n + 16 k, held by every lane) moves to columns through LDS,
 // the column transforms run, and the retained outputs go to plane p of pm.
 // The caller's LDS use of the band region [0, R0) must be over (barrier).
-template <int KC>
-__device__ __forceinline__ void forward_columns(unsigned char* lds, const cx<float> (&xa)[2][PlaneGeo<KC>::KH],
+// `ahead` runs once the outputs are staged, before the copy-out: the
+// persistent operator kernel issues the next plane's loads there.
+template <int KC, typename Ahead>
+__device__ __forceinline__ void forward_columns(int wbase, unsigned char* lds, const cx<float> (&xa)[2][PlaneGeo<KC>::KH],
                                                 const cx<float> (&xb)[2][PlaneGeo<KC>::KH], cx<float>* pm,
                                                 const uint32_t* __restrict__ fcodes, int modes, int np, int planes,
-                                                int p) {
+                                                int p, Ahead&& ahead) {
     using G = PlaneGeo<KC>;
     constexpr int YP = G::YP, KH = G::KH;
     float* yb = reinterpret_cast<float*>(lds);           // band [KC][YP], real or imaginary parts
     cx<float>* st = reinterpret_cast<cx<float>*>(lds);  // then the output staging [modes]
-    const int tid = static_cast<int>(threadIdx.x), wave = tid >> 6, lane = tid & 63, g = lane >> 4, n = lane & 15;
+    const int lane = plane_lane(), tid = wbase + lane, wave = wbase >> 6, g = lane >> 4, n = lane & 15;
     float* wt = reinterpret_cast<float*>(lds + G::R0) + kExchangeFloats * wave;
     const cx<float>* tw = reinterpret_cast<const cx<float>*>(lds + G::R0 + G::WT);
     const int f = 4 * wave + g;
@@ -384,6 +427,13 @@ __device__ __forceinline__ void forward_columns(unsigned char* lds, const cx<flo
         }
     }
     lds_barrier();  // the band is consumed; the region becomes the output staging
+    uint32_t code[G::ROUNDS][8];
+#pragma unroll
+    for (int r = 0; r < G::ROUNDS; ++r) {
+        const int c = 64 * r + 4 * wave + g;
+        if (64 * r + 4 * wave < KC) load_codes(code[r], fcodes, (c < KC ? c : KC - 1) * 16 + n);
+    }
+    ahead();  // after the code loads: the staging below waits for those alone
 
     // 3. column transforms and the retained outputs
 #pragma unroll
@@ -391,13 +441,11 @@ __device__ __forceinline__ void forward_columns(unsigned char* lds, const cx<flo
         const int c = 64 * r + 4 * wave + g;
         if (64 * r + 4 * wave < KC) {
             fft256<false>(cv[r], wt, tw, g, n);
-            uint32_t code[8];
-            load_codes(code, fcodes, (c < KC ? c : KC - 1) * 16 + n);
 #pragma unroll
             for (int k = 0; k < 16; ++k) {
                 if (G::empty(k)) continue;  // no retained k_y there: these outputs are never computed
                 // empty slots (and the columns past KC) write the trash slot
-                st[c < KC ? unpack_code(code, k) : G::Z] = cv[r][k];
+                st[c < KC ? unpack_code(code[r], k) : G::Z] = cv[r][k];
             }
         }
     }
@@ -408,20 +456,24 @@ __device__ __forceinline__ void forward_columns(unsigned char* lds, const cx<flo
     {
         const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(pm, static_cast<uint32_t>(np) * planes * 8u);
         const int pairs = (modes + 1) >> 1;
-        for (int i = tid; i < pairs; i += kPlaneThreads) {
-            const uint32_t at = (static_cast<uint32_t>(i >> 3) * 16u * planes + 16u * p + 2u * (i & 7)) * 8u;
-            raw_buffer_store_f4(*reinterpret_cast<const bf4*>(&st[2 * i]), rs, static_cast<int>(at), 0, 0);
+#pragma unroll
+        for (int k = 0; k < G::IT; ++k) {
+            const int i = tid + kPlaneThreads * k;
+            if (i < pairs) {
+                const uint32_t at = (static_cast<uint32_t>(i >> 3) * 16u * planes + 16u * p + 2u * (i & 7)) * 8u;
+                raw_buffer_store_f4(*reinterpret_cast<const bf4*>(&st[2 * i]), rs, static_cast<int>(at), 0, 0);
+            }
         }
     }
 }
 
 template <int KC>
-__device__ __forceinline__ void forward_plane(unsigned char* lds, const float* real, cx<float>* pm,
+__device__ __forceinline__ void forward_plane(int wbase, unsigned char* lds, const float* real, cx<float>* pm,
                                               const cx<float>* __restrict__ twg, const uint32_t* __restrict__ fcodes,
                                               int modes, int np, int planes, int p) {
     using G = PlaneGeo<KC>;
     constexpr int N = G::N, KH = G::KH;
-    const int tid = static_cast<int>(threadIdx.x), wave = tid >> 6, lane = tid & 63, g = lane >> 4, n = lane & 15;
+    const int lane = plane_lane(), tid = wbase + lane, wave = wbase >> 6, g = lane >> 4, n = lane & 15;
     float* wt = reinterpret_cast<float*>(lds + G::R0) + kExchangeFloats * wave;
     cx<float>* tw = reinterpret_cast<cx<float>*>(lds + G::R0 + G::WT);
     const __amdgpu_buffer_rsrc_t in =
@@ -444,7 +496,7 @@ __device__ __forceinline__ void forward_plane(unsigned char* lds, const float* r
         forward_row_pair<KC>(z, wt, tw, g, n, xa[h], xb[h]);
     }
     lds_barrier();  // (the band region held nothing else; kept for the caller contract)
-    forward_columns<KC>(lds, xa, xb, pm, fcodes, modes, np, planes, p);
+    forward_columns<KC>(wbase, lds, xa, xb, pm, fcodes, modes, np, planes, p, [] {});
 }
 
 template <int KC>
@@ -454,17 +506,21 @@ __global__ __launch_bounds__(kPlaneThreads) void plane_forward_kernel(const floa
                                                                      const uint32_t* __restrict__ fcodes, int modes,
                                                                      int np, int planes) {
     __shared__ __align__(16) unsigned char lds[PlaneGeo<KC>::BYTES];
-    forward_plane<KC>(lds, real, pm, twg, fcodes, modes, np, planes, static_cast<int>(blockIdx.x));
+    forward_plane<KC>(plane_wbase(), lds, real, pm, twg, fcodes, modes, np, planes, static_cast<int>(blockIdx.x));
 }
 
 // ---------------------------------------------------------------------------
-// The retained operator's two transforms of plane blockIdx.x in one
-// workgroup: plane-major values -> inverse -> real plane (written to memory,
-// the physical-space result) and, from the same registers, the forward row
+// The retained operator's two transforms of a plane in one workgroup:
+// plane-major values -> inverse -> real plane (written to memory, the
+// physical-space result) and, from the same registers, the forward row
 // transforms -> forward columns -> plane-major values (may alias the input:
-// each workgroup reads its plane's values in full before it writes them, and
-// no other workgroup touches them).  blockIdx.y is the evaluation of a batch:
-// its arrays sit pm_stride complex and real_stride real elements further on.
+// a plane's values are read in full before they are written, by the one
+// workgroup that owns the plane).  The planes of `count` evaluations
+// (evaluation e's arrays pm_stride complex and real_stride real elements
+// further on) are q = e * planes + p; workgroup b takes q = b, b + grid, ...
+// in turn, one workgroup per CU (the LDS of one).  The next plane's values
+// are loaded into registers while the current plane's outputs are copied
+// out, so a plane does not wait for its input; the twiddles are staged once.
 // ---------------------------------------------------------------------------
 template <int KC>
 __global__ __launch_bounds__(kPlaneThreads) void plane_operator_kernel(const cx<float>* pm_in,
@@ -474,16 +530,28 @@ __global__ __launch_bounds__(kPlaneThreads) void plane_operator_kernel(const cx<
                                                                       const uint32_t* __restrict__ icodes,
                                                                       const uint32_t* __restrict__ fcodes, int modes,
                                                                       int np, int planes, size_t pm_stride,
-                                                                      size_t real_stride) {
+                                                                      size_t real_stride, int total) {
     __shared__ __align__(16) unsigned char lds[PlaneGeo<KC>::BYTES];
-    const int p = static_cast<int>(blockIdx.x);
-    pm_in += pm_stride * blockIdx.y;
-    pm_out += pm_stride * blockIdx.y;
-    real += real_stride * blockIdx.y;
-    cx<float> xa[2][PlaneGeo<KC>::KH], xb[2][PlaneGeo<KC>::KH];
-    inverse_plane<KC, true>(lds, pm_in, real, twg, icodes, modes, np, planes, p, xa, xb);
-    lds_barrier();  // every row-buffer read of the inverse is done before the band overwrites it
-    forward_columns<KC>(lds, xa, xb, pm_out, fcodes, modes, np, planes, p);
+    const int step = static_cast<int>(gridDim.x);
+    int q = static_cast<int>(blockIdx.x);
+    if (q >= total) return;
+    const int wbase = plane_wbase();
+    bf4 v[PlaneGeo<KC>::IT];
+    load_plane_values<KC>(wbase, v, pm_in + pm_stride * (q / planes), modes, np, planes, q % planes);
+    stage_twiddles<KC>(wbase, lds, twg);
+    for (; q < total; q += step) {
+        const int e = q / planes, p = q - e * planes;
+        if (q != static_cast<int>(blockIdx.x)) lds_barrier();  // the previous plane's copy-out has read the staging
+        cx<float> xa[2][PlaneGeo<KC>::KH], xb[2][PlaneGeo<KC>::KH];
+        inverse_plane<KC, true>(wbase, lds, v, real + real_stride * e, icodes, modes, p, xa, xb);
+        lds_barrier();  // every row-buffer read of the inverse is done before the band overwrites it
+        const int qn = q + step;
+        forward_columns<KC>(wbase, lds, xa, xb, pm_out + pm_stride * e, fcodes, modes, np, planes, p, [&] {
+            const bool live = qn < total;
+            const int qv = live ? qn : q;
+            load_plane_values<KC>(wbase, v, pm_in + pm_stride * (qv / planes), modes, np, planes, qv % planes, live);
+        });
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -702,6 +770,10 @@ void PlaneTables::build(const skb_workload& w, uint64_t kc_, uint64_t modes_, co
             t[16 * k + n] = cx<float>{static_cast<float>(std::cos(a)), static_cast<float>(-std::sin(a))};
         }
     tw.upload(t.data(), t.size() * sizeof(t[0]));
+    int dev = 0, cus = 0;
+    check_hip(hipGetDevice(&dev), "hipGetDevice");
+    check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    workgroups = std::max(1, cus);
 }
 
 void plane_inverse(const PlaneTables& t, const void* pm, void* real, int planes, hipStream_t s) {
@@ -720,10 +792,12 @@ void plane_forward(const PlaneTables& t, const void* real, void* pm, int planes,
 
 void plane_operator(const PlaneTables& t, const void* pm_in, void* pm_out, void* real, int planes, hipStream_t s,
                     int count, size_t pm_stride, size_t real_stride) {
-    hipLaunchKernelGGL((plane_operator_kernel<kPlaneKc>), dim3(planes, count), dim3(kPlaneThreads), 0, s,
+    const int total = planes * count;
+    const int grid = std::min(total, t.workgroups);
+    hipLaunchKernelGGL((plane_operator_kernel<kPlaneKc>), dim3(grid), dim3(kPlaneThreads), 0, s,
                        static_cast<const cx<float>*>(pm_in), static_cast<cx<float>*>(pm_out),
                        static_cast<float*>(real), t.tw.as<cx<float>>(), t.icodes.as<uint32_t>(),
-                       t.fcodes.as<uint32_t>(), t.modes, t.np, planes, pm_stride, real_stride);
+                       t.fcodes.as<uint32_t>(), t.modes, t.np, planes, pm_stride, real_stride, total);
     check_hip(hipGetLastError(), "plane operator launch");
 }
 

@@ -33,6 +33,7 @@ struct PlaneTables {
     DeviceBuffer tw, fcodes, icodes;
     DeviceBuffer cmask;  // per 16-mode chunk: bit m % 16 set when mode m is stored conjugated
     int modes = 0, np = 0;
+    int workgroups = 0;  // persistent operator workgroups: the device's CUs (one plane workgroup fits a CU)
     void build(const skb_workload& w, uint64_t kc, uint64_t modes, const std::vector<int>& fmap,
                const std::vector<int>& imap);
 };
