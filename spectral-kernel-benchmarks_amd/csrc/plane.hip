@@ -54,6 +54,7 @@ namespace {
 
 constexpr int kPlaneN = 256;
 constexpr int kPlaneThreads = 1024;
+constexpr int kTwiddlePitch = 18;  // complex entries per lane's twiddle row in LDS (16 used)
 
 
 // Makes the LDS accesses of one wavefront before it visible to the lanes of the
@@ -106,21 +107,28 @@ struct PlaneGeo {
     static constexpr int SMAX = kPlaneMaxModes;    // retained values staged per plane
     static constexpr int IT = (SMAX / 2 + kPlaneThreads - 1) / kPlaneThreads;  // 16-byte input loads per thread
     static constexpr int Z = SMAX;                 // staged slot holding zero (inverse) / trash (forward)
-    static constexpr int R0 = std::max(std::max(H * RP * 8, KC * YP * 4), (SMAX + 1) * 8);
+    // (rounded to 16 bytes: the twiddle rows after the exchange scratch are read 16 bytes at a time)
+    static constexpr int R0 = (std::max(std::max(H * RP * 8, KC * YP * 4), (SMAX + 1) * 8) + 15) / 16 * 16;
     static constexpr int WT = 16 * kExchangeFloats * 4;  // one exchange scratch per wavefront
-    static constexpr int BYTES = R0 + WT + N * 8;
+    static constexpr int BYTES = R0 + WT + 16 * kTwiddlePitch * 8;
     static_assert(KC <= 128 && BYTES <= 160 * 1024, "plane workgroup exceeds the CU's LDS");
 };
 
 // One 256-point FFT per 16-lane group: lane n holds x[16 j + n] in v[j]; on
-// return it holds X[n + 16 k] in v[k].  tw[16 k + n] = exp(-2 pi i n k / 256)
-// in LDS (the 16 lanes of a group read 16 consecutive entries, no bank
-// conflicts); INV uses the conjugate (unnormalised inverse).
+// return it holds X[n + 16 k] in v[k].  Lane n's twiddles exp(-2 pi i n k /
+// 256), k = 0..15, are one LDS row tw[kTwiddlePitch * n + k], read 16 bytes
+// (two twiddles) at a time; rows 144 bytes apart put the 16 lanes of a
+// group on distinct banks.  INV uses the conjugate (unnormalised inverse).
 template <bool INV>
 __device__ __forceinline__ void fft256(cx<float>* v, float* s, const cx<float>* tw, int g, int n) {
     rdft<16, INV>(v);
+    const bf4* row = reinterpret_cast<const bf4*>(tw + kTwiddlePitch * n);
 #pragma unroll
-    for (int k = 1; k < 16; ++k) v[k] = INV ? cmulc(v[k], tw[16 * k + n]) : cmul(v[k], tw[16 * k + n]);
+    for (int h = 0; h < 8; ++h) {
+        const bf4 w = row[h];
+        if (h) v[2 * h] = INV ? cmulc(v[2 * h], cx<float>{w.x, w.y}) : cmul(v[2 * h], cx<float>{w.x, w.y});
+        v[2 * h + 1] = INV ? cmulc(v[2 * h + 1], cx<float>{w.z, w.w}) : cmul(v[2 * h + 1], cx<float>{w.z, w.w});
+    }
     wave_exchange16(s, v, g, n);
     rdft<16, INV>(v);
 }
@@ -185,6 +193,17 @@ __device__ __forceinline__ int plane_wbase() {
     return __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x)) & ~63;
 }
 
+// Byte offset of the 16-byte pair `tid` (modes 2 tid, 2 tid + 1) of plane p in
+// the chunked plane-major block, and the step to pair tid + 1024 (128 chunks
+// on).  tid < 1024, so the chunk index fits the 24-bit multiply.
+__device__ __forceinline__ uint32_t pm_pair_offset(int tid, int planes, int p) {
+    return (__umul24(static_cast<uint32_t>(tid) >> 3, 16u * static_cast<uint32_t>(planes)) + 16u * p +
+            2u * (static_cast<uint32_t>(tid) & 7u)) * 8u;
+}
+__device__ __forceinline__ uint32_t pm_pair_step(int planes) {
+    return (kPlaneThreads / 8) * 16u * 8u * static_cast<uint32_t>(planes);
+}
+
 // The plane's retained values as 16-byte pairs of its chunk lines (past an
 // odd count a pair reads the chunk's padding), into registers: issued ahead
 // of their use so that the plane waits for one memory latency, or none when
@@ -196,13 +215,12 @@ __device__ __forceinline__ void load_plane_values(int wbase, bf4 (&v)[PlaneGeo<K
                                                   int np, int planes, int p, bool live = true) {
     const int tid = wbase + plane_lane();
     const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(pm, live ? static_cast<uint32_t>(np) * planes * 8u : 0u);
-    const int pairs = (modes + 1) >> 1;
+    // pair i = tid + 1024 k sits in chunk i / 8 at 8 (16 P (i / 8) + 16 p + 2 (i % 8)); a pair
+    // past the last one reads the last chunk's padding or, in a later chunk,
+    // past the buffer (zero), and is never staged
+    const uint32_t at = pm_pair_offset(tid, planes, p), step = pm_pair_step(planes);
 #pragma unroll
-    for (int k = 0; k < PlaneGeo<KC>::IT; ++k) {
-        const int i = tid + kPlaneThreads * k;
-        const uint32_t at = (static_cast<uint32_t>(i >> 3) * 16u * planes + 16u * p + 2u * (i & 7)) * 8u;
-        v[k] = raw_buffer_load_f4(rs, static_cast<int>(i < pairs ? at : kBufferPast), 0, 0);
-    }
+    for (int k = 0; k < PlaneGeo<KC>::IT; ++k) v[k] = raw_buffer_load_f4(rs, static_cast<int>(at + k * step), 0, 0);
 }
 
 // The twiddle table into its LDS region (read by every FFT of the workgroup;
@@ -212,7 +230,7 @@ __device__ __forceinline__ void stage_twiddles(int wbase, unsigned char* lds, co
     using G = PlaneGeo<KC>;
     const int tid = wbase + plane_lane();
     cx<float>* tw = reinterpret_cast<cx<float>*>(lds + G::R0 + G::WT);
-    if (tid < G::N) tw[tid] = twg[tid];
+    if (tid < G::N) tw[kTwiddlePitch * (tid & 15) + (tid >> 4)] = twg[tid];
 }
 
 template <int KC, bool FWD>
@@ -456,13 +474,12 @@ __device__ __forceinline__ void forward_columns(int wbase, unsigned char* lds, c
     {
         const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(pm, static_cast<uint32_t>(np) * planes * 8u);
         const int pairs = (modes + 1) >> 1;
+        const uint32_t at = pm_pair_offset(tid, planes, p), step = pm_pair_step(planes);
 #pragma unroll
         for (int k = 0; k < G::IT; ++k) {
             const int i = tid + kPlaneThreads * k;
-            if (i < pairs) {
-                const uint32_t at = (static_cast<uint32_t>(i >> 3) * 16u * planes + 16u * p + 2u * (i & 7)) * 8u;
-                raw_buffer_store_f4(*reinterpret_cast<const bf4*>(&st[2 * i]), rs, static_cast<int>(at), 0, 0);
-            }
+            if (i < pairs)
+                raw_buffer_store_f4(*reinterpret_cast<const bf4*>(&st[2 * i]), rs, static_cast<int>(at + k * step), 0, 0);
         }
     }
 }
@@ -479,7 +496,7 @@ __device__ __forceinline__ void forward_plane(int wbase, unsigned char* lds, con
     const __amdgpu_buffer_rsrc_t in =
         buffer_rsrc(real + static_cast<size_t>(p) * N * N, static_cast<uint32_t>(N * N * 4));
     const int f = 4 * wave + g;
-    if (tid < N) tw[tid] = twg[tid];
+    if (tid < N) tw[kTwiddlePitch * (tid & 15) + (tid >> 4)] = twg[tid];
 
     // 1. rows: X_a, X_b (rows 2f, 2f + 1 of each half) for kx = n + 16 k < KC
     cx<float> xa[2][KH], xb[2][KH];
