@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Retained-operator throughput of the hip provider on one MI355X.
 
-One step = one batch of --batch (32) independent retained-operator
+One step = one batch of --batch (64) independent retained-operator
 evaluations of the 256^2 x Nz65 x 3 workload, distinct inputs and outputs
 resident in HBM, enqueued by ONE C-ABI call
 (skb_retained_operator_execute_batch), each evaluation:
@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=40,
                     help="timed steps; one step is one batch of --batch evaluations")
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--batch", type=int, default=32,
+    ap.add_argument("--batch", type=int, default=64,
                     help="independent evaluations per step (one execute_batch call)")
     ap.add_argument("--batch-mode", choices=("stage", "lanes"), default="stage",
                     help="stage: one launch per stage over the batch; lanes: evaluations over --lanes streams")
@@ -75,6 +75,7 @@ def parse():
 
 
 COPY_BYTES = 256 << 20
+BATCH_CAP = 64  # evaluations per stage-batched launch (skb_retained_operator_reserve_batch's maximum)
 KERNEL_NAMES = {  # substring of the device kernel name -> stage of the (stage-batched) evaluation
     "pm_to_pm_batch_kernel": "layout",
     "plane_operator_kernel<86>": "transforms",
@@ -587,7 +588,7 @@ def run(args, np, torch, S, device):
         # the launches the roofline is timed on: every stage of the
         # stage-batched execute over the headline's batch, in blocks
         B = args.batch
-        f32.op.reserve_batch(min(B, 32))
+        f32.op.reserve_batch(min(B, BATCH_CAP))
         xs = [f32.x * (1.0 + 1e-3 * k) for k in range(B)]
         outs = [torch.empty_like(f32.modal) for _ in range(B)]
         batch_call = f32.op.bind_execute_batch(xs, outs)
@@ -626,7 +627,7 @@ def run(args, np, torch, S, device):
     # HBM-resident inputs and outputs), stage-batched (or over lanes)
     B = args.batch
     if args.batch_mode == "stage":
-        f32.op.reserve_batch(min(B, 32))
+        f32.op.reserve_batch(min(B, BATCH_CAP))
         lanes, lanes_apart = 1, True
     else:
         f32.op.reserve_lanes(args.lanes)
@@ -713,7 +714,7 @@ def run(args, np, torch, S, device):
     kernels_batched = None
     if batched_stages:
         # one launch per group of `cap` evaluations; a call holds B / cap launches
-        cap = min(B, 32)
+        cap = min(B, BATCH_CAP)
         launches = (B + cap - 1) // cap
         kernels_batched = {}
         for n, ms in batched_stages.items():

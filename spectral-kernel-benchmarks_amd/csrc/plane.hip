@@ -142,28 +142,40 @@ __device__ __forceinline__ float from_partner_lane(float v) {
     return __int_as_float(x);
 }
 
-// Code k (0..15) of a lane's 16 packed 16-bit column codes: the staged index
-// of the slot's value (its mode, or kPlaneMaxModes, a staged zero, where the
-// (kx, ky) slot holds no retained value), plus kPartner on the inverse's
-// k_x = 0 Hermitian partners, which take the conjugate.
+// The code of register k (0..15) of a lane's column: the staged index of the
+// slot's value (its mode, or kPlaneMaxModes, a staged zero, where the (kx,
+// ky) slot holds no retained value), plus kPartner on the inverse's k_x = 0
+// Hermitian partners, which take the conjugate.  Registers in the band no
+// 2/3-rule set retains (PlaneGeo::empty) have no code: the others' 16-bit
+// codes sit in order (code_slot), two per 32-bit word, in the first six
+// words of the lane's 32-byte table entry, so the loads bring in no word
+// that goes unused (an unused loaded register would be reused at once and
+// stall on its load).
 constexpr int kPartner = 0x8000, kCodeIndex = 0x7FFF;
 static_assert(kPlaneMaxModes <= kCodeIndex, "staged indices need 15 bits");
+template <int KC>
+constexpr int code_slot(int k) {
+    return k < PlaneGeo<KC>::J0 ? k : k - (PlaneGeo<KC>::J1 - PlaneGeo<KC>::J0 + 1);
+}
+static_assert(16 - (PlaneGeo<kPlaneKc>::J1 - PlaneGeo<kPlaneKc>::J0 + 1) <= 12, "codes take six words");
+
+template <int KC>
 __device__ __forceinline__ int unpack_code(const uint32_t* c, int k) {
-    const uint32_t w = c[k >> 1];
-    return static_cast<int>((k & 1) ? (w >> 16) : (w & 0xFFFFu));
+    const int sl = code_slot<KC>(k);
+    const uint32_t w = c[sl >> 1];
+    return static_cast<int>((sl & 1) ? (w >> 16) : (w & 0xFFFFu));
 }
 
 __device__ __forceinline__ void load_codes(uint32_t* c, const uint32_t* __restrict__ table, int lane_slot) {
     const bf4* src = reinterpret_cast<const bf4*>(table) + 2 * lane_slot;
-    const bf4 a = src[0], b = src[1];
+    const bf4 a = src[0];
+    const bf2 b = *reinterpret_cast<const bf2*>(src + 1);
     c[0] = __float_as_uint(a.x);
     c[1] = __float_as_uint(a.y);
     c[2] = __float_as_uint(a.z);
     c[3] = __float_as_uint(a.w);
     c[4] = __float_as_uint(b.x);
     c[5] = __float_as_uint(b.y);
-    c[6] = __float_as_uint(b.z);
-    c[7] = __float_as_uint(b.w);
 }
 
 }  // namespace
@@ -233,6 +245,19 @@ __device__ __forceinline__ void stage_twiddles(int wbase, unsigned char* lds, co
     if (tid < G::N) tw[kTwiddlePitch * (tid & 15) + (tid >> 4)] = twg[tid];
 }
 
+// The 16 packed codes of each of this lane's column rounds (the same for
+// every plane), issued ahead of their use like the values.
+template <int KC>
+__device__ __forceinline__ void load_column_codes(int wbase, uint32_t (&code)[PlaneGeo<KC>::ROUNDS][8],
+                                                  const uint32_t* __restrict__ table) {
+    const int lane = plane_lane(), wave = wbase >> 6, g = lane >> 4, n = lane & 15;
+#pragma unroll
+    for (int r = 0; r < PlaneGeo<KC>::ROUNDS; ++r) {
+        const int c = 64 * r + 4 * wave + g;
+        if (64 * r + 4 * wave < KC) load_codes(code[r], table, (c < KC ? c : KC - 1) * 16 + n);
+    }
+}
+
 template <int KC, bool FWD>
 __device__ __forceinline__ void inverse_plane(int wbase, unsigned char* lds, const bf4 (&v)[PlaneGeo<KC>::IT], float* real,
                                               const uint32_t* __restrict__ icodes, int modes, int p,
@@ -261,11 +286,7 @@ __device__ __forceinline__ void inverse_plane(int wbase, unsigned char* lds, con
     }
     const int uw = __builtin_amdgcn_readfirstlane(wave);  // wavefront index, uniform
     uint32_t code[G::ROUNDS][8];
-#pragma unroll
-    for (int r = 0; r < G::ROUNDS; ++r) {
-        const int c = 64 * r + 4 * wave + g;
-        if (64 * r + 4 * wave < KC) load_codes(code[r], icodes, (c < KC ? c : KC - 1) * 16 + n);
-    }
+    load_column_codes<KC>(wbase, code, icodes);
     lds_barrier();
 
     // 2. column inputs from the staged values
@@ -280,7 +301,7 @@ __device__ __forceinline__ void inverse_plane(int wbase, unsigned char* lds, con
                     y[r][j] = cx<float>{0.f, 0.f};
                     continue;
                 }
-                const int e = unpack_code(code[r], j);
+                const int e = unpack_code<KC>(code[r], j);
                 const cx<float> x = r0[e & kCodeIndex];
                 // partners live in column 0 alone (wave 0, round 0)
                 y[r][j] = (r == 0 && uw == 0 && (e & kPartner)) ? cx<float>{x.x, -x.y} : x;
@@ -397,13 +418,14 @@ __device__ __forceinline__ void forward_row_pair(cx<float>* z, float* wt, const 
 // halves at kx = n + 16 k, held by every lane) moves to columns through LDS,
 // the column transforms run, and the retained outputs go to plane p of pm.
 // The caller's LDS use of the band region [0, R0) must be over (barrier).
-// `ahead` runs once the outputs are staged, before the copy-out: the
-// persistent operator kernel issues the next plane's loads there.
-template <int KC, typename Ahead>
+// `ahead` runs before the column transforms and `behind` once the outputs
+// are staged, before the copy-out: the persistent operator kernel issues the
+// next plane's value loads and its column codes there.
+template <int KC, typename Ahead, typename Behind>
 __device__ __forceinline__ void forward_columns(int wbase, unsigned char* lds, const cx<float> (&xa)[2][PlaneGeo<KC>::KH],
                                                 const cx<float> (&xb)[2][PlaneGeo<KC>::KH], cx<float>* pm,
                                                 const uint32_t* __restrict__ fcodes, int modes, int np, int planes,
-                                                int p, Ahead&& ahead) {
+                                                int p, Ahead&& ahead, Behind&& behind) {
     using G = PlaneGeo<KC>;
     constexpr int YP = G::YP, KH = G::KH;
     float* yb = reinterpret_cast<float*>(lds);           // band [KC][YP], real or imaginary parts
@@ -414,11 +436,17 @@ __device__ __forceinline__ void forward_columns(int wbase, unsigned char* lds, c
     const int f = 4 * wave + g;
 
     // 2. the band through LDS, real parts then imaginary parts; the column
-    //    lanes gather their inputs after each
+    //    lanes gather their inputs after each.  The output codes are loaded
+    //    between the parts (once the real parts are out of registers), so
+    //    that they have arrived by the staging below.
+    uint32_t code[G::ROUNDS][8];
     cx<float> cv[G::ROUNDS][16];
 #pragma unroll
     for (int part = 0; part < 2; ++part) {
-        if (part) lds_barrier();  // every column lane has read the real parts
+        if (part) {
+            load_column_codes<KC>(wbase, code, fcodes);
+            lds_barrier();  // every column lane has read the real parts
+        }
 #pragma unroll
         for (int h = 0; h < 2; ++h)
 #pragma unroll
@@ -445,12 +473,6 @@ __device__ __forceinline__ void forward_columns(int wbase, unsigned char* lds, c
         }
     }
     lds_barrier();  // the band is consumed; the region becomes the output staging
-    uint32_t code[G::ROUNDS][8];
-#pragma unroll
-    for (int r = 0; r < G::ROUNDS; ++r) {
-        const int c = 64 * r + 4 * wave + g;
-        if (64 * r + 4 * wave < KC) load_codes(code[r], fcodes, (c < KC ? c : KC - 1) * 16 + n);
-    }
     ahead();  // after the code loads: the staging below waits for those alone
 
     // 3. column transforms and the retained outputs
@@ -463,10 +485,11 @@ __device__ __forceinline__ void forward_columns(int wbase, unsigned char* lds, c
             for (int k = 0; k < 16; ++k) {
                 if (G::empty(k)) continue;  // no retained k_y there: these outputs are never computed
                 // empty slots (and the columns past KC) write the trash slot
-                st[c < KC ? unpack_code(code[r], k) : G::Z] = cv[r][k];
+                st[c < KC ? unpack_code<KC>(code[r], k) : G::Z] = cv[r][k];
             }
         }
     }
+    behind();
     lds_barrier();
 
     // 4. copy out (16-byte pairs of one chunk; the pair past an odd count
@@ -513,7 +536,7 @@ __device__ __forceinline__ void forward_plane(int wbase, unsigned char* lds, con
         forward_row_pair<KC>(z, wt, tw, g, n, xa[h], xb[h]);
     }
     lds_barrier();  // (the band region held nothing else; kept for the caller contract)
-    forward_columns<KC>(wbase, lds, xa, xb, pm, fcodes, modes, np, planes, p, [] {});
+    forward_columns<KC>(wbase, lds, xa, xb, pm, fcodes, modes, np, planes, p, [] {}, [] {});
 }
 
 template <int KC>
@@ -567,7 +590,7 @@ __global__ __launch_bounds__(kPlaneThreads) void plane_operator_kernel(const cx<
             const bool live = qn < total;
             const int qv = live ? qn : q;
             load_plane_values<KC>(wbase, v, pm_in + pm_stride * (qv / planes), modes, np, planes, qv % planes, live);
-        });
+        }, [] {});
     }
 }
 
@@ -756,8 +779,9 @@ void PlaneTables::build(const skb_workload& w, uint64_t kc_, uint64_t modes_, co
     modes = static_cast<int>(modes_);
     np = static_cast<int>(plane_np(modes_));
     const int N = kPlaneN, KC = kPlaneKc;
-    // per (column c, lane n) 16 codes, two per 32-bit word
-    std::vector<uint16_t> f(static_cast<size_t>(KC) * 16 * 16), i(f.size());
+    // per (column c, lane n) a 16-entry slot: the codes of the registers
+    // outside the never-retained band at code_slot(k), two per 32-bit word
+    std::vector<uint16_t> f(static_cast<size_t>(KC) * 16 * 16, 0), i(f.size(), 0);
     // Slot codes (2*mode + conj from the column lists) -> staged indices.  A
     // slot the forward writes holds its mode's slot value, so the inverse
     // reads it as it is; the inverse's extra slots are the k_x = 0 partners,
@@ -772,9 +796,13 @@ void PlaneTables::build(const skb_workload& w, uint64_t kc_, uint64_t modes_, co
                 const int ic = imap[static_cast<size_t>(c) * N + 16 * k + n];
                 if (fc >= 0 && (fc & 1)) cm[static_cast<size_t>(fc >> 1) / 16] |= 1u << ((fc >> 1) % 16);
                 if (fi >= 0 && fi != ic) fail(SKB_LOGIC_ERROR, "plane codes: forward and inverse slots disagree");
-                f[(static_cast<size_t>(c) * 16 + n) * 16 + k] = static_cast<uint16_t>(fc < 0 ? kPlaneMaxModes : fc >> 1);
-                i[(static_cast<size_t>(c) * 16 + n) * 16 + k] = static_cast<uint16_t>(
-                    ic < 0 ? kPlaneMaxModes : (ic >> 1) | (fi < 0 ? kPartner : 0));
+                if (PlaneGeo<kPlaneKc>::empty(k)) {
+                    if (fc >= 0 || ic >= 0) fail(SKB_LOGIC_ERROR, "plane codes: a retained slot in the empty band");
+                    continue;
+                }
+                const size_t at = (static_cast<size_t>(c) * 16 + n) * 16 + code_slot<kPlaneKc>(k);
+                f[at] = static_cast<uint16_t>(fc < 0 ? kPlaneMaxModes : fc >> 1);
+                i[at] = static_cast<uint16_t>(ic < 0 ? kPlaneMaxModes : (ic >> 1) | (fi < 0 ? kPartner : 0));
             }
     cmask.upload(cm.data(), cm.size() * sizeof(uint32_t));
     fcodes.upload(f.data(), f.size() * sizeof(uint16_t));
