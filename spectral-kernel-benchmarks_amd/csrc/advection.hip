@@ -2,15 +2,23 @@
 // (src/pointwise_advection.cpp:152-166).  Six input streams and one output
 // stream; pure HBM-bound elementwise work, so the kernel moves 16 bytes per
 // lane per stream with a grid-stride loop and keeps the sum in the order
-// u*qx + v*qy + w*qz, as the reference's serial form does.
+// u*qx + v*qy + w*qz, as the reference's serial form does (FP32 streams sum
+// in FP64 and round once).
 #include "common.hpp"
 
 namespace skb {
 
 template <typename R>
 __device__ __forceinline__ R advect(R u, R v, R w, R qx, R qy, R qz, R scale) {
-    const R s = u * qx + v * qy + w * qz;
-    return -scale * s;
+    if constexpr (sizeof(R) == 4) {
+        // FP32 operands: the products are exact in FP64, the sum and the
+        // scaling run in FP64 and the target is rounded once to FP32
+        const double s = static_cast<double>(u) * qx + static_cast<double>(v) * qy + static_cast<double>(w) * qz;
+        return static_cast<R>(-static_cast<double>(scale) * s);
+    } else {
+        const R s = u * qx + v * qy + w * qz;
+        return -scale * s;
+    }
 }
 
 template <typename R>

@@ -843,10 +843,18 @@ __device__ __forceinline__ void flux_rows_item(const cx<R>* __restrict__ tin, cx
     cx<R> sh[3][N2];
 #pragma unroll
     for (int d = 0; d < 3; ++d) flux_c2r<R, NX>(sh[d], tin, buf, twl, rin, ps_in, kc, d * nz + z0, z0, nz);
+    // The final summation of the flux, u q_x + v q_y + w q_z, runs in FP64
+    // for FP32 plans of 256 and 512 columns: each product of two FP32 values
+    // is exact in FP64, so the target is the FP64 sum rounded once to FP32.
+    // Those kernels hold one wavefront per SIMD either way; at 64 and 128
+    // columns the FP64 accumulators would halve the wavefronts per SIMD and
+    // at 1024 they would spill, so there the sum is FP32 fused multiply-adds.
+    constexpr bool kWideSum = sizeof(R) == 4 && (NX == 256 || NX == 512);
+    using A = typename std::conditional<kWideSum, double, R>::type;
     for (int tg = 0; tg < targets; ++tg) {
-        cx<R> acc[N2];
+        A ax[N2], ay[N2];
 #pragma unroll
-        for (int k = 0; k < N2; ++k) acc[k] = cx<R>{R(0), R(0)};
+        for (int k = 0; k < N2; ++k) ax[k] = ay[k] = A(0);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const cx<R>* sd = sh[d];
@@ -854,12 +862,14 @@ __device__ __forceinline__ void flux_rows_item(const cx<R>* __restrict__ tin, cx
             flux_c2r<R, NX>(q, tin, buf, twl, rin, ps_in, kc, (3 + 3 * tg + d) * nz + z0, z0, nz);
 #pragma unroll
             for (int k = 0; k < N2; ++k) {
-                acc[k].x += sd[k].x * q[k].x;
-                acc[k].y += sd[k].y * q[k].y;
+                ax[k] += static_cast<A>(sd[k].x) * static_cast<A>(q[k].x);
+                ay[k] += static_cast<A>(sd[k].y) * static_cast<A>(q[k].y);
             }
         }
+        cx<R> acc[N2];
 #pragma unroll
-        for (int k = 0; k < N2; ++k) acc[k] = cx<R>{-scale * acc[k].x, -scale * acc[k].y};
+        for (int k = 0; k < N2; ++k)
+            acc[k] = cx<R>{static_cast<R>(-static_cast<A>(scale) * ax[k]), static_cast<R>(-static_cast<A>(scale) * ay[k])};
         flux_r2c_store<R, NX>(acc, tout, buf, twl, rout, ps_out, kc, tg * nz + z0, z0, nz);
     }
 }

@@ -236,6 +236,30 @@ def test_pointwise_advection_known_answer(device, precision, volume):
     assert O.max_relative_error(host(out), expected) <= bound
 
 
+@pytest.mark.parametrize("volume", [4096, 4099])
+def test_pointwise_advection_fp32_sums_in_fp64(device, volume):
+    """FP32 streams: u*qx + v*qy + w*qz is summed in FP64 and rounded once.
+    With u = qx = 1 + 2^-12 (product 1 + 2^-11 + 2^-24), v*qy = -1 and
+    w*qz = 2^-20, the exact sum 2^-11 + 2^-20 + 2^-24 is an FP32 value; a sum
+    in FP32 drops the product's 2^-24 (relative error 2^-13 = 1.2e-4).  Each
+    element scales u, v and w by 2^(i mod 8 - 4), which scales the sum
+    exactly; the oracle's FP64 sum is exact on these inputs."""
+    import torch
+
+    note("advection fp64 sum v=%d" % volume)
+    i = np.arange(volume)
+    f = np.ldexp(1.0, (i % 8) - 4)
+    a = 1.0 + 2.0 ** -12
+    shared = np.concatenate([a * f, -1.0 * f, 2.0 ** -20 * f])
+    deriv = np.concatenate([np.full(volume, a), np.ones(volume), np.ones(volume)])
+    expected = O.pointwise_advection(volume, 1.0, shared, deriv)
+    assert np.allclose(expected, -(2.0 ** -11 + 2.0 ** -20 + 2.0 ** -24) * f, rtol=0, atol=0)
+    d_s, d_d = dev(shared, torch.float32, device), dev(deriv, torch.float32, device)
+    out = torch.empty(volume, dtype=torch.float32, device=device)
+    S.pointwise_advection(volume, 1.0, d_s, d_d, out, S.SKB_F32)
+    assert np.array_equal(host(out).astype(np.float64), expected)
+
+
 # ---- composed retained operator --------------------------------------------------------
 @pytest.mark.parametrize("kind", ["qr32", "k2"])
 def test_retained_operator_fp64_and_fp32_against_fp64_oracle(device, kind):
