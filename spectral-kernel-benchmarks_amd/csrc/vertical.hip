@@ -952,9 +952,12 @@ __global__ __launch_bounds__(64 * kPmWaves, 2) void gemm_pm_range_kernel(const G
             if (cj) v.y = -v.y;
             buffer_store_cx<float>(v, rout, q == 0 && jok ? ocol + static_cast<uint32_t>(16 * MB) * CB : kBufferPast);
         }
-        // rows 16 mb + 4 q + e of column j on the matrix cores: for each k
-        // block t, B split into bf16 planes [re/im][hi, mid, lo], then the
-        // products of every row block
+        // the product transposed on the matrix cores, B^T A^T: the data
+        // fragment is the A operand and the matrix fragment the B operand, in
+        // the registers they occupy as B and A, so lane (n, q) ends with rows
+        // 16 mb + n of modes 4 q + e (e = 0..3).  For each k block t, B split
+        // into bf16 planes [re/im][hi, mid, lo], then the products of every
+        // row block.
         floatx4 r[MB], im[MB];
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb) r[mb] = im[mb] = floatx4{0, 0, 0, 0};
@@ -981,33 +984,40 @@ __global__ __launch_bounds__(64 * kPmWaves, 2) void gemm_pm_range_kernel(const G
             for (int mb = 0; mb < MB; ++mb) {
                 const bf16x8* A = &sa[((mb * NK + t) * 3) * 64 + lane];
                 const bf16x8 a0 = A[0], a1 = A[64], a2 = A[128];
-                r[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, bb[0][0], r[mb], 0, 0, 0);
-                im[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, bb[1][0], im[mb], 0, 0, 0);
-                r[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bb[0][1], r[mb], 0, 0, 0);
-                im[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bb[1][1], im[mb], 0, 0, 0);
-                r[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bb[0][2], r[mb], 0, 0, 0);
-                im[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bb[1][2], im[mb], 0, 0, 0);
-                r[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bb[0][0], r[mb], 0, 0, 0);
-                im[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bb[1][0], im[mb], 0, 0, 0);
-                r[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bb[0][1], r[mb], 0, 0, 0);
-                im[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bb[1][1], im[mb], 0, 0, 0);
-                r[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bb[0][0], r[mb], 0, 0, 0);
-                im[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bb[1][0], im[mb], 0, 0, 0);
+                r[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bb[0][0], a2, r[mb], 0, 0, 0);
+                im[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bb[1][0], a2, im[mb], 0, 0, 0);
+                r[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bb[0][1], a1, r[mb], 0, 0, 0);
+                im[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bb[1][1], a1, im[mb], 0, 0, 0);
+                r[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bb[0][2], a0, r[mb], 0, 0, 0);
+                im[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bb[1][2], a0, im[mb], 0, 0, 0);
+                r[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bb[0][0], a1, r[mb], 0, 0, 0);
+                im[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bb[1][0], a1, im[mb], 0, 0, 0);
+                r[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bb[0][1], a0, r[mb], 0, 0, 0);
+                im[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bb[1][1], a0, im[mb], 0, 0, 0);
+                r[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bb[0][0], a0, r[mb], 0, 0, 0);
+                im[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bb[1][0], a0, im[mb], 0, 0, 0);
             }
         }
+        // k = KB of modes 4 q + e, from the lanes that loaded them (mode = lane % 16)
+        cx<float> bt[4];
+#pragma unroll
+        for (int e4 = 0; e4 < 4; ++e4) {
+            bt[e4].x = __shfl(b.t.x, 4 * q + e4);
+            bt[e4].y = __shfl(b.t.y, 4 * q + e4);
+        }
+        const uint32_t cbits = cword >> (rng.col_begin & 15);
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb) {
+            const float at = stail[16 * mb + j];
 #pragma unroll
             for (int e4 = 0; e4 < 4; ++e4) {
-                const float at = stail[16 * mb + 4 * q + e4];
-                r[mb][e4] += at * b.t.x;
-                im[mb][e4] += at * b.t.y;
+                r[mb][e4] += at * bt[e4].x;
+                im[mb][e4] += at * bt[e4].y;
+                const int md = 4 * q + e4;  // mode of this element; row 16 mb + j
+                const float y = (cbits >> md) & 1u ? -im[mb][e4] : im[mb][e4];
+                const uint32_t o = static_cast<uint32_t>((F * md + f) * M + 16 * mb + j) * CB;
+                raw_buffer_store_f2(bf2{r[mb][e4], y}, rout, static_cast<int>(md < rng.ncols ? o : kBufferPast), 0, 0);
             }
-            if (cj) im[mb] = -im[mb];
-            const uint32_t at = jok ? ocol + static_cast<uint32_t>(16 * mb + 4 * q) * CB : kBufferPast;
-            raw_buffer_store_f4(bf4{r[mb][0], im[mb][0], r[mb][1], im[mb][1]}, rout, static_cast<int>(at), 0, 0);
-            raw_buffer_store_f4(bf4{r[mb][2], im[mb][2], r[mb][3], im[mb][3]}, rout,
-                                static_cast<int>(jok ? at + 2 * CB : kBufferPast), 0, 0);
         }
     };
     for (int i = 0; i < n; ++i) {  // uniform per wavefront
