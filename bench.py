@@ -470,7 +470,9 @@ def bench_batched(args, S, torch, device):
         batch.launch(side)
         ms = _timed_events(torch, lambda: batch.launch(side), 10, stream=side)
         out[name] = {"batches_per_s": 1e3 / ms, "ms_per_batch": ms, "instance_evals_per_s": 96e3 / ms,
-                     "autotuned_grids": [[st["grid"] for st in c["stages"]] for c in tune]}
+                     "autotuned_grids": [[st["grid"] for st in c["stages"]] for c in tune["classes"]],
+                     "autotuned_products": [c["stages"][4].get("products") for c in tune["classes"]],
+                     "tuning_spec": tune["spec"]}
         del batch, keep
         torch.cuda.synchronize()
     out["graph"] = "one hipGraph replay per batch (5 kernel launches per class, %d classes)" % len(chunks)

@@ -590,11 +590,21 @@ int skb_batch_run(skb_batch* batch, void* stream);
  * stream); skb_batch_launch replays it. */
 int skb_batch_capture(skb_batch* batch, void* stream);
 int skb_batch_launch(skb_batch* batch, void* stream);
-/* Times every launch-shape candidate of every stage of every class (reps
- * launches each) and keeps the fastest; writes a JSON report.  Invalidates a
- * captured graph.  Results agree with the default launch shapes to rounding
- * (see skb_pruned_plan_set_grid). */
+/* Times every candidate of every stage of every class (reps launches each)
+ * and keeps the fastest: each stage's launch shape and, for the vertical
+ * product of FP32 square families, split-BF16 or FP32 products; writes a
+ * JSON report.  Invalidates a captured graph.  Results agree with the
+ * defaults to the FP32 tolerance (see skb_pruned_plan_set_grid,
+ * skb_vertical_plan_set_products). */
 int skb_batch_autotune(skb_batch* batch, int reps, void* stream, char* report, size_t capacity);
+/* The batch's current choices as a spec string
+ * ("c<class>.s<stage>=<grid>;...;c<class>.p=<products>;"), and a spec
+ * applied (all entries checked before any is applied; drops a captured
+ * graph): a kept tuning is reused without timing.  No reference
+ * counterpart (the reference's worker-count tuner keeps a manifest,
+ * tools/run_portable_machine_tuning.py:147, 1452-1497). */
+int skb_batch_tuning(const skb_batch* batch, char* spec, size_t capacity);
+int skb_batch_set_tuning(skb_batch* batch, const char* spec);
 
 /* ---- allocation accounting ---------------------------------------------------
  * Device allocations (count, bytes) the library has made in this process,

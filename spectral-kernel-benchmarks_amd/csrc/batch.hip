@@ -12,11 +12,16 @@
 // skb_batch_launch replays it with a single launch call.  The kernels clear
 // nothing with hipMemsetAsync, so the graph replays correctly.
 //
-// skb_batch_autotune times every launch-shape candidate of every stage of
-// every class (workgroups per launch: one per work item, or a persistent
-// grid) with HIP events and keeps the fastest.  Persistent kernels are
-// separate instantiations, so results agree across launch shapes to
-// rounding, not bit for bit.
+// skb_batch_autotune times every candidate of every stage of every class
+// with HIP events and keeps the fastest: the launch shape (workgroups per
+// launch: one per work item, or a persistent grid) of each stage and, for
+// the vertical product of FP32 square families, its arithmetic (split-BF16
+// or FP32 matrix cores).  Persistent kernels are separate instantiations and
+// the two product forms round differently, so results agree across choices
+// to the FP32 tolerance, not bit for bit.  skb_batch_tuning writes the
+// choices as a spec string ("c<class>.s<stage>=<grid>;c<class>.p=<products>;
+// ..."), and skb_batch_set_tuning applies one, so a tuning can be kept and
+// reused without timing again.
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -36,6 +41,7 @@ struct skb_batch_class {
     void* modal = nullptr;    // caller-owned output
     skb::DeviceBuffer real, back;
     int grid[5] = {0, 0, 0, 0, 0};  // pruned stages 0..3, vertical GEMM
+    int products = SKB_PRODUCTS_SPLIT_BF16;
     ~skb_batch_class() {
         if (pruned) skb_pruned_plan_destroy(pruned);
         if (vertical) skb_vertical_plan_destroy(vertical);
@@ -93,6 +99,31 @@ void apply_grid(skb_batch_class& c, int stage, int grid) {
     if (stage < 4) check_status(skb_pruned_plan_set_grid(c.pruned, stage, grid));
     else check_status(skb_vertical_plan_set_grid(c.vertical, grid));
     c.grid[stage] = grid;
+}
+
+void apply_products(skb_batch_class& c, int products) {
+    check_status(skb_vertical_plan_set_products(c.vertical, products));
+    c.products = products;
+}
+
+// The product forms a class's vertical GEMM can take: split-BF16 and FP32
+// where the family allows the split, FP32 (FP64) alone elsewhere.
+std::vector<int> product_candidates(skb_batch_class& c) {
+    const int was = c.products;
+    apply_products(c, SKB_PRODUCTS_SPLIT_BF16);
+    const bool split = skb_vertical_plan_products(c.vertical, 0) == SKB_PRODUCTS_SPLIT_BF16;
+    apply_products(c, was);
+    return split ? std::vector<int>{SKB_PRODUCTS_SPLIT_BF16, SKB_PRODUCTS_FP32} : std::vector<int>{SKB_PRODUCTS_FP32};
+}
+
+std::string tuning_spec(const skb_batch& b) {
+    std::ostringstream o;
+    for (size_t ci = 0; ci < b.classes.size(); ++ci) {
+        const auto& c = *b.classes[ci];
+        for (int st = 0; st < 5; ++st) o << "c" << ci << ".s" << st << "=" << c.grid[st] << ";";
+        o << "c" << ci << ".p=" << c.products << ";";
+    }
+    return o.str();
 }
 
 }  // namespace
@@ -217,26 +248,36 @@ int skb_batch_autotune(skb_batch* batch, int reps, void* stream, char* report, s
                     std::vector<int> cand = {0};
                     for (uint64_t g : {256ull, 512ull, 1024ull, 2048ull, 4096ull})
                         if (g < items) cand.push_back(static_cast<int>(g));
-                    int best = 0;
+                    const std::vector<int> prods = stage == 4 ? product_candidates(c) : std::vector<int>{c.products};
+                    int best = 0, best_p = prods[0];
                     float best_ms = 1e30f;
-                    for (int g : cand) {
-                        apply_grid(c, stage, g);
-                        run_stage(c, stage, s);  // warm
-                        check_hip(hipEventRecord(a, s), "hipEventRecord");
-                        for (int r = 0; r < reps; ++r) run_stage(c, stage, s);
-                        check_hip(hipEventRecord(e, s), "hipEventRecord");
-                        check_hip(hipEventSynchronize(e), "hipEventSynchronize");
-                        float ms = 0.f;
-                        check_hip(hipEventElapsedTime(&ms, a, e), "hipEventElapsedTime");
-                        // a persistent grid must win by 2% to replace one workgroup per item
-                        if (ms < (g == 0 ? best_ms : 0.98f * best_ms)) {
-                            best_ms = ms;
-                            best = g;
+                    for (int pr : prods) {
+                        if (stage == 4) apply_products(c, pr);
+                        for (int g : cand) {
+                            apply_grid(c, stage, g);
+                            run_stage(c, stage, s);  // warm
+                            check_hip(hipEventRecord(a, s), "hipEventRecord");
+                            for (int r = 0; r < reps; ++r) run_stage(c, stage, s);
+                            check_hip(hipEventRecord(e, s), "hipEventRecord");
+                            check_hip(hipEventSynchronize(e), "hipEventSynchronize");
+                            float ms = 0.f;
+                            check_hip(hipEventElapsedTime(&ms, a, e), "hipEventElapsedTime");
+                            // anything but the default (one workgroup per item, the
+                            // first product form) must win by 2% to replace it
+                            const bool first = g == 0 && pr == prods[0];
+                            if (ms < (first ? best_ms : 0.98f * best_ms)) {
+                                best_ms = ms;
+                                best = g;
+                                best_p = pr;
+                            }
                         }
                     }
+                    if (stage == 4) apply_products(c, best_p);
                     apply_grid(c, stage, best);
                     js << (stage ? "," : "") << "{\"stage\":" << stage << ",\"items\":" << items
-                       << ",\"grid\":" << best << ",\"us\":" << 1e3 * best_ms / reps << "}";
+                       << ",\"grid\":" << best << ",\"us\":" << 1e3 * best_ms / reps;
+                    if (stage == 4) js << ",\"products\":" << best_p << ",\"product_forms\":" << prods.size();
+                    js << "}";
                 }
                 js << "]}";
             }
@@ -255,6 +296,56 @@ int skb_batch_autotune(skb_batch* batch, int reps, void* stream, char* report, s
             std::memcpy(report, out.data(), n);
             report[n] = '\0';
         }
+    });
+}
+
+int skb_batch_tuning(const skb_batch* batch, char* spec, size_t capacity) {
+    return guarded([&] {
+        if (!batch || !spec || capacity == 0) fail(SKB_INVALID_ARGUMENT, "null argument");
+        const std::string out = tuning_spec(*batch);
+        if (out.size() + 1 > capacity) fail(SKB_INVALID_ARGUMENT, "tuning spec buffer too small");
+        std::memcpy(spec, out.c_str(), out.size() + 1);
+    });
+}
+
+int skb_batch_set_tuning(skb_batch* batch, const char* spec) {
+    return guarded([&] {
+        if (!batch || !spec) fail(SKB_INVALID_ARGUMENT, "null argument");
+        // parse everything first, so a bad spec changes nothing
+        struct Choice {
+            int cls, stage, value;  // stage 5 = products
+        };
+        std::vector<Choice> choices;
+        const std::string text(spec);
+        size_t at = 0;
+        while (at < text.size()) {
+            size_t end = text.find(';', at);
+            if (end == std::string::npos) end = text.size();
+            const std::string tok = text.substr(at, end - at);
+            at = end + 1;
+            if (tok.empty()) continue;
+            int cls = -1, stage = -1, value = 0, used = 0;
+            if (std::sscanf(tok.c_str(), "c%d.s%d=%d%n", &cls, &stage, &value, &used) == 3 &&
+                used == static_cast<int>(tok.size())) {
+                if (stage < 0 || stage > 4 || value < 0) fail(SKB_INVALID_ARGUMENT, "bad tuning entry: " + tok);
+            } else if (std::sscanf(tok.c_str(), "c%d.p=%d%n", &cls, &value, &used) == 2 &&
+                       used == static_cast<int>(tok.size())) {
+                stage = 5;
+                if (value != SKB_PRODUCTS_FP32 && value != SKB_PRODUCTS_SPLIT_BF16)
+                    fail(SKB_INVALID_ARGUMENT, "bad tuning entry: " + tok);
+            } else {
+                fail(SKB_INVALID_ARGUMENT, "bad tuning entry: " + tok);
+            }
+            if (cls < 0 || cls >= static_cast<int>(batch->classes.size()))
+                fail(SKB_INVALID_ARGUMENT, "tuning entry for a class the batch does not have: " + tok);
+            choices.push_back({cls, stage, value});
+        }
+        for (const auto& ch : choices) {
+            auto& c = *batch->classes[ch.cls];
+            if (ch.stage == 5) apply_products(c, ch.value);
+            else apply_grid(c, ch.stage, ch.value);
+        }
+        drop_graph(*batch);
     });
 }
 

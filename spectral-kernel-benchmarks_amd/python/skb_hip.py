@@ -160,6 +160,8 @@ def _load():
         "skb_batch_capture": (i32, [vp, vp]),
         "skb_batch_launch": (i32, [vp, vp]),
         "skb_batch_autotune": (i32, [vp, i32, vp, ctypes.c_char_p, ctypes.c_size_t]),
+        "skb_batch_tuning": (i32, [vp, ctypes.c_char_p, ctypes.c_size_t]),
+        "skb_batch_set_tuning": (i32, [vp, ctypes.c_char_p]),
         "skb_vertical_wvm_forward": (i32, [vp, vp, vp, vp]),
         "skb_vertical_wvm_inverse": (i32, [vp, vp, vp, vp]),
         "skb_vertical_wvm_modal_elements": (u64, [vp]),
@@ -241,6 +243,7 @@ EXPORTED = [
     "skb_advection_flux_execute",
     "skb_vertical_derivative_matrix", "skb_batch_create", "skb_batch_destroy", "skb_batch_add_class",
     "skb_batch_bind", "skb_batch_run", "skb_batch_capture", "skb_batch_launch", "skb_batch_autotune",
+    "skb_batch_tuning", "skb_batch_set_tuning",
     "skb_advection_flux_device_bytes", "skb_fft_gather_retained_modal", "skb_synthetic_modal_weights",
     "skb_modal_work", "skb_modal_work_split", "skb_vertical_wvm_modal_work", "skb_spectral_pipeline_create",
     "skb_spectral_pipeline_destroy", "skb_spectral_pipeline_execute", "skb_spectral_pipeline_stages",
@@ -1272,12 +1275,58 @@ class Batch:
     def launch(self, stream):
         check(lib.skb_batch_launch(self.h, _stream(stream)))
 
-    def autotune(self, reps: int = 20, stream=None):
-        import json
+    def tuning(self) -> str:
+        """The current choices (launch shape per stage, products per class) as a spec string."""
+        buf = ctypes.create_string_buffer(1 << 14)
+        check(lib.skb_batch_tuning(self.h, buf, len(buf)))
+        return buf.value.decode()
 
+    def set_tuning(self, spec: str):
+        check(lib.skb_batch_set_tuning(self.h, spec.encode()))
+
+    def signature(self) -> dict:
+        """What a kept tuning is valid for: the device, the library and every class's shape."""
+        device = ""
+        try:
+            import torch
+
+            device = torch.cuda.get_device_name(0)
+        except Exception:  # noqa: BLE001 - the name is informative only
+            pass
+        return {"device": device, "library": version(), "precision": self.precision,
+                "classes": [[c["w"].nx, c["w"].ny, c["w"].nz, c["w"].fields, c["instances"], c["modes"], c["nj"]]
+                            for c in self.classes]}
+
+    def autotune(self, reps: int = 20, stream=None, cache=None):
+        """Times the candidates of every stage of every class and keeps the
+        fastest (skb_batch_autotune); returns the per-class report.  With
+        `cache` (a JSON file path), a tuning kept there for this batch's
+        signature is applied without timing (the report then carries
+        "cached": true), and a new tuning is added to the file, which keeps
+        one entry per signature (written to a temporary file, then renamed)."""
+        import json
+        import os
+
+        key = json.dumps(self.signature(), sort_keys=True)
+        manifest = {}
+        if cache and os.path.exists(cache):
+            with open(cache) as f:
+                manifest = json.load(f)
+            entry = manifest.get(key)
+            if entry:
+                self.set_tuning(entry["spec"])
+                return {"cached": True, "spec": entry["spec"], "classes": entry["report"]}
         buf = ctypes.create_string_buffer(1 << 16)
         check(lib.skb_batch_autotune(self.h, reps, _stream(stream), buf, len(buf)))
-        return json.loads(buf.value.decode())
+        report = json.loads(buf.value.decode())
+        spec = self.tuning()
+        if cache:
+            manifest[key] = {"spec": spec, "report": report, "reps": reps}
+            tmp = "%s.tmp%d" % (cache, os.getpid())
+            with open(tmp, "w") as f:
+                json.dump(manifest, f, indent=1, sort_keys=True)
+            os.replace(tmp, cache)
+        return {"cached": False, "spec": spec, "classes": report}
 
     def close(self):
         if getattr(self, "h", None) and lib is not None:

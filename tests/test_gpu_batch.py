@@ -53,7 +53,7 @@ def alone(w, modes, groups, fwd, inv, x, precision, device):
 
 
 @pytest.mark.parametrize("precision", [S.SKB_F32, S.SKB_F64])
-def test_batch_mix_equals_alone_and_graph_replays(device, precision):
+def test_batch_mix_equals_alone_and_graph_replays(device, precision, tmp_path):
     import torch
 
     shapes = [((128, 128, 33, 3), 3), ((256, 256, 65, 3), 2), ((64, 64, 17, 4), 2)]
@@ -92,14 +92,36 @@ def test_batch_mix_equals_alone_and_graph_replays(device, precision):
         side.synchronize()
         for c, ref in zip(cases, eager):
             assert np.array_equal(host(c[-1]), ref), r
-    # autotuned launch shapes give the same numbers
-    report = batch.autotune(5, side)
-    assert len(report) == len(shapes) and all(len(c["stages"]) == 5 for c in report)
+    # autotuned launch shapes (and, for FP32 square families, product forms)
+    # give the same numbers to the tolerance; the tuning is kept in a
+    # manifest and reapplied without timing
+    cache = str(tmp_path / "tuning.json")
+    tune = batch.autotune(5, side, cache=cache)
+    report = tune["classes"]
+    assert not tune["cached"] and len(report) == len(shapes) and all(len(c["stages"]) == 5 for c in report)
+    if precision == S.SKB_F32:  # the 33- and 65-level square families offer two product forms
+        assert [c["stages"][4]["product_forms"] for c in report] == [2, 2, 1]
+    assert tune["spec"] == batch.tuning()
     batch.capture(side)
     batch.launch(side)
     side.synchronize()
     for c, ref in zip(cases, eager):
         assert O.max_relative_error(host(c[-1]).astype(np.complex128), ref.astype(np.complex128)) <= tol
+    again = batch.autotune(5, side, cache=cache)
+    assert again["cached"] and again["spec"] == tune["spec"] and again["classes"] == report
+    # a spec applies whole or not at all
+    with pytest.raises(ValueError):
+        batch.set_tuning("c0.s1=256;c9.s0=0;")
+    with pytest.raises(ValueError):
+        batch.set_tuning("c0.s7=0;")
+    assert batch.tuning() == tune["spec"]
+    # a partial spec changes only its entries
+    batch.set_tuning("c0.s0=256;")
+    changed = batch.tuning()
+    assert "c0.s0=256;" in changed
+    assert changed.replace("c0.s0=256;", "") == tune["spec"].replace(tune["spec"].split(";")[0] + ";", "")
+    batch.set_tuning(tune["spec"])
+    assert batch.tuning() == tune["spec"]
 
 
 def test_batch_requires_bound_arrays(device):
