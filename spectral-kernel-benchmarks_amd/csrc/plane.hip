@@ -133,6 +133,34 @@ __device__ __forceinline__ void fft256(cx<float>* v, float* s, const cx<float>* 
     rdft<16, INV>(v);
 }
 
+// Two independent 256-point FFTs per 16-lane group (the two column rounds of
+// one wavefront), interleaved: a's exchange drains while b's first pass
+// runs, and a's second pass overlaps b's exchange.  The exchanges share the
+// wavefront's scratch; LDS operations of a wavefront complete in order, so
+// b's writes follow a's reads.
+template <bool INV>
+__device__ __forceinline__ void fft256x2(cx<float>* a, cx<float>* b, float* s, const cx<float>* tw, int g, int n) {
+    const bf4* row = reinterpret_cast<const bf4*>(tw + kTwiddlePitch * n);
+    rdft<16, INV>(a);
+#pragma unroll
+    for (int h = 0; h < 8; ++h) {
+        const bf4 w = row[h];
+        if (h) a[2 * h] = INV ? cmulc(a[2 * h], cx<float>{w.x, w.y}) : cmul(a[2 * h], cx<float>{w.x, w.y});
+        a[2 * h + 1] = INV ? cmulc(a[2 * h + 1], cx<float>{w.z, w.w}) : cmul(a[2 * h + 1], cx<float>{w.z, w.w});
+    }
+    wave_exchange16(s, a, g, n);
+    rdft<16, INV>(b);
+#pragma unroll
+    for (int h = 0; h < 8; ++h) {
+        const bf4 w = row[h];
+        if (h) b[2 * h] = INV ? cmulc(b[2 * h], cx<float>{w.x, w.y}) : cmul(b[2 * h], cx<float>{w.x, w.y});
+        b[2 * h + 1] = INV ? cmulc(b[2 * h + 1], cx<float>{w.z, w.w}) : cmul(b[2 * h + 1], cx<float>{w.z, w.w});
+    }
+    wave_exchange16(s, b, g, n);
+    rdft<16, INV>(a);
+    rdft<16, INV>(b);
+}
+
 // Lane i of each 16-lane row receives the value of lane (16 - i) mod 16
 // (row_mirror: lane i <- 15 - i, then row_ror:1: lane i <- i - 1).
 __device__ __forceinline__ float from_partner_lane(float v) {
@@ -314,15 +342,15 @@ __device__ __forceinline__ void inverse_plane(int wbase, unsigned char* lds, con
     //    y[r][k]; rows 0..127 (k < 8) go to the row buffer at once, rows
     //    128..255 stay in registers for the second half
     const int f = 4 * wave + g;
+    static_assert(G::ROUNDS == 2, "two column rounds");
+    if (64 + 4 * wave < KC) fft256x2<true>(y[0], y[1], wt, tw, g, n);
+    else fft256<true>(y[0], wt, tw, g, n);
 #pragma unroll
     for (int r = 0; r < G::ROUNDS; ++r) {
         const int c = 64 * r + 4 * wave + g;
-        if (64 * r + 4 * wave < KC) {
-            fft256<true>(y[r], wt, tw, g, n);
-            if (c < KC) {
+        if (64 * r + 4 * wave < KC && c < KC) {
 #pragma unroll
-                for (int k = 0; k < 8; ++k) r0[(n + 16 * k) * RP + c] = y[r][k];
-            }
+            for (int k = 0; k < 8; ++k) r0[(n + 16 * k) * RP + c] = y[r][k];
         }
     }
 
