@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Retained-operator throughput of the hip provider on one MI355X.
 
-One step = one batch of --batch (64) independent retained-operator
+One step = one batch of --batch (128) independent retained-operator
 evaluations of the 256^2 x Nz65 x 3 workload, distinct inputs and outputs
 resident in HBM, enqueued by ONE C-ABI call
 (skb_retained_operator_execute_batch), each evaluation:
@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=40,
                     help="timed steps; one step is one batch of --batch evaluations")
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--batch", type=int, default=64,
+    ap.add_argument("--batch", type=int, default=128,
                     help="independent evaluations per step (one execute_batch call)")
     ap.add_argument("--batch-mode", choices=("stage", "lanes"), default="stage",
                     help="stage: one launch per stage over the batch; lanes: evaluations over --lanes streams")
@@ -75,7 +75,7 @@ def parse():
 
 
 COPY_BYTES = 256 << 20
-BATCH_CAP = 64  # evaluations per stage-batched launch (skb_retained_operator_reserve_batch's maximum)
+BATCH_CAP = 128  # evaluations per stage-batched launch (skb_retained_operator_reserve_batch's maximum)
 KERNEL_NAMES = {  # substring of the device kernel name -> stage of the (stage-batched) evaluation
     "pm_to_pm_batch_kernel": "layout",
     "plane_operator_kernel<86>": "transforms",

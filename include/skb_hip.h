@@ -297,7 +297,7 @@ int skb_retained_operator_fused(const skb_retained_operator* op);
 int skb_retained_operator_reserve_lanes(skb_retained_operator* op, int lanes);
 /* Lanes reserved (>= 1); *apart = 1 when every lane has a queue of its own. */
 int skb_retained_operator_lanes(const skb_retained_operator* op, int* apart);
-/* Stage-batched evaluation.  reserve_batch(n), n in [2, 64], gives the plan
+/* Stage-batched evaluation.  reserve_batch(n), n in [2, 128], gives the plan
  * the plane-major blocks and real planes of n evaluations (it allocates; 0
  * or 1 releases them).  execute_batch then takes the evaluations in groups
  * of up to n and runs each stage as ONE launch over the group: the layout

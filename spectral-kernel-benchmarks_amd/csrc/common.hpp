@@ -97,7 +97,7 @@ inline bool is_pow2(uint64_t v) { return v && !(v & (v - 1)); }
 
 // Evaluations one batched launch takes at most (their pointers travel as
 // kernel arguments).
-constexpr int kBatchMax = 64;
+constexpr int kBatchMax = 128;
 
 // XCD-aware work mapping.  MI355X dispatches workgroup b to XCD b % 8, and
 // each XCD has its own L2.  With one workgroup per item, item ranges are

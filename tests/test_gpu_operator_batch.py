@@ -122,7 +122,7 @@ def test_batch_allocates_nothing_and_matches_fp64(device):
         op.reserve_lanes(0)
 
 
-@pytest.mark.parametrize("nz,cap", [(65, 4), (65, 64), (33, 8)])
+@pytest.mark.parametrize("nz,cap", [(65, 4), (65, 128), (33, 8)])
 def test_stage_batch_equals_alone_bit_for_bit(device, nz, cap):
     """reserve_batch: every stage one launch over a group of evaluations."""
     import torch
@@ -196,7 +196,7 @@ def test_stage_batch_reservation_limits(device):
     w = S.Workload(256, 256, 65, 3)
     op, modes, groups, f, i, xs = make(w, S.SKB_F32, device, 1)
     with pytest.raises(ValueError):
-        op.reserve_batch(65)
+        op.reserve_batch(129)
     with pytest.raises(ValueError):
         op.reserve_batch(-1)
     op64 = S.OperatorPlan(w, modes, w.nz, groups, f, i, S.SKB_F64)
